@@ -1,0 +1,12 @@
+"""atmos -- host side of the MI355X (gfx950) atmosphere hot-path library.
+
+* `model.Model`: ctypes binding of libatmos_hip.so with the reference's
+  module-procedure names (cgrid_dynamics_step, perform_vertical_remapping,
+  calculate_radiative_heating, ...).
+* `driver`: init_model / run_model / calculate_dt of src/mod_model.f90.
+* `config`: config.nml namelists and ISO-8601 durations.
+* `ic`: seeded synthetic initial conditions.
+"""
+from . import config, driver, ic, model  # noqa: F401
+
+__all__ = ["config", "driver", "ic", "model"]

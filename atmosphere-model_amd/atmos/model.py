@@ -1,0 +1,204 @@
+"""ctypes binding of libatmos_hip.so (include/atm.h) with the reference's
+module-procedure names, so the same host driver runs the GPU library and
+the CPU oracle.  Field arrays cross the boundary in the reference's halo'd
+Fortran layout (numpy shape (nk, ny+14, nx+14), i fastest).
+
+There is no fallback: if the HIP library is missing or no GPU is visible,
+constructing a Model raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HALO = 7
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libatmos_hip.so")
+
+FIELDS = {"dp": 0, "pt": 1, "ud": 2, "vd": 3, "ts": 4, "gz": 5, "net_flux": 6,
+          "plev": 7, "pkap": 8, "radius": 9, "dpc": 10, "ptc": 11, "uc": 12, "vc": 13}
+STATUS = {0: "ok", 1: "argument", 2: "hip", 3: "table range", 4: "non-finite", 5: "state"}
+EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm_set_coordinate",
+           "atm_put_field", "atm_get_field", "atm_put_field_device", "atm_get_field_device",
+           "atm_init_prognostic_fields", "atm_initial_halo_exchange", "atm_init_sw_dyn",
+           "atm_cgrid_step", "atm_cgrid_halo", "atm_dgrid_step", "atm_dgrid_halo",
+           "atm_remap_init", "atm_remap", "atm_radiation_init", "atm_radiation",
+           "atm_radiation_halo", "atm_halo_exchange", "atm_check_stability", "atm_sync",
+           "atm_build_info"]
+
+
+class AtmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"atm status {code} ({STATUS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nlay", C.c_int32),
+                ("dx", C.c_double), ("dy", C.c_double)]
+
+
+class Params(C.Structure):
+    _fields_ = [("gravity", C.c_double), ("coriolis_parameter", C.c_double),
+                ("divergence_damping_coefficient", C.c_double),
+                ("vorticity_damping_coefficient", C.c_double)]
+
+
+_LIB = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libatmos_hip.so and declare its signatures (no GPU needed)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    L = C.CDLL(path)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
+    L.atm_create.argtypes = [C.POINTER(Grid), C.POINTER(Params), ip, C.POINTER(vp)]
+    L.atm_last_error.restype = C.c_char_p
+    L.atm_last_error.argtypes = [vp]
+    L.atm_build_info.restype = C.c_char_p
+    for n in ["atm_destroy", "atm_init_prognostic_fields", "atm_initial_halo_exchange",
+              "atm_init_sw_dyn", "atm_cgrid_halo", "atm_dgrid_halo", "atm_remap_init", "atm_remap",
+              "atm_radiation_init", "atm_radiation_halo", "atm_sync"]:
+        getattr(L, n).argtypes = [vp]
+    L.atm_set_stream.argtypes = [vp, vp]
+    L.atm_set_coordinate.argtypes = [vp, dp, dp]
+    for n in ["atm_put_field", "atm_get_field"]:
+        getattr(L, n).argtypes = [vp, ip, dp]
+    for n in ["atm_put_field_device", "atm_get_field_device"]:
+        getattr(L, n).argtypes = [vp, ip, vp]
+    for n in ["atm_cgrid_step", "atm_dgrid_step", "atm_radiation"]:
+        getattr(L, n).argtypes = [vp, C.c_double]
+    L.atm_halo_exchange.argtypes = [vp, ip]
+    L.atm_check_stability.argtypes = [vp, C.POINTER(C.c_int)]
+    _LIB = L
+    return L
+
+
+def _dptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Model:
+    """One model instance on the current GPU (FP64 drop-in path)."""
+
+    def __init__(self, nx, ny, nlay, dx, dy=None, gravity=9.807, coriolis=1e-5,
+                 divergence_damping=0.15, vorticity_damping=0.005):
+        self.nx, self.ny, self.nlay = nx, ny, nlay
+        self.NI, self.NJ = nx + 2 * HALO, ny + 2 * HALO
+        self._L = load_library()
+        h = C.c_void_p()
+        grid = Grid(nx, ny, nlay, dx, dx if dy is None else dy)
+        prm = Params(gravity, coriolis, divergence_damping, vorticity_damping)
+        rc = self._L.atm_create(C.byref(grid), C.byref(prm), 0, C.byref(h))
+        if rc != 0:
+            raise AtmError(rc, "atm_create failed (no GPU or bad sizes)")
+        self._h = h
+        self._ak = self._bk = None
+        self._gzs = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.atm_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise AtmError(rc, self._L.atm_last_error(self._h).decode())
+
+    def nk(self, name):
+        return {"ts": 1, "radius": 1, "gz": self.nlay + 1, "net_flux": self.nlay + 1,
+                "plev": self.nlay + 1, "pkap": self.nlay + 1}.get(name, self.nlay)
+
+    # ---- field transfer (halo'd layout) ----
+    def put_full(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.nk(name), self.NJ, self.NI)
+        self._ck(self._L.atm_put_field(self._h, FIELDS[name], _dptr(a)))
+
+    def get_full(self, name):
+        a = np.empty((self.nk(name), self.NJ, self.NI))
+        self._ck(self._L.atm_get_field(self._h, FIELDS[name], _dptr(a)))
+        return a
+
+    def put(self, name, arr):
+        """Interior values (nk, ny, nx) or (ny, nx); the halo is zeroed."""
+        arr = np.asarray(arr, dtype=np.float64)
+        if name in ("ak", "bk"):
+            if name == "ak":
+                self._ak = np.ascontiguousarray(arr)
+            else:
+                self._bk = np.ascontiguousarray(arr)
+            if self._ak is not None and self._bk is not None:
+                self._ck(self._L.atm_set_coordinate(self._h, _dptr(self._ak), _dptr(self._bk)))
+            return
+        if name == "gzs":
+            full = np.zeros((self.nlay + 1, self.NJ, self.NI))
+            full[0, HALO:HALO + self.ny, HALO:HALO + self.nx] = arr
+            self.put_full("gz", full)
+            return
+        full = np.zeros((self.nk(name), self.NJ, self.NI))
+        full[..., HALO:HALO + self.ny, HALO:HALO + self.nx] = arr
+        self.put_full(name, full)
+
+    def get(self, name):
+        a = self.get_full(name)[..., HALO:HALO + self.ny, HALO:HALO + self.nx]
+        return np.ascontiguousarray(a[0] if self.nk(name) == 1 else a)
+
+    # ---- reference module procedures ----
+    def init_prognostic_fields(self):
+        self._ck(self._L.atm_init_prognostic_fields(self._h))
+
+    def initial_halo_exchange(self):
+        self._ck(self._L.atm_initial_halo_exchange(self._h))
+
+    def init_sw_dyn(self):
+        self._ck(self._L.atm_init_sw_dyn(self._h))
+
+    def allocate_radiation_arrays(self):
+        self._ck(self._L.atm_radiation_init(self._h))
+
+    def allocate_remapping_arrays(self):
+        self._ck(self._L.atm_remap_init(self._h))
+
+    def calculate_radiative_heating(self, time):
+        self._ck(self._L.atm_radiation(self._h, time))
+
+    def radiation_halo_exchange(self):
+        self._ck(self._L.atm_radiation_halo(self._h))
+
+    def cgrid_dynamics_step(self, dt):
+        self._ck(self._L.atm_cgrid_step(self._h, dt))
+
+    def cgrid_halo_exchange(self):
+        self._ck(self._L.atm_cgrid_halo(self._h))
+
+    def dgrid_dynamics_step(self, dt):
+        self._ck(self._L.atm_dgrid_step(self._h, dt))
+
+    def dgrid_halo_exchange(self):
+        self._ck(self._L.atm_dgrid_halo(self._h))
+
+    def perform_vertical_remapping(self):
+        self._ck(self._L.atm_remap(self._h))
+
+    def halo_exchange(self, name):
+        self._ck(self._L.atm_halo_exchange(self._h, FIELDS[name]))
+
+    def check_stability(self):
+        s = C.c_int(0)
+        self._ck(self._L.atm_check_stability(self._h, C.byref(s)))
+        return bool(s.value)
+
+    def sync(self):
+        self._ck(self._L.atm_sync(self._h))
+
+    def set_stream(self, stream_handle):
+        self._ck(self._L.atm_set_stream(self._h, C.c_void_p(stream_handle)))

@@ -1,0 +1,317 @@
+// atm_api.hip -- C ABI entry points (include/atm.h) of the gfx950 library.
+//
+// Each entry point replaces one Fortran module procedure of the reference
+// (cited per function); device buffers mirror the reference's module
+// arrays in its halo'd Fortran layout.
+#include <cstdio>
+#include <cstring>
+
+#include "atm_common.hpp"
+
+int atm_fail(atm_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define CHECK_CTX(c) \
+    if (!(c)) return ATM_ERR_ARGUMENT
+
+static double *field_ptr(atm_ctx *c, int id, int *nk) {
+    const int L = c->g.L;
+    switch (id) {
+    case ATM_DP: *nk = L; return c->dp;
+    case ATM_PT: *nk = L; return c->pt;
+    case ATM_UD: *nk = L; return c->ud;
+    case ATM_VD: *nk = L; return c->vd;
+    case ATM_TS: *nk = 1; return c->ts;
+    case ATM_GZ: *nk = L + 1; return c->gz;
+    case ATM_NET_FLUX: *nk = L + 1; return c->net_flux;
+    case ATM_PLEV: *nk = L + 1; return c->plev;
+    case ATM_PKAP: *nk = L + 1; return c->pkap;
+    case ATM_RADIUS: *nk = 1; return c->radius;
+    case ATM_DPC: *nk = L; return c->dpc;
+    case ATM_PTC: *nk = L; return c->ptc;
+    case ATM_UC: *nk = L; return c->uc;
+    case ATM_VC: *nk = L; return c->vc;
+    default: *nk = 0; return nullptr;
+    }
+}
+
+static int dalloc(atm_ctx *c, double **p, size_t n) {
+    ATM_HIP(c, hipMalloc(p, sizeof(double) * (n ? n : 1)));
+    ATM_HIP(c, hipMemsetAsync(*p, 0, sizeof(double) * (n ? n : 1), c->stream));
+    return ATM_OK;
+}
+
+static void release(atm_ctx *c) {
+    double **all[] = {&c->ak, &c->bk, &c->dp, &c->pt, &c->ud, &c->vd, &c->ts, &c->gz, &c->net_flux,
+                      &c->plev, &c->pkap, &c->play, &c->playkap, &c->radius, &c->dpc, &c->ptc,
+                      &c->uc, &c->vc, &c->heat, &c->pkapb, &c->gzb, &c->tmp, &c->lap, &c->cx,
+                      &c->cy, &c->denx, &c->deny, &c->dpfx, &c->dpfy, &c->fxq, &c->fyq, &c->vort,
+                      &c->ke, &c->pgfx, &c->pgfy, &c->divg, &c->ua, &c->va, &c->ub, &c->vb};
+    for (auto p : all)
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (c->flag) (void)hipFree(c->flag);
+    if (c->flag_host) (void)hipHostFree(c->flag_host);
+    atm_remap_free(c);
+    atm_rad_free(c);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+extern "C" {
+
+int atm_create(const atm_grid *grid, const atm_params *params, int precision, atm_ctx **out) {
+    if (!grid || !params || !out) return ATM_ERR_ARGUMENT;
+    *out = nullptr;
+    if (precision != ATM_PRECISION_FP64) return ATM_ERR_ARGUMENT;
+    // the halo is 7 wide and vertical_ppm needs at least 4 layers
+    if (grid->nx < ATM_HALO || grid->ny < ATM_HALO || grid->nlay < 4) return ATM_ERR_ARGUMENT;
+    if (!(grid->dx > 0) || !(grid->dy > 0)) return ATM_ERR_ARGUMENT;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return ATM_ERR_HIP;
+    atm_ctx *c = new atm_ctx();
+    c->g.nx = grid->nx;
+    c->g.ny = grid->ny;
+    c->g.L = grid->nlay;
+    c->g.NI = grid->nx + 2 * ATM_HALO;
+    c->g.NJ = grid->ny + 2 * ATM_HALO;
+    c->dx = grid->dx;
+    c->dy = grid->dy;
+    c->prm = *params;
+    int rc = ATM_OK;
+    auto fail = [&](int code) {
+        release(c);
+        delete c;
+        return code;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) return fail(ATM_ERR_HIP);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(ATM_ERR_HIP);
+    c->own_stream = true;
+    const size_t p = c->g.plane(), L = c->g.L;
+    struct { double **ptr; size_t n; } bufs[] = {
+        {&c->ak, L + 1}, {&c->bk, L + 1}, {&c->dp, p * L}, {&c->pt, p * L}, {&c->ud, p * L},
+        {&c->vd, p * L}, {&c->ts, p}, {&c->gz, p * (L + 1)}, {&c->net_flux, p * (L + 1)},
+        {&c->plev, p * (L + 1)}, {&c->pkap, p * (L + 1)}, {&c->play, p * L}, {&c->playkap, p * L},
+        {&c->radius, p}, {&c->dpc, p * L}, {&c->ptc, p * L}, {&c->uc, p * L}, {&c->vc, p * L},
+        {&c->heat, p * L}, {&c->pkapb, p * (L + 1)}, {&c->gzb, p * (L + 1)}, {&c->tmp, p * L},
+        {&c->lap, p * L}, {&c->cx, p * L}, {&c->cy, p * L}, {&c->denx, p * L}, {&c->deny, p * L},
+        {&c->dpfx, p * L}, {&c->dpfy, p * L}, {&c->fxq, p * L}, {&c->fyq, p * L}, {&c->vort, p * L},
+        {&c->ke, p * L}, {&c->pgfx, p * L}, {&c->pgfy, p * L}, {&c->divg, p * L}, {&c->ua, p},
+        {&c->va, p}, {&c->ub, p}, {&c->vb, p}};
+    for (auto &b : bufs) {
+        rc = dalloc(c, b.ptr, b.n);
+        if (rc) return fail(rc);
+    }
+    if (hipMalloc(&c->flag, sizeof(int)) != hipSuccess) return fail(ATM_ERR_HIP);
+    if (hipHostMalloc((void **)&c->flag_host, sizeof(int), hipHostMallocDefault) != hipSuccess)
+        return fail(ATM_ERR_HIP);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(ATM_ERR_HIP);
+    *out = c;
+    return ATM_OK;
+}
+
+int atm_destroy(atm_ctx *c) {
+    CHECK_CTX(c);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    release(c);
+    delete c;
+    return ATM_OK;
+}
+
+const char *atm_last_error(const atm_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int atm_set_stream(atm_ctx *c, void *s) {
+    CHECK_CTX(c);
+    ATM_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->own_stream) {
+        if (s == nullptr) return ATM_OK;
+        (void)hipStreamDestroy(c->stream);
+        c->own_stream = false;
+        c->stream = (hipStream_t)s;
+        return ATM_OK;
+    }
+    if (s == nullptr) {
+        ATM_HIP(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+    } else {
+        c->stream = (hipStream_t)s;
+    }
+    return ATM_OK;
+}
+
+int atm_set_coordinate(atm_ctx *c, const double *ak, const double *bk) {
+    CHECK_CTX(c);
+    if (!ak || !bk) return atm_fail(c, ATM_ERR_ARGUMENT, "null coordinate array");
+    const size_t n = c->g.L + 1;
+    ATM_HIP(c, hipMemcpyAsync(c->ak, ak, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    ATM_HIP(c, hipMemcpyAsync(c->bk, bk, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    ATM_HIP(c, hipStreamSynchronize(c->stream));
+    c->ak0 = ak[0];
+    c->have_coord = true;
+    return ATM_OK;
+}
+
+static int xfer(atm_ctx *c, int id, void *dst, const void *src, bool to_dev, hipMemcpyKind kind, bool block) {
+    CHECK_CTX(c);
+    int nk = 0;
+    double *d = field_ptr(c, id, &nk);
+    if (!d) return atm_fail(c, ATM_ERR_ARGUMENT, "unknown field id " + std::to_string(id));
+    if (!(to_dev ? src : dst)) return atm_fail(c, ATM_ERR_ARGUMENT, "null array");
+    size_t bytes = sizeof(double) * c->g.plane() * nk;
+    if (to_dev)
+        ATM_HIP(c, hipMemcpyAsync(d, src, bytes, kind, c->stream));
+    else
+        ATM_HIP(c, hipMemcpyAsync(dst, d, bytes, kind, c->stream));
+    if (block) ATM_HIP(c, hipStreamSynchronize(c->stream));
+    return ATM_OK;
+}
+
+int atm_put_field(atm_ctx *c, int id, const double *h) {
+    return xfer(c, id, nullptr, h, true, hipMemcpyHostToDevice, true);
+}
+int atm_get_field(atm_ctx *c, int id, double *h) {
+    return xfer(c, id, h, nullptr, false, hipMemcpyDeviceToHost, true);
+}
+int atm_put_field_device(atm_ctx *c, int id, const double *d) {
+    return xfer(c, id, nullptr, d, true, hipMemcpyDeviceToDevice, false);
+}
+int atm_get_field_device(atm_ctx *c, int id, double *d) {
+    return xfer(c, id, d, nullptr, false, hipMemcpyDeviceToDevice, false);
+}
+
+// init_prognostic_fields (mod_input.f90:229-231)
+int atm_init_prognostic_fields(atm_ctx *c) {
+    CHECK_CTX(c);
+    if (!c->have_coord) return atm_fail(c, ATM_ERR_STATE, "atm_set_coordinate first");
+    const size_t p = c->g.plane();
+    int rc = atm_launch_fill(c, c->plev + p * c->g.L, p, c->ak0);
+    if (rc) return rc;
+    return atm_launch_fill(c, c->pkap + p * c->g.L, p, pow(c->ak0, ATM_KAPPA));
+}
+
+// initial_halo_exchange (mod_input.f90:247-252)
+int atm_initial_halo_exchange(atm_ctx *c) {
+    CHECK_CTX(c);
+    const int L = c->g.L;
+    double *f[] = {c->dp, c->pt, c->gz, c->ud, c->vd};
+    for (double *a : f) {
+        int rc = atm_launch_halo(c, a, L);
+        if (rc) return rc;
+    }
+    return atm_launch_halo(c, c->ts, 1);
+}
+
+int atm_init_sw_dyn(atm_ctx *c) {
+    CHECK_CTX(c);
+    if (!c->have_coord) return atm_fail(c, ATM_ERR_STATE, "atm_set_coordinate first");
+    int rc = atm_ref_init_sw_dyn(c);
+    if (rc) return rc;
+    c->sw_init = true;
+    return ATM_OK;
+}
+
+int atm_cgrid_step(atm_ctx *c, double dt) {
+    CHECK_CTX(c);
+    if (!c->sw_init) return atm_fail(c, ATM_ERR_STATE, "atm_init_sw_dyn first");
+    return atm_ref_cgrid(c, dt);
+}
+
+int atm_cgrid_halo(atm_ctx *c) {
+    CHECK_CTX(c);
+    double *f[] = {c->dpc, c->ptc, c->uc, c->vc};
+    for (double *a : f) {
+        int rc = atm_launch_halo(c, a, c->g.L);
+        if (rc) return rc;
+    }
+    return ATM_OK;
+}
+
+int atm_dgrid_step(atm_ctx *c, double dt) {
+    CHECK_CTX(c);
+    if (!c->sw_init) return atm_fail(c, ATM_ERR_STATE, "atm_init_sw_dyn first");
+    return atm_ref_dgrid(c, dt);
+}
+
+int atm_dgrid_halo(atm_ctx *c) {
+    CHECK_CTX(c);
+    double *f[] = {c->dp, c->pt, c->ud, c->vd};
+    for (double *a : f) {
+        int rc = atm_launch_halo(c, a, c->g.L);
+        if (rc) return rc;
+    }
+    return ATM_OK;
+}
+
+int atm_remap_init(atm_ctx *c) {
+    CHECK_CTX(c);
+    if (!c->have_coord) return atm_fail(c, ATM_ERR_STATE, "atm_set_coordinate first");
+    int rc = atm_remap_alloc(c);
+    if (rc) return rc;
+    c->remap_init = true;
+    return ATM_OK;
+}
+
+int atm_remap(atm_ctx *c) {
+    CHECK_CTX(c);
+    if (!c->remap_init) return atm_fail(c, ATM_ERR_STATE, "atm_remap_init first");
+    return atm_remap_run(c);
+}
+
+int atm_radiation_init(atm_ctx *c) {
+    CHECK_CTX(c);
+    int rc = atm_rad_alloc(c);
+    if (rc) return rc;
+    c->rad_init = true;
+    return ATM_OK;
+}
+
+int atm_radiation(atm_ctx *c, double time) {
+    CHECK_CTX(c);
+    if (!c->rad_init) return atm_fail(c, ATM_ERR_STATE, "atm_radiation_init first");
+    ATM_HIP(c, hipMemsetAsync(c->flag, 0, sizeof(int), c->stream));
+    int rc = atm_rad_run(c, time);
+    if (rc) return rc;
+    ATM_HIP(c, hipMemcpyAsync(c->flag_host, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    ATM_HIP(c, hipStreamSynchronize(c->stream));
+    if (*c->flag_host & 2)
+        return atm_fail(c, ATM_ERR_TABLE_RANGE,
+                        "gas-optics lookup outside the tabulated pressure/temperature range");
+    return ATM_OK;
+}
+
+int atm_radiation_halo(atm_ctx *c) {
+    CHECK_CTX(c);
+    return atm_launch_halo(c, c->net_flux, c->g.L + 1);
+}
+
+int atm_halo_exchange(atm_ctx *c, int id) {
+    CHECK_CTX(c);
+    int nk = 0;
+    double *d = field_ptr(c, id, &nk);
+    if (!d) return atm_fail(c, ATM_ERR_ARGUMENT, "unknown field id " + std::to_string(id));
+    if (id == ATM_GZ) nk = c->g.L;  // the 5-argument specific sees gz(:,:,1:nlay)
+    return atm_launch_halo(c, d, nk);
+}
+
+int atm_check_stability(atm_ctx *c, int *stable) {
+    CHECK_CTX(c);
+    if (!stable) return atm_fail(c, ATM_ERR_ARGUMENT, "null result pointer");
+    int rc = atm_launch_stability(c);
+    if (rc) return rc;
+    ATM_HIP(c, hipMemcpyAsync(c->flag_host, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    ATM_HIP(c, hipStreamSynchronize(c->stream));
+    *stable = (*c->flag_host & 1) ? 0 : 1;
+    return ATM_OK;
+}
+
+int atm_sync(atm_ctx *c) {
+    CHECK_CTX(c);
+    ATM_HIP(c, hipStreamSynchronize(c->stream));
+    return ATM_OK;
+}
+
+const char *atm_build_info(void) {
+    return "atmosphere-model_amd gfx950 FP64 (reference-structured kernels), -ffp-contract=off";
+}
+
+}  // extern "C"

@@ -1,0 +1,199 @@
+// atm_common.hpp -- shared definitions of the gfx950 HIP library.
+//
+// Layout: every field is FP64 in the reference's halo'd Fortran layout,
+// index (I, J, K) with I = i - is in [0, NI), J = j - js in [0, NJ),
+// K = k - 1, NI = nx + 14, NJ = ny + 14, element ((K*NJ)+J)*NI + I.
+// Reference index "is+a" is I = a; "ie-b" is I = NI-1-b; the interior
+// isd..ied is I = 7 .. 7+nx-1.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/atm.h"
+
+#define ATM_HALO 7
+
+// Physical constants of src/mod_constants.f90:8-14.  Default-real literals
+// are binary32-rounded then widened (pi, R_dry); cp_dry is an integer.
+#define ATM_PI ((double)3.1415926535897932384626433f)
+#define ATM_R_DRY ((double)287.052874f)
+#define ATM_CP_DRY 1005.0
+#define ATM_CV_DRY (ATM_CP_DRY - ATM_R_DRY)
+#define ATM_KAPPA (ATM_R_DRY / ATM_CP_DRY)
+// single-precision constant expressions (7./12., 1./12., 2./3.) widened
+#define ATM_W7_12 ((double)(7.0f / 12.0f))
+#define ATM_W1_12 ((double)(1.0f / 12.0f))
+#define ATM_W2_3 ((double)(2.0f / 3.0f))
+
+enum { V_PCM = 0, V_PPM_U = 1, V_PPM_C = 2, V_PPM_NN = 3 };  // mod_sw_dyn.f90:73-76
+
+struct Geo {
+    int NI, NJ, L, nx, ny;
+    __host__ __device__ size_t plane() const { return (size_t)NI * NJ; }
+    __host__ __device__ size_t ix(int I, int J, int K) const {
+        return ((size_t)K * NJ + (size_t)J) * NI + (size_t)I;
+    }
+    __host__ __device__ size_t ix2(int I, int J) const { return (size_t)J * NI + (size_t)I; }
+};
+
+// Per-step scalars of mod_sw_dyn (init_sw_dyn 958-1023 and dt-derived).
+struct DynCoef {
+    double dt, dtdx, dtdy;
+    double rdx, rdy, rdx2, rdy2, rdx2rdy2, rA;
+    double vx, vy, divdamp, coriolis, gravity;
+};
+
+// ---------------------------------------------------------------------------
+// Device numerics shared by all dynamics kernels (mod_sw_dyn.f90:781-919).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double dmax3(double a, double b, double c) {
+    double m = a > b ? a : b;
+    return m > c ? m : c;
+}
+__device__ __forceinline__ double dmin3(double a, double b, double c) {
+    double m = a < b ? a : b;
+    return m < c ? m : c;
+}
+__device__ __forceinline__ double dmax2(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin2(double a, double b) { return a < b ? a : b; }
+// Fortran SIGN(a, b)
+__device__ __forceinline__ double fsign(double a, double b) { return copysign(fabs(a), b); }
+
+// dqi_mono of the middle cell from (q(m-1), q(m), q(m+1)): ppm 870-875
+__device__ __forceinline__ double dq_mono(double qm, double q0, double qp) {
+    double d = (qp - qm) / 4.;
+    double dmax = dmax3(qp, q0, qm) - q0;
+    double dmin = q0 - dmin3(qp, q0, qm);
+    return fsign(dmin3(fabs(d), dmin, dmax), d);
+}
+
+// Edge anomalies of the middle cell from q(m-2..m+2): ppm 877-911.
+template <int V>
+__device__ __forceinline__ void ppm_edges(double qm2, double qm1, double q0, double qp1, double qp2,
+                                          double &eL, double &eR) {
+    double dmm = dq_mono(qm2, qm1, q0);
+    double dm0 = dq_mono(qm1, q0, qp1);
+    double dmp = dq_mono(q0, qp1, qp2);
+    eL = (qm1 - q0) / 2. + (dmm - dm0) / 3.;
+    eR = (qp1 - q0) / 2. + (dm0 - dmp) / 3.;
+    if (V == V_PPM_C || V == V_PPM_NN) {
+        double d = (qp1 - qm1) / 4.;
+        eL = -fsign(dmin2(2. * fabs(d), fabs(eL)), d);
+        eR = +fsign(dmin2(2. * fabs(d), fabs(eR)), d);
+        if (V == V_PPM_NN) {
+            eL = dmax2(eL, -q0);
+            eR = dmax2(eR, -q0);
+            double a = eR - eL;
+            double b = -3. * (eR + eL);
+            if ((fabs(a) < -b) && (q0 + a * a / (4. * b) + b / 12. < 0.)) {
+                if (eL * eR > 0.) {
+                    eL = 0.;
+                    eR = 0.;
+                } else if (eL > eR) {
+                    eL = 2. * fabs(eR);
+                } else {
+                    eR = 2. * fabs(eL);
+                }
+            }
+        }
+    }
+}
+
+// Upwind PPM value at the face between cells i-1 and i, from
+// q(i-3..i+2) and the Courant number c(i): flux 802-829.
+template <int V>
+__device__ __forceinline__ double ppm_face(double qm3, double qm2, double qm1, double q0, double qp1,
+                                           double qp2, double c) {
+    double s = fabs(c);
+    double k1 = (1. - 2. * s) * (1. - s);
+    double k2 = s * (s - 1.);
+    if (V == V_PPM_C) {
+        double eLm, eRm, eL0, eR0;
+        ppm_edges<V>(qm3, qm2, qm1, q0, qp1, eLm, eRm);
+        ppm_edges<V>(qm2, qm1, q0, qp1, qp2, eL0, eR0);
+        if ((3. * fabs(eL0 + eR0) > fabs(eL0 - eR0)) && (3. * fabs(eLm + eRm) > fabs(eLm - eRm)))
+            return c > 0. ? qm1 : q0;
+        if (c > 0.) return qm1 + k1 * eRm + k2 * eLm;
+        return q0 + k1 * eL0 + k2 * eR0;
+    } else {
+        // only the upwind cell's edges are needed
+        bool up = c > 0.;
+        double a2 = up ? qm3 : qm2, a1 = up ? qm2 : qm1, a0 = up ? qm1 : q0;
+        double b1 = up ? q0 : qp1, b2 = up ? qp1 : qp2;
+        double eL, eR;
+        ppm_edges<V>(a2, a1, a0, b1, b2, eL, eR);
+        if (up) return a0 + k1 * eR + k2 * eL;
+        return a0 + k1 * eL + k2 * eR;
+    }
+}
+
+// Piecewise-constant upwind value at the face between cells i-1 and i: flux 832-838.
+__device__ __forceinline__ double pcm_face(double qm1, double q0, double c) { return c > 0. ? qm1 : q0; }
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct RemapWork;  // k_remap.hip
+struct RadWork;    // k_rad.hip
+
+struct atm_ctx {
+    Geo g;
+    double dx, dy;
+    atm_params prm;
+    DynCoef coef;  // dt-independent part filled by init_sw_dyn
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    bool have_coord = false, sw_init = false, remap_init = false, rad_init = false;
+    double ak0 = 0;  // Ak(1) = top pressure
+    double *ak = nullptr, *bk = nullptr;  // device, nlay+1
+    // mod_fields
+    double *dp = nullptr, *pt = nullptr, *ud = nullptr, *vd = nullptr, *ts = nullptr;
+    double *gz = nullptr, *net_flux = nullptr, *plev = nullptr, *pkap = nullptr;
+    double *play = nullptr, *playkap = nullptr, *radius = nullptr;
+    // mod_sw_dyn module arrays
+    double *dpc = nullptr, *ptc = nullptr, *uc = nullptr, *vc = nullptr, *heat = nullptr;
+    double *pkapb = nullptr, *gzb = nullptr;
+    // per-level 2-D work arrays, one plane per level
+    double *tmp = nullptr, *lap = nullptr, *cx = nullptr, *cy = nullptr, *denx = nullptr,
+           *deny = nullptr, *dpfx = nullptr, *dpfy = nullptr, *fxq = nullptr, *fyq = nullptr,
+           *vort = nullptr, *ke = nullptr, *pgfx = nullptr, *pgfy = nullptr, *divg = nullptr;
+    double *ua = nullptr, *va = nullptr, *ub = nullptr, *vb = nullptr;  // 2-D (last level)
+    int *flag = nullptr;        // device status word
+    int *flag_host = nullptr;   // pinned mirror
+    RemapWork *remap = nullptr;
+    RadWork *rad = nullptr;
+};
+
+// error helpers
+int atm_fail(atm_ctx *c, int code, const std::string &msg);
+#define ATM_HIP(c, call)                                                                 \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return atm_fail((c), ATM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// launch helpers: 2-D tiles over I x J, z = level
+static inline dim3 grid3(int nI, int nJ, int nK, dim3 b) {
+    return dim3((nI + b.x - 1) / b.x, (nJ + b.y - 1) / b.y, nK);
+}
+
+// kernels implemented in the .hip files (host launchers)
+int atm_launch_halo(atm_ctx *c, double *a, int nk);
+int atm_ref_cgrid(atm_ctx *c, double dt);
+int atm_ref_dgrid(atm_ctx *c, double dt);
+int atm_ref_init_sw_dyn(atm_ctx *c);
+int atm_remap_alloc(atm_ctx *c);
+void atm_remap_free(atm_ctx *c);
+int atm_remap_run(atm_ctx *c);
+int atm_rad_alloc(atm_ctx *c);
+void atm_rad_free(atm_ctx *c);
+int atm_rad_run(atm_ctx *c, double time);
+int atm_launch_stability(atm_ctx *c);
+int atm_launch_fill(atm_ctx *c, double *a, size_t n, double v);

@@ -1,0 +1,72 @@
+// k_halo.hip -- periodic halo fill and the stability check (gfx950).
+//
+// halo_exchange on one image (src/mod_sync.f90:41-217) copies interior
+// strips into the 7-wide halo of the same tile: every halo cell (i, j)
+// receives q(wrap(i), wrap(j)), edges and corners alike.  One thread per
+// halo cell; the halo cells of a level are enumerated as two full-height
+// side strips (7 x NJ each, corners included) and two interior-width
+// bottom/top strips (nx x 7 each).  Reads touch the interior only and
+// writes the halo only, so the fill is race-free in one launch.
+#include "atm_common.hpp"
+
+namespace {
+
+__global__ void k_halo_fill(Geo g, double *__restrict__ a, int nk) {
+    const int H = ATM_HALO;
+    const int side = H * g.NJ;       // per side strip
+    const int tb = g.nx * H;         // per top/bottom strip
+    const int per = 2 * side + 2 * tb;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int K = blockIdx.y;
+    if (t >= per || K >= nk) return;
+    int I, J;
+    if (t < 2 * side) {
+        int s = t / side, r = t - s * side;
+        J = r / H;
+        int m = r - J * H;
+        I = s == 0 ? m : H + g.nx + m;
+    } else {
+        int r = t - 2 * side;
+        int s = r / tb;
+        r -= s * tb;
+        int m = r / g.nx;
+        I = H + (r - m * g.nx);
+        J = s == 0 ? m : H + g.ny + m;
+    }
+    int Is = I < H ? I + g.nx : (I >= H + g.nx ? I - g.nx : I);
+    int Js = J < H ? J + g.ny : (J >= H + g.ny ? J - g.ny : J);
+    a[g.ix(I, J, K)] = a[g.ix(Is, Js, K)];
+}
+
+// check_stability (mod_fields.f90:69-106): flag bit set when any interior
+// value of dp, pt, ud, vd is not finite.
+__global__ void k_finite(Geo g, const double *__restrict__ a, const double *__restrict__ b,
+                         const double *__restrict__ c, const double *__restrict__ d, int *flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int K = blockIdx.z;
+    if (i >= g.nx) return;
+    size_t x = g.ix(i + ATM_HALO, j + ATM_HALO, K);
+    bool bad = !isfinite(a[x]) || !isfinite(b[x]) || !isfinite(c[x]) || !isfinite(d[x]);
+    if (bad) atomicOr(flag, 1);
+}
+
+}  // namespace
+
+int atm_launch_halo(atm_ctx *c, double *a, int nk) {
+    const Geo &g = c->g;
+    int per = 2 * ATM_HALO * g.NJ + 2 * g.nx * ATM_HALO;
+    dim3 b(256), gr((per + 255) / 256, nk);
+    hipLaunchKernelGGL(k_halo_fill, gr, b, 0, c->stream, g, a, nk);
+    ATM_HIP(c, hipGetLastError());
+    return ATM_OK;
+}
+
+int atm_launch_stability(atm_ctx *c) {
+    const Geo &g = c->g;
+    ATM_HIP(c, hipMemsetAsync(c->flag, 0, sizeof(int), c->stream));
+    dim3 b(256), gr((g.nx + 255) / 256, g.ny, g.L);
+    hipLaunchKernelGGL(k_finite, gr, b, 0, c->stream, g, c->dp, c->pt, c->ud, c->vd, c->flag);
+    ATM_HIP(c, hipGetLastError());
+    return ATM_OK;
+}

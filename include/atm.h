@@ -1,0 +1,140 @@
+/*
+ * atm.h -- C ABI of the MI355X (gfx950) HIP library for the atmosphere
+ * model's per-timestep hot path.
+ *
+ * Drop-in boundary for felix-goddard/atmosphere-model.  The reference has no
+ * FFI: its hot path is a set of Fortran module procedures that share module
+ * arrays (src/mod_fields.f90:15-37).  Each entry point below replaces one
+ * of those procedures; a Fortran host binds them through ISO_C_BINDING
+ * (see INTEGRATION.md).  All functions return an atm_status (0 = success);
+ * atm_last_error() gives the message of the last failure on a context.
+ *
+ * Data layout.  Every field crossing this boundary is FP64 in the
+ * reference's halo'd Fortran layout: (is:ie, js:je[, k]) column-major with
+ * is = 1-7, ie = nx+7 (halo width 7, src/mod_tiles.f90:12), i.e. a
+ * contiguous array of (nx+14)*(ny+14)*nk doubles, i fastest.  Layer fields
+ * have nk = nlay, level fields nk = nlay+1, surface fields nk = 1.
+ *
+ * Threading.  Calls on one context are stream-ordered and asynchronous;
+ * atm_get_field*, atm_sync, atm_check_stability and atm_radiation block.
+ * A context is not thread-safe; separate contexts may be used from
+ * separate threads.
+ */
+#ifndef ATM_H
+#define ATM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct atm_ctx atm_ctx;
+
+typedef enum atm_status {
+    ATM_OK = 0,
+    ATM_ERR_ARGUMENT = 1,      /* bad pointer, size, id or sequence */
+    ATM_ERR_HIP = 2,           /* HIP runtime error or missing device */
+    ATM_ERR_TABLE_RANGE = 3,   /* gas-optics lookup outside its table
+                                  (reference aborts: mod_gas_optics.f90:4562-4592) */
+    ATM_ERR_NONFINITE = 4,     /* state failed the stability check */
+    ATM_ERR_STATE = 5          /* call out of order (e.g. step before init) */
+} atm_status;
+
+typedef enum atm_field {
+    ATM_DP = 0,       /* layer pressure thickness dp (nlay)        mod_fields.f90:17 */
+    ATM_PT = 1,       /* potential temperature pt (nlay)           mod_fields.f90:18 */
+    ATM_UD = 2,       /* D-grid u (nlay)                           mod_fields.f90:19 */
+    ATM_VD = 3,       /* D-grid v (nlay)                           mod_fields.f90:20 */
+    ATM_TS = 4,       /* surface temperature ts (1)                mod_fields.f90:21 */
+    ATM_GZ = 5,       /* geopotential on interfaces gz (nlay+1)    mod_fields.f90:29 */
+    ATM_NET_FLUX = 6, /* net radiative flux on interfaces (nlay+1) mod_fields.f90:32 */
+    ATM_PLEV = 7,     /* interface pressure plev (nlay+1)          mod_fields.f90:27 */
+    ATM_PKAP = 8,     /* plev**kappa (nlay+1)                      mod_fields.f90:28 */
+    ATM_RADIUS = 9,   /* distance from the domain centre (1)       mod_fields.f90:37 */
+    /* mod_sw_dyn module-private C-grid state (mod_sw_dyn.f90:23-27); the
+       reference host never sees these, they are exposed for testing. */
+    ATM_DPC = 10, ATM_PTC = 11, ATM_UC = 12, ATM_VC = 13,
+    ATM_NFIELDS = 14
+} atm_field;
+
+typedef struct atm_grid {
+    int32_t nx, ny, nlay;      /* interior sizes (mod_config.f90:22) */
+    double dx, dy;             /* grid spacing, m */
+} atm_grid;
+
+typedef struct atm_params {
+    double gravity;                  /* physics_parameters namelist, config.nml:16-21 */
+    double coriolis_parameter;
+    double divergence_damping_coefficient;
+    double vorticity_damping_coefficient;
+} atm_params;
+
+enum { ATM_PRECISION_FP64 = 0 };
+
+/* Context lifetime.  Allocates every device buffer for one model instance.
+   Replaces allocate_fields (mod_fields.f90:41-67) and the module
+   allocations of init_sw_dyn / allocate_remapping_arrays /
+   allocate_radiation_arrays. */
+int atm_create(const atm_grid *grid, const atm_params *params, int precision, atm_ctx **out);
+int atm_destroy(atm_ctx *ctx);
+const char *atm_last_error(const atm_ctx *ctx);
+/* Run on an existing HIP stream (hipStream_t passed as void*); NULL = the
+   context's own stream. */
+int atm_set_stream(atm_ctx *ctx, void *hip_stream);
+
+/* Hybrid coordinate coefficients Ak, Bk (nlay+1 each) -- the reference's
+   coord_Ak/coord_Bk read by init_prognostic_fields (mod_input.f90:161-168). */
+int atm_set_coordinate(atm_ctx *ctx, const double *ak, const double *bk);
+
+/* Field transfer, host arrays in the halo'd layout described above.  A put
+   replaces the whole array (halo included); a get returns exactly what the
+   reference's module array would hold at that point. */
+int atm_put_field(atm_ctx *ctx, int field, const double *host);
+int atm_get_field(atm_ctx *ctx, int field, double *host);
+/* Same, device pointers (same layout), stream-ordered. */
+int atm_put_field_device(atm_ctx *ctx, int field, const double *dev);
+int atm_get_field_device(atm_ctx *ctx, int field, double *dev);
+
+/* init_prognostic_fields tail (mod_input.f90:229-231): plev/pkap top level. */
+int atm_init_prognostic_fields(atm_ctx *ctx);
+/* initial_halo_exchange (mod_input.f90:247-252): dp, pt, gz(1:nlay), ud, vd, ts. */
+int atm_initial_halo_exchange(atm_ctx *ctx);
+
+/* mod_sw_dyn public procedures (mod_sw_dyn.f90:17-19) */
+int atm_init_sw_dyn(atm_ctx *ctx);                 /* init_sw_dyn          958-1023 */
+int atm_cgrid_step(atm_ctx *ctx, double dt);       /* cgrid_dynamics_step   80-416  */
+int atm_cgrid_halo(atm_ctx *ctx);                  /* cgrid_halo_exchange  418-422  */
+int atm_dgrid_step(atm_ctx *ctx, double dt);       /* dgrid_dynamics_step  424-773  */
+int atm_dgrid_halo(atm_ctx *ctx);                  /* dgrid_halo_exchange  775-779  */
+
+/* mod_remap public procedures (mod_remap.f90:13) */
+int atm_remap_init(atm_ctx *ctx);                  /* allocate_remapping_arrays 358-388 */
+int atm_remap(atm_ctx *ctx);                       /* perform_vertical_remapping 27-163 */
+
+/* mod_radiation public procedures (mod_radiation.f90:20-21) */
+int atm_radiation_init(atm_ctx *ctx);              /* allocate_radiation_arrays 406-446 */
+int atm_radiation(atm_ctx *ctx, double time);      /* calculate_radiative_heating 41-156;
+                                                      blocks; ATM_ERR_TABLE_RANGE on a
+                                                      lookup outside the tables */
+int atm_radiation_halo(atm_ctx *ctx);              /* radiation_halo_exchange 400-404 */
+
+/* mod_sync generic halo_exchange (mod_sync.f90:14-17) for one field: a
+   periodic fill of the 7-wide halo, edges and corners.  For ATM_GZ only
+   levels 1..nlay are filled (the 5-argument specific's view of gz). */
+int atm_halo_exchange(atm_ctx *ctx, int field);
+
+/* check_stability (mod_fields.f90:69-106): *stable = 1 when dp, pt, ud, vd
+   are finite over the interior.  Blocks. */
+int atm_check_stability(atm_ctx *ctx, int *stable);
+
+/* Wait for all work queued on the context. */
+int atm_sync(atm_ctx *ctx);
+
+/* Library build information (kernel path, compile flags), for logs. */
+const char *atm_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,305 @@
+"""Pins of the CPU oracle (oracle/atm_oracle.c) -- what makes it trustworthy.
+
+* gas optics and constants: bit-identical to the reference's own compiled
+  functions (committed golden vectors from oracle/_ref; live check where the
+  reference is present);
+* halo fill: the periodic closed form;
+* adding method: brute-force solution of the two-stream linear system;
+* vertical PPM: the tridiagonal stage against scipy's banded solver and its
+  exactness for linear profiles on non-uniform grids;
+* remap_column: identity, exact overlap integrals (40-digit) with the
+  binary32 2./3. of the reference, conservation, constants;
+* full steps: mass conservation, rest state, inertial oscillation in closed
+  form, x<->y transpose symmetry, translation equivariance and the stale
+  wind halo of the remap (Q5).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "gas_optics_golden.npz")
+
+
+def _eval_oracle(q):
+    out = []
+    for kind, g, p, t in q:
+        g = int(g)
+        if kind == 1:
+            v, e = O.sw_abs(g, p, t)
+        elif kind == 2:
+            v, e = O.sw_sca(g), 0
+        elif kind == 3:
+            v, e = O.lw_abs(g, p, t)
+        else:
+            v, e = O.planck(g, t)
+        assert e == 0
+        out.append(v)
+    return np.array(out)
+
+
+def test_gas_optics_bitwise_vs_reference_golden():
+    d = np.load(GOLD)
+    mine = _eval_oracle(d["queries"])
+    assert np.array_equal(mine, d["values"]), np.flatnonzero(mine != d["values"])[:10]
+
+
+def test_constants_bitwise_vs_reference():
+    d = np.load(GOLD)
+    c = O.constants()
+    assert [c[k] for k in ("pi", "R_dry", "cp_dry", "cv_dry", "kappa")] == list(d["constants"])
+    # Q1: binary32-rounded literals
+    assert c["pi"] == 3.1415927410125732 and c["kappa"] == 0.28562476314715485
+
+
+def test_gas_optics_live_reference():
+    rng = np.random.default_rng(11)
+    d = np.load(GOLD)
+    q = d["queries"][rng.permutation(len(d["queries"]))[:600]]
+    q[:, 3] = np.where(q[:, 0] == 4, q[:, 3] * 0 + rng.uniform(120, 350, len(q)), q[:, 3])
+    out = O.reference_gas_optics(q)
+    if out is None:
+        pytest.skip("compiled reference (oracle/_ref) not present")
+    assert np.array_equal(_eval_oracle(q), out[0])
+
+
+def test_table_range_errors():
+    assert O.sw_abs(1, 0.5, 200.0)[1] != 0          # below the lowest pressure
+    assert O.lw_abs(1, 1.2e5, 280.0)[1] != 0        # above the highest pressure
+    assert O.sw_abs(1, 1e5, 100.0)[1] != 0          # temperature below the row
+    assert O.planck(1, 119.0)[1] != 0 and O.planck(1, 351.0)[1] != 0
+
+
+def test_halo_closed_form():
+    o = O.Oracle(9, 12, 3, 1e5, 1e5)
+    rng = np.random.default_rng(1)
+    a = o.field(O.DP)
+    a[...] = rng.standard_normal(a.shape)
+    before = a.copy()
+    o.halo_exchange(O.DP)
+    ii = (np.arange(9 + 14) - 7) % 9 + 7
+    jj = (np.arange(12 + 14) - 7) % 12 + 7
+    assert np.array_equal(a, before[:, jj][:, :, ii])
+
+
+def test_adding_method_brute_force():
+    rng = np.random.default_rng(3)
+    for L in (1, 2, 3, 4, 7):
+        R = rng.uniform(0, 0.6, L)
+        T = rng.uniform(0, 1, L) * (1 - R)
+        Su, Sd = rng.uniform(0, 5, L), rng.uniform(0, 5, L)
+        toa, sfc_up, alb = 3.0, 1.7, 0.35
+        fu, fd = O.calculate_fluxes(R, T, Su, Sd, toa, sfc_up, alb)
+        # unknowns up(0..L), dn(0..L) (level 0 = surface)
+        n = L + 1
+        A = np.zeros((2 * n, 2 * n))
+        b = np.zeros(2 * n)
+        r = 0
+        for k in range(L):
+            # up(k+1) = R dn(k+1) + T up(k) + Su ; dn(k) = T dn(k+1) + R up(k) + Sd
+            A[r, k + 1] = 1; A[r, n + k + 1] = -R[k]; A[r, k] = -T[k]; b[r] = Su[k]; r += 1
+            A[r, n + k] = 1; A[r, n + k + 1] = -T[k]; A[r, k] = -R[k]; b[r] = Sd[k]; r += 1
+        A[r, n + L] = 1; b[r] = toa; r += 1
+        A[r, 0] = 1; A[r, n] = -alb; b[r] = sfc_up
+        x = np.linalg.solve(A, b)
+        np.testing.assert_allclose(fu, x[:n], rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(fd, x[n:], rtol=1e-12, atol=1e-12)
+
+
+def _random_grid(rng, L):
+    d = rng.uniform(0.5, 2.0, L)
+    c = np.concatenate([[0.0], np.cumsum(d)])
+    return 1e5 - c * 1000.0  # decreasing like pressure
+
+
+def test_vertical_ppm_linear_exact_and_banded():
+    from scipy.linalg import solve_banded
+    rng = np.random.default_rng(5)
+    for L in (4, 9, 20):
+        x = _random_grid(rng, L)
+        # interface values are exact for a profile linear in the coordinate
+        # (the interior equation is exact to second order, the top and
+        # bottom closures to first order)
+        a0, a1 = rng.standard_normal(2)
+        s = (x - x[0]) / 1000.0
+        avg = a0 + a1 * (s[1:] + s[:-1]) / 2
+        _, _, _, raw = O.vertical_ppm(avg, x)
+        exact = a0 + a1 * s
+        np.testing.assert_allclose(raw, exact, rtol=0, atol=1e-11 * np.abs(exact).max())
+        # the same system through a banded solver (coefficients of the
+        # elimination at mod_remap.f90:296-312)
+        v = rng.standard_normal(L)
+        _, _, _, raw = O.vertical_ppm(v, x)
+        Ab = np.zeros((3, L + 1))
+        rhs = np.zeros(L + 1)
+        al = (x[2] - x[1]) / (x[1] - x[0])
+        Ab[1, 0] = al * (.5 + al); Ab[0, 1] = 1 + al * (1.5 + al)
+        rhs[0] = 2 * al * (1 + al) * v[0] + v[1]
+        for k in range(1, L):  # interior equation for interface k (0-based)
+            a = (x[k + 1] - x[k]) / (x[k] - x[k - 1])
+            Ab[2, k - 1] = a; Ab[1, k] = 2 + 2 * a; Ab[0, k + 1] = 1
+            rhs[k] = 3 * (v[k] + a * v[k - 1])
+        a = (x[L - 1] - x[L - 2]) / (x[L] - x[L - 1])
+        Ab[2, L - 1] = 1 + a * (1.5 + a); Ab[1, L] = a * (.5 + a)
+        rhs[L] = 2 * a * (1 + a) * v[L - 1] + v[L - 2]
+        sol = solve_banded((1, 1), Ab, rhs)
+        np.testing.assert_allclose(raw, sol, rtol=1e-10, atol=1e-10 * np.abs(sol).max())
+
+
+def _exact_remap(v, x, y, top, btm, curv):
+    """Overlap integrals of each source layer's PPM parabola over each
+    destination layer in 40-digit arithmetic (antiderivative of
+    a(s) = btm + s (top - btm) + curv s (1 - s)), plus the one deliberate
+    deviation of the reference: the case where the destination top lies
+    inside the source layer integrates with the binary32 literal 2./3.
+    (mod_remap.f90:243, Q1), which adds curv s2^3 (2/3 - w) / 2."""
+    import mpmath as mp
+    mp.mp.dps = 40
+    w = mp.mpf(float(np.float32(2.0) / np.float32(3.0)))
+    X = [mp.mpf(a) for a in x]
+    Y = [mp.mpf(a) for a in y]
+    L = len(v)
+
+    def F(k, s):
+        b, t, c6 = mp.mpf(btm[k]), mp.mpf(top[k]), mp.mpf(curv[k])
+        return b * s + (t - b) * s ** 2 / 2 + c6 * (s ** 2 / 2 - s ** 3 / 3)
+
+    out = []
+    for m in range(L):
+        acc = mp.mpf(0)
+        for k in range(L):
+            lo, hi = max(Y[m + 1], X[k + 1]), min(Y[m], X[k])
+            if hi <= lo:
+                continue
+            ds = X[k + 1] - X[k]
+            sa, sb = (hi - X[k]) / ds, (lo - X[k]) / ds
+            acc += (F(k, sb) - F(k, sa)) * ds
+            if X[k + 1] < Y[m + 1] < X[k] <= Y[m]:  # destination top inside the source
+                acc += mp.mpf(curv[k]) * sb ** 3 * (mp.mpf(2) / 3 - w) / 2 * ds
+        out.append(float(acc / (Y[m + 1] - Y[m])))
+    return np.array(out)
+
+
+def test_remap_column_vs_exact_integrals():
+    rng = np.random.default_rng(9)
+    for L in (5, 20, 64):
+        x = _random_grid(rng, L)
+        v = rng.uniform(1, 2, L)
+        assert np.array_equal(O.remap_column(v, x, x), v)  # identity (case 5 only)
+        y = x.copy()
+        y[1:-1] = np.sort(x[1:-1] + rng.uniform(-300, 300, L - 1))[::-1]
+        out = O.remap_column(v, x, y)
+        top, btm, curv, _ = O.vertical_ppm(v, x)
+        np.testing.assert_allclose(out, _exact_remap(v, x, y, top, btm, curv), rtol=1e-11)
+        # each source parabola keeps its layer mean, so the column integral is
+        # conserved up to the binary32 2/3 term (~1e-9 relative)
+        np.testing.assert_allclose(btm + (top - btm) / 2 + curv / 6, v, rtol=1e-13)
+        total_src = np.sum(v * (x[:-1] - x[1:]))
+        total_dst = np.sum(out * (y[:-1] - y[1:]))
+        assert abs(total_dst - total_src) <= 1e-8 * abs(total_src)
+        const = O.remap_column(np.full(L, 3.25), x, y)
+        np.testing.assert_allclose(const, 3.25, rtol=1e-13)
+
+
+def test_flux_pencil_constants_and_upwind():
+    n = 16
+    q = np.full(n, 2.5)
+    for var in (O.PPM_UNCONSTRAINED, O.PPM_CONSTRAINED, O.PPM_NONNEGATIVE):
+        c = np.linspace(-0.8, 0.8, n)
+        f = O.flux_pencil(q, c, var)
+        assert np.array_equal(f[3:n - 2], q[3:n - 2])
+    rng = np.random.default_rng(2)
+    q = rng.standard_normal(n)
+    c = rng.uniform(-1, 1, n)
+    f = O.flux_pencil(q, c, O.PCM)
+    expect = np.where(c[1:] > 0, q[:-1], q[1:])
+    assert np.array_equal(f[1:], expect)
+
+
+# ---- full steps -----------------------------------------------------------------
+def _run(ic, steps, f=1e-5, remap=300.0, dt=60.0):
+    from atmos import driver
+    from atmos.config import Config
+    cfg = Config(dt_max=dt, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12,
+                 dt_remap=remap, radiation_on=False, physics_on=False, coriolis_parameter=f)
+    L = ic["dp"].shape[0]
+    ny, nx = ic["dp"].shape[1:]
+    o = O.Oracle(nx, ny, L, ic["dx"], ic["dy"], coriolis=f)
+    st = driver.init_model(o, ic, cfg)
+    driver.run_model(o, cfg, st, max_steps=steps)
+    return o
+
+
+def _ic(nx=24, ny=24, L=10, **kw):
+    from atmos.ic import make_initial_state
+    return make_initial_state(nx, ny, L, 100e3, **kw)
+
+
+def test_mass_conservation_with_remap():
+    ic = _ic()
+    o = _run(ic, 10)
+    m0, m1 = ic["dp"].sum(), o.interior(O.DP).sum()
+    assert abs(m1 - m0) <= 1e-13 * m0
+
+
+def test_rest_state_is_stationary():
+    ic = _ic(bump=0.0, wind_noise=0.0, u0=0.0, v0=0.0)
+    o = _run(ic, 6)
+    assert np.array_equal(o.interior(O.UD), np.zeros_like(ic["u"]))
+    assert np.array_equal(o.interior(O.VD), np.zeros_like(ic["v"]))
+    np.testing.assert_allclose(o.interior(O.DP), ic["dp"], rtol=1e-14)
+    np.testing.assert_allclose(o.interior(O.PT), ic["pt"], rtol=1e-13)
+
+
+def test_uniform_flow_inertial_oscillation_closed_form():
+    from atmos import driver
+    from atmos.config import Config
+    f, dt, u, v = 1e-4, 60.0, 7.0, -3.0
+    ic = _ic(16, 16, 8, bump=0.0, wind_noise=0.0, u0=u, v0=v)
+    o = O.Oracle(16, 16, 8, 100e3, 100e3, coriolis=f)
+    cfg = Config(dt_max=dt, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12,
+                 dt_remap=1e12, radiation_on=False, physics_on=False, coriolis_parameter=f)
+    st = driver.init_model(o, ic, cfg)
+    for _ in range(3):
+        driver.step(o, cfg, st, 0.0)
+        uc, vc = u + (dt / 2) * (v * f), v - (dt / 2) * (u * f)  # C-grid half step
+        u, v = u + dt * (vc * f), v - dt * (uc * f)               # D-grid step
+        assert np.abs(o.interior(O.UD) - u).max() <= 1e-14
+        assert np.abs(o.interior(O.VD) - v).max() <= 1e-14
+
+
+def test_transpose_symmetry():
+    ic = _ic(20, 20, 8)
+    icT = dict(ic)
+    for k in ("dp", "pt"):
+        icT[k] = ic[k].transpose(0, 2, 1).copy()
+    icT["u"] = ic["v"].transpose(0, 2, 1).copy()
+    icT["v"] = ic["u"].transpose(0, 2, 1).copy()
+    a, b = _run(ic, 4, f=0.0, remap=1e12), _run(icT, 4, f=0.0, remap=1e12)
+    for fa, fb in ((O.DP, O.DP), (O.PT, O.PT), (O.UD, O.VD), (O.VD, O.UD)):
+        x, y = a.interior(fa), b.interior(fb).transpose(0, 2, 1)
+        assert np.abs(x - y).max() <= 1e-12 * np.abs(x).max()
+
+
+def test_translation_and_stale_wind_ring():
+    ic = _ic(20, 20, 8)
+    sh = dict(ic)
+    for k in ("dp", "pt", "u", "v"):
+        sh[k] = np.roll(np.roll(ic[k], 5, axis=2), 3, axis=1)
+    roll = lambda a: np.roll(np.roll(a, 5, axis=2), 3, axis=1)
+    a, b = _run(ic, 3, remap=1e12), _run(sh, 3, remap=1e12)
+    for f in (O.DP, O.PT, O.UD, O.VD):
+        assert np.array_equal(roll(a.interior(f)), b.interior(f))
+    # with a remap (step 5) the outermost interior rows/columns of pt see the
+    # previous step's winds in the halo ring (Q5), so equivariance breaks
+    # there and only there
+    a, b = _run(ic, 5, remap=300.0), _run(sh, 5, remap=300.0)
+    for f in (O.DP, O.UD, O.VD):
+        assert np.array_equal(roll(a.interior(f)), b.interior(f))
+    d = np.abs(roll(a.interior(O.PT)) - b.interior(O.PT)).max(axis=0)
+    ring = np.zeros((20, 20), bool)
+    ring[[0, -1], :] = True
+    ring[:, [0, -1]] = True
+    assert d.max() > 0 and not np.any(d[~(ring | roll(ring[None])[0])] > 0)
