@@ -26,7 +26,9 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_cgrid_step", "atm_cgrid_halo", "atm_dgrid_step", "atm_dgrid_halo",
            "atm_remap_init", "atm_remap", "atm_radiation_init", "atm_radiation",
            "atm_radiation_halo", "atm_halo_exchange", "atm_check_stability", "atm_sync",
-           "atm_build_info"]
+           "atm_build_info", "atm_set_path", "atm_get_path", "atm_timer_enable", "atm_timer_read"]
+PATHS = {"fused": 0, "reference": 1}
+TIMERS = {"c1": 0, "c2": 1, "d1": 2, "d2": 3, "scan": 4, "halo": 5, "remap": 6, "radiation": 7}
 
 
 class AtmError(RuntimeError):
@@ -76,6 +78,10 @@ def load_library(path: str = LIB_PATH):
         getattr(L, n).argtypes = [vp, C.c_double]
     L.atm_halo_exchange.argtypes = [vp, ip]
     L.atm_check_stability.argtypes = [vp, C.POINTER(C.c_int)]
+    L.atm_set_path.argtypes = [vp, ip]
+    L.atm_get_path.argtypes = [vp, C.POINTER(C.c_int)]
+    L.atm_timer_enable.argtypes = [vp, ip]
+    L.atm_timer_read.argtypes = [vp, ip, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     _LIB = L
     return L
 
@@ -199,6 +205,24 @@ class Model:
 
     def sync(self):
         self._ck(self._L.atm_sync(self._h))
+
+    @property
+    def path(self):
+        p = C.c_int(0)
+        self._ck(self._L.atm_get_path(self._h, C.byref(p)))
+        return {v: k for k, v in PATHS.items()}[p.value]
+
+    def set_path(self, name):
+        self._ck(self._L.atm_set_path(self._h, PATHS[name]))
+
+    def timer_enable(self, on=True):
+        self._ck(self._L.atm_timer_enable(self._h, int(bool(on))))
+
+    def timer_read(self, name):
+        """(launches, total milliseconds) of one kernel group since enabling."""
+        n, ms = C.c_int64(0), C.c_double(0)
+        self._ck(self._L.atm_timer_read(self._h, TIMERS[name], C.byref(n), C.byref(ms)))
+        return n.value, ms.value
 
     def set_stream(self, stream_handle):
         self._ck(self._L.atm_set_stream(self._h, C.c_void_p(stream_handle)))

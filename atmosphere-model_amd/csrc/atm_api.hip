@@ -6,7 +6,10 @@
 #include <cstdio>
 #include <cstring>
 
+#include <cstdlib>
+
 #include "atm_common.hpp"
+#include "atm_fused.hpp"
 
 int atm_fail(atm_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
@@ -31,8 +34,8 @@ static double *field_ptr(atm_ctx *c, int id, int *nk) {
     case ATM_RADIUS: *nk = 1; return c->radius;
     case ATM_DPC: *nk = L; return c->dpc;
     case ATM_PTC: *nk = L; return c->ptc;
-    case ATM_UC: *nk = L; return c->uc;
-    case ATM_VC: *nk = L; return c->vc;
+    case ATM_UC: *nk = L; return c->fused ? c->uc2 : c->uc;
+    case ATM_VC: *nk = L; return c->fused ? c->vc2 : c->vc;
     default: *nk = 0; return nullptr;
     }
 }
@@ -44,18 +47,65 @@ static int dalloc(atm_ctx *c, double **p, size_t n) {
 }
 
 static void release(atm_ctx *c) {
-    double **all[] = {&c->ak, &c->bk, &c->dp, &c->pt, &c->ud, &c->vd, &c->ts, &c->gz, &c->net_flux,
+    c->dp = c->pt = c->ud = c->vd = nullptr;  // owned through pbuf
+    double **all[] = {&c->ak, &c->bk, &c->ts, &c->gz, &c->net_flux,
                       &c->plev, &c->pkap, &c->play, &c->playkap, &c->radius, &c->dpc, &c->ptc,
                       &c->uc, &c->vc, &c->heat, &c->pkapb, &c->gzb, &c->tmp, &c->lap, &c->cx,
                       &c->cy, &c->denx, &c->deny, &c->dpfx, &c->dpfy, &c->fxq, &c->fyq, &c->vort,
                       &c->ke, &c->pgfx, &c->pgfy, &c->divg, &c->ua, &c->va, &c->ub, &c->vb};
     for (auto p : all)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
+    for (int f = 0; f < 4; f++)
+        for (int b = 0; b < 2; b++)
+            if (c->pbuf[f][b]) { (void)hipFree(c->pbuf[f][b]); c->pbuf[f][b] = nullptr; }
+    double **fz[] = {&c->uc2, &c->vc2, &c->heatd};
+    for (auto p : fz)
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    for (auto &t : c->timers) {
+        for (int i = 0; i < t.cap; i++) (void)hipEventDestroy(t.ev[i]);
+        delete[] t.ev;
+        t.ev = nullptr;
+        t.cap = t.used = 0;
+    }
     if (c->flag) (void)hipFree(c->flag);
     if (c->flag_host) (void)hipHostFree(c->flag_host);
     atm_remap_free(c);
     atm_rad_free(c);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// dp/pt/ud/vd point at the current copy of the prognostic fields
+void atm_point_current(atm_ctx *c) {
+    c->dp = c->pbuf[0][c->cur];
+    c->pt = c->pbuf[1][c->cur];
+    c->ud = c->pbuf[2][c->cur];
+    c->vd = c->pbuf[3][c->cur];
+}
+
+static FusedBufs bufs_of(atm_ctx *c, int which) {
+    return FusedBufs{c->pbuf[0][which], c->pbuf[1][which], c->pbuf[2][which], c->pbuf[3][which]};
+}
+
+void atm_timer_begin(atm_ctx *c, int id) {
+    if (!c->timing) return;
+    AtmTimer &t = c->timers[id];
+    if (t.used + 2 > t.cap) {
+        int ncap = t.cap ? 2 * t.cap : 256;
+        hipEvent_t *ev = new hipEvent_t[ncap];
+        for (int i = 0; i < t.cap; i++) ev[i] = t.ev[i];
+        for (int i = t.cap; i < ncap; i++) (void)hipEventCreate(&ev[i]);
+        delete[] t.ev;
+        t.ev = ev;
+        t.cap = ncap;
+    }
+    (void)hipEventRecord(t.ev[t.used], c->stream);
+}
+
+void atm_timer_end(atm_ctx *c, int id) {
+    if (!c->timing) return;
+    AtmTimer &t = c->timers[id];
+    (void)hipEventRecord(t.ev[t.used + 1], c->stream);
+    t.used += 2;
 }
 
 extern "C" {
@@ -88,9 +138,11 @@ int atm_create(const atm_grid *grid, const atm_params *params, int precision, at
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(ATM_ERR_HIP);
     c->own_stream = true;
     const size_t p = c->g.plane(), L = c->g.L;
+    const char *path = getenv("ATM_PATH");
+    c->fused = atm_fused_supported(c->g) && !(path && std::string(path) == "reference");
     struct { double **ptr; size_t n; } bufs[] = {
-        {&c->ak, L + 1}, {&c->bk, L + 1}, {&c->dp, p * L}, {&c->pt, p * L}, {&c->ud, p * L},
-        {&c->vd, p * L}, {&c->ts, p}, {&c->gz, p * (L + 1)}, {&c->net_flux, p * (L + 1)},
+        {&c->pbuf[0][0], p * L}, {&c->pbuf[1][0], p * L}, {&c->pbuf[2][0], p * L}, {&c->pbuf[3][0], p * L},
+        {&c->ak, L + 1}, {&c->bk, L + 1}, {&c->ts, p}, {&c->gz, p * (L + 1)}, {&c->net_flux, p * (L + 1)},
         {&c->plev, p * (L + 1)}, {&c->pkap, p * (L + 1)}, {&c->play, p * L}, {&c->playkap, p * L},
         {&c->radius, p}, {&c->dpc, p * L}, {&c->ptc, p * L}, {&c->uc, p * L}, {&c->vc, p * L},
         {&c->heat, p * L}, {&c->pkapb, p * (L + 1)}, {&c->gzb, p * (L + 1)}, {&c->tmp, p * L},
@@ -102,6 +154,15 @@ int atm_create(const atm_grid *grid, const atm_params *params, int precision, at
         rc = dalloc(c, b.ptr, b.n);
         if (rc) return fail(rc);
     }
+    if (c->fused) {
+        double **fz[] = {&c->pbuf[0][1], &c->pbuf[1][1], &c->pbuf[2][1], &c->pbuf[3][1], &c->uc2, &c->vc2,
+                         &c->heatd};
+        for (auto q : fz) {
+            rc = dalloc(c, q, p * L);
+            if (rc) return fail(rc);
+        }
+    }
+    atm_point_current(c);
     if (hipMalloc(&c->flag, sizeof(int)) != hipSuccess) return fail(ATM_ERR_HIP);
     if (hipHostMalloc((void **)&c->flag_host, sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(ATM_ERR_HIP);
@@ -147,6 +208,7 @@ int atm_set_coordinate(atm_ctx *c, const double *ak, const double *bk) {
     ATM_HIP(c, hipMemcpyAsync(c->bk, bk, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     ATM_HIP(c, hipStreamSynchronize(c->stream));
     c->ak0 = ak[0];
+    c->pktop = pow(ak[0], ATM_KAPPA);  // host libm, as the reference's top_pressure**kappa
     c->have_coord = true;
     return ATM_OK;
 }
@@ -186,7 +248,7 @@ int atm_init_prognostic_fields(atm_ctx *c) {
     const size_t p = c->g.plane();
     int rc = atm_launch_fill(c, c->plev + p * c->g.L, p, c->ak0);
     if (rc) return rc;
-    return atm_launch_fill(c, c->pkap + p * c->g.L, p, pow(c->ak0, ATM_KAPPA));
+    return atm_launch_fill(c, c->pkap + p * c->g.L, p, c->pktop);
 }
 
 // initial_halo_exchange (mod_input.f90:247-252)
@@ -213,11 +275,17 @@ int atm_init_sw_dyn(atm_ctx *c) {
 int atm_cgrid_step(atm_ctx *c, double dt) {
     CHECK_CTX(c);
     if (!c->sw_init) return atm_fail(c, ATM_ERR_STATE, "atm_init_sw_dyn first");
+    if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "atm_dgrid_halo must follow atm_dgrid_step");
+    if (c->fused) {
+        FusedBufs b = bufs_of(c, c->cur);
+        return atm_fused_cgrid(c, b, dt);
+    }
     return atm_ref_cgrid(c, dt);
 }
 
 int atm_cgrid_halo(atm_ctx *c) {
     CHECK_CTX(c);
+    if (c->fused) return atm_fused_cgrid_halo(c);
     double *f[] = {c->dpc, c->ptc, c->uc, c->vc};
     for (double *a : f) {
         int rc = atm_launch_halo(c, a, c->g.L);
@@ -226,19 +294,36 @@ int atm_cgrid_halo(atm_ctx *c) {
     return ATM_OK;
 }
 
+// The fused D step writes the other copy of dp, pt, ud, vd and makes it
+// current; the previous copy keeps the halo the wind remap reads (Q5) until
+// atm_dgrid_halo.  Between atm_dgrid_step and atm_dgrid_halo the interior
+// of dp/pt/ud/vd is the reference's; their halo is refreshed by the fill.
 int atm_dgrid_step(atm_ctx *c, double dt) {
     CHECK_CTX(c);
     if (!c->sw_init) return atm_fail(c, ATM_ERR_STATE, "atm_init_sw_dyn first");
+    if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "atm_dgrid_halo must follow atm_dgrid_step");
+    if (c->fused) {
+        FusedBufs cur = bufs_of(c, c->cur), nxt = bufs_of(c, 1 - c->cur);
+        int rc = atm_fused_dgrid(c, cur, nxt, dt);
+        if (rc) return rc;
+        c->cur = 1 - c->cur;
+        atm_point_current(c);
+        c->after_d = true;
+        return ATM_OK;
+    }
     return atm_ref_dgrid(c, dt);
 }
 
 int atm_dgrid_halo(atm_ctx *c) {
     CHECK_CTX(c);
     double *f[] = {c->dp, c->pt, c->ud, c->vd};
+    atm_timer_begin(c, ATM_T_HALO);
     for (double *a : f) {
         int rc = atm_launch_halo(c, a, c->g.L);
         if (rc) return rc;
     }
+    atm_timer_end(c, ATM_T_HALO);
+    c->after_d = false;
     return ATM_OK;
 }
 
@@ -254,6 +339,18 @@ int atm_remap_init(atm_ctx *c) {
 int atm_remap(atm_ctx *c) {
     CHECK_CTX(c);
     if (!c->remap_init) return atm_fail(c, ATM_ERR_STATE, "atm_remap_init first");
+    if (c->fused && c->after_d) {
+        // read the D step's state (current copy) with the wind ring of the
+        // previous copy; write the remapped interior into the previous copy
+        const int o = 1 - c->cur;
+        FusedBufs cur = bufs_of(c, c->cur), prv = bufs_of(c, o);
+        int rc = atm_remap_run_split(c, cur.dp, cur.pt, cur.ud, cur.vd, prv.ud, prv.vd, prv.dp, prv.pt, prv.ud,
+                                     prv.vd);
+        if (rc) return rc;
+        c->cur = o;
+        atm_point_current(c);
+        return ATM_OK;
+    }
     return atm_remap_run(c);
 }
 
@@ -304,6 +401,51 @@ int atm_check_stability(atm_ctx *c, int *stable) {
     return ATM_OK;
 }
 
+int atm_set_path(atm_ctx *c, int path) {
+    CHECK_CTX(c);
+    if (path != ATM_PATH_FUSED && path != ATM_PATH_REFERENCE) return atm_fail(c, ATM_ERR_ARGUMENT, "bad path");
+    if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "finish the step (atm_dgrid_halo) first");
+    if (path == ATM_PATH_FUSED) {
+        if (!atm_fused_supported(c->g) || !c->uc2)
+            return atm_fail(c, ATM_ERR_ARGUMENT, "fused path needs nx % 32 == 0 and ny % 8 == 0");
+        c->fused = true;
+    } else {
+        c->fused = false;
+    }
+    return ATM_OK;
+}
+
+int atm_get_path(atm_ctx *c, int *path) {
+    CHECK_CTX(c);
+    if (!path) return ATM_ERR_ARGUMENT;
+    *path = c->fused ? ATM_PATH_FUSED : ATM_PATH_REFERENCE;
+    return ATM_OK;
+}
+
+int atm_timer_enable(atm_ctx *c, int on) {
+    CHECK_CTX(c);
+    ATM_HIP(c, hipStreamSynchronize(c->stream));
+    for (auto &t : c->timers) t.used = 0;
+    c->timing = on != 0;
+    return ATM_OK;
+}
+
+int atm_timer_read(atm_ctx *c, int kernel, int64_t *launches, double *total_ms) {
+    CHECK_CTX(c);
+    if (kernel < 0 || kernel >= ATM_T_COUNT || !launches || !total_ms) return ATM_ERR_ARGUMENT;
+    ATM_HIP(c, hipStreamSynchronize(c->stream));
+    AtmTimer &t = c->timers[kernel];
+    double sum = 0;
+    for (int i = 0; i + 1 < t.used; i += 2) {
+        float ms = 0;
+        ATM_HIP(c, hipEventElapsedTime(&ms, t.ev[i], t.ev[i + 1]));
+        sum += ms;
+    }
+    *launches = t.used / 2;
+    *total_ms = sum;
+    return ATM_OK;
+}
+
 int atm_sync(atm_ctx *c) {
     CHECK_CTX(c);
     ATM_HIP(c, hipStreamSynchronize(c->stream));
@@ -311,7 +453,8 @@ int atm_sync(atm_ctx *c) {
 }
 
 const char *atm_build_info(void) {
-    return "atmosphere-model_amd gfx950 FP64 (reference-structured kernels), -ffp-contract=off";
+    return "atmosphere-model_amd gfx950 FP64: fused LDS-tiled dynamics (32x8 tiles) with the "
+           "reference-structured kernels as the alternative path; -O3 -ffp-contract=off";
 }
 
 }  // extern "C"

@@ -140,6 +140,16 @@ __device__ __forceinline__ double pcm_face(double qm1, double q0, double c) { re
 struct RemapWork;  // k_remap.hip
 struct RadWork;    // k_rad.hip
 
+// kernels timed with HIP events when timing is enabled (atm_timer_*)
+enum {
+    ATM_T_C1 = 0, ATM_T_C2, ATM_T_D1, ATM_T_D2, ATM_T_SCAN, ATM_T_HALO, ATM_T_REMAP,
+    ATM_T_RAD, ATM_T_COUNT
+};
+struct AtmTimer {
+    hipEvent_t *ev = nullptr;  // pairs (start, stop)
+    int cap = 0, used = 0;
+};
+
 struct atm_ctx {
     Geo g;
     double dx, dy;
@@ -150,7 +160,19 @@ struct atm_ctx {
     bool own_stream = false;
     std::string err;
     bool have_coord = false, sw_init = false, remap_init = false, rad_init = false;
-    double ak0 = 0;  // Ak(1) = top pressure
+    double ak0 = 0;    // Ak(1) = top pressure
+    double pktop = 0;  // Ak(1)**kappa, evaluated on the host as the reference does
+    // execution path: fused LDS-tiled kernels (default where the grid tiles
+    // evenly) or the reference-structured kernels of k_dyn_ref.hip
+    bool fused = false;
+    // prognostic fields: two copies each for the fused path; dp/pt/ud/vd
+    // below point at the current copy, prv at the previous one
+    double *pbuf[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    int cur = 0;
+    bool after_d = false;  // a fused D step ran and its halo fill is pending
+    double *uc2 = nullptr, *vc2 = nullptr, *heatd = nullptr;  // fused-path C-step outputs
+    bool timing = false;
+    AtmTimer timers[ATM_T_COUNT];
     double *ak = nullptr, *bk = nullptr;  // device, nlay+1
     // mod_fields
     double *dp = nullptr, *pt = nullptr, *ud = nullptr, *vd = nullptr, *ts = nullptr;
@@ -197,3 +219,8 @@ void atm_rad_free(atm_ctx *c);
 int atm_rad_run(atm_ctx *c, double time);
 int atm_launch_stability(atm_ctx *c);
 int atm_launch_fill(atm_ctx *c, double *a, size_t n, double v);
+void atm_timer_begin(atm_ctx *c, int id);
+void atm_timer_end(atm_ctx *c, int id);
+int atm_remap_run_split(atm_ctx *c, const double *dp, const double *pt, const double *ud_in,
+                        const double *vd_in, const double *ud_ring, const double *vd_ring, double *dp_out,
+                        double *pt_out, double *ud_out, double *vd_out);

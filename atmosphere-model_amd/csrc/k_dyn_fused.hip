@@ -1,0 +1,591 @@
+// k_dyn_fused.hip -- fused, LDS-tiled dynamics kernels (gfx950, FP64).
+//
+// The C-grid and D-grid steps of src/mod_sw_dyn.f90 as four tile kernels
+// per step (C1, C2, D1, D2) plus column scans, instead of one kernel per
+// reference loop nest.  A workgroup of 256 threads owns a TX x TY tile of
+// one level; every intermediate of the reference's level loop (C winds,
+// denominators, inner PCM steps, Laplacians, outer PPM fluxes, vorticity,
+// kinetic energy, divergence damping, B-grid pressure gradient) lives in LDS
+// over the tile plus the halo it needs, so HBM sees only each kernel's
+// inputs and outputs.
+//
+// Every value is computed with the same expression, operand order and
+// rounding as the reference (and as k_dyn_ref.hip), so the two paths agree
+// bit for bit.  Tile-local recomputation of an intermediate in the halo of
+// neighbouring tiles repeats identical arithmetic on identical inputs.
+//
+// The reference evaluates phase 1 of each step on the ring around the
+// interior as well; those ring values are periodic images of interior values
+// because every input is periodic there.  Here each kernel writes the
+// interior only and a periodic halo fill provides the ring, which is the
+// same data.
+//
+// Tile coordinates: x = I - I0, y = J - J0 with (I0, J0) the tile origin in
+// the halo'd index space; LDS arrays cover [x0, x1) x [y0, y1) regions.
+#include "atm_common.hpp"
+#include "atm_fused.hpp"
+
+namespace {
+
+constexpr int TX = ATM_TILE_X;
+constexpr int TY = ATM_TILE_Y;
+constexpr int NT = 256;
+static_assert(TX * TY == NT, "one output cell per thread");
+
+template <int X0, int X1, int Y0, int Y1>
+struct Reg {
+    static constexpr int x0 = X0, x1 = X1, y0 = Y0, y1 = Y1;
+    static constexpr int w = X1 - X0, h = Y1 - Y0, n = w * h;
+    __device__ static int at(int x, int y) { return (y - Y0) * w + (x - X0); }
+};
+
+#define FOR_REG(R)                                                   \
+    for (int t_ = threadIdx.x; t_ < R::n; t_ += NT) {                \
+        const int x = R::x0 + t_ % R::w, y = R::y0 + t_ / R::w;
+#define END_FOR }
+
+// ---------------------------------------------------------------------------
+// C1 / D1: the dp and pt updates of one level (cgrid 94-268, dgrid 439-598).
+// CGRID: the C winds come from the D winds (109-126) and are also written
+// out; otherwise uc/vc of the C step are read (the D step).
+// ---------------------------------------------------------------------------
+template <int VAR, bool CGRID>
+__global__ __launch_bounds__(NT) void k_pair(Geo g, DynCoef c, const double *__restrict__ dp,
+                                             const double *__restrict__ pt, const double *__restrict__ uw,
+                                             const double *__restrict__ vw, const double *__restrict__ heat,
+                                             double *__restrict__ dpo, double *__restrict__ pto,
+                                             double *__restrict__ uco, double *__restrict__ vco,
+                                             double *__restrict__ ua, double *__restrict__ va) {
+    using RUC = Reg<0, TX + 1, -3, TY + 3>;
+    using RVC = Reg<-3, TX + 3, 0, TY + 1>;
+    using RDX = Reg<0, TX, -3, TY + 3>;  // denx and tmp of the inner x step
+    using RDY = Reg<-3, TX + 3, 0, TY>;  // deny and tmp of the inner y step
+    using RQ = Reg<-3, TX + 3, -3, TY + 3>;
+    using RL = Reg<-1, TX + 1, -1, TY + 1>;
+    using RFX = Reg<0, TX + 1, 0, TY>;
+    using RFY = Reg<0, TX, 0, TY + 1>;
+    __shared__ double sUC[RUC::n], sVC[RVC::n], sDX[RDX::n], sDY[RDY::n], sQ[RQ::n];
+    __shared__ double sTX[RDY::n], sTY[RDX::n], sL[RL::n];
+    __shared__ double sDPFX[RFX::n], sDPFY[RFY::n], sFX[RFX::n], sFY[RFY::n];
+
+    const int K = blockIdx.z;
+    const int I0 = ATM_HALO + blockIdx.x * TX, J0 = ATM_HALO + blockIdx.y * TY;
+#define G(a, x, y) a[g.ix(I0 + (x), J0 + (y), K)]
+#define UC(x, y) sUC[RUC::at(x, y)]
+#define VC(x, y) sVC[RVC::at(x, y)]
+#define Q(x, y) sQ[RQ::at(x, y)]
+#define LP(x, y) sL[RL::at(x, y)]
+
+    // C winds (cgrid 109-117) or the C step's winds (dgrid)
+    FOR_REG(RUC)
+    if (CGRID) {
+        double ub = .5 * (G(uw, x, y) + G(uw, x - 1, y));
+        UC(x, y) = .5 * (ub + .5 * (G(uw, x, y + 1) + G(uw, x - 1, y + 1)));
+    } else {
+        UC(x, y) = G(uw, x, y);
+    }
+    END_FOR
+    FOR_REG(RVC)
+    if (CGRID) {
+        double vb = .5 * (G(vw, x, y) + G(vw, x, y - 1));
+        VC(x, y) = .5 * (vb + .5 * (G(vw, x + 1, y) + G(vw, x + 1, y - 1)));
+    } else {
+        VC(x, y) = G(vw, x, y);
+    }
+    END_FOR
+    FOR_REG(RQ) Q(x, y) = G(dp, x, y); END_FOR
+    __syncthreads();
+
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    double dpv = 0., dpcv = 0.;
+    for (int pass = 0; pass < 2; pass++) {
+        // inner y step (tmp of the outer x step), with deny; inner x step with denx; Laplacian
+        FOR_REG(RDY)
+        const double v0 = VC(x, y), v1 = VC(x, y + 1);
+        double dny;
+        if (pass == 0) {
+            dny = 1. / (1. - c.dtdy * (v1 - v0));
+            sDY[RDY::at(x, y)] = dny;
+        } else {
+            dny = sDY[RDY::at(x, y)];
+        }
+        const double q0 = Q(x, y);
+        const double f0 = pcm_face(Q(x, y - 1), q0, c.dtdx * v0);  // Q2: cy = dt/dx * vc
+        const double f1 = pcm_face(q0, Q(x, y + 1), c.dtdx * v1);
+        sTX[RDY::at(x, y)] = .5 * (q0 + dny * (q0 - c.rA * (f1 - f0)));
+        END_FOR
+        FOR_REG(RDX)
+        const double u0 = UC(x, y), u1 = UC(x + 1, y);
+        double dnx;
+        if (pass == 0) {
+            dnx = 1. / (1. - c.dtdx * (u1 - u0));
+            sDX[RDX::at(x, y)] = dnx;
+        } else {
+            dnx = sDX[RDX::at(x, y)];
+        }
+        const double q0 = Q(x, y);
+        const double f0 = pcm_face(Q(x - 1, y), q0, c.dtdx * u0);
+        const double f1 = pcm_face(q0, Q(x + 1, y), c.dtdx * u1);
+        sTY[RDX::at(x, y)] = .5 * (q0 + dnx * (q0 - c.rA * (f1 - f0)));
+        END_FOR
+        FOR_REG(RL)
+        LP(x, y) = -2. * Q(x, y) * c.rdx2rdy2 + (Q(x + 1, y) + Q(x - 1, y)) * c.rdx2 +
+                   (Q(x, y + 1) + Q(x, y - 1)) * c.rdy2;
+        END_FOR
+        __syncthreads();
+        // outer steps: PPM fluxes on faces
+        FOR_REG(RFX)
+#define T(m) sTX[RDY::at(x + (m), y)]
+        const double cx = c.dtdx * UC(x, y);
+        const double f = ppm_face<VAR>(T(-3), T(-2), T(-1), T(0), T(1), T(2), cx);
+#undef T
+        if (pass == 0) sDPFX[RFX::at(x, y)] = f * cx;
+        else sFX[RFX::at(x, y)] = f;
+        END_FOR
+        FOR_REG(RFY)
+#define T(m) sTY[RDX::at(x, y + (m))]
+        const double cy = c.dtdx * VC(x, y);  // Q2
+        const double f = ppm_face<VAR>(T(-3), T(-2), T(-1), T(0), T(1), T(2), cy);
+#undef T
+        if (pass == 0) sDPFY[RFY::at(x, y)] = f * cy;
+        else sFY[RFY::at(x, y)] = f;
+        END_FOR
+        __syncthreads();
+        const double l0 = LP(tx, ty);
+        const double lxp = LP(tx + 1, ty), lxm = LP(tx - 1, ty), lyp = LP(tx, ty + 1), lym = LP(tx, ty - 1);
+        const double fx0 = sDPFX[RFX::at(tx, ty)], fx1 = sDPFX[RFX::at(tx + 1, ty)];
+        const double fy0 = sDPFY[RFY::at(tx, ty)], fy1 = sDPFY[RFY::at(tx, ty + 1)];
+        if (pass == 0) {
+            dpv = Q(tx, ty);
+            double d = dpv - (fx1 - fx0 + c.dtdx * c.vx * (lxp - 2. * l0 + lxm));
+            d = d - (fy1 - fy0 + c.dtdy * c.vy * (lyp - 2. * l0 + lym));
+            dpcv = d;
+            G(dpo, tx, ty) = d;
+            __syncthreads();
+            FOR_REG(RQ) Q(x, y) = G(pt, x, y); END_FOR
+            __syncthreads();
+        } else {
+            const double ptv = Q(tx, ty);
+            double p = dpv * ptv - (sFX[RFX::at(tx + 1, ty)] * fx1 - sFX[RFX::at(tx, ty)] * fx0 +
+                                    c.vx * ((lxp - l0) * fx1 - (l0 - lxm) * fx0));
+            p = (p - (sFY[RFY::at(tx, ty + 1)] * fy1 - sFY[RFY::at(tx, ty)] * fy0 +
+                      c.vy * ((lyp - l0) * fy1 - (l0 - lym) * fy0)) +
+                 c.dt * dpv * G(heat, tx, ty)) /
+                dpcv;
+            G(pto, tx, ty) = p;
+        }
+    }
+    if (CGRID) {
+        const double u = UC(tx, ty), v = VC(tx, ty);
+        G(uco, tx, ty) = u;
+        G(vco, tx, ty) = v;
+        if (K == g.L - 1) {  // A winds of the last level (119-125), read by the KE pass
+            const size_t p2 = g.ix2(I0 + tx, J0 + ty);
+            ua[p2] = .5 * (u + .25 * (G(uw, tx + 1, ty) + G(uw, tx, ty) + G(uw, tx + 1, ty + 1) + G(uw, tx, ty + 1)));
+            va[p2] = .5 * (v + .25 * (G(vw, tx, ty + 1) + G(vw, tx, ty) + G(vw, tx + 1, ty + 1) + G(vw, tx + 1, ty)));
+        }
+    }
+#undef G
+#undef UC
+#undef VC
+#undef Q
+#undef LP
+}
+
+// ---------------------------------------------------------------------------
+// C2: kinetic energy, vorticity, pressure gradient and the uc/vc update of
+// one level (cgrid 294-414).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_c2(Geo g, DynCoef c, const double *__restrict__ uc1,
+                                           const double *__restrict__ vc1, const double *__restrict__ ud,
+                                           const double *__restrict__ vd, const double *__restrict__ pkap,
+                                           const double *__restrict__ gz, const double *__restrict__ ua,
+                                           const double *__restrict__ va, double *__restrict__ uco,
+                                           double *__restrict__ vco) {
+    using RB = Reg<-3, TX + 3, -3, TY + 3>;
+    using RW = Reg<-2, TX + 3, -2, TY + 3>;
+    using RK = Reg<-1, TX, -1, TY>;
+    using RTV = Reg<0, TX, -2, TY + 3>;
+    using RTW = Reg<-2, TX + 3, 0, TY>;
+    using RLV = Reg<0, TX + 1, 0, TY + 1>;
+    __shared__ double sU[RB::n], sV[RB::n], sW[RW::n], sK[RK::n], sTV[RTV::n], sTW[RTW::n], sL[RLV::n];
+    const int K = blockIdx.z;
+    const int I0 = ATM_HALO + blockIdx.x * TX, J0 = ATM_HALO + blockIdx.y * TY;
+#define G(a, x, y) a[g.ix(I0 + (x), J0 + (y), K)]
+#define U(x, y) sU[RB::at(x, y)]
+#define V(x, y) sV[RB::at(x, y)]
+#define W(x, y) sW[RW::at(x, y)]
+    FOR_REG(RB)
+    U(x, y) = G(uc1, x, y);
+    V(x, y) = G(vc1, x, y);
+    END_FOR
+    __syncthreads();
+    FOR_REG(RW)  // vorticity (319-323)
+    W(x, y) = c.coriolis - (U(x, y) - U(x, y - 1)) * c.rdy + (V(x, y) - V(x - 1, y)) * c.rdx;
+    END_FOR
+    FOR_REG(RK)  // kinetic energy (299-317), last-level A winds, Q4 shifts
+    const size_t p2 = g.ix2(I0 + x, J0 + y);
+    const double vaij = va[p2], uaij = ua[p2];
+    const double fy = ppm_face<V_PPM_U>(V(x, y - 2), V(x, y - 1), V(x, y), V(x, y + 1), V(x, y + 2), V(x, y + 3),
+                                        c.dtdy * vaij);
+    const double fx = ppm_face<V_PPM_U>(U(x - 2, y), U(x - 1, y), U(x, y), U(x + 1, y), U(x + 2, y), U(x + 3, y),
+                                        c.dtdx * uaij);
+    const double k = vaij * fy;
+    sK[RK::at(x, y)] = .5 * (k + uaij * fx);
+    END_FOR
+    __syncthreads();
+    FOR_REG(RTV)  // inner x step of vorticity, Courant cx shifted by one (353-367, Q4)
+    const double w0 = W(x, y);
+    const double um = G(ud, x - 1, y), u0 = G(ud, x, y);
+    const double dnx = 1. / (1. - c.dtdx * (u0 - um));
+    const double f0 = pcm_face(W(x - 1, y), w0, c.dtdx * um);
+    const double f1 = pcm_face(w0, W(x + 1, y), c.dtdx * u0);
+    sTV[RTV::at(x, y)] = .5 * (w0 + dnx * (w0 - c.rA * (f1 - f0)));
+    END_FOR
+    FOR_REG(RTW)  // inner y step of vorticity, Courant cy shifted by one (388-397, Q4)
+    const double w0 = W(x, y);
+    const double vm = G(vd, x, y - 1), v0 = G(vd, x, y);
+    const double dny = 1. / (1. - c.dtdy * (v0 - vm));
+    const double f0 = pcm_face(W(x, y - 1), w0, c.dtdx * vm);
+    const double f1 = pcm_face(w0, W(x, y + 1), c.dtdx * v0);
+    sTW[RTW::at(x, y)] = .5 * (w0 + dny * (w0 - c.rA * (f1 - f0)));
+    END_FOR
+    FOR_REG(RLV)
+    sL[RLV::at(x, y)] = -2. * W(x, y) * c.rdx2rdy2 + (W(x + 1, y) + W(x - 1, y)) * c.rdx2 +
+                        (W(x, y + 1) + W(x, y - 1)) * c.rdy2;
+    END_FOR
+    __syncthreads();
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int I = I0 + tx, J = J0 + ty;
+    // pressure gradient force (325-339)
+    double pgfx, pgfy;
+    {
+        double a = pkap[g.ix(I, J, K)], d = pkap[g.ix(I, J, K + 1)];
+        double b = pkap[g.ix(I - 1, J, K + 1)], cc = pkap[g.ix(I - 1, J, K)];
+        pgfx = ((a - b) * (gz[g.ix(I - 1, J, K)] - gz[g.ix(I, J, K + 1)]) +
+                (cc - d) * (gz[g.ix(I - 1, J, K + 1)] - gz[g.ix(I, J, K)])) /
+               (a - b + cc - d);
+        b = pkap[g.ix(I, J - 1, K + 1)];
+        cc = pkap[g.ix(I, J - 1, K)];
+        pgfy = ((a - b) * (gz[g.ix(I, J - 1, K)] - gz[g.ix(I, J, K + 1)]) +
+                (cc - d) * (gz[g.ix(I, J - 1, K + 1)] - gz[g.ix(I, J, K)])) /
+               (a - b + cc - d);
+    }
+    const double udv = G(ud, tx, ty), vdv = G(vd, tx, ty);
+    // uc (369-382)
+    {
+#define T(m) sTV[RTV::at(tx, ty + 1 + (m))]
+        const double fy = ppm_face<V_PPM_U>(T(-3), T(-2), T(-1), T(0), T(1), T(2), c.dtdx * vdv);
+#undef T
+        G(uco, tx, ty) = U(tx, ty) + c.dt * (vdv * fy + c.vy * (sL[RLV::at(tx, ty + 1)] - sL[RLV::at(tx, ty)])) -
+                         c.dtdx * (sK[RK::at(tx, ty)] - sK[RK::at(tx - 1, ty)] - pgfx);
+    }
+    // vc (399-412)
+    {
+#define T(m) sTW[RTW::at(tx + 1 + (m), ty)]
+        const double fx = ppm_face<V_PPM_U>(T(-3), T(-2), T(-1), T(0), T(1), T(2), c.dtdx * udv);
+#undef T
+        G(vco, tx, ty) = V(tx, ty) - c.dt * (udv * fx + c.vx * (sL[RLV::at(tx + 1, ty)] - sL[RLV::at(tx, ty)])) -
+                         c.dtdy * (sK[RK::at(tx, ty)] - sK[RK::at(tx, ty - 1)] - pgfy);
+    }
+#undef G
+#undef U
+#undef V
+#undef W
+}
+
+// ---------------------------------------------------------------------------
+// D2: kinetic energy, vorticity, divergence damping, B-grid pressure
+// gradient and the ud/vd update of one level (dgrid 606-771).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double corner_lds(const double *q, int w, int x0, int y0, int x, int y) {
+#define QQ(a, b) q[((y) + (b) - y0) * w + ((x) + (a) - x0)]
+#define TXV(b) (ATM_W7_12 * (QQ(0, b) + QQ(-1, b)) - ATM_W1_12 * (QQ(1, b) + QQ(-2, b)))
+#define TYV(a) (ATM_W7_12 * (QQ(a, 0) + QQ(a, -1)) - ATM_W1_12 * (QQ(a, 1) + QQ(a, -2)))
+    double o = (9. / 32.) * (TXV(0) + TXV(-1)) - (1. / 32.) * (TXV(1) + TXV(-2));
+    o = o + (9. / 32.) * (TYV(0) + TYV(-1)) - (1. / 32.) * (TYV(1) + TYV(-2));
+    return o;
+#undef QQ
+#undef TXV
+#undef TYV
+}
+
+__global__ __launch_bounds__(NT) void k_d2(Geo g, DynCoef c, double pkb_top, const double *__restrict__ ud,
+                                           const double *__restrict__ vd, const double *__restrict__ uc,
+                                           const double *__restrict__ vc, const double *__restrict__ pkap,
+                                           const double *__restrict__ gz, double *__restrict__ udo,
+                                           double *__restrict__ vdo) {
+    using RU = Reg<-3, TX + 3, -3, TY + 4>;
+    using RV = Reg<-3, TX + 4, -3, TY + 3>;
+    using RKE = Reg<0, TX + 1, 0, TY + 1>;
+    using RW = Reg<-3, TX + 3, -3, TY + 3>;
+    using RDV = Reg<-2, TX + 3, -2, TY + 3>;
+    using RD2 = Reg<-1, TX + 2, -1, TY + 2>;
+    using RTU = Reg<0, TX, -3, TY + 3>;
+    using RTW = Reg<-3, TX + 3, 0, TY>;
+    using RLP = Reg<-1, TX, -1, TY>;
+    using RQS = Reg<-2, TX + 2, -2, TY + 2>;
+    __shared__ double sU[RU::n], sV[RV::n], sKE[RKE::n], sW[RW::n], sDV[RDV::n], sD2[RD2::n], sD4[RKE::n];
+    __shared__ double sTU[RTU::n], sTW[RTW::n], sL[RLP::n], sQS[RQS::n], sCB[4][RKE::n];
+    const int K = blockIdx.z;
+    const int I0 = ATM_HALO + blockIdx.x * TX, J0 = ATM_HALO + blockIdx.y * TY;
+#define G(a, x, y) a[g.ix(I0 + (x), J0 + (y), K)]
+#define U(x, y) sU[RU::at(x, y)]
+#define V(x, y) sV[RV::at(x, y)]
+#define W(x, y) sW[RW::at(x, y)]
+#define DV(x, y) sDV[RDV::at(x, y)]
+#define D2(x, y) sD2[RD2::at(x, y)]
+    FOR_REG(RU) U(x, y) = G(ud, x, y); END_FOR
+    FOR_REG(RV) V(x, y) = G(vd, x, y); END_FOR
+    __syncthreads();
+    FOR_REG(RKE)  // kinetic energy (639-657), PPM_CONSTRAINED, last-level B winds
+    const int Lm = g.L - 1;
+    const double ubij = .5 * (uc[g.ix(I0 + x, J0 + y, Lm)] + uc[g.ix(I0 + x, J0 + y - 1, Lm)]);
+    const double vbij = .5 * (vc[g.ix(I0 + x, J0 + y, Lm)] + vc[g.ix(I0 + x - 1, J0 + y, Lm)]);
+    const double fy = ppm_face<V_PPM_C>(V(x, y - 3), V(x, y - 2), V(x, y - 1), V(x, y), V(x, y + 1), V(x, y + 2),
+                                        c.dtdy * vbij);
+    const double fx = ppm_face<V_PPM_C>(U(x - 3, y), U(x - 2, y), U(x - 1, y), U(x, y), U(x + 1, y), U(x + 2, y),
+                                        c.dtdx * ubij);
+    const double k = vbij * fy;
+    sKE[RKE::at(x, y)] = .5 * (k + ubij * fx);
+    END_FOR
+    FOR_REG(RW)  // vorticity (659-663)
+    W(x, y) = c.coriolis - (U(x, y + 1) - U(x, y)) * c.rdy + (V(x + 1, y) - V(x, y)) * c.rdx;
+    END_FOR
+    FOR_REG(RDV)  // divergence (682-685)
+    DV(x, y) = c.rdx * (U(x, y) - U(x - 1, y)) + c.rdy * (V(x, y) - V(x, y - 1));
+    END_FOR
+    __syncthreads();
+    FOR_REG(RD2)  // del^2 divergence (688-692)
+    D2(x, y) = -2. * DV(x, y) * c.rdx2rdy2 + (DV(x + 1, y) + DV(x - 1, y)) * c.rdx2 +
+               (DV(x, y + 1) + DV(x, y - 1)) * c.rdy2;
+    END_FOR
+    // The reference's second level loop (634-771) reuses the 2-D arrays cx,
+    // cy, denom_x, denom_y (and ub, vb) that its first loop (441-454) left
+    // at the last level, k = nlay: the vorticity fluxes of every level use
+    // the C winds of level nlay for their Courant numbers and denominators.
+    const int Lm = g.L - 1;
+#define GL(a, x, y) a[g.ix(I0 + (x), J0 + (y), Lm)]
+    FOR_REG(RTU)  // inner x step of vorticity for ud (708-716)
+    const double w0 = W(x, y);
+    const double u0 = GL(uc, x, y), u1 = GL(uc, x + 1, y);
+    const double dnx = 1. / (1. - c.dtdx * (u1 - u0));
+    const double f0 = pcm_face(W(x - 1, y), w0, c.dtdx * u0);
+    const double f1 = pcm_face(w0, W(x + 1, y), c.dtdx * u1);
+    sTU[RTU::at(x, y)] = .5 * (w0 + dnx * (w0 - c.rA * (f1 - f0)));
+    END_FOR
+    FOR_REG(RTW)  // inner y step of vorticity for vd (744-752)
+    const double w0 = W(x, y);
+    const double v0 = GL(vc, x, y), v1 = GL(vc, x, y + 1);
+    const double dny = 1. / (1. - c.dtdy * (v1 - v0));
+    const double f0 = pcm_face(W(x, y - 1), w0, c.dtdx * v0);  // Q2
+    const double f1 = pcm_face(w0, W(x, y + 1), c.dtdx * v1);
+    sTW[RTW::at(x, y)] = .5 * (w0 + dny * (w0 - c.rA * (f1 - f0)));
+    END_FOR
+    FOR_REG(RLP)  // Laplacian of vorticity (717-720)
+    sL[RLP::at(x, y)] = -2. * W(x, y) * c.rdx2rdy2 + (W(x + 1, y) + W(x - 1, y)) * c.rdx2 +
+                        (W(x, y + 1) + W(x, y - 1)) * c.rdy2;
+    END_FOR
+    __syncthreads();
+    FOR_REG(RKE)  // del^4 divergence times the damping coefficient (695-702)
+    sD4[RKE::at(x, y)] = c.divdamp * (-2. * D2(x, y) * c.rdx2rdy2 + (D2(x + 1, y) + D2(x - 1, y)) * c.rdx2 +
+                                      (D2(x, y + 1) + D2(x, y - 1)) * c.rdy2);
+    END_FOR
+    // B-grid pkap and gz of levels k and k+1 (606-632 and init 991, 995)
+    for (int f = 0; f < 4; f++) {
+        const double *src = f < 2 ? pkap : gz;
+        const int Kq = K + (f & 1);
+        if (f == 1 && Kq == g.L) {
+            FOR_REG(RKE) sCB[f][RKE::at(x, y)] = pkb_top; END_FOR
+            continue;
+        }
+        __syncthreads();
+        FOR_REG(RQS) sQS[RQS::at(x, y)] = src[g.ix(I0 + x, J0 + y, Kq)]; END_FOR
+        __syncthreads();
+        FOR_REG(RKE) sCB[f][RKE::at(x, y)] = corner_lds(sQS, RQS::w, RQS::x0, RQS::y0, x, y); END_FOR
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+#define CB(f, x, y) sCB[f][RKE::at(x, y)]
+    double pgfx, pgfy;
+    {  // 665-679
+        double a = CB(0, tx + 1, ty), b = CB(1, tx, ty), cc = CB(0, tx, ty), d = CB(1, tx + 1, ty);
+        pgfx = ((a - b) * (CB(2, tx, ty) - CB(3, tx + 1, ty)) + (cc - d) * (CB(3, tx, ty) - CB(2, tx + 1, ty))) /
+               (a - b + cc - d);
+        a = CB(0, tx, ty + 1);
+        d = CB(1, tx, ty + 1);
+        pgfy = ((a - b) * (CB(2, tx, ty) - CB(3, tx, ty + 1)) + (cc - d) * (CB(3, tx, ty) - CB(2, tx, ty + 1))) /
+               (a - b + cc - d);
+    }
+#undef CB
+    const double ucv = G(uc, tx, ty), vcv = G(vc, tx, ty);
+    const double ucl = GL(uc, tx, ty), vcl = GL(vc, tx, ty);
+#undef GL
+    {  // ud (724-738)
+#define T(m) sTU[RTU::at(tx, ty + (m))]
+        const double fy = ppm_face<V_PPM_C>(T(-3), T(-2), T(-1), T(0), T(1), T(2), c.dtdx * vcl);
+#undef T
+        G(udo, tx, ty) = U(tx, ty) +
+                         c.dt * (vcv * fy + c.vy * (sL[RLP::at(tx, ty)] - sL[RLP::at(tx, ty - 1)])) -
+                         c.dtdx * (sKE[RKE::at(tx + 1, ty)] - sKE[RKE::at(tx, ty)] - sD4[RKE::at(tx + 1, ty)] +
+                                   sD4[RKE::at(tx, ty)] - pgfx);
+    }
+    {  // vd (755-769)
+#define T(m) sTW[RTW::at(tx + (m), ty)]
+        const double fx = ppm_face<V_PPM_C>(T(-3), T(-2), T(-1), T(0), T(1), T(2), c.dtdx * ucl);
+#undef T
+        G(vdo, tx, ty) = V(tx, ty) -
+                         c.dt * (ucv * fx + c.vx * (sL[RLP::at(tx, ty)] - sL[RLP::at(tx - 1, ty)])) -
+                         c.dtdy * (sKE[RKE::at(tx, ty + 1)] - sKE[RKE::at(tx, ty)] - sD4[RKE::at(tx, ty + 1)] +
+                                   sD4[RKE::at(tx, ty)] - pgfy);
+    }
+#undef G
+#undef U
+#undef V
+#undef W
+#undef DV
+#undef D2
+}
+
+// ---------------------------------------------------------------------------
+// Column scans.  One thread per column, i fastest across the wavefront.
+// ---------------------------------------------------------------------------
+// heating rate of the C step from dp (cgrid 89-106) on interior columns
+__global__ void k_scan_heat(Geo g, double ptop, double pktop, double gravity, const double *__restrict__ src,
+                            const double *__restrict__ nf, double *__restrict__ heat) {
+    const int I = ATM_HALO + blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = ATM_HALO + blockIdx.y;
+    if (I >= ATM_HALO + g.nx) return;
+    double p = ptop, pk1 = pktop;
+    double nf1 = nf[g.ix(I, J, g.L)];
+    for (int K = g.L - 1; K >= 0; K--) {
+        const double s = src[g.ix(I, J, K)];
+        p = p + s;
+        const double pk = pow(p, ATM_KAPPA);
+        const double plk = (pk1 - pk) / log(pk1 / pk);
+        const double nf0 = nf[g.ix(I, J, K)];
+        heat[g.ix(I, J, K)] = (nf1 - nf0) / (ATM_CP_DRY * s / gravity) / plk;
+        pk1 = pk;
+        nf1 = nf0;
+    }
+}
+
+// pkap and gz from (src, th) over all columns (cgrid 276-292, dgrid 433-437 +
+// 606-626); optional plev output and optional heating rate of the D step.
+__global__ void k_scan_pg(Geo g, double ptop, double pktop, double gravity, const double *__restrict__ src,
+                          const double *__restrict__ th, const double *__restrict__ nf, double *__restrict__ plev,
+                          double *__restrict__ pkap, double *__restrict__ gz, double *__restrict__ heat) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y;
+    if (I >= g.NI) return;
+    double p = ptop, pk1 = pktop;
+    double nf1 = heat ? nf[g.ix(I, J, g.L)] : 0.;
+    const bool interior = I >= ATM_HALO && I < ATM_HALO + g.nx && J >= ATM_HALO && J < ATM_HALO + g.ny;
+    for (int K = g.L - 1; K >= 0; K--) {
+        const double s = src[g.ix(I, J, K)];
+        p = p + s;
+        const double pk = pow(p, ATM_KAPPA);
+        if (plev) plev[g.ix(I, J, K)] = p;
+        pkap[g.ix(I, J, K)] = pk;
+        if (heat && interior) {
+            const double plk = (pk1 - pk) / log(pk1 / pk);
+            const double nf0 = nf[g.ix(I, J, K)];
+            heat[g.ix(I, J, K)] = (nf1 - nf0) / (ATM_CP_DRY * s / gravity) / plk;
+            nf1 = nf0;
+        }
+        pk1 = pk;
+    }
+    double z = gz[g.ix(I, J, 0)];
+    double pk = pkap[g.ix(I, J, 0)];
+    for (int K = 0; K < g.L; K++) {
+        const double pkn = K + 1 == g.L ? pktop : pkap[g.ix(I, J, K + 1)];
+        z = z + ATM_CP_DRY * th[g.ix(I, J, K)] * (pk - pkn);
+        gz[g.ix(I, J, K + 1)] = z;
+        pk = pkn;
+    }
+}
+
+}  // namespace
+
+bool atm_fused_supported(const Geo &g) { return g.nx % TX == 0 && g.ny % TY == 0; }
+
+// ---------------------------------------------------------------------------
+// Step orchestration of the fused path.  Buffers: prognostic fields have two
+// copies (cur, nxt); the C step writes its own arrays; the D step writes nxt.
+// ---------------------------------------------------------------------------
+int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
+    const Geo &g = c->g;
+    c->coef.dt = dt;
+    c->coef.dtdx = dt * c->coef.rdx;
+    c->coef.dtdy = dt * c->coef.rdy;
+    const DynCoef &k = c->coef;
+    const double pktop = c->pktop;
+    atm_timer_begin(c, ATM_T_SCAN);
+    hipLaunchKernelGGL(k_scan_heat, dim3((g.nx + 63) / 64, g.ny), dim3(64), 0, c->stream, g, c->ak0, pktop,
+                       k.gravity, b.dp, c->net_flux, c->heat);
+    atm_timer_end(c, ATM_T_SCAN);
+    dim3 grid(g.nx / TX, g.ny / TY, g.L), blk(NT);
+    atm_timer_begin(c, ATM_T_C1);
+    hipLaunchKernelGGL((k_pair<V_PPM_U, true>), grid, blk, 0, c->stream, g, k, b.dp, b.pt, b.ud, b.vd, c->heat,
+                       c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va);
+    atm_timer_end(c, ATM_T_C1);
+    double *fill[] = {c->dpc, c->ptc, c->uc, c->vc};
+    atm_timer_begin(c, ATM_T_HALO);
+    for (double *a : fill) {
+        int rc = atm_launch_halo(c, a, g.L);
+        if (rc) return rc;
+    }
+    int rc = atm_launch_halo(c, c->ua, 1);
+    if (!rc) rc = atm_launch_halo(c, c->va, 1);
+    atm_timer_end(c, ATM_T_HALO);
+    if (rc) return rc;
+    atm_timer_begin(c, ATM_T_SCAN);
+    hipLaunchKernelGGL(k_scan_pg, dim3((g.NI + 63) / 64, g.NJ), dim3(64), 0, c->stream, g, c->ak0, pktop, k.gravity,
+                       c->dpc, c->ptc, c->net_flux, (double *)nullptr, c->pkap, c->gz, c->heatd);
+    atm_timer_end(c, ATM_T_SCAN);
+    atm_timer_begin(c, ATM_T_C2);
+    hipLaunchKernelGGL(k_c2, grid, blk, 0, c->stream, g, k, c->uc, c->vc, b.ud, b.vd, c->pkap, c->gz, c->ua, c->va,
+                       c->uc2, c->vc2);
+    atm_timer_end(c, ATM_T_C2);
+    ATM_HIP(c, hipGetLastError());
+    return ATM_OK;
+}
+
+// the halo fill of cgrid_halo_exchange for the fused path: dpc/ptc are
+// already periodic, uc/vc were updated on the interior
+int atm_fused_cgrid_halo(atm_ctx *c) {
+    atm_timer_begin(c, ATM_T_HALO);
+    int rc = atm_launch_halo(c, c->uc2, c->g.L);
+    if (!rc) rc = atm_launch_halo(c, c->vc2, c->g.L);
+    atm_timer_end(c, ATM_T_HALO);
+    return rc;
+}
+
+int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
+    const Geo &g = c->g;
+    c->coef.dt = dt;
+    c->coef.dtdx = dt * c->coef.rdx;
+    c->coef.dtdy = dt * c->coef.rdy;
+    const DynCoef &k = c->coef;
+    dim3 grid(g.nx / TX, g.ny / TY, g.L), blk(NT);
+    atm_timer_begin(c, ATM_T_D1);
+    hipLaunchKernelGGL((k_pair<V_PPM_NN, false>), grid, blk, 0, c->stream, g, k, cur.dp, cur.pt, c->uc2, c->vc2,
+                       c->heatd, nxt.dp, nxt.pt, (double *)nullptr, (double *)nullptr, (double *)nullptr,
+                       (double *)nullptr);
+    atm_timer_end(c, ATM_T_D1);
+    atm_timer_begin(c, ATM_T_HALO);
+    int rc = atm_launch_halo(c, nxt.dp, g.L);
+    if (!rc) rc = atm_launch_halo(c, nxt.pt, g.L);
+    atm_timer_end(c, ATM_T_HALO);
+    if (rc) return rc;
+    atm_timer_begin(c, ATM_T_SCAN);
+    hipLaunchKernelGGL(k_scan_pg, dim3((g.NI + 63) / 64, g.NJ), dim3(64), 0, c->stream, g, c->ak0, c->pktop,
+                       k.gravity, nxt.dp, nxt.pt, c->net_flux, c->plev, c->pkap, c->gz, (double *)nullptr);
+    atm_timer_end(c, ATM_T_SCAN);
+    atm_timer_begin(c, ATM_T_D2);
+    hipLaunchKernelGGL(k_d2, grid, blk, 0, c->stream, g, k, c->pktop, cur.ud, cur.vd, c->uc2, c->vc2, c->pkap,
+                       c->gz, nxt.ud, nxt.vd);
+    atm_timer_end(c, ATM_T_D2);
+    ATM_HIP(c, hipGetLastError());
+    return ATM_OK;
+}

@@ -261,27 +261,32 @@ __global__ void k_c_denoms2(Geo g, Rng r, DynCoef c, const double *__restrict__ 
 
 // inner x PCM of vorticity with Courant cx shifted by sh (C grid: sh = -1, Q4;
 // D grid: 0), tmp and laplacian (353-367 / D 708-722)
-__global__ void k_vort_inner_x(Geo g, Rng r, DynCoef c, int sh, const double *__restrict__ vort,
+// kc >= 0 reads cx/denx of level kc instead of the current level (D grid:
+// those 2-D arrays still hold the first level loop's last level, see
+// atm_ref_dgrid)
+__global__ void k_vort_inner_x(Geo g, Rng r, DynCoef c, int sh, int kc, const double *__restrict__ vort,
                                const double *__restrict__ cx, const double *__restrict__ denx,
                                double *__restrict__ tmp, double *__restrict__ lap) {
     RANGE_GUARD(r);
+    const int Kc = kc >= 0 ? kc : K;
     double w0 = vort[g.ix(I, J, K)];
-    double f0 = pcm_face(vort[g.ix(I - 1, J, K)], w0, cx[g.ix(I + sh, J, K)]);
-    double f1 = pcm_face(w0, vort[g.ix(I + 1, J, K)], cx[g.ix(I + 1 + sh, J, K)]);
-    tmp[g.ix(I, J, K)] = .5 * (w0 + denx[g.ix(I, J, K)] * (w0 - c.rA * (f1 - f0)));
+    double f0 = pcm_face(vort[g.ix(I - 1, J, K)], w0, cx[g.ix(I + sh, J, Kc)]);
+    double f1 = pcm_face(w0, vort[g.ix(I + 1, J, K)], cx[g.ix(I + 1 + sh, J, Kc)]);
+    tmp[g.ix(I, J, K)] = .5 * (w0 + denx[g.ix(I, J, Kc)] * (w0 - c.rA * (f1 - f0)));
     lap[g.ix(I, J, K)] = -2. * w0 * c.rdx2rdy2 + (vort[g.ix(I + 1, J, K)] + vort[g.ix(I - 1, J, K)]) * c.rdx2 +
                          (vort[g.ix(I, J + 1, K)] + vort[g.ix(I, J - 1, K)]) * c.rdy2;
 }
 
 // inner y PCM of vorticity with Courant cy shifted by sh (388-397 / D 744-753)
-__global__ void k_vort_inner_y(Geo g, Rng r, DynCoef c, int sh, const double *__restrict__ vort,
+__global__ void k_vort_inner_y(Geo g, Rng r, DynCoef c, int sh, int kc, const double *__restrict__ vort,
                                const double *__restrict__ cy, const double *__restrict__ deny,
                                double *__restrict__ tmp) {
     RANGE_GUARD(r);
+    const int Kc = kc >= 0 ? kc : K;
     double w0 = vort[g.ix(I, J, K)];
-    double f0 = pcm_face(vort[g.ix(I, J - 1, K)], w0, cy[g.ix(I, J + sh, K)]);
-    double f1 = pcm_face(w0, vort[g.ix(I, J + 1, K)], cy[g.ix(I, J + 1 + sh, K)]);
-    tmp[g.ix(I, J, K)] = .5 * (w0 + deny[g.ix(I, J, K)] * (w0 - c.rA * (f1 - f0)));
+    double f0 = pcm_face(vort[g.ix(I, J - 1, K)], w0, cy[g.ix(I, J + sh, Kc)]);
+    double f1 = pcm_face(w0, vort[g.ix(I, J + 1, K)], cy[g.ix(I, J + 1 + sh, Kc)]);
+    tmp[g.ix(I, J, K)] = .5 * (w0 + deny[g.ix(I, J, Kc)] * (w0 - c.rA * (f1 - f0)));
 }
 
 // uc update (369-382): outer PPM_U along y on tmp shifted by +1 (Q4)
@@ -417,7 +422,7 @@ __global__ void k_del2(Geo g, Rng r, DynCoef c, const double *__restrict__ a, do
     out[g.ix(I, J, K)] = scale ? c.divdamp * v : v;
 }
 
-// ud update (724-738)
+// ud update (724-738); cy of the last level (see atm_ref_dgrid)
 __global__ void k_d_ud(Geo g, Rng r, DynCoef c, const double *__restrict__ tmp,
                        const double *__restrict__ cy, const double *__restrict__ vc,
                        const double *__restrict__ lap, const double *__restrict__ ke,
@@ -425,7 +430,7 @@ __global__ void k_d_ud(Geo g, Rng r, DynCoef c, const double *__restrict__ tmp,
                        double *__restrict__ ud) {
     RANGE_GUARD(r);
 #define TQ(m) tmp[g.ix(I, J + (m), K)]
-    double fy = ppm_face<V_PPM_C>(TQ(-3), TQ(-2), TQ(-1), TQ(0), TQ(1), TQ(2), cy[g.ix(I, J, K)]);
+    double fy = ppm_face<V_PPM_C>(TQ(-3), TQ(-2), TQ(-1), TQ(0), TQ(1), TQ(2), cy[g.ix(I, J, g.L - 1)]);
 #undef TQ
     ud[g.ix(I, J, K)] = ud[g.ix(I, J, K)] +
                         c.dt * (vc[g.ix(I, J, K)] * fy + c.vy * (lap[g.ix(I, J, K)] - lap[g.ix(I, J - 1, K)])) -
@@ -433,7 +438,7 @@ __global__ void k_d_ud(Geo g, Rng r, DynCoef c, const double *__restrict__ tmp,
                                   div[g.ix(I, J, K)] - pgfx[g.ix(I, J, K)]);
 }
 
-// vd update (755-769)
+// vd update (755-769); cx of the last level (see atm_ref_dgrid)
 __global__ void k_d_vd(Geo g, Rng r, DynCoef c, const double *__restrict__ tmp,
                        const double *__restrict__ cx, const double *__restrict__ uc,
                        const double *__restrict__ lap, const double *__restrict__ ke,
@@ -441,7 +446,7 @@ __global__ void k_d_vd(Geo g, Rng r, DynCoef c, const double *__restrict__ tmp,
                        double *__restrict__ vd) {
     RANGE_GUARD(r);
 #define TQ(m) tmp[g.ix(I + (m), J, K)]
-    double fx = ppm_face<V_PPM_C>(TQ(-3), TQ(-2), TQ(-1), TQ(0), TQ(1), TQ(2), cx[g.ix(I, J, K)]);
+    double fx = ppm_face<V_PPM_C>(TQ(-3), TQ(-2), TQ(-1), TQ(0), TQ(1), TQ(2), cx[g.ix(I, J, g.L - 1)]);
 #undef TQ
     vd[g.ix(I, J, K)] = vd[g.ix(I, J, K)] -
                         c.dt * (uc[g.ix(I, J, K)] * fx + c.vx * (lap[g.ix(I, J, K)] - lap[g.ix(I - 1, J, K)])) -
@@ -544,15 +549,19 @@ int atm_ref_cgrid(atm_ctx *c, double dt) {
     LAUNCH(k_c_vort, R(g, 2, 2), L, k, c->uc, c->vc, c->vort);
     LAUNCH(k_c_pgf, R(g, 7, 7), L, c->pkap, c->gz, c->pgfx, c->pgfy);
     LAUNCH(k_c_denoms2, R(g, 1, 1), L, k, c->ud, c->vd, c->denx, c->deny, c->cx, c->cy);
-    LAUNCH(k_vort_inner_x, R(g, 3, 3), L, k, -1, c->vort, c->cx, c->denx, c->tmp, c->lap);
+    LAUNCH(k_vort_inner_x, R(g, 3, 3), L, k, -1, -1, c->vort, c->cx, c->denx, c->tmp, c->lap);
     LAUNCH(k_c_uc, R(g, 7, 7), L, k, c->tmp, c->cy, c->vd, c->lap, c->ke, c->pgfx, c->uc);
-    LAUNCH(k_vort_inner_y, R(g, 3, 3), L, k, -1, c->vort, c->cy, c->deny, c->tmp);
+    LAUNCH(k_vort_inner_y, R(g, 3, 3), L, k, -1, -1, c->vort, c->cy, c->deny, c->tmp);
     LAUNCH(k_c_vc, R(g, 7, 7), L, k, c->tmp, c->cx, c->ud, c->lap, c->ke, c->pgfy, c->vc);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;
 }
 
-// dgrid_dynamics_step (424-773)
+// dgrid_dynamics_step (424-773).  The second level loop (634-771) does not
+// recompute the 2-D work arrays cx, cy, denom_x, denom_y, ub, vb set by the
+// first loop (441-454), so every level's vorticity fluxes and kinetic energy
+// use their values from the last level, k = nlay; the kernels below read
+// level nlay of those arrays.
 int atm_ref_dgrid(atm_ctx *c, double dt) {
     const Geo &g = c->g;
     set_dt(c, dt);
@@ -593,9 +602,9 @@ int atm_ref_dgrid(atm_ctx *c, double dt) {
     LAUNCH(k_d_div, R(g, 4, 4), L, k, c->ud, c->vd, c->divg);
     LAUNCH(k_del2, R(g, 5, 5), L, k, c->divg, c->tmp, 0);
     LAUNCH(k_del2, R(g, 6, 6), L, k, c->tmp, c->divg, 1);
-    LAUNCH(k_vort_inner_x, R(g, 4, 4), L, k, 0, c->vort, c->cx, c->denx, c->tmp, c->lap);
+    LAUNCH(k_vort_inner_x, R(g, 4, 4), L, k, 0, L - 1, c->vort, c->cx, c->denx, c->tmp, c->lap);
     LAUNCH(k_d_ud, R(g, 7, 7), L, k, c->tmp, c->cy, c->vc, c->lap, c->ke, c->divg, c->pgfx, c->ud);
-    LAUNCH(k_vort_inner_y, R(g, 4, 4), L, k, 0, c->vort, c->cy, c->deny, c->tmp);
+    LAUNCH(k_vort_inner_y, R(g, 4, 4), L, k, 0, L - 1, c->vort, c->cy, c->deny, c->tmp);
     LAUNCH(k_d_vd, R(g, 7, 7), L, k, c->tmp, c->cx, c->uc, c->lap, c->ke, c->divg, c->pgfy, c->vd);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;

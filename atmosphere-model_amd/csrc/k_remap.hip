@@ -157,6 +157,17 @@ __device__ void remap_column(const Col &c, const Slots &S, int L, double *out, s
 #undef DC
 }
 
+// The winds remap reads ud/vd on the interior and on the ring around it.
+// The ring holds the halo values the previous step's halo exchange left
+// there (the D-grid step updates the interior only, Q5); with separate
+// input buffers for interior and ring, `pick` returns the value the
+// reference's single array holds at that point.
+__device__ __forceinline__ double pick(const Geo &g, const double *__restrict__ in,
+                                       const double *__restrict__ ring, int I, int J, int K) {
+    const bool inside = I >= ATM_HALO && I < ATM_HALO + g.nx && J >= ATM_HALO && J < ATM_HALO + g.ny;
+    return inside ? in[g.ix(I, J, K)] : ring[g.ix(I, J, K)];
+}
+
 // current and target grids and gz over isd-3..ied+3 (42-78)
 __global__ void k_remap_grid(Geo g, double top, const double *__restrict__ ak, const double *__restrict__ bk,
                              const double *__restrict__ dp, const double *__restrict__ pt,
@@ -202,7 +213,8 @@ __global__ void k_remap_grid(Geo g, double top, const double *__restrict__ ak, c
 // winds on faces over isd-1..ied+1 (80-105)
 __global__ void k_remap_winds(Geo g, double *__restrict__ scr, const double *__restrict__ plev,
                               const double *__restrict__ tplev, const double *__restrict__ ud,
-                              const double *__restrict__ vd, double *__restrict__ ur, double *__restrict__ vr) {
+                              const double *__restrict__ vd, const double *__restrict__ udr,
+                              const double *__restrict__ vdr, double *__restrict__ ur, double *__restrict__ vr) {
     const int I = 6 + blockIdx.x * blockDim.x + threadIdx.x;
     const int J = 6 + blockIdx.y * blockDim.y + threadIdx.y;
     if (I > g.NI - 7 || J > g.NJ - 7) return;
@@ -215,7 +227,7 @@ __global__ void k_remap_winds(Geo g, double *__restrict__ scr, const double *__r
         c.at(S.dst(), K + 1) = ATM_W7_12 * (tplev[g.ix(I, J, K)] + tplev[g.ix(I, J - 1, K)]) -
                                ATM_W1_12 * (tplev[g.ix(I, J + 1, K)] + tplev[g.ix(I, J - 2, K)]);
     }
-    for (int K = 0; K < L; K++) c.at(S.var(), K + 1) = ud[g.ix(I, J, K)];
+    for (int K = 0; K < L; K++) c.at(S.var(), K + 1) = pick(g, ud, udr, I, J, K);
     remap_column(c, S, L, ur + g.ix(I, J, 0), g.plane());
     for (int K = 0; K <= L; K++) {
         c.at(S.src(), K + 1) = ATM_W7_12 * (plev[g.ix(I, J, K)] + plev[g.ix(I - 1, J, K)]) -
@@ -223,7 +235,7 @@ __global__ void k_remap_winds(Geo g, double *__restrict__ scr, const double *__r
         c.at(S.dst(), K + 1) = ATM_W7_12 * (tplev[g.ix(I, J, K)] + tplev[g.ix(I - 1, J, K)]) -
                                ATM_W1_12 * (tplev[g.ix(I + 1, J, K)] + tplev[g.ix(I - 2, J, K)]);
     }
-    for (int K = 0; K < L; K++) c.at(S.var(), K + 1) = vd[g.ix(I, J, K)];
+    for (int K = 0; K < L; K++) c.at(S.var(), K + 1) = pick(g, vd, vdr, I, J, K);
     remap_column(c, S, L, vr + g.ix(I, J, 0), g.plane());
 }
 
@@ -233,6 +245,7 @@ __global__ void k_remap_energy(Geo g, double *__restrict__ scr, const double *__
                                const double *__restrict__ tplk, const double *__restrict__ playkap,
                                const double *__restrict__ gz, const double *__restrict__ pt,
                                const double *__restrict__ ud, const double *__restrict__ vd,
+                               const double *__restrict__ udr, const double *__restrict__ vdr,
                                const double *__restrict__ ur, const double *__restrict__ vr,
                                double *__restrict__ ptr) {
     const int I = 7 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -241,8 +254,8 @@ __global__ void k_remap_energy(Geo g, double *__restrict__ scr, const double *__
     const int L = g.L;
     Slots S{L};
     Col c{scr, g.plane(), g.ix2(I, J)};
-#define UD(a, b) ud[g.ix(I + (a), J + (b), K)]
-#define VD(a, b) vd[g.ix(I + (a), J + (b), K)]
+#define UD(a, b) pick(g, ud, udr, I + (a), J + (b), K)
+#define VD(a, b) pick(g, vd, vdr, I + (a), J + (b), K)
     for (int K = 0; K < L; K++) {
         double ua = (.25 * (UD(0, 0) + UD(0, 1)) + .125 * (UD(-1, 0) + UD(-1, 1) + UD(1, 0) + UD(1, 1)));
         double va = (.25 * (VD(0, 0) + VD(1, 0)) + .125 * (VD(0, -1) + VD(1, -1) + VD(0, 1) + VD(1, 1)));
@@ -337,20 +350,29 @@ void atm_remap_free(atm_ctx *c) {
     c->remap = nullptr;
 }
 
-int atm_remap_run(atm_ctx *c) {
+int atm_remap_run_split(atm_ctx *c, const double *dp, const double *pt, const double *ud_in,
+                        const double *vd_in, const double *ud_ring, const double *vd_ring, double *dp_out,
+                        double *pt_out, double *ud_out, double *vd_out) {
     const Geo &g = c->g;
     RemapWork *w = c->remap;
     dim3 b(64, 2);
+    atm_timer_begin(c, ATM_T_REMAP);
     hipLaunchKernelGGL(k_remap_grid, grid3(g.nx + 6, g.ny + 6, 1, b), b, 0, c->stream, g, c->ak0, c->ak, c->bk,
-                       c->dp, c->pt, c->plev, c->pkap, c->playkap, c->gz, w->tplev, w->tpkap, w->tplk);
+                       dp, pt, c->plev, c->pkap, c->playkap, c->gz, w->tplev, w->tpkap, w->tplk);
     hipLaunchKernelGGL(k_remap_winds, grid3(g.nx + 2, g.ny + 2, 1, b), b, 0, c->stream, g, w->scr, c->plev,
-                       w->tplev, c->ud, c->vd, w->uremap, w->vremap);
+                       w->tplev, ud_in, vd_in, ud_ring, vd_ring, w->uremap, w->vremap);
     hipLaunchKernelGGL(k_remap_energy, grid3(g.nx, g.ny, 1, b), b, 0, c->stream, g, w->scr, c->plev, w->tplev,
-                       w->tpkap, w->tplk, c->playkap, c->gz, c->pt, c->ud, c->vd, w->uremap, w->vremap,
-                       w->ptremap);
+                       w->tpkap, w->tplk, c->playkap, c->gz, pt, ud_in, vd_in, ud_ring, vd_ring, w->uremap,
+                       w->vremap, w->ptremap);
     dim3 bw(64, 4);
     hipLaunchKernelGGL(k_remap_write, grid3(g.nx, g.ny, g.L, bw), bw, 0, c->stream, g, w->tplev, w->uremap,
-                       w->vremap, w->ptremap, c->dp, c->ud, c->vd, c->pt);
+                       w->vremap, w->ptremap, dp_out, ud_out, vd_out, pt_out);
+    atm_timer_end(c, ATM_T_REMAP);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;
+}
+
+// in-place form (reference-structured path)
+int atm_remap_run(atm_ctx *c) {
+    return atm_remap_run_split(c, c->dp, c->pt, c->ud, c->vd, c->ud, c->vd, c->dp, c->pt, c->ud, c->vd);
 }

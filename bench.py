@@ -1,0 +1,153 @@
+#!/usr/bin/env python3
+"""Throughput of the FP64 dynamics + remap hot path on one MI355X.
+
+Workload (the plain FP64 GPU drop-in configuration): 512 x 512 x 64 doubly
+periodic grid, dx = dy = 25 km, dt = 20 s, vertical remap every 5 steps
+(every 100 s), radiation and physics off, seeded synthetic initial state
+(atmos.ic).  One step is the reference's main-loop body: C-grid half step,
+C halo fill, D-grid step, the remap when due, D halo fill and the
+stability check (src/mod_model.f90:117-177).
+
+Prints one JSON line: value = grid-cell-steps per second
+(nx * ny * nlay * steps / seconds) with all state resident in HBM, a
+roofline entry for the dominant kernel (algorithmic bytes per launch over
+its HIP-event-measured average duration, against 8 TB/s) and a CPU
+baseline: the oracle (a single-core C restatement of the reference
+algorithm) timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "atmosphere-model_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "cell-steps/s FP64 vs mixed at stated error; adjoint:forward cost ratio"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic HBM bytes per interior cell for one launch of each kernel group
+# (compulsory reads + writes of the fields each kernel consumes/produces)
+ALG_BYTES_PER_CELL = {
+    "c1": 8 * (5 + 4),   # read dp, pt, ud, vd, heat; write dpc, ptc, uc, vc
+    "c2": 8 * (6 + 2),   # read uc, vc, ud, vd, pkap, gz; write uc, vc
+    "d1": 8 * (5 + 2),   # read dp, pt, uc, vc, heat; write dp, pt
+    "d2": 8 * (6 + 2),   # read ud, vd, uc, vc, pkap, gz; write ud, vd
+    "remap": 8 * (4 + 4),  # read dp, pt, ud, vd; write dp, pt, ud, vd
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--ny", type=int, default=512)
+    ap.add_argument("--nlay", type=int, default=64)
+    ap.add_argument("--dx", type=float, default=25e3)
+    ap.add_argument("--dt", type=float, default=20.0)
+    ap.add_argument("--remap-every", type=int, default=5)
+    ap.add_argument("--path", choices=["fused", "reference"], default="fused")
+    ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
+    ap.add_argument("--cpu-nx", type=int, default=128, help="bounded CPU sample: nx = ny")
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    return ap.parse_args()
+
+
+def cpu_baseline(args):
+    """Single-core oracle (C restatement of the reference) on a bounded sample."""
+    import oracle as O
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+
+    n, L = args.cpu_nx, args.nlay
+    ic = make_initial_state(n, n, L, args.dx)
+    cfg = Config(dt_max=args.dt, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12,
+                 dt_remap=args.remap_every * args.dt, radiation_on=False, physics_on=False)
+    o = O.Oracle(n, n, L, args.dx, args.dx)
+    st = driver.init_model(o, ic, cfg)
+    t0 = time.perf_counter()
+    _, steps, _ = driver.run_model(o, cfg, st, max_steps=args.cpu_steps)
+    sec = time.perf_counter() - t0
+    return {"value": n * n * L * steps / sec, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n}x{n}x{L}, {steps} steps incl. {steps // args.remap_every} remap, "
+                      f"{sec:.1f} s on 1 host core (oracle/atm_oracle.c, -O2, glibc libm)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(the path shards by 2-D tiles with halo exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np  # noqa: F401
+
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+
+    nx, ny, L = args.nx, args.ny, args.nlay
+    ic = make_initial_state(nx, ny, L, args.dx)
+    cfg = Config(dt_max=args.dt, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12,
+                 dt_remap=args.remap_every * args.dt, radiation_on=False, physics_on=False)
+    m = Model(nx, ny, L, args.dx)
+    if args.path != m.path:
+        m.set_path(args.path)
+    st = driver.init_model(m, ic, cfg)
+    time_s = 0.0
+    for _ in range(args.warmup):
+        time_s += driver.step(m, cfg, st, time_s)
+        assert m.check_stability()
+    m.sync()
+    m.timer_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        time_s += driver.step(m, cfg, st, time_s)
+        if not m.check_stability():
+            raise SystemExit("unstable")
+    m.sync()
+    sec = time.perf_counter() - t0
+    cells = nx * ny * L
+    value = cells * args.steps / sec
+
+    kernels = {}
+    for name in ("c1", "c2", "d1", "d2", "scan", "halo", "remap"):
+        n, ms = m.timer_read(name)
+        if n:
+            kernels[name] = (n, ms)
+    m.timer_enable(False)
+    dom = max((k for k in kernels if k in ALG_BYTES_PER_CELL), key=lambda k: kernels[k][1])
+    n, ms = kernels[dom]
+    avg_s = ms / n / 1e3
+    alg = ALG_BYTES_PER_CELL[dom] * cells
+    achieved = alg / avg_s / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get(f"{args.path}:{dom}:{nx}x{ny}x{L}")
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": sec / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"FP64 dyn+remap {nx}x{ny}x{L}, dx={args.dx / 1e3:g} km, dt={args.dt:g} s, "
+                               f"remap every {args.remap_every} steps, radiation/physics off, seeded IC",
+                   "nx": nx, "ny": ny, "nlay": L, "path": args.path, "precision": "fp64"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": alg, "avg_launch_ms": avg_s * 1e3},
+        "kernels_ms_per_step": {k: v[1] / args.steps for k, v in kernels.items()},
+    }
+    if args.cpu_baseline == "on":
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

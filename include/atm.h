@@ -128,6 +128,23 @@ int atm_halo_exchange(atm_ctx *ctx, int field);
    are finite over the interior.  Blocks. */
 int atm_check_stability(atm_ctx *ctx, int *stable);
 
+/* Execution path of the dynamics (no counterpart in the reference; both
+   give identical results).  ATM_PATH_FUSED (default where nx % 32 == 0 and
+   ny % 8 == 0): LDS-tiled kernels, four per step.  ATM_PATH_REFERENCE: one
+   kernel per reference loop nest, in place.  Settable between steps; the
+   environment variable ATM_PATH=reference selects it at creation. */
+enum { ATM_PATH_FUSED = 0, ATM_PATH_REFERENCE = 1 };
+int atm_set_path(atm_ctx *ctx, int path);
+int atm_get_path(atm_ctx *ctx, int *path);
+
+/* Instrumentation (no counterpart in the reference): HIP-event timing of
+   the library's kernels on the context stream.  Enabling resets the
+   counters; atm_timer_read blocks and returns the number of timed launches
+   and their summed duration for one kernel group:
+   0 C1, 1 C2, 2 D1, 3 D2, 4 column scans, 5 halo fills, 6 remap, 7 radiation. */
+int atm_timer_enable(atm_ctx *ctx, int on);
+int atm_timer_read(atm_ctx *ctx, int kernel, int64_t *launches, double *total_ms);
+
 /* Wait for all work queued on the context. */
 int atm_sync(atm_ctx *ctx);
 
