@@ -53,9 +53,23 @@ def parse():
     ap.add_argument("--remap-every", type=int, default=5)
     ap.add_argument("--path", choices=["fused", "reference"], default="fused")
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
-    ap.add_argument("--cpu-nx", type=int, default=128, help="bounded CPU sample: nx = ny")
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-nx", type=int, default=256, help="bounded CPU sample: nx = ny")
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of this many MiB (read + write) a few times, "
+                         "to scale HBM counters (FETCH_SIZE/WRITE_SIZE) against a known size")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_mib": mib, "bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8}))
 
 
 def cpu_baseline(args):
@@ -81,6 +95,9 @@ def cpu_baseline(args):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               f"(the path shards by 2-D tiles with halo exchange, see DESIGN.md)", file=sys.stderr)
