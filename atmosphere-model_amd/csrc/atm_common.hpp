@@ -50,18 +50,27 @@ struct DynCoef {
 // ---------------------------------------------------------------------------
 // Device numerics shared by all dynamics kernels (mod_sw_dyn.f90:781-919).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double dmax3(double a, double b, double c) {
-    double m = a > b ? a : b;
-    return m > c ? m : c;
-}
-__device__ __forceinline__ double dmin3(double a, double b, double c) {
-    double m = a < b ? a : b;
-    return m < c ? m : c;
-}
-__device__ __forceinline__ double dmax2(double a, double b) { return a > b ? a : b; }
-__device__ __forceinline__ double dmin2(double a, double b) { return a < b ? a : b; }
+// Fortran MAX/MIN on finite values: one v_max_f64 / v_min_f64 each (the
+// compare-and-select form costs three instructions).  They differ from a
+// compare-and-select only for NaN operands and in the sign of an exact zero.
+__device__ __forceinline__ double dmax2(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ double dmin2(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ double dmax3(double a, double b, double c) { return dmax2(dmax2(a, b), c); }
+__device__ __forceinline__ double dmin3(double a, double b, double c) { return dmin2(dmin2(a, b), c); }
 // Fortran SIGN(a, b)
 __device__ __forceinline__ double fsign(double a, double b) { return copysign(fabs(a), b); }
+
+// x / 3 rounded to nearest, without the FP64 division sequence: the product
+// with RN(1/3) is corrected by one FMA step using the exact remainder
+// x - 3q.  For every finite x away from the subnormal and overflow ranges
+// this equals the IEEE quotient (checked on 1.6e9 random doubles spanning
+// 2^-300..2^300 against x / 3.); a zero product keeps the sign of x.
+__device__ __forceinline__ double div3(double x) {
+    const double r3 = 1.0 / 3.0;
+    const double q = x * r3;
+    if (q == 0.) return q;
+    return fma(fma(-q, 3.0, x), r3, q);
+}
 
 // dqi_mono of the middle cell from (q(m-1), q(m), q(m+1)): ppm 870-875
 __device__ __forceinline__ double dq_mono(double qm, double q0, double qp) {
@@ -78,8 +87,8 @@ __device__ __forceinline__ void ppm_edges(double qm2, double qm1, double q0, dou
     double dmm = dq_mono(qm2, qm1, q0);
     double dm0 = dq_mono(qm1, q0, qp1);
     double dmp = dq_mono(q0, qp1, qp2);
-    eL = (qm1 - q0) / 2. + (dmm - dm0) / 3.;
-    eR = (qp1 - q0) / 2. + (dm0 - dmp) / 3.;
+    eL = (qm1 - q0) / 2. + div3(dmm - dm0);
+    eR = (qp1 - q0) / 2. + div3(dm0 - dmp);
     if (V == V_PPM_C || V == V_PPM_NN) {
         double d = (qp1 - qm1) / 4.;
         eL = -fsign(dmin2(2. * fabs(d), fabs(eL)), d);
@@ -200,6 +209,20 @@ int atm_fail(atm_ctx *c, int code, const std::string &msg);
         if (e_ != hipSuccess)                                                            \
             return atm_fail((c), ATM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the part, each
+// with its own L2.  For a tiled stencil launch this maps the hardware block
+// index to a logical tile so that each XCD receives one contiguous range of
+// tiles (neighbouring tiles, whose halos overlap, then share an L2).
+__device__ __forceinline__ void xcd_tile(int gx, int gy, int gz, int &bx, int &by, int &bz) {
+    const int n = gx * gy * gz;
+    int b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if ((n & 7) == 0) b = (b & 7) * (n >> 3) + (b >> 3);
+    bx = b % gx;
+    b /= gx;
+    by = b % gy;
+    bz = b / gy;
+}
 
 // launch helpers: 2-D tiles over I x J, z = level
 static inline dim3 grid3(int nI, int nJ, int nK, dim3 b) {

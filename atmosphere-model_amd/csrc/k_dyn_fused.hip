@@ -68,8 +68,9 @@ __global__ __launch_bounds__(NT) void k_pair(Geo g, DynCoef c, const double *__r
     __shared__ double sTX[RDY::n], sTY[RDX::n], sL[RL::n];
     __shared__ double sDPFX[RFX::n], sDPFY[RFY::n], sFX[RFX::n], sFY[RFY::n];
 
-    const int K = blockIdx.z;
-    const int I0 = ATM_HALO + blockIdx.x * TX, J0 = ATM_HALO + blockIdx.y * TY;
+    int bx, by, K;
+    xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
+    const int I0 = ATM_HALO + bx * TX, J0 = ATM_HALO + by * TY;
 #define G(a, x, y) a[g.ix(I0 + (x), J0 + (y), K)]
 #define UC(x, y) sUC[RUC::at(x, y)]
 #define VC(x, y) sVC[RVC::at(x, y)]
@@ -209,8 +210,9 @@ __global__ __launch_bounds__(NT) void k_c2(Geo g, DynCoef c, const double *__res
     using RTW = Reg<-2, TX + 3, 0, TY>;
     using RLV = Reg<0, TX + 1, 0, TY + 1>;
     __shared__ double sU[RB::n], sV[RB::n], sW[RW::n], sK[RK::n], sTV[RTV::n], sTW[RTW::n], sL[RLV::n];
-    const int K = blockIdx.z;
-    const int I0 = ATM_HALO + blockIdx.x * TX, J0 = ATM_HALO + blockIdx.y * TY;
+    int bx, by, K;
+    xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
+    const int I0 = ATM_HALO + bx * TX, J0 = ATM_HALO + by * TY;
 #define G(a, x, y) a[g.ix(I0 + (x), J0 + (y), K)]
 #define U(x, y) sU[RB::at(x, y)]
 #define V(x, y) sV[RB::at(x, y)]
@@ -327,8 +329,9 @@ __global__ __launch_bounds__(NT) void k_d2(Geo g, DynCoef c, double pkb_top, con
     using RQS = Reg<-2, TX + 2, -2, TY + 2>;
     __shared__ double sU[RU::n], sV[RV::n], sKE[RKE::n], sW[RW::n], sDV[RDV::n], sD2[RD2::n], sD4[RKE::n];
     __shared__ double sTU[RTU::n], sTW[RTW::n], sL[RLP::n], sQS[RQS::n], sCB[4][RKE::n];
-    const int K = blockIdx.z;
-    const int I0 = ATM_HALO + blockIdx.x * TX, J0 = ATM_HALO + blockIdx.y * TY;
+    int bx, by, K;
+    xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
+    const int I0 = ATM_HALO + bx * TX, J0 = ATM_HALO + by * TY;
 #define G(a, x, y) a[g.ix(I0 + (x), J0 + (y), K)]
 #define U(x, y) sU[RU::at(x, y)]
 #define V(x, y) sV[RV::at(x, y)]
