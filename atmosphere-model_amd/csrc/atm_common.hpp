@@ -53,10 +53,10 @@ struct DynCoef {
 // Fortran MAX/MIN on finite values: one v_max_f64 / v_min_f64 each (the
 // compare-and-select form costs three instructions).  They differ from a
 // compare-and-select only for NaN operands and in the sign of an exact zero.
-__device__ __forceinline__ double dmax2(double a, double b) { return __builtin_fmax(a, b); }
-__device__ __forceinline__ double dmin2(double a, double b) { return __builtin_fmin(a, b); }
-__device__ __forceinline__ double dmax3(double a, double b, double c) { return dmax2(dmax2(a, b), c); }
-__device__ __forceinline__ double dmin3(double a, double b, double c) { return dmin2(dmin2(a, b), c); }
+__host__ __device__ __forceinline__ double dmax2(double a, double b) { return __builtin_fmax(a, b); }
+__host__ __device__ __forceinline__ double dmin2(double a, double b) { return __builtin_fmin(a, b); }
+__host__ __device__ __forceinline__ double dmax3(double a, double b, double c) { return dmax2(dmax2(a, b), c); }
+__host__ __device__ __forceinline__ double dmin3(double a, double b, double c) { return dmin2(dmin2(a, b), c); }
 // Fortran SIGN(a, b)
 __device__ __forceinline__ double fsign(double a, double b) { return copysign(fabs(a), b); }
 

@@ -1,171 +1,82 @@
 // k_remap.hip -- vertical Lagrangian -> hybrid sigma-p remap (gfx950).
 //
-// src/mod_remap.f90:27-163.  One thread per column.  Per-column work arrays
-// (face coordinates, tridiagonal coefficients, PPM edges) live in a device
-// scratch laid out [slot][column] so that a wavefront's 64 columns read and
-// write each slot contiguously.
+// src/mod_remap.f90:27-163.  One thread per column, columns flattened so
+// every wavefront is full; each remap_column (165-285) runs in two passes:
 //
-// remap_column (165-285) tests every (source k, destination m) pair.  For
-// monotone coordinates the pairs that contribute (the test at 203-204 fails)
-// are, for each m, a contiguous run of k that only moves upward as m grows;
-// the kernel walks exactly those pairs, for each m in ascending k, with the
-// reference's formulas, so every destination value is accumulated from the
-// same terms in the same order.
+//   forward   the elimination sweep of vertical_ppm (287-356) from k = nlay
+//             down to 2, the only part that runs against the final order;
+//             it leaves the eliminated edge values and gam in a scratch
+//             laid out [slot][column] (two slots per level, coalesced);
+//   merge     one ascending pass that back-substitutes, clamps (316-319),
+//             applies the monotonicity test (325-334) and the Huynh
+//             constraint (340-352), forms each source layer's PPM
+//             coefficients in registers as the layer comes up, and walks
+//             the source/destination overlaps (remap_column 193-283).
+//
+// For monotone coordinates the (source k, destination m) pairs that pass
+// the overlap tests are, for each m, a contiguous run of k that only moves
+// upward as m grows; the merge visits exactly those pairs, for each m in
+// ascending k, with the reference's formulas, so every destination value
+// is accumulated from the same terms in the same order.  Layers are
+// produced in ascending order only, so the per-layer limiter state is a
+// window of a few registers (gam(k-2..k+2), var(k..k+1), edges(k..k+1)).
 #include "atm_common.hpp"
+#include "remap_column.hpp"
 
 struct RemapWork {
-    double *scr = nullptr;      // [slots][plane]
+    double *scr = nullptr;      // [3 * (L+2)][ncol]: eliminated edges, gam, energy
     double *tplev = nullptr;    // target_plev  (NI x NJ x L+1), used on isd-3..ied+3
     double *tpkap = nullptr;    // target_pkap
     double *tplk = nullptr;     // target_playkap (L)
-    double *uremap = nullptr, *vremap = nullptr, *ptremap = nullptr;  // NI x NJ x L
-    int slots = 0;
+    double *uremap = nullptr, *vremap = nullptr;  // NI x NJ x L
+    size_t ncol = 0;            // columns of the widest remap domain ((nx+2) x (ny+2))
 };
 
 namespace {
 
-// scratch slot bases (multiples of the plane size); L+2 per array
-struct Slots {
-    int L;
-    __device__ int src() const { return 0; }
-    __device__ int dst() const { return (L + 2); }
-    __device__ int gam() const { return 2 * (L + 2); }
-    __device__ int lv() const { return 3 * (L + 2); }
-    __device__ int top() const { return 4 * (L + 2); }
-    __device__ int btm() const { return 5 * (L + 2); }
-    __device__ int crv() const { return 6 * (L + 2); }
-    __device__ int var() const { return 7 * (L + 2); }
-    static int count(int L) { return 8 * (L + 2); }
-};
-
-struct Col {
+// Column scratch: slot s, 1-based level k -> scr[(s*(L+2) + k) * ncol + t]
+struct Scr {
     double *s;
-    size_t plane, col;
-    __device__ double &at(int slot, int k) const { return s[(size_t)(slot + k) * plane + col]; }
+    size_t ncol, t;
+    int L;
+    __device__ double *lv() const { return s + t; }
+    __device__ double *gm() const { return s + (size_t)(L + 2) * ncol + t; }
+    __device__ double *var() const { return s + (size_t)2 * (L + 2) * ncol + t; }
 };
 
-// vertical_ppm (287-356) with 1-based k as in the reference.
-__device__ void vertical_ppm(const Col &c, const Slots &S, int L) {
-#define V(k) c.at(S.var(), (k))
-#define X(k) c.at(S.src(), (k))
-#define G(k) c.at(S.gam(), (k))
-#define LV(k) c.at(S.lv(), (k))
-#define TOP(k) c.at(S.top(), (k))
-#define BTM(k) c.at(S.btm(), (k))
-    double alpha = (X(L) - X(L - 1)) / (X(L + 1) - X(L));
-    double beta = alpha * (.5 + alpha);
-    LV(L + 1) = (2. * alpha * (1. + alpha) * V(L) + V(L - 1)) / beta;
-    G(L) = (1. + alpha * (1.5 + alpha)) / beta;
-    for (int k = L; k >= 2; k--) {
-        alpha = (X(k + 1) - X(k)) / (X(k) - X(k - 1));
-        beta = 2. + 2. * alpha - G(k);
-        LV(k) = (3. * (V(k) + alpha * V(k - 1)) - LV(k + 1)) / beta;
-        G(k - 1) = alpha / beta;
+// Column sources.  X(k): source face coordinate, D(k): destination face
+// coordinate (k = 1..L+1), V(k): layer value (k = 1..L).
+//
+// Winds (80-105): face coordinates are the 4-point averages of plev /
+// target_plev along J (u) or I (v); the layer values are the D-grid winds.
+struct WindCol {
+    const double *plev, *tplev, *v;
+    size_t base, s, plane;
+    __device__ double avg(const double *a, int k) const {
+        const size_t o = base + (size_t)(k - 1) * plane;
+        return ATM_W7_12 * (a[o] + a[o - s]) - ATM_W1_12 * (a[o + s] + a[o - 2 * s]);
     }
-    beta = 1. + alpha * (1.5 + alpha);
-    LV(1) = (2. * alpha * (1. + alpha) * V(1) + V(2) - beta * LV(2)) / (alpha * (.5 + alpha) - beta * G(1));
-    {
-        double prev = LV(1);
-        for (int k = 2; k <= L + 1; k++) {
-            double v = LV(k) - G(k - 1) * prev;
-            LV(k) = v;
-            prev = v;
-        }
-    }
-    LV(L) = dmin2(LV(L), dmax2(V(L), V(L - 1)));
-    LV(L) = dmax2(LV(L), dmin2(V(L), V(L - 1)));
-    LV(2) = dmin2(LV(2), dmax2(V(2), V(1)));
-    LV(2) = dmax2(LV(2), dmin2(V(2), V(1)));
-    // monotonicity (325-334).  At k = L-1 the reference reads gam(L+1) out of
-    // bounds (Q7); every branch is then a no-op on the already clamped
-    // level_var(L), so that iteration is skipped.
-    for (int k = 2; k <= L - 2; k++) {
-        double gk2 = G(k + 2), gk = G(k);
-        double lo = dmin2(V(k), V(k + 1)), hi = dmax2(V(k), V(k + 1));
-        double x = LV(k + 1);
-        if (gk2 * gk > 0) {
-            x = dmin2(x, hi);
-            x = dmax2(x, lo);
-        } else if (gk2 > 0) {
-            x = dmax2(x, lo);
-        } else {
-            x = dmin2(x, hi);
-        }
-        LV(k + 1) = x;
-    }
-    for (int k = 1; k <= L; k++) {
-        TOP(k) = LV(k + 1);
-        BTM(k) = LV(k);
-    }
-    // Huynh constraint (340-352), Q8
-    for (int k = 2; k <= L - 1; k++) {
-        double vk1 = V(k + 1);
-        double pmp1 = vk1 - 2. * G(k);
-        double lac1 = pmp1 + 1.5 * G(k - 1);
-        TOP(k + 1) = dmin2(dmax2(TOP(k + 1), dmin3(vk1, pmp1, lac1)), dmax3(vk1, pmp1, lac1));
-        pmp1 = vk1 + 2. * G(k + 1);
-        lac1 = pmp1 - 1.5 * G(k);
-        BTM(k + 1) = dmin2(dmax2(BTM(k + 1), dmin3(vk1, pmp1, lac1)), dmax3(vk1, pmp1, lac1));
-    }
-    for (int k = 1; k <= L; k++) c.at(S.crv(), k) = 6. * V(k) - 3. * (TOP(k) + BTM(k));
-#undef V
-#undef X
-#undef G
-#undef LV
-#undef TOP
-#undef BTM
-}
+    __device__ double X(int k) const { return avg(plev, k); }
+    __device__ double D(int k) const { return avg(tplev, k); }
+    __device__ double V(int k) const { return v[base + (size_t)(k - 1) * plane]; }
+};
 
-// remap_column (165-285): the source variable is in slot var, coordinates in
-// slots src/dst; result written to out(m) = out[(m-1)*ostride].
-__device__ void remap_column(const Col &c, const Slots &S, int L, double *out, size_t ostride) {
-    vertical_ppm(c, S, L);
-#define SC(k) c.at(S.src(), (k))
-#define DC(k) c.at(S.dst(), (k))
-    int k0 = 1;
-    for (int m = 1; m <= L; m++) {
-        const double dm0 = DC(m), dm1 = DC(m + 1);
-        while (k0 <= L && SC(k0 + 1) >= dm0) k0++;
-        double acc = 0.;
-        for (int k = k0; k <= L; k++) {
-            const double sk = SC(k), sk1 = SC(k + 1);
-            if (sk <= dm1) break;
-            const double dsrc = sk1 - sk;
-            const double s1 = (dm0 - sk) / dsrc;
-            const double s2 = (dm1 - sk) / dsrc;
-            const double factor = dsrc / (dm1 - dm0);
-            const double tmp3 = 2. * (s1 * s1);
-            const double b = c.at(S.btm(), k), t = c.at(S.top(), k), cv = c.at(S.crv(), k);
-            if (dm1 <= sk1 && sk1 < dm0 && dm0 < sk) {  // case 3
-                double tmp1 = s1 - 1., tmp2 = s1 + 1.;
-                acc = acc + tmp1 / 6. * (3. * (b * tmp1 - t * tmp2) + cv * (tmp3 - tmp2)) * factor;
-            }
-            if (sk1 < dm1 && dm1 < sk && sk <= dm0) {  // case 4
-                acc = acc + (s2 / 2.) * (b * (2. - s2) + s2 * (t + cv * (1. - ATM_W2_3 * s2))) * factor;
-            }
-            if (dm1 <= sk1 && sk <= dm0) {  // case 5
-                acc = acc + c.at(S.var(), k) * factor;
-            }
-            if (sk1 < dm1 && dm0 < sk) {  // case 6
-                double tmp1 = s1 + s2;
-                acc = acc + (s1 - s2) / 6. * (3. * (b * (tmp1 - 2.) - t * tmp1) + cv * (tmp3 + tmp1 * (2. * s2 - 3.))) * factor;
-            }
-        }
-        out[(size_t)(m - 1) * ostride] = acc;
-    }
-#undef SC
-#undef DC
-}
+// Total energy (111-150): coordinates are plev / target_plev of the column,
+// the layer values are formed once into the scratch's third slot.
+struct EnergyCol {
+    const double *plev, *tplev, *var;
+    size_t base, plane, ncol;
+    __device__ double X(int k) const { return plev[base + (size_t)(k - 1) * plane]; }
+    __device__ double D(int k) const { return tplev[base + (size_t)(k - 1) * plane]; }
+    __device__ double V(int k) const { return var[(size_t)k * ncol]; }
+};
 
-// The winds remap reads ud/vd on the interior and on the ring around it.
-// The ring holds the halo values the previous step's halo exchange left
-// there (the D-grid step updates the interior only, Q5); with separate
-// input buffers for interior and ring, `pick` returns the value the
-// reference's single array holds at that point.
-__device__ __forceinline__ double pick(const Geo &g, const double *__restrict__ in,
-                                       const double *__restrict__ ring, int I, int J, int K) {
-    const bool inside = I >= ATM_HALO && I < ATM_HALO + g.nx && J >= ATM_HALO && J < ATM_HALO + g.ny;
-    return inside ? in[g.ix(I, J, K)] : ring[g.ix(I, J, K)];
+// Logical block of a 1-D launch, consecutive logical blocks kept on one XCD
+// (hardware block b runs on XCD b % 8) so that neighbouring rows of columns
+// share that XCD's L2.  The grid is padded to a multiple of 8 blocks.
+__device__ __forceinline__ unsigned xcd_block1d() {
+    const unsigned n = gridDim.x, b = blockIdx.x;
+    return (b & 7u) * (n >> 3) + (b >> 3);
 }
 
 // current and target grids and gz over isd-3..ied+3 (42-78)
@@ -210,102 +121,118 @@ __global__ void k_remap_grid(Geo g, double top, const double *__restrict__ ak, c
     }
 }
 
-// winds on faces over isd-1..ied+1 (80-105)
-__global__ void k_remap_winds(Geo g, double *__restrict__ scr, const double *__restrict__ plev,
-                              const double *__restrict__ tplev, const double *__restrict__ ud,
-                              const double *__restrict__ vd, const double *__restrict__ udr,
-                              const double *__restrict__ vdr, double *__restrict__ ur, double *__restrict__ vr) {
-    const int I = 6 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int J = 6 + blockIdx.y * blockDim.y + threadIdx.y;
-    if (I > g.NI - 7 || J > g.NJ - 7) return;
+// winds on faces over isd-1..ied+1 (80-105).  The ring of the D-grid winds
+// holds the halo values the previous step's halo exchange left there (the
+// D-grid step updates the interior only): a column outside the interior
+// reads its winds from `ring`, an interior column from `in`.
+__global__ void __launch_bounds__(256) k_remap_winds(Geo g, double *__restrict__ scr, size_t ncol,
+                                                     const double *__restrict__ plev,
+                                                     const double *__restrict__ tplev,
+                                                     const double *__restrict__ ud, const double *__restrict__ vd,
+                                                     const double *__restrict__ udr,
+                                                     const double *__restrict__ vdr, double *__restrict__ ur,
+                                                     double *__restrict__ vr) {
+    const int W = g.nx + 2;
+    const size_t t = (size_t)xcd_block1d() * blockDim.x + threadIdx.x;
+    if (t >= ncol) return;
+    const int I = 6 + (int)(t % W), J = 6 + (int)(t / W);
     const int L = g.L;
-    Slots S{L};
-    Col c{scr, g.plane(), g.ix2(I, J)};
-    for (int K = 0; K <= L; K++) {
-        c.at(S.src(), K + 1) = ATM_W7_12 * (plev[g.ix(I, J, K)] + plev[g.ix(I, J - 1, K)]) -
-                               ATM_W1_12 * (plev[g.ix(I, J + 1, K)] + plev[g.ix(I, J - 2, K)]);
-        c.at(S.dst(), K + 1) = ATM_W7_12 * (tplev[g.ix(I, J, K)] + tplev[g.ix(I, J - 1, K)]) -
-                               ATM_W1_12 * (tplev[g.ix(I, J + 1, K)] + tplev[g.ix(I, J - 2, K)]);
+    const bool inside = I >= ATM_HALO && I < ATM_HALO + g.nx && J >= ATM_HALO && J < ATM_HALO + g.ny;
+    const size_t base = g.ix2(I, J), plane = g.plane();
+    Scr s{scr, ncol, t, L};
+    {
+        WindCol c{plev, tplev, inside ? ud : udr, base, (size_t)g.NI, plane};
+        ppm_forward(c, L, s.lv(), s.gm(), ncol);
+        auto out = [&](int m, double acc, double, double) { ur[base + (size_t)(m - 1) * plane] = acc; };
+        ppm_merge(c, L, s.lv(), s.gm(), ncol, out);
     }
-    for (int K = 0; K < L; K++) c.at(S.var(), K + 1) = pick(g, ud, udr, I, J, K);
-    remap_column(c, S, L, ur + g.ix(I, J, 0), g.plane());
-    for (int K = 0; K <= L; K++) {
-        c.at(S.src(), K + 1) = ATM_W7_12 * (plev[g.ix(I, J, K)] + plev[g.ix(I - 1, J, K)]) -
-                               ATM_W1_12 * (plev[g.ix(I + 1, J, K)] + plev[g.ix(I - 2, J, K)]);
-        c.at(S.dst(), K + 1) = ATM_W7_12 * (tplev[g.ix(I, J, K)] + tplev[g.ix(I - 1, J, K)]) -
-                               ATM_W1_12 * (tplev[g.ix(I + 1, J, K)] + tplev[g.ix(I - 2, J, K)]);
+    {
+        WindCol c{plev, tplev, inside ? vd : vdr, base, (size_t)1, plane};
+        ppm_forward(c, L, s.lv(), s.gm(), ncol);
+        auto out = [&](int m, double acc, double, double) { vr[base + (size_t)(m - 1) * plane] = acc; };
+        ppm_merge(c, L, s.lv(), s.gm(), ncol, out);
     }
-    for (int K = 0; K < L; K++) c.at(S.var(), K + 1) = pick(g, vd, vdr, I, J, K);
-    remap_column(c, S, L, vr + g.ix(I, J, 0), g.plane());
 }
 
-// total energy remap and pt reconstruction on the interior (111-155)
-__global__ void k_remap_energy(Geo g, double *__restrict__ scr, const double *__restrict__ plev,
-                               const double *__restrict__ tplev, const double *__restrict__ tpkap,
-                               const double *__restrict__ tplk, const double *__restrict__ playkap,
-                               const double *__restrict__ gz, const double *__restrict__ pt,
-                               const double *__restrict__ ud, const double *__restrict__ vd,
-                               const double *__restrict__ udr, const double *__restrict__ vdr,
-                               const double *__restrict__ ur, const double *__restrict__ vr,
-                               double *__restrict__ ptr) {
-    const int I = 7 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int J = 7 + blockIdx.y * blockDim.y + threadIdx.y;
-    if (I > g.NI - 8 || J > g.NJ - 8) return;
+// Total energy remap, pt reconstruction (111-155) and the write-back of
+// the interior (157-161).  With write_winds the remapped winds are copied
+// into ud_out / vd_out here (only when those are not the buffers the
+// energy of neighbouring columns is formed from).
+__global__ void __launch_bounds__(256) k_remap_energy(
+    Geo g, double *__restrict__ scr, size_t ncol, const double *__restrict__ plev,
+    const double *__restrict__ tplev, const double *__restrict__ tpkap, const double *__restrict__ tplk,
+    const double *__restrict__ playkap, const double *__restrict__ gz, const double *pt,
+    const double *__restrict__ ud, const double *__restrict__ vd, const double *__restrict__ udr,
+    const double *__restrict__ vdr, const double *__restrict__ ur, const double *__restrict__ vr, double *pt_out,
+    double *__restrict__ dp_out, double *__restrict__ ud_out, double *__restrict__ vd_out, int write_winds) {
+    const int W = g.nx;
+    const size_t t = (size_t)xcd_block1d() * blockDim.x + threadIdx.x;
+    if (t >= ncol) return;
+    const int I = 7 + (int)(t % W), J = 7 + (int)(t / W);
     const int L = g.L;
-    Slots S{L};
-    Col c{scr, g.plane(), g.ix2(I, J)};
-#define UD(a, b) pick(g, ud, udr, I + (a), J + (b), K)
-#define VD(a, b) pick(g, vd, vdr, I + (a), J + (b), K)
-    for (int K = 0; K < L; K++) {
-        double ua = (.25 * (UD(0, 0) + UD(0, 1)) + .125 * (UD(-1, 0) + UD(-1, 1) + UD(1, 0) + UD(1, 1)));
-        double va = (.25 * (VD(0, 0) + VD(1, 0)) + .125 * (VD(0, -1) + VD(1, -1) + VD(0, 1) + VD(1, 1)));
-        c.at(S.var(), K + 1) = ATM_CV_DRY * pt[g.ix(I, J, K)] * playkap[g.ix(I, J, K)] +
-                               .5 * (gz[g.ix(I, J, K + 1)] + gz[g.ix(I, J, K)] + ua * ua + va * va);
+    const size_t plane = g.plane(), base = g.ix2(I, J);
+    const size_t sN = g.NI;  // +1 in J
+    Scr s{scr, ncol, t, L};
+    // wind source at (I+a, J+b): interior points from ud / vd, the ring from udr / vdr
+    auto wsrc = [&](const double *in, const double *ring, int a, int b) {
+        const int i = I + a, j = J + b;
+        const bool inside = i >= ATM_HALO && i < ATM_HALO + g.nx && j >= ATM_HALO && j < ATM_HALO + g.ny;
+        return (inside ? in : ring) + g.ix2(i, j);
+    };
+    {
+        const double *u00 = wsrc(ud, udr, 0, 0), *u01 = wsrc(ud, udr, 0, 1), *um0 = wsrc(ud, udr, -1, 0),
+                     *um1 = wsrc(ud, udr, -1, 1), *up0 = wsrc(ud, udr, 1, 0), *up1 = wsrc(ud, udr, 1, 1);
+        const double *v00 = wsrc(vd, vdr, 0, 0), *v10 = wsrc(vd, vdr, 1, 0), *v0m = wsrc(vd, vdr, 0, -1),
+                     *v1m = wsrc(vd, vdr, 1, -1), *v01 = wsrc(vd, vdr, 0, 1), *v11 = wsrc(vd, vdr, 1, 1);
+        double *var = s.var();
+        for (int K = 0; K < L; K++) {
+            const size_t o = (size_t)K * plane;
+            double ua = (.25 * (u00[o] + u01[o]) + .125 * (um0[o] + um1[o] + up0[o] + up1[o]));
+            double va = (.25 * (v00[o] + v10[o]) + .125 * (v0m[o] + v1m[o] + v01[o] + v11[o]));
+            var[(size_t)(K + 1) * ncol] = ATM_CV_DRY * pt[base + o] * playkap[base + o] +
+                                          .5 * (gz[base + o + plane] + gz[base + o] + ua * ua + va * va);
+        }
     }
-#undef UD
-#undef VD
-    for (int K = 0; K <= L; K++) {
-        c.at(S.src(), K + 1) = plev[g.ix(I, J, K)];
-        c.at(S.dst(), K + 1) = tplev[g.ix(I, J, K)];
-    }
-    // energy_remap(m) goes to ptr (overwritten by pt_remap below, level by level)
-    remap_column(c, S, L, ptr + g.ix(I, J, 0), g.plane());
-#define UR(a, b) ur[g.ix(I + (a), J + (b), K)]
-#define VR(a, b) vr[g.ix(I + (a), J + (b), K)]
-    double gzr = gz[g.ix(I, J, 0)];
-    for (int K = 0; K < L; K++) {
-        double ua = (.25 * (UR(0, 0) + UR(0, 1)) + .125 * (UR(-1, 0) + UR(-1, 1) + UR(1, 0) + UR(1, 1)));
-        double va = (.25 * (VR(0, 0) + VR(1, 0)) + .125 * (VR(0, -1) + VR(1, -1) + VR(0, 1) + VR(1, 1)));
-        double tk = tpkap[g.ix(I, J, K)], tk1 = tpkap[g.ix(I, J, K + 1)];
-        double e = ptr[g.ix(I, J, K)];
-        double p = (e - gzr - .5 * (ua * ua + va * va)) /
-                   (ATM_CV_DRY * tplk[g.ix(I, J, K)] + .5 * ATM_CP_DRY * (tk - tk1));
-        ptr[g.ix(I, J, K)] = p;
+    EnergyCol c{plev, tplev, s.var(), base, plane, ncol};
+    ppm_forward(c, L, s.lv(), s.gm(), ncol);
+    double gzr = gz[base];
+    auto out = [&](int m, double e, double dm0, double dm1) {
+        const size_t x = base + (size_t)(m - 1) * plane;
+        double ua = (.25 * (ur[x] + ur[x + sN]) + .125 * (ur[x - 1] + ur[x - 1 + sN] + ur[x + 1] + ur[x + 1 + sN]));
+        double va = (.25 * (vr[x] + vr[x + 1]) + .125 * (vr[x - sN] + vr[x + 1 - sN] + vr[x + sN] + vr[x + 1 + sN]));
+        double tk = tpkap[x], tk1 = tpkap[x + plane];
+        double p = (e - gzr - .5 * (ua * ua + va * va)) / (ATM_CV_DRY * tplk[x] + .5 * ATM_CP_DRY * (tk - tk1));
+        pt_out[x] = p;
         gzr = gzr + ATM_CP_DRY * p * (tk - tk1);
-    }
-#undef UR
-#undef VR
+        dp_out[x] = dm0 - dm1;  // target_plev(k) - target_plev(k+1)
+        if (write_winds) {
+            ud_out[x] = ur[x];
+            vd_out[x] = vr[x];
+        }
+    };
+    ppm_merge(c, L, s.lv(), s.gm(), ncol, out);
 }
 
-// write back on the interior (157-161)
-__global__ void k_remap_write(Geo g, const double *__restrict__ tplev, const double *__restrict__ ur,
-                              const double *__restrict__ vr, const double *__restrict__ ptr,
-                              double *__restrict__ dp, double *__restrict__ ud, double *__restrict__ vd,
-                              double *__restrict__ pt) {
+// interior wind write-back when the output winds are the input buffers
+__global__ void k_remap_write_winds(Geo g, const double *__restrict__ ur, const double *__restrict__ vr,
+                                    double *__restrict__ ud, double *__restrict__ vd) {
     const int I = 7 + blockIdx.x * blockDim.x + threadIdx.x;
     const int J = 7 + blockIdx.y * blockDim.y + threadIdx.y;
     const int K = blockIdx.z;
     if (I > g.NI - 8 || J > g.NJ - 8) return;
     size_t x = g.ix(I, J, K);
-    dp[x] = tplev[x] - tplev[g.ix(I, J, K + 1)];
     ud[x] = ur[x];
     vd[x] = vr[x];
-    pt[x] = ptr[x];
 }
 
 __global__ void k_fill_plane(double *a, size_t n, double v) {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) a[t] = v;
+}
+
+unsigned blocks8(size_t n, unsigned bs) {
+    size_t b = (n + bs - 1) / bs;
+    return (unsigned)((b + 7) / 8 * 8);
 }
 
 }  // namespace
@@ -315,15 +242,14 @@ int atm_remap_alloc(atm_ctx *c) {
     if (!c->remap) {
         RemapWork *w = new RemapWork();
         c->remap = w;
-        w->slots = Slots::count(g.L);
+        w->ncol = (size_t)(g.nx + 2) * (g.ny + 2);
         size_t p = g.plane();
-        ATM_HIP(c, hipMalloc(&w->scr, sizeof(double) * p * w->slots));
+        ATM_HIP(c, hipMalloc(&w->scr, sizeof(double) * w->ncol * 3 * (g.L + 2)));
         ATM_HIP(c, hipMalloc(&w->tplev, sizeof(double) * p * (g.L + 1)));
         ATM_HIP(c, hipMalloc(&w->tpkap, sizeof(double) * p * (g.L + 1)));
         ATM_HIP(c, hipMalloc(&w->tplk, sizeof(double) * p * g.L));
         ATM_HIP(c, hipMalloc(&w->uremap, sizeof(double) * p * g.L));
         ATM_HIP(c, hipMalloc(&w->vremap, sizeof(double) * p * g.L));
-        ATM_HIP(c, hipMalloc(&w->ptremap, sizeof(double) * p * g.L));
     }
     // target_plev(:,:,nlay+1) = top; target_pkap(:,:,nlay+1) = top**kappa (381-382)
     RemapWork *w = c->remap;
@@ -345,7 +271,6 @@ void atm_remap_free(atm_ctx *c) {
     (void)hipFree(w->tplk);
     (void)hipFree(w->uremap);
     (void)hipFree(w->vremap);
-    (void)hipFree(w->ptremap);
     delete w;
     c->remap = nullptr;
 }
@@ -359,14 +284,23 @@ int atm_remap_run_split(atm_ctx *c, const double *dp, const double *pt, const do
     atm_timer_begin(c, ATM_T_REMAP);
     hipLaunchKernelGGL(k_remap_grid, grid3(g.nx + 6, g.ny + 6, 1, b), b, 0, c->stream, g, c->ak0, c->ak, c->bk,
                        dp, pt, c->plev, c->pkap, c->playkap, c->gz, w->tplev, w->tpkap, w->tplk);
-    hipLaunchKernelGGL(k_remap_winds, grid3(g.nx + 2, g.ny + 2, 1, b), b, 0, c->stream, g, w->scr, c->plev,
+    const size_t nw = (size_t)(g.nx + 2) * (g.ny + 2), ne = (size_t)g.nx * g.ny;
+    const unsigned bs = 256;
+    hipLaunchKernelGGL(k_remap_winds, dim3(blocks8(nw, bs)), dim3(bs), 0, c->stream, g, w->scr, nw, c->plev,
                        w->tplev, ud_in, vd_in, ud_ring, vd_ring, w->uremap, w->vremap);
-    hipLaunchKernelGGL(k_remap_energy, grid3(g.nx, g.ny, 1, b), b, 0, c->stream, g, w->scr, c->plev, w->tplev,
-                       w->tpkap, w->tplk, c->playkap, c->gz, pt, ud_in, vd_in, ud_ring, vd_ring, w->uremap,
-                       w->vremap, w->ptremap);
-    dim3 bw(64, 4);
-    hipLaunchKernelGGL(k_remap_write, grid3(g.nx, g.ny, g.L, bw), bw, 0, c->stream, g, w->tplev, w->uremap,
-                       w->vremap, w->ptremap, dp_out, ud_out, vd_out, pt_out);
+    // the remapped winds go to the output buffers from the energy kernel
+    // unless those are the input winds, whose interior neighbouring columns
+    // still read there (a ring-only source buffer is read outside the
+    // interior only)
+    const int write_winds = (ud_out != ud_in && vd_out != vd_in) ? 1 : 0;
+    hipLaunchKernelGGL(k_remap_energy, dim3(blocks8(ne, bs)), dim3(bs), 0, c->stream, g, w->scr, ne, c->plev,
+                       w->tplev, w->tpkap, w->tplk, c->playkap, c->gz, pt, ud_in, vd_in, ud_ring, vd_ring,
+                       w->uremap, w->vremap, pt_out, dp_out, ud_out, vd_out, write_winds);
+    if (!write_winds) {
+        dim3 bw(64, 4);
+        hipLaunchKernelGGL(k_remap_write_winds, grid3(g.nx, g.ny, g.L, bw), bw, 0, c->stream, g, w->uremap,
+                           w->vremap, ud_out, vd_out);
+    }
     atm_timer_end(c, ATM_T_REMAP);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;
