@@ -213,17 +213,42 @@ int atm_set_coordinate(atm_ctx *c, const double *ak, const double *bk) {
     return ATM_OK;
 }
 
+// Between atm_dgrid_step and atm_dgrid_halo the double-buffered path keeps
+// the new D-grid winds in the current copy and the wind ring the reference's
+// arrays still hold there (the previous step's halo, read by the remap) in
+// the previous copy.  Transfers of UD / VD in that window see the
+// reference's array: reads compose ring and interior, writes go to both.
+static bool split_winds(const atm_ctx *c, int id) {
+    return c->fused && c->after_d && (id == ATM_UD || id == ATM_VD);
+}
+
 static int xfer(atm_ctx *c, int id, void *dst, const void *src, bool to_dev, hipMemcpyKind kind, bool block) {
     CHECK_CTX(c);
     int nk = 0;
     double *d = field_ptr(c, id, &nk);
     if (!d) return atm_fail(c, ATM_ERR_ARGUMENT, "unknown field id " + std::to_string(id));
     if (!(to_dev ? src : dst)) return atm_fail(c, ATM_ERR_ARGUMENT, "null array");
-    size_t bytes = sizeof(double) * c->g.plane() * nk;
-    if (to_dev)
+    const Geo &g = c->g;
+    size_t bytes = sizeof(double) * g.plane() * nk;
+    if (split_winds(c, id)) {
+        double *ring = c->pbuf[id == ATM_UD ? 2 : 3][1 - c->cur];
+        if (to_dev) {
+            ATM_HIP(c, hipMemcpyAsync(d, src, bytes, kind, c->stream));
+            ATM_HIP(c, hipMemcpyAsync(ring, src, bytes, kind, c->stream));
+        } else {
+            ATM_HIP(c, hipMemcpyAsync(dst, ring, bytes, kind, c->stream));
+            const size_t pitch = sizeof(double) * g.NI;
+            for (int K = 0; K < nk; K++) {
+                const size_t o = g.ix(ATM_HALO, ATM_HALO, K);
+                ATM_HIP(c, hipMemcpy2DAsync((double *)dst + o, pitch, d + o, pitch, sizeof(double) * g.nx, g.ny,
+                                            kind, c->stream));
+            }
+        }
+    } else if (to_dev) {
         ATM_HIP(c, hipMemcpyAsync(d, src, bytes, kind, c->stream));
-    else
+    } else {
         ATM_HIP(c, hipMemcpyAsync(dst, d, bytes, kind, c->stream));
+    }
     if (block) ATM_HIP(c, hipStreamSynchronize(c->stream));
     return ATM_OK;
 }

@@ -451,59 +451,107 @@ __global__ __launch_bounds__(NT) void k_d2(Geo g, DynCoef c, double pkb_top, con
 }
 
 // ---------------------------------------------------------------------------
-// Column scans.  One thread per column, i fastest across the wavefront.
+// Column scans.  One thread per column, i fastest across the wavefront.  The
+// running sum of dp is the only serial dependence: levels go in chunks of
+// SCAN_CH whose pow / log evaluations are independent of each other, so a
+// wavefront keeps SCAN_CH of them in flight.
 // ---------------------------------------------------------------------------
+#define SCAN_CH 4
+
 // heating rate of the C step from dp (cgrid 89-106) on interior columns
-__global__ void k_scan_heat(Geo g, double ptop, double pktop, double gravity, const double *__restrict__ src,
-                            const double *__restrict__ nf, double *__restrict__ heat) {
+__global__ void __launch_bounds__(64) k_scan_heat(Geo g, double ptop, double pktop, double gravity,
+                                                  const double *__restrict__ src, const double *__restrict__ nf,
+                                                  double *__restrict__ heat) {
     const int I = ATM_HALO + blockIdx.x * blockDim.x + threadIdx.x;
     const int J = ATM_HALO + blockIdx.y;
     if (I >= ATM_HALO + g.nx) return;
+    const size_t pl = g.plane(), c0 = g.ix2(I, J);
     double p = ptop, pk1 = pktop;
-    double nf1 = nf[g.ix(I, J, g.L)];
-    for (int K = g.L - 1; K >= 0; K--) {
-        const double s = src[g.ix(I, J, K)];
-        p = p + s;
-        const double pk = pow(p, ATM_KAPPA);
+    double nf1 = nf[c0 + (size_t)g.L * pl];
+    auto level = [&](int K, double s, double pk, double nf0) {
         const double plk = (pk1 - pk) / log(pk1 / pk);
-        const double nf0 = nf[g.ix(I, J, K)];
-        heat[g.ix(I, J, K)] = (nf1 - nf0) / (ATM_CP_DRY * s / gravity) / plk;
+        heat[c0 + (size_t)K * pl] = (nf1 - nf0) / (ATM_CP_DRY * s / gravity) / plk;
         pk1 = pk;
         nf1 = nf0;
+    };
+    int K = g.L - 1;
+    for (; K >= SCAN_CH - 1; K -= SCAN_CH) {
+        double s[SCAN_CH], pv[SCAN_CH], pk[SCAN_CH], nf0[SCAN_CH];
+#pragma unroll
+        for (int u = 0; u < SCAN_CH; u++) {
+            s[u] = src[c0 + (size_t)(K - u) * pl];
+            nf0[u] = nf[c0 + (size_t)(K - u) * pl];
+        }
+#pragma unroll
+        for (int u = 0; u < SCAN_CH; u++) {
+            p = p + s[u];
+            pv[u] = p;
+        }
+#pragma unroll
+        for (int u = 0; u < SCAN_CH; u++) pk[u] = pow(pv[u], ATM_KAPPA);
+#pragma unroll
+        for (int u = 0; u < SCAN_CH; u++) level(K - u, s[u], pk[u], nf0[u]);
+    }
+    for (; K >= 0; K--) {
+        const double s = src[c0 + (size_t)K * pl];
+        p = p + s;
+        level(K, s, pow(p, ATM_KAPPA), nf[c0 + (size_t)K * pl]);
     }
 }
 
 // pkap and gz from (src, th) over all columns (cgrid 276-292, dgrid 433-437 +
 // 606-626); optional plev output and optional heating rate of the D step.
-__global__ void k_scan_pg(Geo g, double ptop, double pktop, double gravity, const double *__restrict__ src,
-                          const double *__restrict__ th, const double *__restrict__ nf, double *__restrict__ plev,
-                          double *__restrict__ pkap, double *__restrict__ gz, double *__restrict__ heat) {
+__global__ void __launch_bounds__(64) k_scan_pg(Geo g, double ptop, double pktop, double gravity,
+                                                const double *__restrict__ src, const double *__restrict__ th,
+                                                const double *__restrict__ nf, double *__restrict__ plev,
+                                                double *__restrict__ pkap, double *__restrict__ gz,
+                                                double *__restrict__ heat) {
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     const int J = blockIdx.y;
     if (I >= g.NI) return;
-    double p = ptop, pk1 = pktop;
-    double nf1 = heat ? nf[g.ix(I, J, g.L)] : 0.;
+    const size_t pl = g.plane(), c0 = g.ix2(I, J);
     const bool interior = I >= ATM_HALO && I < ATM_HALO + g.nx && J >= ATM_HALO && J < ATM_HALO + g.ny;
-    for (int K = g.L - 1; K >= 0; K--) {
-        const double s = src[g.ix(I, J, K)];
-        p = p + s;
-        const double pk = pow(p, ATM_KAPPA);
-        if (plev) plev[g.ix(I, J, K)] = p;
-        pkap[g.ix(I, J, K)] = pk;
-        if (heat && interior) {
+    const bool want_heat = heat && interior;
+    double p = ptop, pk1 = pktop;
+    double nf1 = want_heat ? nf[c0 + (size_t)g.L * pl] : 0.;
+    auto level = [&](int K, double s, double pv, double pk) {
+        const size_t x = c0 + (size_t)K * pl;
+        if (plev) plev[x] = pv;
+        pkap[x] = pk;
+        if (want_heat) {
             const double plk = (pk1 - pk) / log(pk1 / pk);
-            const double nf0 = nf[g.ix(I, J, K)];
-            heat[g.ix(I, J, K)] = (nf1 - nf0) / (ATM_CP_DRY * s / gravity) / plk;
+            const double nf0 = nf[x];
+            heat[x] = (nf1 - nf0) / (ATM_CP_DRY * s / gravity) / plk;
             nf1 = nf0;
         }
         pk1 = pk;
+    };
+    int K = g.L - 1;
+    for (; K >= SCAN_CH - 1; K -= SCAN_CH) {
+        double s[SCAN_CH], pv[SCAN_CH], pk[SCAN_CH];
+#pragma unroll
+        for (int u = 0; u < SCAN_CH; u++) s[u] = src[c0 + (size_t)(K - u) * pl];
+#pragma unroll
+        for (int u = 0; u < SCAN_CH; u++) {
+            p = p + s[u];
+            pv[u] = p;
+        }
+#pragma unroll
+        for (int u = 0; u < SCAN_CH; u++) pk[u] = pow(pv[u], ATM_KAPPA);
+#pragma unroll
+        for (int u = 0; u < SCAN_CH; u++) level(K - u, s[u], pv[u], pk[u]);
     }
-    double z = gz[g.ix(I, J, 0)];
-    double pk = pkap[g.ix(I, J, 0)];
-    for (int K = 0; K < g.L; K++) {
-        const double pkn = K + 1 == g.L ? pktop : pkap[g.ix(I, J, K + 1)];
-        z = z + ATM_CP_DRY * th[g.ix(I, J, K)] * (pk - pkn);
-        gz[g.ix(I, J, K + 1)] = z;
+    for (; K >= 0; K--) {
+        const double s = src[c0 + (size_t)K * pl];
+        p = p + s;
+        level(K, s, p, pow(p, ATM_KAPPA));
+    }
+    double z = gz[c0];
+    double pk = pkap[c0];
+    for (K = 0; K < g.L; K++) {
+        const double pkn = K + 1 == g.L ? pktop : pkap[c0 + (size_t)(K + 1) * pl];
+        z = z + ATM_CP_DRY * th[c0 + (size_t)K * pl] * (pk - pkn);
+        gz[c0 + (size_t)(K + 1) * pl] = z;
         pk = pkn;
     }
 }

@@ -79,44 +79,83 @@ __device__ __forceinline__ unsigned xcd_block1d() {
     return (b & 7u) * (n >> 3) + (b >> 3);
 }
 
-// current and target grids and gz over isd-3..ied+3 (42-78)
+// current and target grids and gz over isd-3..ied+3 (42-78).  Levels go in
+// chunks of 4 whose pow / log evaluations are independent.
 __global__ void k_remap_grid(Geo g, double top, const double *__restrict__ ak, const double *__restrict__ bk,
                              const double *__restrict__ dp, const double *__restrict__ pt,
                              double *__restrict__ plev, double *__restrict__ pkap, double *__restrict__ playkap,
                              double *__restrict__ gz, double *__restrict__ tplev, double *__restrict__ tpkap,
                              double *__restrict__ tplk) {
+    constexpr int CH = 4;
     const int I = 4 + blockIdx.x * blockDim.x + threadIdx.x;
     const int J = 4 + blockIdx.y * blockDim.y + threadIdx.y;
     if (I > g.NI - 5 || J > g.NJ - 5) return;
     const int L = g.L;
-    double p = plev[g.ix(I, J, L)];
-    double pk1 = pkap[g.ix(I, J, L)];
-    for (int K = L - 1; K >= 0; K--) {
-        p = p + dp[g.ix(I, J, K)];
-        plev[g.ix(I, J, K)] = p;
-        double pk = pow(p, ATM_KAPPA);
-        pkap[g.ix(I, J, K)] = pk;
-        double plog = log(pk1 / pk);
-        playkap[g.ix(I, J, K)] = (pk1 - pk) / plog;
-        pk1 = pk;
+    const size_t pl = g.plane(), c0 = g.ix2(I, J);
+    double p = plev[c0 + (size_t)L * pl];
+    double pk1 = pkap[c0 + (size_t)L * pl];
+    // plev, pkap, playkap of the current grid from the top down (42-58)
+    {
+        auto level = [&](int K, double pv, double pk) {
+            const size_t x = c0 + (size_t)K * pl;
+            plev[x] = pv;
+            pkap[x] = pk;
+            double plog = log(pk1 / pk);
+            playkap[x] = (pk1 - pk) / plog;
+            pk1 = pk;
+        };
+        int K = L - 1;
+        for (; K >= CH - 1; K -= CH) {
+            double pv[CH], pk[CH];
+#pragma unroll
+            for (int u = 0; u < CH; u++) {
+                p = p + dp[c0 + (size_t)(K - u) * pl];
+                pv[u] = p;
+            }
+#pragma unroll
+            for (int u = 0; u < CH; u++) pk[u] = pow(pv[u], ATM_KAPPA);
+#pragma unroll
+            for (int u = 0; u < CH; u++) level(K - u, pv[u], pk[u]);
+        }
+        for (; K >= 0; K--) {
+            p = p + dp[c0 + (size_t)K * pl];
+            level(K, p, pow(p, ATM_KAPPA));
+        }
     }
+    // target grid from the surface pressure (60-70)
     const double ps = p;
-    double tk1 = tpkap[g.ix(I, J, L)];
-    for (int K = L - 1; K >= 0; K--) {
-        double tp = ak[K] + bk[K] * (ps - top);
-        tplev[g.ix(I, J, K)] = tp;
-        double tk = pow(tp, ATM_KAPPA);
-        tpkap[g.ix(I, J, K)] = tk;
-        double plog = log(tk1 / tk);
-        tplk[g.ix(I, J, K)] = (tk1 - tk) / plog;
-        tk1 = tk;
+    double tk1 = tpkap[c0 + (size_t)L * pl];
+    {
+        auto level = [&](int K, double tp, double tk) {
+            const size_t x = c0 + (size_t)K * pl;
+            tplev[x] = tp;
+            tpkap[x] = tk;
+            double plog = log(tk1 / tk);
+            tplk[x] = (tk1 - tk) / plog;
+            tk1 = tk;
+        };
+        int K = L - 1;
+        for (; K >= CH - 1; K -= CH) {
+            double tp[CH], tk[CH];
+#pragma unroll
+            for (int u = 0; u < CH; u++) tp[u] = ak[K - u] + bk[K - u] * (ps - top);
+#pragma unroll
+            for (int u = 0; u < CH; u++) tk[u] = pow(tp[u], ATM_KAPPA);
+#pragma unroll
+            for (int u = 0; u < CH; u++) level(K - u, tp[u], tk[u]);
+        }
+        for (; K >= 0; K--) {
+            const double tp = ak[K] + bk[K] * (ps - top);
+            level(K, tp, pow(tp, ATM_KAPPA));
+        }
     }
-    double z = gz[g.ix(I, J, 0)];
-    double pk = pkap[g.ix(I, J, 0)];
+    // geopotential (72-78)
+    double z = gz[c0];
+    double pk = pkap[c0];
     for (int K = 0; K < L; K++) {
-        double pkn = pkap[g.ix(I, J, K + 1)];
-        z = z + ATM_CP_DRY * pt[g.ix(I, J, K)] * (pk - pkn);
-        gz[g.ix(I, J, K + 1)] = z;
+        double pkn = pkap[c0 + (size_t)(K + 1) * pl];
+        z = z + ATM_CP_DRY * pt[c0 + (size_t)K * pl] * (pk - pkn);
+        gz[c0 + (size_t)(K + 1) * pl] = z;
         pk = pkn;
     }
 }

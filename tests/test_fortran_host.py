@@ -1,0 +1,150 @@
+"""The Fortran drop-in: a Fortran host calling the library through the
+modules with the reference's names (atmosphere-model_amd/fortran:
+mod_sw_dyn, mod_remap, mod_radiation, mod_sync over atm_c_api).
+
+tests/fortran/drop_in_host runs run_model's call order
+(src/mod_model.f90:115-193) on a state the oracle initialised, with the
+step schedule (dt, radiation and remap steps) the host driver's
+calculate_dt produced, in both coherence modes: mirrored (every call moves
+the host arrays) and resident (the state stays on the device).  Its final
+dp, pt, ud, vd and net_flux must match the oracle's to 1e-12 norm-relative.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from atmos import driver
+from atmos.config import Config
+from atmos.ic import make_initial_state
+
+from conftest import norm_rel
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST = os.path.join(ROOT, "tests", "fortran", "_build", "drop_in_host")
+FIELDS_IN = (O.DP, O.PT, O.UD, O.VD, O.TS, O.GZ, O.PLEV, O.PKAP, O.NET_FLUX, O.RADIUS)
+
+
+class _Recorder:
+    """Forwards to the oracle and records the schedule of one run."""
+
+    def __init__(self, inner):
+        self._inner = inner
+        self.rad, self.dt, self.remap = [], [], []
+
+    def calculate_radiative_heating(self, time):
+        self.rad.append((len(self.dt), time))
+        return self._inner.calculate_radiative_heating(time)
+
+    def cgrid_dynamics_step(self, half_dt):
+        self.dt.append(2.0 * half_dt)
+        return self._inner.cgrid_dynamics_step(half_dt)
+
+    def perform_vertical_remapping(self):
+        self.remap.append(len(self.dt) - 1)
+        return self._inner.perform_vertical_remapping()
+
+    def __getattr__(self, name):
+        return getattr(self._inner, name)
+
+
+def _write_input(path, o, dx, cfg, rec, resident, params):
+    n = len(rec.dt)
+    times = np.zeros(n)
+    t = cfg.t_initial
+    for i, d in enumerate(rec.dt):
+        times[i] = t
+        t = t + d
+    rad = np.zeros(n, np.int32)
+    for step, _ in rec.rad:
+        rad[step] = 1
+    remap = np.zeros(n, np.int32)
+    remap[rec.remap] = 1
+    with open(path, "wb") as fh:
+        np.array([o.nx, o.ny, o.nlay, n, resident], np.int32).tofile(fh)
+        np.array([dx, dx, *params], np.float64).tofile(fh)
+        o.field(O.AK).astype(np.float64).tofile(fh)
+        o.field(O.BK).astype(np.float64).tofile(fh)
+        times.tofile(fh)
+        np.asarray(rec.dt, np.float64).tofile(fh)
+        rad.tofile(fh)
+        remap.tofile(fh)
+
+
+def _run_case(tmp_path, nx, ny, nlay, dx, steps, radiation, resident):
+    ic = make_initial_state(nx, ny, nlay, dx)
+    cfg = Config(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=240.0, dt_physics=1e12,
+                 dt_remap=180.0, radiation_on=radiation, physics_on=False)
+    params = (9.807, 1e-5, 0.15, 0.005)
+    o = O.Oracle(nx, ny, nlay, dx, dx, gravity=params[0], coriolis=params[1],
+                 divergence_damping=params[2], vorticity_damping=params[3])
+    st = driver.init_model(o, ic, cfg)
+    inp = str(tmp_path / f"in_{resident}.bin")
+    snapshot = [np.array(o.field(f), dtype=np.float64) for f in FIELDS_IN]
+    rec = _Recorder(o)
+    time_s = 0.0
+    for _ in range(steps):
+        time_s += driver.step(rec, cfg, st, time_s)
+    _write_input(inp, o, dx, cfg, rec, resident, params)
+    with open(inp, "ab") as fh:
+        for a in snapshot:
+            a.tofile(fh)
+    out = str(tmp_path / f"out_{resident}.bin")
+    res = subprocess.run([HOST, inp, out], capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    with open(out, "rb") as fh:
+        done, stable = np.fromfile(fh, np.int32, 2)
+        got = {}
+        for f in (O.DP, O.PT, O.UD, O.VD, O.NET_FLUX):
+            shape = o.field(f).shape
+            got[f] = np.fromfile(fh, np.float64, int(np.prod(shape))).reshape(shape)
+    assert done == steps and stable == 1
+    h = O.HALO
+    worst = {}
+    for f in (O.DP, O.PT, O.UD, O.VD, O.NET_FLUX):
+        a = got[f][..., h:h + ny, h:h + nx]
+        b = o.interior(f)
+        worst[f] = norm_rel(a, b)
+    return worst, rec
+
+
+def test_host_built():
+    """The drop-in modules and the test host compile and link (flang)."""
+    if not os.path.exists("/opt/rocm/lib/llvm/bin/flang"):
+        pytest.skip("flang not in this image")
+    assert os.path.exists(HOST), "tests/fortran/_build/drop_in_host missing: run build()"
+
+
+def test_host_without_gpu_reports_fatal(tmp_path):
+    """Without a usable GPU the drop-in reports through the logger and
+    aborts (the reference's fatal-error convention)."""
+    if not os.path.exists(HOST):
+        pytest.skip("test host not built")
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    o = O.Oracle(16, 16, 6, 100e3, 100e3)
+    inp = str(tmp_path / "in.bin")
+    with open(inp, "wb") as fh:
+        np.array([16, 16, 6, 0, 0], np.int32).tofile(fh)
+        np.array([100e3, 100e3, 9.807, 1e-5, 0.15, 0.005], np.float64).tofile(fh)
+        np.zeros(2 * 7, np.float64).tofile(fh)
+        for f in FIELDS_IN:
+            np.zeros(o.field(f).size, np.float64).tofile(fh)
+    res = subprocess.run([HOST, inp, str(tmp_path / "out.bin")], capture_output=True, text=True, timeout=60)
+    assert res.returncode != 0
+    assert "FATAL [atm_device] could not create the device context" in res.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("resident", [0, 1])
+def test_fortran_drop_in_matches_oracle(gpu_available, tmp_path, resident):
+    if not os.path.exists(HOST):
+        pytest.skip("test host not built")
+    print(f"fortran drop-in: 32x32x12, 8 steps, radiation + remap, resident={resident}", flush=True)
+    worst, rec = _run_case(tmp_path, 32, 32, 12, 100e3, 8, True, resident)
+    assert rec.remap and len(rec.rad) >= 2  # the schedule exercised both
+    for f, e in worst.items():
+        assert e <= 1e-12, (f, e)
