@@ -6,15 +6,15 @@ reference module procedures by their reference names (the HIP `Model`, or
 the CPU oracle in the tests):
 
     put(name, interior_array), init_prognostic_fields(), initial_halo_exchange(),
-    init_sw_dyn(), allocate_radiation_arrays(), allocate_remapping_arrays(),
-    calculate_radiative_heating(time), radiation_halo_exchange(),
+    init_sw_dyn(), allocate_radiation_arrays(), allocate_physics_arrays(),
+    allocate_remapping_arrays(), calculate_radiative_heating(time),
+    radiation_halo_exchange(), calculate_physics_tendencies(),
     cgrid_dynamics_step(dt), cgrid_halo_exchange(), dgrid_dynamics_step(dt),
-    perform_vertical_remapping(), dgrid_halo_exchange(), check_stability()
+    apply_physics_tendencies(dt), perform_vertical_remapping(),
+    dgrid_halo_exchange(), physics_halo_exchange(), check_stability()
 
-Physics (src/mod_physics.f90) is not part of this library; a config with
-physics_on must supply a backend that implements
-calculate_physics_tendencies / apply_physics_tendencies /
-physics_halo_exchange, otherwise run_model raises.
+radiation_on / physics_on are harness switches (the reference has none,
+Q11): off means the procedures are not called and net_flux stays zero.
 """
 from __future__ import annotations
 
@@ -54,6 +54,7 @@ def init_model(backend, ic: dict, config: Config) -> DriverState:
     backend.initial_halo_exchange()
     backend.init_sw_dyn()
     backend.allocate_radiation_arrays()
+    backend.allocate_physics_arrays()
     backend.allocate_remapping_arrays()
     t0 = config.t_initial
     return DriverState(t0, t0, t0, t0, calculate_radiation_this_step=True,
@@ -119,8 +120,6 @@ def run_model(backend, config: Config, st: DriverState, max_steps: int | None = 
     """run_model (mod_model.f90:91-199) without output; returns
     (time, steps, stable).  check_every > 1 checks stability less often
     (the reference checks every step)."""
-    if config.physics_on and not hasattr(backend, "apply_physics_tendencies"):
-        raise NotImplementedError("physics_on needs a backend with the physics procedures")
     time = config.t_initial
     n = 0
     while time < config.t_final:

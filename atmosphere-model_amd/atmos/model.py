@@ -26,9 +26,11 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_cgrid_step", "atm_cgrid_halo", "atm_dgrid_step", "atm_dgrid_halo",
            "atm_remap_init", "atm_remap", "atm_radiation_init", "atm_radiation",
            "atm_radiation_halo", "atm_halo_exchange", "atm_check_stability", "atm_sync",
-           "atm_build_info", "atm_set_path", "atm_get_path", "atm_timer_enable", "atm_timer_read"]
+           "atm_build_info", "atm_set_path", "atm_get_path", "atm_timer_enable", "atm_timer_read",
+           "atm_physics_init", "atm_physics_tendencies", "atm_apply_physics", "atm_physics_halo"]
 PATHS = {"fused": 0, "reference": 1}
-TIMERS = {"c1": 0, "c2": 1, "d1": 2, "d2": 3, "scan": 4, "halo": 5, "remap": 6, "radiation": 7}
+TIMERS = {"c1": 0, "c2": 1, "d1": 2, "d2": 3, "scan": 4, "halo": 5, "remap": 6, "radiation": 7,
+          "physics": 8}
 
 
 class AtmError(RuntimeError):
@@ -66,7 +68,8 @@ def load_library(path: str = LIB_PATH):
     L.atm_build_info.restype = C.c_char_p
     for n in ["atm_destroy", "atm_init_prognostic_fields", "atm_initial_halo_exchange",
               "atm_init_sw_dyn", "atm_cgrid_halo", "atm_dgrid_halo", "atm_remap_init", "atm_remap",
-              "atm_radiation_init", "atm_radiation_halo", "atm_sync"]:
+              "atm_radiation_init", "atm_radiation_halo", "atm_sync", "atm_physics_init",
+              "atm_physics_tendencies", "atm_physics_halo"]:
         getattr(L, n).argtypes = [vp]
     L.atm_set_stream.argtypes = [vp, vp]
     L.atm_set_coordinate.argtypes = [vp, dp, dp]
@@ -74,7 +77,7 @@ def load_library(path: str = LIB_PATH):
         getattr(L, n).argtypes = [vp, ip, dp]
     for n in ["atm_put_field_device", "atm_get_field_device"]:
         getattr(L, n).argtypes = [vp, ip, vp]
-    for n in ["atm_cgrid_step", "atm_dgrid_step", "atm_radiation"]:
+    for n in ["atm_cgrid_step", "atm_dgrid_step", "atm_radiation", "atm_apply_physics"]:
         getattr(L, n).argtypes = [vp, C.c_double]
     L.atm_halo_exchange.argtypes = [vp, ip]
     L.atm_check_stability.argtypes = [vp, C.POINTER(C.c_int)]
@@ -173,6 +176,18 @@ class Model:
 
     def allocate_remapping_arrays(self):
         self._ck(self._L.atm_remap_init(self._h))
+
+    def allocate_physics_arrays(self):
+        self._ck(self._L.atm_physics_init(self._h))
+
+    def calculate_physics_tendencies(self):
+        self._ck(self._L.atm_physics_tendencies(self._h))
+
+    def apply_physics_tendencies(self, dt):
+        self._ck(self._L.atm_apply_physics(self._h, dt))
+
+    def physics_halo_exchange(self):
+        self._ck(self._L.atm_physics_halo(self._h))
 
     def calculate_radiative_heating(self, time):
         self._ck(self._L.atm_radiation(self._h, time))

@@ -71,6 +71,7 @@ static void release(atm_ctx *c) {
     if (c->flag_host) (void)hipHostFree(c->flag_host);
     atm_remap_free(c);
     atm_rad_free(c);
+    atm_phys_free(c);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -404,6 +405,32 @@ int atm_radiation(atm_ctx *c, double time) {
 int atm_radiation_halo(atm_ctx *c) {
     CHECK_CTX(c);
     return atm_launch_halo(c, c->net_flux, c->g.L + 1);
+}
+
+int atm_physics_init(atm_ctx *c) {
+    CHECK_CTX(c);
+    int rc = atm_phys_alloc(c);
+    if (rc) return rc;
+    c->phys_init = true;
+    return ATM_OK;
+}
+
+int atm_physics_tendencies(atm_ctx *c) {
+    CHECK_CTX(c);
+    if (!c->phys_init) return atm_fail(c, ATM_ERR_STATE, "atm_physics_init first");
+    if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "physics tendencies come before the C-grid step");
+    return atm_phys_tendencies(c);
+}
+
+int atm_apply_physics(atm_ctx *c, double dt) {
+    CHECK_CTX(c);
+    if (!c->phys_init) return atm_fail(c, ATM_ERR_STATE, "atm_physics_init first");
+    return atm_phys_apply(c, dt);
+}
+
+int atm_physics_halo(atm_ctx *c) {
+    CHECK_CTX(c);
+    return atm_launch_halo(c, c->ts, 1);
 }
 
 int atm_halo_exchange(atm_ctx *c, int id) {

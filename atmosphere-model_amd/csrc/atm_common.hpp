@@ -148,11 +148,12 @@ __device__ __forceinline__ double pcm_face(double qm1, double q0, double c) { re
 // ---------------------------------------------------------------------------
 struct RemapWork;  // k_remap.hip
 struct RadWork;    // k_rad.hip
+struct PhysWork;   // k_physics.hip
 
 // kernels timed with HIP events when timing is enabled (atm_timer_*)
 enum {
     ATM_T_C1 = 0, ATM_T_C2, ATM_T_D1, ATM_T_D2, ATM_T_SCAN, ATM_T_HALO, ATM_T_REMAP,
-    ATM_T_RAD, ATM_T_COUNT
+    ATM_T_RAD, ATM_T_PHYS, ATM_T_COUNT
 };
 struct AtmTimer {
     hipEvent_t *ev = nullptr;  // pairs (start, stop)
@@ -168,7 +169,7 @@ struct atm_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string err;
-    bool have_coord = false, sw_init = false, remap_init = false, rad_init = false;
+    bool have_coord = false, sw_init = false, remap_init = false, rad_init = false, phys_init = false;
     double ak0 = 0;    // Ak(1) = top pressure
     double pktop = 0;  // Ak(1)**kappa, evaluated on the host as the reference does
     // execution path: fused LDS-tiled kernels (default where the grid tiles
@@ -199,6 +200,7 @@ struct atm_ctx {
     int *flag_host = nullptr;   // pinned mirror
     RemapWork *remap = nullptr;
     RadWork *rad = nullptr;
+    PhysWork *phys = nullptr;
 };
 
 // error helpers
@@ -237,6 +239,10 @@ int atm_ref_init_sw_dyn(atm_ctx *c);
 int atm_remap_alloc(atm_ctx *c);
 void atm_remap_free(atm_ctx *c);
 int atm_remap_run(atm_ctx *c);
+int atm_phys_alloc(atm_ctx *c);
+void atm_phys_free(atm_ctx *c);
+int atm_phys_tendencies(atm_ctx *c);
+int atm_phys_apply(atm_ctx *c, double dt);
 int atm_rad_alloc(atm_ctx *c);
 void atm_rad_free(atm_ctx *c);
 int atm_rad_run(atm_ctx *c, double time);

@@ -119,6 +119,17 @@ int atm_radiation(atm_ctx *ctx, double time);      /* calculate_radiative_heatin
                                                       lookup outside the tables */
 int atm_radiation_halo(atm_ctx *ctx);              /* radiation_halo_exchange 400-404 */
 
+/* mod_physics public procedures (mod_physics.f90:13-14) on the device.
+   Tendencies are computed when the host's physics interval is due, before
+   the C-grid step; they are applied every step after the D-grid step
+   (mod_model.f90:137-159). */
+int atm_physics_init(atm_ctx *ctx);                /* allocate_physics_arrays 193-214 */
+int atm_physics_tendencies(atm_ctx *ctx);          /* calculate_physics_tendencies 27-78
+                                                      (dry_convective_adjustment 80-150,
+                                                      surface_heating 152-160) */
+int atm_apply_physics(atm_ctx *ctx, double dt);    /* apply_physics_tendencies 162-185 */
+int atm_physics_halo(atm_ctx *ctx);                /* physics_halo_exchange 187-191 (ts) */
+
 /* mod_sync generic halo_exchange (mod_sync.f90:14-17) for one field: a
    periodic fill of the 7-wide halo, edges and corners.  For ATM_GZ only
    levels 1..nlay are filled (the 5-argument specific's view of gz). */
@@ -141,7 +152,8 @@ int atm_get_path(atm_ctx *ctx, int *path);
    the library's kernels on the context stream.  Enabling resets the
    counters; atm_timer_read blocks and returns the number of timed launches
    and their summed duration for one kernel group:
-   0 C1, 1 C2, 2 D1, 3 D2, 4 column scans, 5 halo fills, 6 remap, 7 radiation. */
+   0 C1, 1 C2, 2 D1, 3 D2, 4 column scans, 5 halo fills, 6 remap, 7 radiation,
+   8 physics. */
 int atm_timer_enable(atm_ctx *ctx, int on);
 int atm_timer_read(atm_ctx *ctx, int kernel, int64_t *launches, double *total_ms);
 

@@ -45,8 +45,11 @@ def lib():
         L.orc_field_size.restype = C.c_long
         L.orc_field_size.argtypes = [vp, C.c_int]
         for name in ["orc_init_sw_dyn", "orc_cgrid_halo", "orc_dgrid_halo", "orc_allocate_remapping",
-                     "orc_remap", "orc_allocate_radiation", "orc_radiation_halo"]:
+                     "orc_remap", "orc_allocate_radiation", "orc_radiation_halo", "orc_allocate_physics",
+                     "orc_physics_tendencies", "orc_physics_halo"]:
             getattr(L, name).argtypes = [vp]
+        L.orc_apply_physics.argtypes = [vp, C.c_double]
+        L.orc_physics_column_tend.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp]
         L.orc_cgrid_step.argtypes = [vp, C.c_double]
         L.orc_dgrid_step.argtypes = [vp, C.c_double]
         L.orc_radiation.argtypes = [vp, C.c_double]
@@ -178,6 +181,25 @@ class Oracle:
 
     def radiation_halo_exchange(self):
         self._L.orc_radiation_halo(self._h)
+
+    def allocate_physics_arrays(self):
+        self._L.orc_allocate_physics(self._h)
+
+    def calculate_physics_tendencies(self):
+        self._L.orc_physics_tendencies(self._h)
+
+    def apply_physics_tendencies(self, dt):
+        self._L.orc_apply_physics(self._h, dt)
+
+    def physics_halo_exchange(self):
+        self._L.orc_physics_halo(self._h)
+
+    def physics_column_tendencies(self, i, j):
+        """(u_tend, v_tend, pt_tend) of column (i, j), 1-based reference
+        indices (i = 1 is the first interior point)."""
+        u, v, t = (np.zeros(self.nlay) for _ in range(3))
+        self._L.orc_physics_column_tend(self._h, i, j, _p(u), _p(v), _p(t))
+        return u, v, t
 
     def halo_exchange(self, fid):
         a = self._views[fid]

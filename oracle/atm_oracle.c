@@ -57,6 +57,10 @@ struct orc {
     /* mod_radiation module state (src/mod_radiation.f90:23-37) */
     int risl, riel, rjsl, rjel;
     double *lw_up, *lw_dn, *sw_up, *sw_dn, *layer_T, *level_T;
+    /* mod_physics module state (src/mod_physics.f90:16-23); the tendencies
+       are kept over the whole halo'd plane, of which the reference allocates
+       isd-2:ied+2 x jsd-2:jed+2 */
+    double *dp_tend, *pt_tend, *u_tend, *v_tend, *ts_tend, *pua, *pva;
 };
 
 /* Fortran-layout accessors */
@@ -134,7 +138,8 @@ void orc_destroy(orc_t *o) {
                       &o->edge_L, &o->edge_R, &o->dqi, &o->dqi_mono, &o->cbuf, &o->u_remap,
                       &o->v_remap, &o->pt_remap, &o->target_plev, &o->target_pkap,
                       &o->target_playkap, &o->lw_up, &o->lw_dn, &o->sw_up, &o->sw_dn,
-                      &o->layer_T, &o->level_T};
+                      &o->layer_T, &o->level_T, &o->dp_tend, &o->pt_tend, &o->u_tend,
+                      &o->v_tend, &o->ts_tend, &o->pua, &o->pva};
     for (size_t i = 0; i < sizeof(all) / sizeof(all[0]); i++) free(*all[i]);
     free(o);
 }
@@ -1273,3 +1278,143 @@ int orc_radiation(orc_t *o, double time) {
 }
 
 void orc_radiation_halo(orc_t *o) { halo_fill(o, o->net_flux, o->nlay + 1); } /* 400-404 */
+
+/* ------------------------------------------------------------------------
+ * mod_physics (src/mod_physics.f90): dry convective adjustment, surface
+ * heating and the application of the tendencies.
+ * ------------------------------------------------------------------------ */
+void orc_allocate_physics(orc_t *o) { /* 193-214 */
+    size_t n2 = (size_t)o->ni * o->nj, nl = n2 * o->nlay;
+    if (o->dp_tend) return;
+    o->dp_tend = zalloc(nl); o->pt_tend = zalloc(nl); o->u_tend = zalloc(nl);
+    o->v_tend = zalloc(nl); o->ts_tend = zalloc(n2); o->pua = zalloc(nl); o->pva = zalloc(nl);
+}
+
+/* dry_convective_adjustment: 80-150, columns isd-2..ied+2 x jsd-2..jed+2 */
+static void dry_convective_adjustment(orc_t *o) {
+    const int L = o->nlay;
+    const double min_v2 = (double)1e-4f;           /* default-real literal (Q1) */
+    const double crit = .25, timescale = 600;
+    double *energy = zalloc(L + 1), *u_adj = zalloc(L + 1), *v_adj = zalloc(L + 1);
+    double *gz_adj = zalloc(L + 2), *pt_adj = zalloc(L + 1);
+    for (int j = o->jsd - 2; j <= o->jed + 2; j++)
+        for (int i = o->isd - 2; i <= o->ied + 2; i++) {
+            for (int k = 1; k <= L; k++) {
+                double ua = F3(o->pua, i, j, k), va = F3(o->pva, i, j, k);
+                energy[k] = CV_DRY * F3(o->pt, i, j, k) * F3(o->playkap, i, j, k) +
+                            .5 * (F3(o->gz, i, j, k + 1) + F3(o->gz, i, j, k) + ua * ua + va * va);
+                u_adj[k] = ua;
+                v_adj[k] = va;
+            }
+            for (int k = 1; k <= L - 1; k++) {
+                double dk = F3(o->dp, i, j, k), dk1 = F3(o->dp, i, j, k + 1);
+                double ptk = F3(o->pt, i, j, k), ptk1 = F3(o->pt, i, j, k + 1);
+                double du = F3(o->pua, i, j, k + 1) - F3(o->pua, i, j, k);
+                double dv = F3(o->pva, i, j, k + 1) - F3(o->pva, i, j, k);
+                double ri = (ptk1 - ptk) *
+                            (dk * (F3(o->gz, i, j, k + 1) - F3(o->gz, i, j, k)) +
+                             dk1 * (F3(o->gz, i, j, k + 2) - F3(o->gz, i, j, k + 1))) /
+                            ((dk * ptk + dk1 * ptk1) * (du * du + dv * dv + min_v2));
+                if (ri < crit) {
+                    double m = 1. - fmaxd(0.0, ri / crit);
+                    double mf = m * m * dk * dk1 / (dk + dk1);
+                    double ex = mf * (u_adj[k] - u_adj[k + 1]);
+                    u_adj[k] = u_adj[k] - ex / dk;
+                    u_adj[k + 1] = u_adj[k + 1] + ex / dk1;
+                    ex = mf * (v_adj[k] - v_adj[k + 1]);
+                    v_adj[k] = v_adj[k] - ex / dk;
+                    v_adj[k + 1] = v_adj[k + 1] + ex / dk1;
+                    ex = mf * (energy[k] - energy[k + 1]);
+                    energy[k] = energy[k] - ex / dk;
+                    energy[k + 1] = energy[k + 1] + ex / dk1;
+                }
+            }
+            for (int k = 1; k <= L; k++) {
+                F3(o->u_tend, i, j, k) = F3(o->u_tend, i, j, k) + (u_adj[k] - F3(o->pua, i, j, k)) / timescale;
+                F3(o->v_tend, i, j, k) = F3(o->v_tend, i, j, k) + (v_adj[k] - F3(o->pva, i, j, k)) / timescale;
+            }
+            gz_adj[1] = F3(o->gz, i, j, 1);
+            for (int k = 1; k <= L; k++) {
+                double dpk = F3(o->pkap, i, j, k) - F3(o->pkap, i, j, k + 1);
+                pt_adj[k] = (energy[k] - gz_adj[k] - .5 * (u_adj[k] * u_adj[k] + v_adj[k] * v_adj[k])) /
+                            (CV_DRY * F3(o->playkap, i, j, k) + .5 * CP_DRY * dpk);
+                gz_adj[k + 1] = gz_adj[k] + CP_DRY * pt_adj[k] * dpk;
+            }
+            for (int k = 1; k <= L; k++)
+                F3(o->pt_tend, i, j, k) = F3(o->pt_tend, i, j, k) + (pt_adj[k] - F3(o->pt, i, j, k)) / timescale;
+        }
+    free(energy); free(u_adj); free(v_adj); free(gz_adj); free(pt_adj);
+}
+
+/* calculate_physics_tendencies: 27-78 */
+void orc_physics_tendencies(orc_t *o) {
+    const int L = o->nlay;
+    size_t nl = (size_t)o->ni * o->nj * L;
+    memset(o->dp_tend, 0, nl * sizeof(double));
+    memset(o->pt_tend, 0, nl * sizeof(double));
+    memset(o->u_tend, 0, nl * sizeof(double));
+    memset(o->v_tend, 0, nl * sizeof(double));
+    for (int k = 1; k <= L; k++)
+        for (int j = o->js + 1; j <= o->je - 1; j++)
+            for (int i = o->is + 1; i <= o->ie - 1; i++) {
+                F3(o->pua, i, j, k) = (.25 * (F3(o->ud, i, j, k) + F3(o->ud, i, j + 1, k)) +
+                                       .125 * (F3(o->ud, i - 1, j, k) + F3(o->ud, i - 1, j + 1, k) +
+                                               F3(o->ud, i + 1, j, k) + F3(o->ud, i + 1, j + 1, k)));
+                F3(o->pva, i, j, k) = (.25 * (F3(o->vd, i, j, k) + F3(o->vd, i + 1, j, k)) +
+                                       .125 * (F3(o->vd, i, j - 1, k) + F3(o->vd, i + 1, j - 1, k) +
+                                               F3(o->vd, i, j + 1, k) + F3(o->vd, i + 1, j + 1, k)));
+            }
+    for (int k = L; k >= 1; k--)
+        for (int j = o->js + 1; j <= o->je - 1; j++)
+            for (int i = o->is + 1; i <= o->ie - 1; i++) {
+                F3(o->plev, i, j, k) = F3(o->plev, i, j, k + 1) + F3(o->dp, i, j, k);
+                F3(o->pkap, i, j, k) = pow(F3(o->plev, i, j, k), KAPPA);
+            }
+    for (int k = 1; k <= L; k++)
+        for (int j = o->js + 1; j <= o->je - 1; j++)
+            for (int i = o->is + 1; i <= o->ie - 1; i++) {
+                F3(o->gz, i, j, k + 1) = F3(o->gz, i, j, k) + CP_DRY * F3(o->pt, i, j, k) *
+                                                                  (F3(o->pkap, i, j, k) - F3(o->pkap, i, j, k + 1));
+                F3(o->playkap, i, j, k) = (F3(o->pkap, i, j, k + 1) - F3(o->pkap, i, j, k)) /
+                                          log(F3(o->plev, i, j, k + 1) / F3(o->plev, i, j, k)) / KAPPA;
+            }
+    dry_convective_adjustment(o);
+    /* surface_heating: 152-160, heat capacity of 5 m of water 41680*5 */
+    for (int j = o->jsd; j <= o->jed; j++)
+        for (int i = o->isd; i <= o->ied; i++)
+            F2(o->ts_tend, i, j) = F3(o->net_flux, i, j, 1) / (double)(41680 * 5);
+}
+
+/* apply_physics_tendencies: 162-185 */
+void orc_apply_physics(orc_t *o, double dt) {
+    const int L = o->nlay;
+    for (int k = 1; k <= L; k++)
+        for (int j = o->jsd; j <= o->jed; j++)
+            for (int i = o->isd; i <= o->ied; i++) {
+                F3(o->dp, i, j, k) = F3(o->dp, i, j, k) + dt * F3(o->dp_tend, i, j, k);
+                F3(o->pt, i, j, k) = F3(o->pt, i, j, k) + dt * F3(o->pt_tend, i, j, k);
+            }
+    for (int j = o->jsd; j <= o->jed; j++)
+        for (int i = o->isd; i <= o->ied; i++) F2(o->ts, i, j) = F2(o->ts, i, j) + dt * F2(o->ts_tend, i, j);
+    for (int k = 1; k <= L; k++)
+        for (int j = o->jsd; j <= o->jed; j++)
+            for (int i = o->isd; i <= o->ied; i++) {
+                F3(o->ud, i, j, k) = F3(o->ud, i, j, k) +
+                                     dt * (W7_12 * (F3(o->u_tend, i, j, k) + F3(o->u_tend, i, j - 1, k)) -
+                                           W1_12 * (F3(o->u_tend, i, j + 1, k) + F3(o->u_tend, i, j - 2, k)));
+                F3(o->vd, i, j, k) = F3(o->vd, i, j, k) +
+                                     dt * (W7_12 * (F3(o->v_tend, i, j, k) + F3(o->v_tend, i - 1, j, k)) -
+                                           W1_12 * (F3(o->v_tend, i + 1, j, k) + F3(o->v_tend, i - 2, j, k)));
+            }
+}
+
+void orc_physics_halo(orc_t *o) { halo_fill(o, o->ts, 0); } /* 187-191 */
+
+/* unit entry: the physics tendencies of one column (for pins) */
+void orc_physics_column_tend(orc_t *o, int i, int j, double *u_tend, double *v_tend, double *pt_tend) {
+    for (int k = 1; k <= o->nlay; k++) {
+        u_tend[k - 1] = F3(o->u_tend, i, j, k);
+        v_tend[k - 1] = F3(o->v_tend, i, j, k);
+        pt_tend[k - 1] = F3(o->pt_tend, i, j, k);
+    }
+}

@@ -5,7 +5,7 @@
  * restatement of the reference Fortran model's per-timestep hot path
  * (felix-goddard/atmosphere-model: src/mod_sw_dyn.f90, src/mod_remap.f90,
  * src/mod_radiation.f90, src/mod_gas_optics.f90, src/mod_solar.f90,
- * src/mod_sync.f90 on a single image).  It is the checker the HIP library is
+ * src/mod_physics.f90, src/mod_sync.f90 on a single image).  It is the checker the HIP library is
  * compared against.  Only tests/, __graft_entry__.smoke() and the CPU-baseline
  * leg of bench.py may load it; the product path never does.
  *
@@ -49,6 +49,12 @@ void orc_allocate_radiation(orc_t *o);          /* allocate_radiation_arrays */
 int  orc_radiation(orc_t *o, double time);      /* calculate_radiative_heating; !=0: table range */
 void orc_radiation_halo(orc_t *o);              /* radiation_halo_exchange */
 void orc_halo(orc_t *o, double *a, int nk);     /* halo_exchange (nk==0: 2-D field) */
+void orc_allocate_physics(orc_t *o);            /* allocate_physics_arrays */
+void orc_physics_tendencies(orc_t *o);          /* calculate_physics_tendencies */
+void orc_apply_physics(orc_t *o, double dt);    /* apply_physics_tendencies */
+void orc_physics_halo(orc_t *o);                /* physics_halo_exchange */
+void orc_physics_column_tend(orc_t *o, int i, int j, double *u_tend, double *v_tend,
+                             double *pt_tend); /* tendencies of column (i, j), 1-based i, j */
 
 /* unit-level entry points for the tests */
 double orc_sw_abs(int g, double p, double T, int *err);

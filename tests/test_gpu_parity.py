@@ -116,3 +116,71 @@ def test_fused_path_bitwise_equals_reference_path(gpu_available, shape, steps):
         x, y = a.get_full(name), b.get_full(name)
         assert np.array_equal(x, y), (name, float(np.max(np.abs(x - y))))
     print(f"fused == reference bitwise {shape} x{steps}", flush=True)
+
+
+def _unstable(ic, rng, frac=0.3):
+    """Perturb pt so that some interfaces are statically unstable and the
+    dry convective adjustment mixes there."""
+    pt = ic["pt"].copy()
+    mask = rng.random(pt.shape) < frac
+    pt[mask] *= 1.0 + rng.uniform(0.01, 0.05, mask.sum())
+    ic = dict(ic)
+    ic["pt"] = pt
+    return ic
+
+
+def test_physics_stage(gpu_available):
+    """calculate_physics_tendencies + apply_physics_tendencies (dry
+    convective adjustment, surface heating) on a partly unstable state."""
+    import oracle as O
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+    nx, ny, nlay = 32, 24, 16
+    rng = np.random.default_rng(77)
+    ic = _unstable(make_initial_state(nx, ny, nlay, 100e3), rng)
+    cfg = Config(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=300.0,
+                 dt_remap=300.0, radiation_on=False, physics_on=True)
+    gpu, orc = Model(nx, ny, nlay, 100e3), O.Oracle(nx, ny, nlay, 100e3, 100e3)
+    nf = np.zeros((nlay + 1, ny, nx))
+    nf[0] = rng.uniform(-80.0, 150.0, (ny, nx))
+    for b in (gpu, orc):
+        driver.init_model(b, ic, Config(**cfg.__dict__))
+        b.put("net_flux", nf)
+        b.calculate_physics_tendencies()
+        b.apply_physics_tendencies(60.0)
+    errs = {n: compare(gpu, orc, n) for n in ("dp", "pt", "ud", "vd", "ts")}
+    errs.update({n: compare(gpu, orc, n, (1, 1)) for n in ("gz", "plev", "pkap")})
+    _print("physics stage", **errs)
+    moved = np.max(np.abs(gpu.get("pt") - ic["pt"]))
+    assert moved > 0.0
+    for n, e in errs.items():
+        assert e <= PARITY, (n, e)
+
+
+@pytest.mark.parametrize("path", ["fused", "reference"])
+def test_driver_run_with_physics(gpu_available, path):
+    """The CPU reference configuration's schedule (dry mixing on, physics and
+    remap every 5 minutes, dt = 60 s) for 20 steps with radiation on."""
+    import oracle as O
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+    nx, ny, nlay = 64, 64, 20
+    rng = np.random.default_rng(3)
+    ic = _unstable(make_initial_state(nx, ny, nlay, 100e3), rng, 0.1)
+    cfg = dict(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=600.0, dt_physics=300.0,
+               dt_remap=300.0, radiation_on=True, physics_on=True)
+    gpu, orc = Model(nx, ny, nlay, 100e3), O.Oracle(nx, ny, nlay, 100e3, 100e3)
+    if path != gpu.path:
+        gpu.set_path(path)
+    for b in (gpu, orc):
+        st = driver.init_model(b, ic, Config(**cfg))
+        t, n, stable = driver.run_model(b, Config(**cfg), st, max_steps=20)
+        assert stable and n == 20
+    errs = {n: compare(gpu, orc, n) for n in ("dp", "pt", "ud", "vd", "ts", "net_flux")}
+    _print(f"driver physics {path} 64x64x20 x20", **errs)
+    for n, e in errs.items():
+        assert e <= PARITY, (n, e)

@@ -1,0 +1,135 @@
+"""Pins of the oracle's physics (src/mod_physics.f90) restatement.
+
+* a statically unstable two-layer column mixes completely: both layers end
+  at the mass-weighted mean wind (closed form), so u_tend, v_tend follow;
+* a stable column with no shear is left alone: zero wind tendencies and a
+  potential-temperature tendency at round-off (the energy round trip);
+* the mixing sweep conserves column momentum (sum of dp * tendency);
+* surface heating: ts advances by dt * net_flux(1) / (41680 * 5);
+* apply_physics_tendencies interpolates with the binary32 weights 7/12 and
+  1/12 (Q1) along j for u and along i for v: an x<->y transpose of the
+  state transposes every result.
+"""
+import numpy as np
+
+import oracle as O
+
+N = 8
+H = O.HALO
+
+
+def _column_state(dp, pt, u, v, ptop=1000.0, nx=N, ny=N):
+    L = len(dp)
+    o = O.Oracle(nx, ny, L, 100e3, 100e3)
+    ones = np.ones((L, ny, nx))
+    o.put("dp", ones * np.asarray(dp)[:, None, None])
+    o.put("pt", ones * np.asarray(pt)[:, None, None])
+    o.put("ud", ones * np.asarray(u)[:, None, None])
+    o.put("vd", ones * np.asarray(v)[:, None, None])
+    o.put("gzs", np.zeros((ny, nx)))
+    o.put("ts", np.full((ny, nx), 280.0))
+    ak = np.full(L + 1, ptop)
+    bk = np.linspace(1.0, 0.0, L + 1)
+    o.put("ak", ak)
+    o.put("bk", bk)
+    o.init_prognostic_fields()
+    o.initial_halo_exchange()
+    o.allocate_physics_arrays()
+    return o
+
+
+def test_unstable_two_layers_mix_to_mass_weighted_mean():
+    dp = np.array([40000.0, 25000.0])
+    pt = np.array([320.0, 300.0])  # potential temperature falls with height: unstable
+    u, v = np.array([12.0, -3.0]), np.array([1.5, 7.25])
+    o = _column_state(dp, pt, u, v)
+    o.calculate_physics_tendencies()
+    ut, vt, _ = o.physics_column_tendencies(3, 4)
+    for w, t in ((u, ut), (v, vt)):
+        mean = (dp[0] * w[0] + dp[1] * w[1]) / (dp[0] + dp[1])
+        np.testing.assert_allclose(t, (mean - w) / 600.0, rtol=1e-13, atol=1e-16)
+
+
+def test_stable_column_without_shear_is_untouched():
+    L = 6
+    dp = np.full(L, 15000.0)
+    pt = 300.0 + 10.0 * np.arange(L)  # stable
+    o = _column_state(dp, pt, np.full(L, 5.0), np.full(L, -2.0))
+    o.calculate_physics_tendencies()
+    for j in (1, 4, N):
+        ut, vt, tt = o.physics_column_tendencies(2, j)
+        assert np.all(ut == 0.0) and np.all(vt == 0.0)
+        assert np.max(np.abs(tt)) * 600.0 < 1e-12 * np.max(pt)
+
+
+def test_mixing_conserves_column_momentum():
+    rng = np.random.default_rng(11)
+    L = 20
+    dp = rng.uniform(2000.0, 8000.0, L)
+    pt = 300.0 + np.cumsum(rng.normal(0.0, 3.0, L))  # mixed stability
+    u, v = rng.normal(0, 10, L), rng.normal(0, 10, L)
+    o = _column_state(dp, pt, u, v)
+    o.calculate_physics_tendencies()
+    ut, vt, _ = o.physics_column_tendencies(5, 2)
+    assert np.any(ut != 0.0)  # some interfaces mixed
+    for t, w in ((ut, u), (vt, v)):
+        assert abs(np.sum(dp * t)) * 600.0 <= 1e-12 * np.sum(dp * np.abs(w))
+
+
+def test_surface_heating_closed_form():
+    o = _column_state([30000.0, 30000.0], [300.0, 310.0], [0.0, 0.0], [0.0, 0.0])
+    nf = np.zeros((3, N, N))
+    nf[0] = np.linspace(-50.0, 120.0, N * N).reshape(N, N)
+    o.put("net_flux", nf)
+    ts0 = o.get("ts")
+    o.calculate_physics_tendencies()
+    dt = 60.0
+    o.apply_physics_tendencies(dt)
+    np.testing.assert_array_equal(o.get("ts"), ts0 + dt * (nf[0] / float(41680 * 5)))
+
+
+def test_apply_weights_binary32_and_transpose_symmetry():
+    rng = np.random.default_rng(5)
+    L, n = 8, 12
+    dp = rng.uniform(5000.0, 15000.0, (L, n, n))
+    pt = 300.0 + np.cumsum(rng.normal(0.0, 2.0, (L, n, n)), axis=0)
+    u = rng.normal(0, 8, (L, n, n))
+    v = rng.normal(0, 8, (L, n, n))
+
+    def run(dp, pt, u, v):
+        o = O.Oracle(n, n, L, 100e3, 100e3)
+        o.put("dp", dp)
+        o.put("pt", pt)
+        o.put("ud", u)
+        o.put("vd", v)
+        o.put("gzs", np.zeros((n, n)))
+        o.put("ts", np.full((n, n), 280.0))
+        o.put("ak", np.full(L + 1, 1000.0))
+        o.put("bk", np.linspace(1.0, 0.0, L + 1))
+        o.init_prognostic_fields()
+        o.initial_halo_exchange()
+        o.allocate_physics_arrays()
+        o.calculate_physics_tendencies()
+        o.apply_physics_tendencies(30.0)
+        return {k: o.get(k) for k in ("dp", "pt", "ud", "vd")}
+
+    # D-grid u(i, j) sits at the y-face j - 1/2 and v(i, j) at the x-face
+    # i - 1/2: transposing moves u to v with the same face offsets
+    tr = lambda a: np.ascontiguousarray(np.swapaxes(a, 1, 2))
+    a = run(dp, pt, u, v)
+    b = run(tr(dp), tr(pt), tr(v), tr(u))
+    np.testing.assert_array_equal(a["dp"], tr(b["dp"]))
+    np.testing.assert_allclose(a["pt"], tr(b["pt"]), rtol=1e-14)
+    np.testing.assert_allclose(a["ud"], tr(b["vd"]), rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(a["vd"], tr(b["ud"]), rtol=1e-13, atol=1e-13)
+    # uniform tendency: the 4-point weights sum to 2 * (7/12 - 1/12) in
+    # binary32, 1 - 5.96e-8, not 1
+    o = _column_state([40000.0, 25000.0], [320.0, 300.0], [12.0, -3.0], [1.5, 7.25])
+    o.calculate_physics_tendencies()
+    ut, _, _ = o.physics_column_tendencies(3, 3)
+    u0 = o.get("ud")
+    o.apply_physics_tendencies(10.0)
+    w7, w1 = float(np.float32(7.0 / 12.0)), float(np.float32(1.0 / 12.0))
+    expect = u0[:, 3, 3] + 10.0 * (w7 * (ut + ut) - w1 * (ut + ut))
+    np.testing.assert_array_equal(o.get("ud")[:, 3, 3], expect)
+    assert abs((w7 * 2 - w1 * 2) - 1.0) > 1e-8
