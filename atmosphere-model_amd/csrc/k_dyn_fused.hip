@@ -300,22 +300,10 @@ __global__ __launch_bounds__(NT) void k_c2(Geo g, DynCoef c, const double *__res
 // D2: kinetic energy, vorticity, divergence damping, B-grid pressure
 // gradient and the ud/vd update of one level (dgrid 606-771).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double corner_lds(const double *q, int w, int x0, int y0, int x, int y) {
-#define QQ(a, b) q[((y) + (b) - y0) * w + ((x) + (a) - x0)]
-#define TXV(b) (ATM_W7_12 * (QQ(0, b) + QQ(-1, b)) - ATM_W1_12 * (QQ(1, b) + QQ(-2, b)))
-#define TYV(a) (ATM_W7_12 * (QQ(a, 0) + QQ(a, -1)) - ATM_W1_12 * (QQ(a, 1) + QQ(a, -2)))
-    double o = (9. / 32.) * (TXV(0) + TXV(-1)) - (1. / 32.) * (TXV(1) + TXV(-2));
-    o = o + (9. / 32.) * (TYV(0) + TYV(-1)) - (1. / 32.) * (TYV(1) + TYV(-2));
-    return o;
-#undef QQ
-#undef TXV
-#undef TYV
-}
-
-__global__ __launch_bounds__(NT) void k_d2(Geo g, DynCoef c, double pkb_top, const double *__restrict__ ud,
+__global__ __launch_bounds__(NT) void k_d2(Geo g, DynCoef c, const double *__restrict__ ud,
                                            const double *__restrict__ vd, const double *__restrict__ uc,
-                                           const double *__restrict__ vc, const double *__restrict__ pkap,
-                                           const double *__restrict__ gz, double *__restrict__ udo,
+                                           const double *__restrict__ vc, const double *__restrict__ pkapb,
+                                           const double *__restrict__ gzb, double *__restrict__ udo,
                                            double *__restrict__ vdo) {
     using RU = Reg<-3, TX + 3, -3, TY + 4>;
     using RV = Reg<-3, TX + 4, -3, TY + 3>;
@@ -326,9 +314,8 @@ __global__ __launch_bounds__(NT) void k_d2(Geo g, DynCoef c, double pkb_top, con
     using RTU = Reg<0, TX, -3, TY + 3>;
     using RTW = Reg<-3, TX + 3, 0, TY>;
     using RLP = Reg<-1, TX, -1, TY>;
-    using RQS = Reg<-2, TX + 2, -2, TY + 2>;
     __shared__ double sU[RU::n], sV[RV::n], sKE[RKE::n], sW[RW::n], sDV[RDV::n], sD2[RD2::n], sD4[RKE::n];
-    __shared__ double sTU[RTU::n], sTW[RTW::n], sL[RLP::n], sQS[RQS::n], sCB[4][RKE::n];
+    __shared__ double sTU[RTU::n], sTW[RTW::n], sL[RLP::n];
     int bx, by, K;
     xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
     const int I0 = ATM_HALO + bx * TX, J0 = ATM_HALO + by * TY;
@@ -394,30 +381,22 @@ __global__ __launch_bounds__(NT) void k_d2(Geo g, DynCoef c, double pkb_top, con
     sD4[RKE::at(x, y)] = c.divdamp * (-2. * D2(x, y) * c.rdx2rdy2 + (D2(x + 1, y) + D2(x - 1, y)) * c.rdx2 +
                                       (D2(x, y + 1) + D2(x, y - 1)) * c.rdy2);
     END_FOR
-    // B-grid pkap and gz of levels k and k+1 (606-632 and init 991, 995)
-    for (int f = 0; f < 4; f++) {
-        const double *src = f < 2 ? pkap : gz;
-        const int Kq = K + (f & 1);
-        if (f == 1 && Kq == g.L) {
-            FOR_REG(RKE) sCB[f][RKE::at(x, y)] = pkb_top; END_FOR
-            continue;
-        }
-        __syncthreads();
-        FOR_REG(RQS) sQS[RQS::at(x, y)] = src[g.ix(I0 + x, J0 + y, Kq)]; END_FOR
-        __syncthreads();
-        FOR_REG(RKE) sCB[f][RKE::at(x, y)] = corner_lds(sQS, RQS::w, RQS::x0, RQS::y0, x, y); END_FOR
-    }
     __syncthreads();
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-#define CB(f, x, y) sCB[f][RKE::at(x, y)]
+    // B-grid pressure gradient (665-679) from the corner values of pkap and
+    // gz at levels k and k+1 (k_bcorners; pkapb(nlay+1) and gzb(1) from
+    // init_sw_dyn 991, 995)
+#define CB(a, kq, x, y) a[g.ix(I0 + (x), J0 + (y), K + (kq))]
     double pgfx, pgfy;
-    {  // 665-679
-        double a = CB(0, tx + 1, ty), b = CB(1, tx, ty), cc = CB(0, tx, ty), d = CB(1, tx + 1, ty);
-        pgfx = ((a - b) * (CB(2, tx, ty) - CB(3, tx + 1, ty)) + (cc - d) * (CB(3, tx, ty) - CB(2, tx + 1, ty))) /
+    {
+        double a = CB(pkapb, 0, tx + 1, ty), b = CB(pkapb, 1, tx, ty), cc = CB(pkapb, 0, tx, ty),
+               d = CB(pkapb, 1, tx + 1, ty);
+        const double g00 = CB(gzb, 0, tx, ty), g01 = CB(gzb, 1, tx, ty);
+        pgfx = ((a - b) * (g00 - CB(gzb, 1, tx + 1, ty)) + (cc - d) * (g01 - CB(gzb, 0, tx + 1, ty))) /
                (a - b + cc - d);
-        a = CB(0, tx, ty + 1);
-        d = CB(1, tx, ty + 1);
-        pgfy = ((a - b) * (CB(2, tx, ty) - CB(3, tx, ty + 1)) + (cc - d) * (CB(3, tx, ty) - CB(2, tx, ty + 1))) /
+        a = CB(pkapb, 0, tx, ty + 1);
+        d = CB(pkapb, 1, tx, ty + 1);
+        pgfy = ((a - b) * (g00 - CB(gzb, 1, tx, ty + 1)) + (cc - d) * (g01 - CB(gzb, 0, tx, ty + 1))) /
                (a - b + cc - d);
     }
 #undef CB
@@ -448,6 +427,60 @@ __global__ __launch_bounds__(NT) void k_d2(Geo g, DynCoef c, double pkb_top, con
 #undef W
 #undef DV
 #undef D2
+}
+
+// B-grid corner values of pkap and gz for the D step (dgrid 606-632): for
+// every level k, pkapb(k) = corners(pkap(k)) and gzb(k+1) = corners(gz(k+1))
+// on the interior plus one row and column (what the pressure gradient of
+// the tile reads).  pkapb(nlay+1) and gzb(1) keep their init_sw_dyn values.
+// interpolate_to_corners (921-956) is separable: each corner combines four
+// x-direction 4-point averages (rows j-2..j+1) and four y-direction ones
+// (columns i-2..i+1).  A 32 x 8 tile stages the field, forms each average
+// once in LDS and combines them in the reference's order.
+__global__ void __launch_bounds__(NT) k_bcorners(Geo g, const double *__restrict__ pkap,
+                                                 const double *__restrict__ gz, double *__restrict__ pkapb,
+                                                 double *__restrict__ gzb) {
+    using RQ = Reg<-2, TX + 1, -2, TY + 1>;  // field
+    using RAX = Reg<0, TX, -2, TY + 1>;      // x averages
+    using RAY = Reg<-2, TX + 1, 0, TY>;      // y averages
+    __shared__ double sQ[2][RQ::n], sAX[2][RAX::n], sAY[2][RAY::n];
+    const int I0 = ATM_HALO + blockIdx.x * TX, J0 = ATM_HALO + blockIdx.y * TY, K = blockIdx.z;
+    const double *src[2] = {pkap + g.ix(0, 0, K), gz + g.ix(0, 0, K + 1)};
+    // the last tile row / column holds one valid output row / column; loads
+    // beyond the halo'd plane are not used by it and are skipped
+    for (int f = 0; f < 2; f++) {
+        FOR_REG(RQ)
+        const int I = I0 + x, J = J0 + y;
+        sQ[f][RQ::at(x, y)] = (I < g.NI && J < g.NJ) ? src[f][g.ix2(I, J)] : 0.;
+        END_FOR
+    }
+    __syncthreads();
+#define Q(f, x, y) sQ[f][RQ::at(x, y)]
+    for (int f = 0; f < 2; f++) {
+        FOR_REG(RAX)
+        sAX[f][RAX::at(x, y)] = ATM_W7_12 * (Q(f, x, y) + Q(f, x - 1, y)) - ATM_W1_12 * (Q(f, x + 1, y) + Q(f, x - 2, y));
+        END_FOR
+        FOR_REG(RAY)
+        sAY[f][RAY::at(x, y)] = ATM_W7_12 * (Q(f, x, y) + Q(f, x, y - 1)) - ATM_W1_12 * (Q(f, x, y + 1) + Q(f, x, y - 2));
+        END_FOR
+    }
+#undef Q
+    __syncthreads();
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int I = I0 + tx, J = J0 + ty;
+    if (I > ATM_HALO + g.nx || J > ATM_HALO + g.ny) return;
+    double out[2];
+    for (int f = 0; f < 2; f++) {
+#define AX(b) sAX[f][RAX::at(tx, ty + (b))]
+#define AY(a) sAY[f][RAY::at(tx + (a), ty)]
+        double o = (9. / 32.) * (AX(0) + AX(-1)) - (1. / 32.) * (AX(1) + AX(-2));
+        o = o + (9. / 32.) * (AY(0) + AY(-1)) - (1. / 32.) * (AY(1) + AY(-2));
+        out[f] = o;
+#undef AX
+#undef AY
+    }
+    pkapb[g.ix(I, J, K)] = out[0];
+    gzb[g.ix(I, J, K + 1)] = out[1];
 }
 
 // ---------------------------------------------------------------------------
@@ -632,10 +665,12 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
     atm_timer_begin(c, ATM_T_SCAN);
     hipLaunchKernelGGL(k_scan_pg, dim3((g.NI + 63) / 64, g.NJ), dim3(64), 0, c->stream, g, c->ak0, c->pktop,
                        k.gravity, nxt.dp, nxt.pt, c->net_flux, c->plev, c->pkap, c->gz, (double *)nullptr);
+    hipLaunchKernelGGL(k_bcorners, dim3((g.nx + TX) / TX, (g.ny + TY) / TY, g.L), blk, 0, c->stream, g, c->pkap,
+                       c->gz, c->pkapb, c->gzb);
     atm_timer_end(c, ATM_T_SCAN);
     atm_timer_begin(c, ATM_T_D2);
-    hipLaunchKernelGGL(k_d2, grid, blk, 0, c->stream, g, k, c->pktop, cur.ud, cur.vd, c->uc2, c->vc2, c->pkap,
-                       c->gz, nxt.ud, nxt.vd);
+    hipLaunchKernelGGL(k_d2, grid, blk, 0, c->stream, g, k, cur.ud, cur.vd, c->uc2, c->vc2, c->pkapb, c->gzb,
+                       nxt.ud, nxt.vd);
     atm_timer_end(c, ATM_T_D2);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;

@@ -1,8 +1,8 @@
 ! atm_c_api.f90 -- ISO_C_BINDING interfaces to libatmos_hip.so (include/atm.h).
 !
 ! A Fortran host binds the GPU library through this module; the drop-in
-! modules in this folder (mod_sw_dyn, mod_remap, mod_radiation, mod_sync)
-! call it with the reference's public procedure names.
+! modules in this folder (mod_sw_dyn, mod_remap, mod_radiation, mod_physics,
+! mod_sync) call it with the reference's public procedure names.
 module atm_c_api
    use iso_c_binding, only: c_int, c_int32_t, c_int64_t, c_double, c_ptr, c_null_ptr, c_char
    implicit none
@@ -14,7 +14,8 @@ module atm_c_api
              atm_initial_halo_exchange, atm_init_sw_dyn, atm_cgrid_step, &
              atm_cgrid_halo, atm_dgrid_step, atm_dgrid_halo, atm_remap_init, &
              atm_remap, atm_radiation_init, atm_radiation, atm_radiation_halo, &
-             atm_halo_exchange, atm_check_stability, atm_sync
+             atm_halo_exchange, atm_check_stability, atm_sync, atm_physics_init, &
+             atm_physics_tendencies, atm_apply_physics, atm_physics_halo
    public :: ATM_DP, ATM_PT, ATM_UD, ATM_VD, ATM_TS, ATM_GZ, ATM_NET_FLUX, &
              ATM_PLEV, ATM_PKAP, ATM_RADIUS
 
@@ -114,6 +115,23 @@ module atm_c_api
          real(c_double), value :: time
       end function
       integer(c_int) function atm_radiation_halo(ctx) bind(c, name='atm_radiation_halo')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function atm_physics_init(ctx) bind(c, name='atm_physics_init')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function atm_physics_tendencies(ctx) bind(c, name='atm_physics_tendencies')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function atm_apply_physics(ctx, dt) bind(c, name='atm_apply_physics')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: dt
+      end function
+      integer(c_int) function atm_physics_halo(ctx) bind(c, name='atm_physics_halo')
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
       end function

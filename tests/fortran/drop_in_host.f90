@@ -1,14 +1,14 @@
 ! drop_in_host.f90 -- a Fortran host that drives the gfx950 library through
 ! the drop-in modules with the reference's names, in the call order of
-! run_model (src/mod_model.f90:115-193; physics and output are not part of
-! this test).
+! run_model (src/mod_model.f90:115-193; output is not part of this test).
 !
 ! usage: drop_in_host <input file> <output file>
-!   input (stream, native endian): int32 nx, ny, nlay, nsteps, resident;
-!   float64 dx, dy, gravity, f, c_div, c_vort; ak(nlay+1), bk(nlay+1);
-!   per step float64 time, dt and int32 radiation, remap flags; then the
-!   halo'd arrays dp, pt, ud, vd, ts, gz, plev, pkap, net_flux, radius.
-!   output: int32 steps completed, int32 stable; dp, pt, ud, vd, net_flux.
+!   input (stream, native endian): int32 nx, ny, nlay, nsteps, resident,
+!   physics_on; float64 dx, dy, gravity, f, c_div, c_vort; ak(nlay+1),
+!   bk(nlay+1); per step float64 time, dt and int32 radiation, physics,
+!   remap flags; then the halo'd arrays dp, pt, ud, vd, ts, gz, plev, pkap,
+!   net_flux, radius.
+!   output: int32 steps completed, int32 stable; dp, pt, ud, vd, net_flux, ts.
 program drop_in_host
    use iso_fortran_env, only: int32, real64
    use mod_kinds, only: ik, rk
@@ -23,14 +23,16 @@ program drop_in_host
    use mod_radiation, only: allocate_radiation_arrays, calculate_radiative_heating, &
                             radiation_halo_exchange
    use mod_remap, only: allocate_remapping_arrays, perform_vertical_remapping
+   use mod_physics, only: allocate_physics_arrays, physics_halo_exchange, &
+                          calculate_physics_tendencies, apply_physics_tendencies
    use atm_device, only: device_set_resident, device_check_stability, device_pull_all, device_stop
    implicit none
 
    character(len=512) :: fin, fout
-   integer(int32) :: hdr(5), nsteps, resident, n, done, stable_out
+   integer(int32) :: hdr(6), nsteps, resident, physics_on, n, done, stable_out
    real(real64) :: prm(6)
    real(real64), allocatable :: time(:), dt(:)
-   integer(int32), allocatable :: rad(:), remap(:)
+   integer(int32), allocatable :: rad(:), phys(:), remap(:)
    logical :: stable
    integer :: u
 
@@ -43,6 +45,7 @@ program drop_in_host
    config%nlay = hdr(3)
    nsteps = hdr(4)
    resident = hdr(5)
+   physics_on = hdr(6)
    read (u) prm
    config%dx = prm(1)
    config%dy = prm(2)
@@ -50,8 +53,8 @@ program drop_in_host
    call init_tiles()
    call allocate_fields()
    read (u) coord_Ak, coord_Bk
-   allocate (time(nsteps), dt(nsteps), rad(nsteps), remap(nsteps))
-   read (u) time, dt, rad, remap
+   allocate (time(nsteps), dt(nsteps), rad(nsteps), phys(nsteps), remap(nsteps))
+   read (u) time, dt, rad, phys, remap
    read (u) dp, pt, ud, vd, ts, gz, plev, pkap, net_flux, radius
    close (u)
 
@@ -61,6 +64,7 @@ program drop_in_host
    call halo_exchange(ts)
    call init_sw_dyn()
    call allocate_radiation_arrays()
+   call allocate_physics_arrays()
    call allocate_remapping_arrays()
    if (resident /= 0) call device_set_resident(.true.)
 
@@ -71,11 +75,14 @@ program drop_in_host
          call calculate_radiative_heating(time(n))
          call radiation_halo_exchange()
       end if
+      if (phys(n) /= 0) call calculate_physics_tendencies()
       call cgrid_dynamics_step(dt(n)/2.)
       call cgrid_halo_exchange()
       call dgrid_dynamics_step(dt(n))
+      if (physics_on /= 0) call apply_physics_tendencies(dt(n))
       if (remap(n) /= 0) call perform_vertical_remapping()
       call dgrid_halo_exchange()
+      if (physics_on /= 0) call physics_halo_exchange()
       if (resident /= 0) then
          stable = device_check_stability()
       else
@@ -90,7 +97,7 @@ program drop_in_host
    stable_out = merge(1, 0, stable)
    open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
    write (u) done, stable_out
-   write (u) dp, pt, ud, vd, net_flux
+   write (u) dp, pt, ud, vd, net_flux, ts
    close (u)
    print '(a,i0,a)', 'drop_in_host: ', done, ' steps'
 end program drop_in_host

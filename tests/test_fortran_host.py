@@ -1,13 +1,14 @@
 """The Fortran drop-in: a Fortran host calling the library through the
 modules with the reference's names (atmosphere-model_amd/fortran:
-mod_sw_dyn, mod_remap, mod_radiation, mod_sync over atm_c_api).
+mod_sw_dyn, mod_remap, mod_radiation, mod_physics, mod_sync over atm_c_api).
 
 tests/fortran/drop_in_host runs run_model's call order
 (src/mod_model.f90:115-193) on a state the oracle initialised, with the
-step schedule (dt, radiation and remap steps) the host driver's
+step schedule (dt, radiation, physics and remap steps) the host driver's
 calculate_dt produced, in both coherence modes: mirrored (every call moves
 the host arrays) and resident (the state stays on the device).  Its final
-dp, pt, ud, vd and net_flux must match the oracle's to 1e-12 norm-relative.
+dp, pt, ud, vd, net_flux and ts must match the oracle's to 1e-12
+norm-relative.
 """
 import os
 import subprocess
@@ -32,11 +33,15 @@ class _Recorder:
 
     def __init__(self, inner):
         self._inner = inner
-        self.rad, self.dt, self.remap = [], [], []
+        self.rad, self.dt, self.remap, self.phys = [], [], [], []
 
     def calculate_radiative_heating(self, time):
         self.rad.append((len(self.dt), time))
         return self._inner.calculate_radiative_heating(time)
+
+    def calculate_physics_tendencies(self):
+        self.phys.append(len(self.dt))
+        return self._inner.calculate_physics_tendencies()
 
     def cgrid_dynamics_step(self, half_dt):
         self.dt.append(2.0 * half_dt)
@@ -62,21 +67,29 @@ def _write_input(path, o, dx, cfg, rec, resident, params):
         rad[step] = 1
     remap = np.zeros(n, np.int32)
     remap[rec.remap] = 1
+    phys = np.zeros(n, np.int32)
+    phys[rec.phys] = 1
     with open(path, "wb") as fh:
-        np.array([o.nx, o.ny, o.nlay, n, resident], np.int32).tofile(fh)
+        np.array([o.nx, o.ny, o.nlay, n, resident, int(cfg.physics_on)], np.int32).tofile(fh)
         np.array([dx, dx, *params], np.float64).tofile(fh)
         o.field(O.AK).astype(np.float64).tofile(fh)
         o.field(O.BK).astype(np.float64).tofile(fh)
         times.tofile(fh)
         np.asarray(rec.dt, np.float64).tofile(fh)
         rad.tofile(fh)
+        phys.tofile(fh)
         remap.tofile(fh)
 
 
-def _run_case(tmp_path, nx, ny, nlay, dx, steps, radiation, resident):
+def _run_case(tmp_path, nx, ny, nlay, dx, steps, radiation, physics, resident):
     ic = make_initial_state(nx, ny, nlay, dx)
-    cfg = Config(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=240.0, dt_physics=1e12,
-                 dt_remap=180.0, radiation_on=radiation, physics_on=False)
+    rng = np.random.default_rng(4)
+    pt = ic["pt"].copy()
+    mask = rng.random(pt.shape) < 0.2  # some statically unstable interfaces
+    pt[mask] *= 1.0 + rng.uniform(0.01, 0.04, mask.sum())
+    ic["pt"] = pt
+    cfg = Config(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=240.0, dt_physics=120.0,
+                 dt_remap=180.0, radiation_on=radiation, physics_on=physics)
     params = (9.807, 1e-5, 0.15, 0.005)
     o = O.Oracle(nx, ny, nlay, dx, dx, gravity=params[0], coriolis=params[1],
                  divergence_damping=params[2], vorticity_damping=params[3])
@@ -97,13 +110,13 @@ def _run_case(tmp_path, nx, ny, nlay, dx, steps, radiation, resident):
     with open(out, "rb") as fh:
         done, stable = np.fromfile(fh, np.int32, 2)
         got = {}
-        for f in (O.DP, O.PT, O.UD, O.VD, O.NET_FLUX):
+        for f in (O.DP, O.PT, O.UD, O.VD, O.NET_FLUX, O.TS):
             shape = o.field(f).shape
             got[f] = np.fromfile(fh, np.float64, int(np.prod(shape))).reshape(shape)
     assert done == steps and stable == 1
     h = O.HALO
     worst = {}
-    for f in (O.DP, O.PT, O.UD, O.VD, O.NET_FLUX):
+    for f in (O.DP, O.PT, O.UD, O.VD, O.NET_FLUX, O.TS):
         a = got[f][..., h:h + ny, h:h + nx]
         b = o.interior(f)
         worst[f] = norm_rel(a, b)
@@ -128,7 +141,7 @@ def test_host_without_gpu_reports_fatal(tmp_path):
     o = O.Oracle(16, 16, 6, 100e3, 100e3)
     inp = str(tmp_path / "in.bin")
     with open(inp, "wb") as fh:
-        np.array([16, 16, 6, 0, 0], np.int32).tofile(fh)
+        np.array([16, 16, 6, 0, 0, 0], np.int32).tofile(fh)
         np.array([100e3, 100e3, 9.807, 1e-5, 0.15, 0.005], np.float64).tofile(fh)
         np.zeros(2 * 7, np.float64).tofile(fh)
         for f in FIELDS_IN:
@@ -143,8 +156,9 @@ def test_host_without_gpu_reports_fatal(tmp_path):
 def test_fortran_drop_in_matches_oracle(gpu_available, tmp_path, resident):
     if not os.path.exists(HOST):
         pytest.skip("test host not built")
-    print(f"fortran drop-in: 32x32x12, 8 steps, radiation + remap, resident={resident}", flush=True)
-    worst, rec = _run_case(tmp_path, 32, 32, 12, 100e3, 8, True, resident)
-    assert rec.remap and len(rec.rad) >= 2  # the schedule exercised both
+    print(f"fortran drop-in: 32x32x12, 8 steps, radiation + physics + remap, resident={resident}",
+          flush=True)
+    worst, rec = _run_case(tmp_path, 32, 32, 12, 100e3, 8, True, True, resident)
+    assert rec.remap and len(rec.rad) >= 2 and len(rec.phys) >= 2  # the schedule exercised all
     for f, e in worst.items():
         assert e <= 1e-12, (f, e)
