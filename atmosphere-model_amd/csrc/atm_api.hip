@@ -58,7 +58,7 @@ static void release(atm_ctx *c) {
     for (int f = 0; f < 4; f++)
         for (int b = 0; b < 2; b++)
             if (c->pbuf[f][b]) { (void)hipFree(c->pbuf[f][b]); c->pbuf[f][b] = nullptr; }
-    double **fz[] = {&c->uc2, &c->vc2, &c->heatd};
+    double **fz[] = {&c->uc2, &c->vc2, &c->heatd, &c->plk};
     for (auto p : fz)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     for (auto &t : c->timers) {
@@ -157,7 +157,7 @@ int atm_create(const atm_grid *grid, const atm_params *params, int precision, at
     }
     if (c->fused) {
         double **fz[] = {&c->pbuf[0][1], &c->pbuf[1][1], &c->pbuf[2][1], &c->pbuf[3][1], &c->uc2, &c->vc2,
-                         &c->heatd};
+                         &c->heatd, &c->plk};
         for (auto q : fz) {
             rc = dalloc(c, q, p * L);
             if (rc) return fail(rc);
@@ -231,6 +231,7 @@ static int xfer(atm_ctx *c, int id, void *dst, const void *src, bool to_dev, hip
     if (!(to_dev ? src : dst)) return atm_fail(c, ATM_ERR_ARGUMENT, "null array");
     const Geo &g = c->g;
     size_t bytes = sizeof(double) * g.plane() * nk;
+    if (to_dev && id == ATM_DP) c->plk_valid = false;
     if (split_winds(c, id)) {
         double *ring = c->pbuf[id == ATM_UD ? 2 : 3][1 - c->cur];
         if (to_dev) {
@@ -332,11 +333,13 @@ int atm_dgrid_step(atm_ctx *c, double dt) {
         FusedBufs cur = bufs_of(c, c->cur), nxt = bufs_of(c, 1 - c->cur);
         int rc = atm_fused_dgrid(c, cur, nxt, dt);
         if (rc) return rc;
+        c->plk_valid = true;
         c->cur = 1 - c->cur;
         atm_point_current(c);
         c->after_d = true;
         return ATM_OK;
     }
+    c->plk_valid = false;
     return atm_ref_dgrid(c, dt);
 }
 
@@ -365,6 +368,7 @@ int atm_remap_init(atm_ctx *c) {
 int atm_remap(atm_ctx *c) {
     CHECK_CTX(c);
     if (!c->remap_init) return atm_fail(c, ATM_ERR_STATE, "atm_remap_init first");
+    c->plk_valid = false;
     if (c->fused && c->after_d) {
         // read the D step's state (current copy) with the wind ring of the
         // previous copy; write the remapped interior into the previous copy
@@ -457,6 +461,7 @@ int atm_set_path(atm_ctx *c, int path) {
     CHECK_CTX(c);
     if (path != ATM_PATH_FUSED && path != ATM_PATH_REFERENCE) return atm_fail(c, ATM_ERR_ARGUMENT, "bad path");
     if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "finish the step (atm_dgrid_halo) first");
+    c->plk_valid = false;
     if (path == ATM_PATH_FUSED) {
         if (!atm_fused_supported(c->g) || !c->uc2)
             return atm_fail(c, ATM_ERR_ARGUMENT, "fused path needs nx % 32 == 0 and ny % 8 == 0");

@@ -181,6 +181,11 @@ struct atm_ctx {
     int cur = 0;
     bool after_d = false;  // a fused D step ran and its halo fill is pending
     double *uc2 = nullptr, *vc2 = nullptr, *heatd = nullptr;  // fused-path C-step outputs
+    // fused path: playkap of the current dp on the interior (the C step's
+    // heating rate needs it); left by the D step's scan, recomputed by the
+    // C step when dp changed since (remap, put)
+    double *plk = nullptr;
+    bool plk_valid = false;
     bool timing = false;
     AtmTimer timers[ATM_T_COUNT];
     double *ak = nullptr, *bk = nullptr;  // device, nlay+1

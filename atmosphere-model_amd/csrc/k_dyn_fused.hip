@@ -53,6 +53,7 @@ template <int VAR, bool CGRID>
 __global__ __launch_bounds__(NT) void k_pair(Geo g, DynCoef c, const double *__restrict__ dp,
                                              const double *__restrict__ pt, const double *__restrict__ uw,
                                              const double *__restrict__ vw, const double *__restrict__ heat,
+                                             const double *__restrict__ plk, const double *__restrict__ nf,
                                              double *__restrict__ dpo, double *__restrict__ pto,
                                              double *__restrict__ uco, double *__restrict__ vco,
                                              double *__restrict__ ua, double *__restrict__ va) {
@@ -167,11 +168,20 @@ __global__ __launch_bounds__(NT) void k_pair(Geo g, DynCoef c, const double *__r
             __syncthreads();
         } else {
             const double ptv = Q(tx, ty);
+            // heating rate: the C step forms it from dp and playkap(dp)
+            // (cgrid 99-106); the D step reads the C step's (dgrid 457-465)
+            double hr;
+            if (CGRID) {
+                const size_t xk = g.ix(I0 + tx, J0 + ty, K);
+                hr = (nf[xk + g.plane()] - nf[xk]) / (ATM_CP_DRY * dpv / c.gravity) / plk[xk];
+            } else {
+                hr = G(heat, tx, ty);
+            }
             double p = dpv * ptv - (sFX[RFX::at(tx + 1, ty)] * fx1 - sFX[RFX::at(tx, ty)] * fx0 +
                                     c.vx * ((lxp - l0) * fx1 - (l0 - lxm) * fx0));
             p = (p - (sFY[RFY::at(tx, ty + 1)] * fy1 - sFY[RFY::at(tx, ty)] * fy0 +
                       c.vy * ((lyp - l0) * fy1 - (l0 - lym) * fy0)) +
-                 c.dt * dpv * G(heat, tx, ty)) /
+                 c.dt * dpv * hr) /
                 dpcv;
             G(pto, tx, ty) = p;
         }
@@ -491,60 +501,56 @@ __global__ void __launch_bounds__(NT) k_bcorners(Geo g, const double *__restrict
 // ---------------------------------------------------------------------------
 #define SCAN_CH 4
 
-// heating rate of the C step from dp (cgrid 89-106) on interior columns
-__global__ void __launch_bounds__(64) k_scan_heat(Geo g, double ptop, double pktop, double gravity,
-                                                  const double *__restrict__ src, const double *__restrict__ nf,
-                                                  double *__restrict__ heat) {
+// playkap of the C step from dp (cgrid 89-101) on interior columns; C1
+// forms the heating rate from it (99-106).  Needed only when dp changed
+// after the last D step's scan (which leaves the same values, see below).
+__global__ void __launch_bounds__(64) k_scan_plk(Geo g, double ptop, double pktop,
+                                                 const double *__restrict__ src, double *__restrict__ plk) {
     const int I = ATM_HALO + blockIdx.x * blockDim.x + threadIdx.x;
     const int J = ATM_HALO + blockIdx.y;
     if (I >= ATM_HALO + g.nx) return;
     const size_t pl = g.plane(), c0 = g.ix2(I, J);
     double p = ptop, pk1 = pktop;
-    double nf1 = nf[c0 + (size_t)g.L * pl];
-    auto level = [&](int K, double s, double pk, double nf0) {
-        const double plk = (pk1 - pk) / log(pk1 / pk);
-        heat[c0 + (size_t)K * pl] = (nf1 - nf0) / (ATM_CP_DRY * s / gravity) / plk;
+    auto level = [&](int K, double pk) {
+        plk[c0 + (size_t)K * pl] = (pk1 - pk) / log(pk1 / pk);
         pk1 = pk;
-        nf1 = nf0;
     };
     int K = g.L - 1;
     for (; K >= SCAN_CH - 1; K -= SCAN_CH) {
-        double s[SCAN_CH], pv[SCAN_CH], pk[SCAN_CH], nf0[SCAN_CH];
+        double pv[SCAN_CH], pk[SCAN_CH];
 #pragma unroll
         for (int u = 0; u < SCAN_CH; u++) {
-            s[u] = src[c0 + (size_t)(K - u) * pl];
-            nf0[u] = nf[c0 + (size_t)(K - u) * pl];
-        }
-#pragma unroll
-        for (int u = 0; u < SCAN_CH; u++) {
-            p = p + s[u];
+            p = p + src[c0 + (size_t)(K - u) * pl];
             pv[u] = p;
         }
 #pragma unroll
         for (int u = 0; u < SCAN_CH; u++) pk[u] = pow(pv[u], ATM_KAPPA);
 #pragma unroll
-        for (int u = 0; u < SCAN_CH; u++) level(K - u, s[u], pk[u], nf0[u]);
+        for (int u = 0; u < SCAN_CH; u++) level(K - u, pk[u]);
     }
     for (; K >= 0; K--) {
-        const double s = src[c0 + (size_t)K * pl];
-        p = p + s;
-        level(K, s, pow(p, ATM_KAPPA), nf[c0 + (size_t)K * pl]);
+        p = p + src[c0 + (size_t)K * pl];
+        level(K, pow(p, ATM_KAPPA));
     }
 }
 
 // pkap and gz from (src, th) over all columns (cgrid 276-292, dgrid 433-437 +
-// 606-626); optional plev output and optional heating rate of the D step.
+// 606-626); optional plev output, optional heating rate of the D step and
+// optional playkap on the interior.  After the D step the latter is what
+// the next C step's scan would compute from the same dp, level by level
+// with the same operations (cgrid 89-101).
 __global__ void __launch_bounds__(64) k_scan_pg(Geo g, double ptop, double pktop, double gravity,
                                                 const double *__restrict__ src, const double *__restrict__ th,
                                                 const double *__restrict__ nf, double *__restrict__ plev,
                                                 double *__restrict__ pkap, double *__restrict__ gz,
-                                                double *__restrict__ heat) {
+                                                double *__restrict__ heat, double *__restrict__ plk_out) {
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     const int J = blockIdx.y;
     if (I >= g.NI) return;
     const size_t pl = g.plane(), c0 = g.ix2(I, J);
     const bool interior = I >= ATM_HALO && I < ATM_HALO + g.nx && J >= ATM_HALO && J < ATM_HALO + g.ny;
     const bool want_heat = heat && interior;
+    const bool want_plk = plk_out && interior;
     double p = ptop, pk1 = pktop;
     double nf1 = want_heat ? nf[c0 + (size_t)g.L * pl] : 0.;
     auto level = [&](int K, double s, double pv, double pk) {
@@ -557,6 +563,7 @@ __global__ void __launch_bounds__(64) k_scan_pg(Geo g, double ptop, double pktop
             heat[x] = (nf1 - nf0) / (ATM_CP_DRY * s / gravity) / plk;
             nf1 = nf0;
         }
+        if (want_plk) plk_out[x] = (pk1 - pk) / log(pk1 / pk);
         pk1 = pk;
     };
     int K = g.L - 1;
@@ -604,13 +611,17 @@ int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
     c->coef.dtdy = dt * c->coef.rdy;
     const DynCoef &k = c->coef;
     const double pktop = c->pktop;
-    atm_timer_begin(c, ATM_T_SCAN);
-    hipLaunchKernelGGL(k_scan_heat, dim3((g.nx + 63) / 64, g.ny), dim3(64), 0, c->stream, g, c->ak0, pktop,
-                       k.gravity, b.dp, c->net_flux, c->heat);
-    atm_timer_end(c, ATM_T_SCAN);
+    if (!c->plk_valid) {
+        atm_timer_begin(c, ATM_T_SCAN);
+        hipLaunchKernelGGL(k_scan_plk, dim3((g.nx + 63) / 64, g.ny), dim3(64), 0, c->stream, g, c->ak0, pktop, b.dp,
+                           c->plk);
+        atm_timer_end(c, ATM_T_SCAN);
+        c->plk_valid = true;
+    }
     dim3 grid(g.nx / TX, g.ny / TY, g.L), blk(NT);
     atm_timer_begin(c, ATM_T_C1);
-    hipLaunchKernelGGL((k_pair<V_PPM_U, true>), grid, blk, 0, c->stream, g, k, b.dp, b.pt, b.ud, b.vd, c->heat,
+    hipLaunchKernelGGL((k_pair<V_PPM_U, true>), grid, blk, 0, c->stream, g, k, b.dp, b.pt, b.ud, b.vd,
+                       (const double *)nullptr, c->plk, c->net_flux,
                        c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va);
     atm_timer_end(c, ATM_T_C1);
     double *fill[] = {c->dpc, c->ptc, c->uc, c->vc};
@@ -625,7 +636,8 @@ int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
     if (rc) return rc;
     atm_timer_begin(c, ATM_T_SCAN);
     hipLaunchKernelGGL(k_scan_pg, dim3((g.NI + 63) / 64, g.NJ), dim3(64), 0, c->stream, g, c->ak0, pktop, k.gravity,
-                       c->dpc, c->ptc, c->net_flux, (double *)nullptr, c->pkap, c->gz, c->heatd);
+                       c->dpc, c->ptc, c->net_flux, (double *)nullptr, c->pkap, c->gz, c->heatd,
+                       (double *)nullptr);
     atm_timer_end(c, ATM_T_SCAN);
     atm_timer_begin(c, ATM_T_C2);
     hipLaunchKernelGGL(k_c2, grid, blk, 0, c->stream, g, k, c->uc, c->vc, b.ud, b.vd, c->pkap, c->gz, c->ua, c->va,
@@ -654,7 +666,8 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
     dim3 grid(g.nx / TX, g.ny / TY, g.L), blk(NT);
     atm_timer_begin(c, ATM_T_D1);
     hipLaunchKernelGGL((k_pair<V_PPM_NN, false>), grid, blk, 0, c->stream, g, k, cur.dp, cur.pt, c->uc2, c->vc2,
-                       c->heatd, nxt.dp, nxt.pt, (double *)nullptr, (double *)nullptr, (double *)nullptr,
+                       c->heatd, (const double *)nullptr, (const double *)nullptr, nxt.dp, nxt.pt,
+                       (double *)nullptr, (double *)nullptr, (double *)nullptr,
                        (double *)nullptr);
     atm_timer_end(c, ATM_T_D1);
     atm_timer_begin(c, ATM_T_HALO);
@@ -664,7 +677,8 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
     if (rc) return rc;
     atm_timer_begin(c, ATM_T_SCAN);
     hipLaunchKernelGGL(k_scan_pg, dim3((g.NI + 63) / 64, g.NJ), dim3(64), 0, c->stream, g, c->ak0, c->pktop,
-                       k.gravity, nxt.dp, nxt.pt, c->net_flux, c->plev, c->pkap, c->gz, (double *)nullptr);
+                       k.gravity, nxt.dp, nxt.pt, c->net_flux, c->plev, c->pkap, c->gz, (double *)nullptr,
+                       c->plk);
     hipLaunchKernelGGL(k_bcorners, dim3((g.nx + TX) / TX, (g.ny + TY) / TY, g.L), blk, 0, c->stream, g, c->pkap,
                        c->gz, c->pkapb, c->gzb);
     atm_timer_end(c, ATM_T_SCAN);
