@@ -31,6 +31,7 @@ constexpr int TX = ATM_TILE_X;
 constexpr int TY = ATM_TILE_Y;
 constexpr int NT = 256;
 static_assert(TX * TY == NT, "one output cell per thread");
+static_assert((NT & (NT - 1)) == 0 && NT % 64 == 0, "rounds rotate by whole wavefronts");
 
 template <int X0, int X1, int Y0, int Y1>
 struct Reg {
@@ -39,8 +40,13 @@ struct Reg {
     __device__ static int at(int x, int y) { return (y - Y0) * w + (x - X0); }
 };
 
-#define FOR_REG(R)                                                   \
-    for (int t_ = threadIdx.x; t_ < R::n; t_ += NT) {                \
+// The items of a region are dealt to the block's threads in rounds of NT;
+// the last round of a region is usually partial.  Each region starts its
+// rounds at a different wavefront (__COUNTER__ rotation), so the partial
+// rounds are spread over the block's four wavefronts (and SIMDs) instead
+// of all landing on wavefront 0.
+#define FOR_REG(R)                                                                         \
+    for (int t_ = (threadIdx.x + 64 * (__COUNTER__ & 3)) & (NT - 1); t_ < R::n; t_ += NT) { \
         const int x = R::x0 + t_ % R::w, y = R::y0 + t_ / R::w;
 #define END_FOR }
 
