@@ -65,13 +65,13 @@ __global__ __launch_bounds__(NT) void k_pair(Geo g, DynCoef c, const double *__r
                                              double *__restrict__ ua, double *__restrict__ va) {
     using RUC = Reg<0, TX + 1, -3, TY + 3>;
     using RVC = Reg<-3, TX + 3, 0, TY + 1>;
-    using RDX = Reg<0, TX, -3, TY + 3>;  // denx and tmp of the inner x step
-    using RDY = Reg<-3, TX + 3, 0, TY>;  // deny and tmp of the inner y step
+    using RDX = Reg<0, TX, -3, TY + 3>;  // tmp of the inner x step (denominators recomputed per pass)
+    using RDY = Reg<-3, TX + 3, 0, TY>;  // tmp of the inner y step
     using RQ = Reg<-3, TX + 3, -3, TY + 3>;
     using RL = Reg<-1, TX + 1, -1, TY + 1>;
     using RFX = Reg<0, TX + 1, 0, TY>;
     using RFY = Reg<0, TX, 0, TY + 1>;
-    __shared__ double sUC[RUC::n], sVC[RVC::n], sDX[RDX::n], sDY[RDY::n], sQ[RQ::n];
+    __shared__ double sUC[RUC::n], sVC[RVC::n], sQ[RQ::n];
     __shared__ double sTX[RDY::n], sTY[RDX::n], sL[RL::n];
     __shared__ double sDPFX[RFX::n], sDPFY[RFY::n], sFX[RFX::n], sFY[RFY::n];
 
@@ -110,13 +110,7 @@ __global__ __launch_bounds__(NT) void k_pair(Geo g, DynCoef c, const double *__r
         // inner y step (tmp of the outer x step), with deny; inner x step with denx; Laplacian
         FOR_REG(RDY)
         const double v0 = VC(x, y), v1 = VC(x, y + 1);
-        double dny;
-        if (pass == 0) {
-            dny = 1. / (1. - c.dtdy * (v1 - v0));
-            sDY[RDY::at(x, y)] = dny;
-        } else {
-            dny = sDY[RDY::at(x, y)];
-        }
+        const double dny = 1. / (1. - c.dtdy * (v1 - v0));
         const double q0 = Q(x, y);
         const double f0 = pcm_face(Q(x, y - 1), q0, c.dtdx * v0);  // Q2: cy = dt/dx * vc
         const double f1 = pcm_face(q0, Q(x, y + 1), c.dtdx * v1);
@@ -124,13 +118,7 @@ __global__ __launch_bounds__(NT) void k_pair(Geo g, DynCoef c, const double *__r
         END_FOR
         FOR_REG(RDX)
         const double u0 = UC(x, y), u1 = UC(x + 1, y);
-        double dnx;
-        if (pass == 0) {
-            dnx = 1. / (1. - c.dtdx * (u1 - u0));
-            sDX[RDX::at(x, y)] = dnx;
-        } else {
-            dnx = sDX[RDX::at(x, y)];
-        }
+        const double dnx = 1. / (1. - c.dtdx * (u1 - u0));
         const double q0 = Q(x, y);
         const double f0 = pcm_face(Q(x - 1, y), q0, c.dtdx * u0);
         const double f1 = pcm_face(q0, Q(x + 1, y), c.dtdx * u1);
