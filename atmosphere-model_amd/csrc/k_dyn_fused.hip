@@ -73,7 +73,9 @@ __global__ __launch_bounds__(NT) void k_pair(Geo g, DynCoef c, const double *__r
     using RFY = Reg<0, TX, 0, TY + 1>;
     __shared__ double sUC[RUC::n], sVC[RVC::n], sQ[RQ::n];
     __shared__ double sTX[RDY::n], sTY[RDX::n], sL[RL::n];
-    __shared__ double sDPFX[RFX::n], sDPFY[RFY::n], sFX[RFX::n], sFY[RFY::n];
+    // face values: the dp mass fluxes in the dp pass (each thread keeps the
+    // four around its cell in registers), the pt face values in the pt pass
+    __shared__ double sFX[RFX::n], sFY[RFY::n];
 
     int bx, by, K;
     xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
@@ -106,6 +108,7 @@ __global__ __launch_bounds__(NT) void k_pair(Geo g, DynCoef c, const double *__r
 
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     double dpv = 0., dpcv = 0.;
+    double fx0 = 0., fx1 = 0., fy0 = 0., fy1 = 0.;  // dp mass fluxes on the cell's faces
     for (int pass = 0; pass < 2; pass++) {
         // inner y step (tmp of the outer x step), with deny; inner x step with denx; Laplacian
         FOR_REG(RDY)
@@ -135,23 +138,23 @@ __global__ __launch_bounds__(NT) void k_pair(Geo g, DynCoef c, const double *__r
         const double cx = c.dtdx * UC(x, y);
         const double f = ppm_face<VAR>(T(-3), T(-2), T(-1), T(0), T(1), T(2), cx);
 #undef T
-        if (pass == 0) sDPFX[RFX::at(x, y)] = f * cx;
-        else sFX[RFX::at(x, y)] = f;
+        sFX[RFX::at(x, y)] = pass == 0 ? f * cx : f;
         END_FOR
         FOR_REG(RFY)
 #define T(m) sTY[RDX::at(x, y + (m))]
         const double cy = c.dtdx * VC(x, y);  // Q2
         const double f = ppm_face<VAR>(T(-3), T(-2), T(-1), T(0), T(1), T(2), cy);
 #undef T
-        if (pass == 0) sDPFY[RFY::at(x, y)] = f * cy;
-        else sFY[RFY::at(x, y)] = f;
+        sFY[RFY::at(x, y)] = pass == 0 ? f * cy : f;
         END_FOR
         __syncthreads();
         const double l0 = LP(tx, ty);
         const double lxp = LP(tx + 1, ty), lxm = LP(tx - 1, ty), lyp = LP(tx, ty + 1), lym = LP(tx, ty - 1);
-        const double fx0 = sDPFX[RFX::at(tx, ty)], fx1 = sDPFX[RFX::at(tx + 1, ty)];
-        const double fy0 = sDPFY[RFY::at(tx, ty)], fy1 = sDPFY[RFY::at(tx, ty + 1)];
         if (pass == 0) {
+            fx0 = sFX[RFX::at(tx, ty)];
+            fx1 = sFX[RFX::at(tx + 1, ty)];
+            fy0 = sFY[RFY::at(tx, ty)];
+            fy1 = sFY[RFY::at(tx, ty + 1)];
             dpv = Q(tx, ty);
             double d = dpv - (fx1 - fx0 + c.dtdx * c.vx * (lxp - 2. * l0 + lxm));
             d = d - (fy1 - fy0 + c.dtdy * c.vy * (lyp - 2. * l0 + lym));
@@ -318,7 +321,10 @@ __global__ __launch_bounds__(NT) void k_d2(Geo g, DynCoef c, const double *__res
     using RTU = Reg<0, TX, -3, TY + 3>;
     using RTW = Reg<-3, TX + 3, 0, TY>;
     using RLP = Reg<-1, TX, -1, TY>;
-    __shared__ double sU[RU::n], sV[RV::n], sKE[RKE::n], sW[RW::n], sDV[RDV::n], sD2[RD2::n], sD4[RKE::n];
+    __shared__ double sU[RU::n], sV[RV::n], sKE[RKE::n], sW[RW::n], sDV[RDV::n], sD2[RD2::n];
+    // del^4 divergence reuses the divergence's storage (dead once del^2 is formed)
+    static_assert(RKE::n <= RDV::n, "sD4 fits in sDV");
+    double *const sD4 = sDV;
     __shared__ double sTU[RTU::n], sTW[RTW::n], sL[RLP::n];
     int bx, by, K;
     xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
