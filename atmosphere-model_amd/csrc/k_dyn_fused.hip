@@ -56,7 +56,7 @@ struct Reg {
 // out; otherwise uc/vc of the C step are read (the D step).
 // ---------------------------------------------------------------------------
 template <int VAR, bool CGRID>
-__global__ __launch_bounds__(NT) void k_pair(Geo g, DynCoef c, const double *__restrict__ dp,
+__global__ __launch_bounds__(NT, 6) void k_pair(Geo g, DynCoef c, const double *__restrict__ dp,
                                              const double *__restrict__ pt, const double *__restrict__ uw,
                                              const double *__restrict__ vw, const double *__restrict__ heat,
                                              const double *__restrict__ plk, const double *__restrict__ nf,
