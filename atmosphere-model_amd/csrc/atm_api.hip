@@ -464,7 +464,8 @@ int atm_set_path(atm_ctx *c, int path) {
     c->plk_valid = false;
     if (path == ATM_PATH_FUSED) {
         if (!atm_fused_supported(c->g) || !c->uc2)
-            return atm_fail(c, ATM_ERR_ARGUMENT, "fused path needs nx % 32 == 0 and ny % 8 == 0");
+            return atm_fail(c, ATM_ERR_ARGUMENT, "fused path needs nx % " + std::to_string(ATM_TILE_X) + " == 0 and ny % " +
+                                                      std::to_string(ATM_TILE_Y) + " == 0");
         c->fused = true;
     } else {
         c->fused = false;
@@ -510,7 +511,7 @@ int atm_sync(atm_ctx *c) {
 }
 
 const char *atm_build_info(void) {
-    return "atmosphere-model_amd gfx950 FP64: fused LDS-tiled dynamics (32x8 tiles) with the "
+    return "atmosphere-model_amd gfx950 FP64: fused LDS-tiled dynamics (32x16 tiles) with the "
            "reference-structured kernels as the alternative path; -O3 -ffp-contract=off";
 }
 

@@ -3,9 +3,11 @@
 
 #include "atm_common.hpp"
 
-// output tile of one workgroup (one cell per thread)
+// output tiles of one workgroup (one cell per thread): 32 x 8 for the dp/pt
+// kernels, 32 x 16 for the wind and corner kernels
 #define ATM_TILE_X 32
-#define ATM_TILE_Y 8
+#define ATM_TILE_Y_PAIR 8
+#define ATM_TILE_Y 16
 
 struct FusedBufs {
     double *dp, *pt, *ud, *vd;

@@ -2,7 +2,7 @@
 //
 // The C-grid and D-grid steps of src/mod_sw_dyn.f90 as four tile kernels
 // per step (C1, C2, D1, D2) plus column scans, instead of one kernel per
-// reference loop nest.  A workgroup of 256 threads owns a TX x TY tile of
+// reference loop nest.  A workgroup of TX x TY threads owns a TX x TY tile of
 // one level; every intermediate of the reference's level loop (C winds,
 // denominators, inner PCM steps, Laplacians, outer PPM fluxes, vorticity,
 // kinetic energy, divergence damping, B-grid pressure gradient) lives in LDS
@@ -28,10 +28,16 @@
 namespace {
 
 constexpr int TX = ATM_TILE_X;
-constexpr int TY = ATM_TILE_Y;
-constexpr int NT = 256;
-static_assert(TX * TY == NT, "one output cell per thread");
-static_assert((NT & (NT - 1)) == 0 && NT % 64 == 0, "rounds rotate by whole wavefronts");
+// Tile rows per kernel: the dp/pt kernels (the heaviest in registers) run
+// 32 x 8 tiles, the wind kernels and the corner kernel 32 x 16 (less halo
+// recomputation per cell).  Each kernel declares its own TY and NT (one
+// thread per output cell).
+constexpr int TY_PAIR = ATM_TILE_Y_PAIR;
+constexpr int TY_WIND = ATM_TILE_Y;
+constexpr int NT_PAIR = TX * TY_PAIR, NT_WIND = TX * TY_WIND;
+static_assert((NT_PAIR & (NT_PAIR - 1)) == 0 && NT_PAIR % 64 == 0 && (NT_WIND & (NT_WIND - 1)) == 0 &&
+                  NT_WIND % 64 == 0,
+              "rounds rotate by whole wavefronts");
 
 template <int X0, int X1, int Y0, int Y1>
 struct Reg {
@@ -43,10 +49,10 @@ struct Reg {
 // The items of a region are dealt to the block's threads in rounds of NT;
 // the last round of a region is usually partial.  Each region starts its
 // rounds at a different wavefront (__COUNTER__ rotation), so the partial
-// rounds are spread over the block's four wavefronts (and SIMDs) instead
+// rounds are spread over the block's wavefronts (and SIMDs) instead
 // of all landing on wavefront 0.
 #define FOR_REG(R)                                                                         \
-    for (int t_ = (threadIdx.x + 64 * (__COUNTER__ & 3)) & (NT - 1); t_ < R::n; t_ += NT) { \
+    for (int t_ = (threadIdx.x + 64 * (__COUNTER__ % (NT / 64))) & (NT - 1); t_ < R::n; t_ += NT) { \
         const int x = R::x0 + t_ % R::w, y = R::y0 + t_ / R::w;
 #define END_FOR }
 
@@ -56,13 +62,14 @@ struct Reg {
 // out; otherwise uc/vc of the C step are read (the D step).
 // ---------------------------------------------------------------------------
 template <int VAR, bool CGRID>
-__global__ __launch_bounds__(NT, 6) void k_pair(Geo g, DynCoef c, const double *__restrict__ dp,
+__global__ __launch_bounds__(NT_PAIR, 6) void k_pair(Geo g, DynCoef c, const double *__restrict__ dp,
                                              const double *__restrict__ pt, const double *__restrict__ uw,
                                              const double *__restrict__ vw, const double *__restrict__ heat,
                                              const double *__restrict__ plk, const double *__restrict__ nf,
                                              double *__restrict__ dpo, double *__restrict__ pto,
                                              double *__restrict__ uco, double *__restrict__ vco,
                                              double *__restrict__ ua, double *__restrict__ va) {
+    constexpr int TY = TY_PAIR, NT = NT_PAIR;
     using RUC = Reg<0, TX + 1, -3, TY + 3>;
     using RVC = Reg<-3, TX + 3, 0, TY + 1>;
     using RDX = Reg<0, TX, -3, TY + 3>;  // tmp of the inner x step (denominators recomputed per pass)
@@ -204,12 +211,13 @@ __global__ __launch_bounds__(NT, 6) void k_pair(Geo g, DynCoef c, const double *
 // C2: kinetic energy, vorticity, pressure gradient and the uc/vc update of
 // one level (cgrid 294-414).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_c2(Geo g, DynCoef c, const double *__restrict__ uc1,
+__global__ __launch_bounds__(NT_WIND) void k_c2(Geo g, DynCoef c, const double *__restrict__ uc1,
                                            const double *__restrict__ vc1, const double *__restrict__ ud,
                                            const double *__restrict__ vd, const double *__restrict__ pkap,
                                            const double *__restrict__ gz, const double *__restrict__ ua,
                                            const double *__restrict__ va, double *__restrict__ uco,
                                            double *__restrict__ vco) {
+    constexpr int TY = TY_WIND, NT = NT_WIND;
     using RB = Reg<-3, TX + 3, -3, TY + 3>;
     using RW = Reg<-2, TX + 3, -2, TY + 3>;
     using RK = Reg<-1, TX, -1, TY>;
@@ -307,11 +315,12 @@ __global__ __launch_bounds__(NT) void k_c2(Geo g, DynCoef c, const double *__res
 // D2: kinetic energy, vorticity, divergence damping, B-grid pressure
 // gradient and the ud/vd update of one level (dgrid 606-771).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_d2(Geo g, DynCoef c, const double *__restrict__ ud,
+__global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *__restrict__ ud,
                                            const double *__restrict__ vd, const double *__restrict__ uc,
                                            const double *__restrict__ vc, const double *__restrict__ pkapb,
                                            const double *__restrict__ gzb, double *__restrict__ udo,
                                            double *__restrict__ vdo) {
+    constexpr int TY = TY_WIND, NT = NT_WIND;
     using RU = Reg<-3, TX + 3, -3, TY + 4>;
     using RV = Reg<-3, TX + 4, -3, TY + 3>;
     using RKE = Reg<0, TX + 1, 0, TY + 1>;
@@ -445,11 +454,12 @@ __global__ __launch_bounds__(NT) void k_d2(Geo g, DynCoef c, const double *__res
 // the tile reads).  pkapb(nlay+1) and gzb(1) keep their init_sw_dyn values.
 // interpolate_to_corners (921-956) is separable: each corner combines four
 // x-direction 4-point averages (rows j-2..j+1) and four y-direction ones
-// (columns i-2..i+1).  A 32 x 8 tile stages the field, forms each average
+// (columns i-2..i+1).  A TX x TY tile stages the field, forms each average
 // once in LDS and combines them in the reference's order.
-__global__ void __launch_bounds__(NT) k_bcorners(Geo g, const double *__restrict__ pkap,
+__global__ void __launch_bounds__(NT_WIND) k_bcorners(Geo g, const double *__restrict__ pkap,
                                                  const double *__restrict__ gz, double *__restrict__ pkapb,
                                                  double *__restrict__ gzb) {
+    constexpr int TY = TY_WIND, NT = NT_WIND;
     using RQ = Reg<-2, TX + 1, -2, TY + 1>;  // field
     using RAX = Reg<0, TX, -2, TY + 1>;      // x averages
     using RAY = Reg<-2, TX + 1, 0, TY>;      // y averages
@@ -598,7 +608,7 @@ __global__ void __launch_bounds__(64) k_scan_pg(Geo g, double ptop, double pktop
 
 }  // namespace
 
-bool atm_fused_supported(const Geo &g) { return g.nx % TX == 0 && g.ny % TY == 0; }
+bool atm_fused_supported(const Geo &g) { return g.nx % TX == 0 && g.ny % TY_PAIR == 0 && g.ny % TY_WIND == 0; }
 
 // ---------------------------------------------------------------------------
 // Step orchestration of the fused path.  Buffers: prognostic fields have two
@@ -618,7 +628,8 @@ int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
         atm_timer_end(c, ATM_T_SCAN);
         c->plk_valid = true;
     }
-    dim3 grid(g.nx / TX, g.ny / TY, g.L), blk(NT);
+    dim3 grid(g.nx / TX, g.ny / TY_PAIR, g.L), blk(NT_PAIR);
+    dim3 gridw(g.nx / TX, g.ny / TY_WIND, g.L), blkw(NT_WIND);
     atm_timer_begin(c, ATM_T_C1);
     hipLaunchKernelGGL((k_pair<V_PPM_U, true>), grid, blk, 0, c->stream, g, k, b.dp, b.pt, b.ud, b.vd,
                        (const double *)nullptr, c->plk, c->net_flux,
@@ -640,7 +651,7 @@ int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
                        (double *)nullptr);
     atm_timer_end(c, ATM_T_SCAN);
     atm_timer_begin(c, ATM_T_C2);
-    hipLaunchKernelGGL(k_c2, grid, blk, 0, c->stream, g, k, c->uc, c->vc, b.ud, b.vd, c->pkap, c->gz, c->ua, c->va,
+    hipLaunchKernelGGL(k_c2, gridw, blkw, 0, c->stream, g, k, c->uc, c->vc, b.ud, b.vd, c->pkap, c->gz, c->ua, c->va,
                        c->uc2, c->vc2);
     atm_timer_end(c, ATM_T_C2);
     ATM_HIP(c, hipGetLastError());
@@ -663,7 +674,8 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
     c->coef.dtdx = dt * c->coef.rdx;
     c->coef.dtdy = dt * c->coef.rdy;
     const DynCoef &k = c->coef;
-    dim3 grid(g.nx / TX, g.ny / TY, g.L), blk(NT);
+    dim3 grid(g.nx / TX, g.ny / TY_PAIR, g.L), blk(NT_PAIR);
+    dim3 gridw(g.nx / TX, g.ny / TY_WIND, g.L), blkw(NT_WIND);
     atm_timer_begin(c, ATM_T_D1);
     hipLaunchKernelGGL((k_pair<V_PPM_NN, false>), grid, blk, 0, c->stream, g, k, cur.dp, cur.pt, c->uc2, c->vc2,
                        c->heatd, (const double *)nullptr, (const double *)nullptr, nxt.dp, nxt.pt,
@@ -679,11 +691,11 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
     hipLaunchKernelGGL(k_scan_pg, dim3((g.NI + 63) / 64, g.NJ), dim3(64), 0, c->stream, g, c->ak0, c->pktop,
                        k.gravity, nxt.dp, nxt.pt, c->net_flux, c->plev, c->pkap, c->gz, (double *)nullptr,
                        c->plk);
-    hipLaunchKernelGGL(k_bcorners, dim3((g.nx + TX) / TX, (g.ny + TY) / TY, g.L), blk, 0, c->stream, g, c->pkap,
+    hipLaunchKernelGGL(k_bcorners, dim3((g.nx + TX) / TX, (g.ny + TY_WIND) / TY_WIND, g.L), blkw, 0, c->stream, g, c->pkap,
                        c->gz, c->pkapb, c->gzb);
     atm_timer_end(c, ATM_T_SCAN);
     atm_timer_begin(c, ATM_T_D2);
-    hipLaunchKernelGGL(k_d2, grid, blk, 0, c->stream, g, k, cur.ud, cur.vd, c->uc2, c->vc2, c->pkapb, c->gzb,
+    hipLaunchKernelGGL(k_d2, gridw, blkw, 0, c->stream, g, k, cur.ud, cur.vd, c->uc2, c->vc2, c->pkapb, c->gzb,
                        nxt.ud, nxt.vd);
     atm_timer_end(c, ATM_T_D2);
     ATM_HIP(c, hipGetLastError());

@@ -141,7 +141,7 @@ int atm_check_stability(atm_ctx *ctx, int *stable);
 
 /* Execution path of the dynamics (no counterpart in the reference; both
    give identical results).  ATM_PATH_FUSED (default where nx % 32 == 0 and
-   ny % 8 == 0): LDS-tiled kernels, four per step.  ATM_PATH_REFERENCE: one
+   ny % 16 == 0): LDS-tiled kernels, four per step.  ATM_PATH_REFERENCE: one
    kernel per reference loop nest, in place.  Settable between steps; the
    environment variable ATM_PATH=reference selects it at creation. */
 enum { ATM_PATH_FUSED = 0, ATM_PATH_REFERENCE = 1 };
