@@ -56,6 +56,13 @@ struct Reg {
         const int x = R::x0 + t_ % R::w, y = R::y0 + t_ / R::w;
 #define END_FOR }
 
+// Tile-relative global addressing: a base that is uniform over the
+// workgroup (level k, tile origin moved back by 4 rows and columns) plus a
+// non-negative 32-bit per-lane offset, which the loads and stores take as
+// a scalar base and a 32-bit vector offset.  Offsets x, y >= -4.
+#define TBASE(k) ((size_t)(k) * g.plane() + (size_t)(J0 - 4) * g.NI + (size_t)(I0 - 4))
+#define TOFF(x, y) ((unsigned)(((y) + 4) * g.NI + (x) + 4))
+
 // ---------------------------------------------------------------------------
 // C1 / D1: the dp and pt updates of one level (cgrid 94-268, dgrid 439-598).
 // CGRID: the C winds come from the D winds (109-126) and are also written
@@ -87,7 +94,8 @@ __global__ __launch_bounds__(NT_PAIR, 6) void k_pair(Geo g, DynCoef c, const dou
     int bx, by, K;
     xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
     const int I0 = ATM_HALO + bx * TX, J0 = ATM_HALO + by * TY;
-#define G(a, x, y) a[g.ix(I0 + (x), J0 + (y), K)]
+    const size_t kb0 = TBASE(K);
+#define G(a, x, y) (a + kb0)[TOFF(x, y)]
 #define UC(x, y) sUC[RUC::at(x, y)]
 #define VC(x, y) sVC[RVC::at(x, y)]
 #define Q(x, y) sQ[RQ::at(x, y)]
@@ -176,8 +184,8 @@ __global__ __launch_bounds__(NT_PAIR, 6) void k_pair(Geo g, DynCoef c, const dou
             // (cgrid 99-106); the D step reads the C step's (dgrid 457-465)
             double hr;
             if (CGRID) {
-                const size_t xk = g.ix(I0 + tx, J0 + ty, K);
-                hr = (nf[xk + g.plane()] - nf[xk]) / (ATM_CP_DRY * dpv / c.gravity) / plk[xk];
+                const unsigned o = TOFF(tx, ty);
+                hr = ((nf + TBASE(K + 1))[o] - (nf + kb0)[o]) / (ATM_CP_DRY * dpv / c.gravity) / (plk + kb0)[o];
             } else {
                 hr = G(heat, tx, ty);
             }
@@ -228,7 +236,8 @@ __global__ __launch_bounds__(NT_WIND) void k_c2(Geo g, DynCoef c, const double *
     int bx, by, K;
     xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
     const int I0 = ATM_HALO + bx * TX, J0 = ATM_HALO + by * TY;
-#define G(a, x, y) a[g.ix(I0 + (x), J0 + (y), K)]
+    const size_t kb0 = TBASE(K);
+#define G(a, x, y) (a + kb0)[TOFF(x, y)]
 #define U(x, y) sU[RB::at(x, y)]
 #define V(x, y) sV[RB::at(x, y)]
 #define W(x, y) sW[RW::at(x, y)]
@@ -273,19 +282,19 @@ __global__ __launch_bounds__(NT_WIND) void k_c2(Geo g, DynCoef c, const double *
     END_FOR
     __syncthreads();
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int I = I0 + tx, J = J0 + ty;
     // pressure gradient force (325-339)
     double pgfx, pgfy;
     {
-        double a = pkap[g.ix(I, J, K)], d = pkap[g.ix(I, J, K + 1)];
-        double b = pkap[g.ix(I - 1, J, K + 1)], cc = pkap[g.ix(I - 1, J, K)];
-        pgfx = ((a - b) * (gz[g.ix(I - 1, J, K)] - gz[g.ix(I, J, K + 1)]) +
-                (cc - d) * (gz[g.ix(I - 1, J, K + 1)] - gz[g.ix(I, J, K)])) /
+        const double *pk0 = pkap + kb0, *pk1 = pkap + TBASE(K + 1), *gz0 = gz + kb0, *gz1 = gz + TBASE(K + 1);
+        double a = pk0[TOFF(tx, ty)], d = pk1[TOFF(tx, ty)];
+        double b = pk1[TOFF(tx - 1, ty)], cc = pk0[TOFF(tx - 1, ty)];
+        pgfx = ((a - b) * (gz0[TOFF(tx - 1, ty)] - gz1[TOFF(tx, ty)]) +
+                (cc - d) * (gz1[TOFF(tx - 1, ty)] - gz0[TOFF(tx, ty)])) /
                (a - b + cc - d);
-        b = pkap[g.ix(I, J - 1, K + 1)];
-        cc = pkap[g.ix(I, J - 1, K)];
-        pgfy = ((a - b) * (gz[g.ix(I, J - 1, K)] - gz[g.ix(I, J, K + 1)]) +
-                (cc - d) * (gz[g.ix(I, J - 1, K + 1)] - gz[g.ix(I, J, K)])) /
+        b = pk1[TOFF(tx, ty - 1)];
+        cc = pk0[TOFF(tx, ty - 1)];
+        pgfy = ((a - b) * (gz0[TOFF(tx, ty - 1)] - gz1[TOFF(tx, ty)]) +
+                (cc - d) * (gz1[TOFF(tx, ty - 1)] - gz0[TOFF(tx, ty)])) /
                (a - b + cc - d);
     }
     const double udv = G(ud, tx, ty), vdv = G(vd, tx, ty);
@@ -338,7 +347,8 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
     int bx, by, K;
     xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
     const int I0 = ATM_HALO + bx * TX, J0 = ATM_HALO + by * TY;
-#define G(a, x, y) a[g.ix(I0 + (x), J0 + (y), K)]
+    const size_t kb0 = TBASE(K);
+#define G(a, x, y) (a + kb0)[TOFF(x, y)]
 #define U(x, y) sU[RU::at(x, y)]
 #define V(x, y) sV[RV::at(x, y)]
 #define W(x, y) sW[RW::at(x, y)]
@@ -349,8 +359,9 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
     __syncthreads();
     FOR_REG(RKE)  // kinetic energy (639-657), PPM_CONSTRAINED, last-level B winds
     const int Lm = g.L - 1;
-    const double ubij = .5 * (uc[g.ix(I0 + x, J0 + y, Lm)] + uc[g.ix(I0 + x, J0 + y - 1, Lm)]);
-    const double vbij = .5 * (vc[g.ix(I0 + x, J0 + y, Lm)] + vc[g.ix(I0 + x - 1, J0 + y, Lm)]);
+    const double *ucl = uc + TBASE(Lm), *vcl = vc + TBASE(Lm);
+    const double ubij = .5 * (ucl[TOFF(x, y)] + ucl[TOFF(x, y - 1)]);
+    const double vbij = .5 * (vcl[TOFF(x, y)] + vcl[TOFF(x - 1, y)]);
     const double fy = ppm_face<V_PPM_C>(V(x, y - 3), V(x, y - 2), V(x, y - 1), V(x, y), V(x, y + 1), V(x, y + 2),
                                         c.dtdy * vbij);
     const double fx = ppm_face<V_PPM_C>(U(x - 3, y), U(x - 2, y), U(x - 1, y), U(x, y), U(x + 1, y), U(x + 2, y),
@@ -374,7 +385,8 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
     // at the last level, k = nlay: the vorticity fluxes of every level use
     // the C winds of level nlay for their Courant numbers and denominators.
     const int Lm = g.L - 1;
-#define GL(a, x, y) a[g.ix(I0 + (x), J0 + (y), Lm)]
+    const size_t kbL = TBASE(Lm);
+#define GL(a, x, y) (a + kbL)[TOFF(x, y)]
     FOR_REG(RTU)  // inner x step of vorticity for ud (708-716)
     const double w0 = W(x, y);
     const double u0 = GL(uc, x, y), u1 = GL(uc, x + 1, y);
@@ -405,7 +417,8 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
     // B-grid pressure gradient (665-679) from the corner values of pkap and
     // gz at levels k and k+1 (k_bcorners; pkapb(nlay+1) and gzb(1) from
     // init_sw_dyn 991, 995)
-#define CB(a, kq, x, y) a[g.ix(I0 + (x), J0 + (y), K + (kq))]
+    const size_t kb1 = TBASE(K + 1);
+#define CB(a, kq, x, y) (a + ((kq) ? kb1 : kb0))[TOFF(x, y)]
     double pgfx, pgfy;
     {
         double a = CB(pkapb, 0, tx + 1, ty), b = CB(pkapb, 1, tx, ty), cc = CB(pkapb, 0, tx, ty),
