@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic HBM bytes per interior cell for one launch of each kernel group
 # (compulsory reads + writes of the fields each kernel consumes/produces)
 ALG_BYTES_PER_CELL = {
-    "c1": 8 * (5 + 4),   # read dp, pt, ud, vd, heat; write dpc, ptc, uc, vc
+    "c1": 8 * (6 + 4),   # read dp, pt, ud, vd, playkap, net_flux; write dpc, ptc, uc, vc
     "c2": 8 * (6 + 2),   # read uc, vc, ud, vd, pkap, gz; write uc, vc
     "d1": 8 * (5 + 2),   # read dp, pt, uc, vc, heat; write dp, pt
     "d2": 8 * (6 + 2),   # read ud, vd, uc, vc, pkap, gz; write ud, vd
