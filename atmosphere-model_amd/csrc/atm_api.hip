@@ -346,12 +346,11 @@ int atm_dgrid_step(atm_ctx *c, double dt) {
 int atm_dgrid_halo(atm_ctx *c) {
     CHECK_CTX(c);
     double *f[] = {c->dp, c->pt, c->ud, c->vd};
+    const int nk[] = {c->g.L, c->g.L, c->g.L, c->g.L};
     atm_timer_begin(c, ATM_T_HALO);
-    for (double *a : f) {
-        int rc = atm_launch_halo(c, a, c->g.L);
-        if (rc) return rc;
-    }
+    int rc = atm_launch_halo_multi(c, f, nk, 4);
     atm_timer_end(c, ATM_T_HALO);
+    if (rc) return rc;
     c->after_d = false;
     return ATM_OK;
 }

@@ -238,6 +238,9 @@ static inline dim3 grid3(int nI, int nJ, int nK, dim3 b) {
 
 // kernels implemented in the .hip files (host launchers)
 int atm_launch_halo(atm_ctx *c, double *a, int nk);
+#define ATM_HALO_MAXF 6
+// periodic halo fill of n <= ATM_HALO_MAXF fields (nk[i] levels each) in one launch
+int atm_launch_halo_multi(atm_ctx *c, double *const *a, const int *nk, int n);
 int atm_ref_cgrid(atm_ctx *c, double dt);
 int atm_ref_dgrid(atm_ctx *c, double dt);
 int atm_ref_init_sw_dyn(atm_ctx *c);

@@ -648,16 +648,14 @@ int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
                        (const double *)nullptr, c->plk, c->net_flux,
                        c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va);
     atm_timer_end(c, ATM_T_C1);
-    double *fill[] = {c->dpc, c->ptc, c->uc, c->vc};
-    atm_timer_begin(c, ATM_T_HALO);
-    for (double *a : fill) {
-        int rc = atm_launch_halo(c, a, g.L);
+    {
+        double *fill[] = {c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va};
+        const int nk[] = {g.L, g.L, g.L, g.L, 1, 1};
+        atm_timer_begin(c, ATM_T_HALO);
+        int rc = atm_launch_halo_multi(c, fill, nk, 6);
+        atm_timer_end(c, ATM_T_HALO);
         if (rc) return rc;
     }
-    int rc = atm_launch_halo(c, c->ua, 1);
-    if (!rc) rc = atm_launch_halo(c, c->va, 1);
-    atm_timer_end(c, ATM_T_HALO);
-    if (rc) return rc;
     atm_timer_begin(c, ATM_T_SCAN);
     hipLaunchKernelGGL(k_scan_pg, dim3((g.NI + 63) / 64, g.NJ), dim3(64), 0, c->stream, g, c->ak0, pktop, k.gravity,
                        c->dpc, c->ptc, c->net_flux, (double *)nullptr, c->pkap, c->gz, c->heatd,
@@ -674,9 +672,10 @@ int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
 // the halo fill of cgrid_halo_exchange for the fused path: dpc/ptc are
 // already periodic, uc/vc were updated on the interior
 int atm_fused_cgrid_halo(atm_ctx *c) {
+    double *fill[] = {c->uc2, c->vc2};
+    const int nk[] = {c->g.L, c->g.L};
     atm_timer_begin(c, ATM_T_HALO);
-    int rc = atm_launch_halo(c, c->uc2, c->g.L);
-    if (!rc) rc = atm_launch_halo(c, c->vc2, c->g.L);
+    int rc = atm_launch_halo_multi(c, fill, nk, 2);
     atm_timer_end(c, ATM_T_HALO);
     return rc;
 }
@@ -695,11 +694,14 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
                        (double *)nullptr, (double *)nullptr, (double *)nullptr,
                        (double *)nullptr);
     atm_timer_end(c, ATM_T_D1);
-    atm_timer_begin(c, ATM_T_HALO);
-    int rc = atm_launch_halo(c, nxt.dp, g.L);
-    if (!rc) rc = atm_launch_halo(c, nxt.pt, g.L);
-    atm_timer_end(c, ATM_T_HALO);
-    if (rc) return rc;
+    {
+        double *fill[] = {nxt.dp, nxt.pt};
+        const int nk[] = {g.L, g.L};
+        atm_timer_begin(c, ATM_T_HALO);
+        int rc = atm_launch_halo_multi(c, fill, nk, 2);
+        atm_timer_end(c, ATM_T_HALO);
+        if (rc) return rc;
+    }
     atm_timer_begin(c, ATM_T_SCAN);
     hipLaunchKernelGGL(k_scan_pg, dim3((g.NI + 63) / 64, g.NJ), dim3(64), 0, c->stream, g, c->ak0, c->pktop,
                        k.gravity, nxt.dp, nxt.pt, c->net_flux, c->plev, c->pkap, c->gz, (double *)nullptr,

@@ -11,7 +11,16 @@
 
 namespace {
 
-__global__ void k_halo_fill(Geo g, double *__restrict__ a, int nk) {
+// up to ATM_HALO_MAXF fields in one launch (grid z = field)
+struct HaloSet {
+    double *a[ATM_HALO_MAXF];
+    int nk[ATM_HALO_MAXF];
+};
+
+__global__ void k_halo_fill(Geo g, HaloSet set) {
+    const int f = blockIdx.z;
+    double *__restrict__ a = set.a[f];
+    const int nk = set.nk[f];
     const int H = ATM_HALO;
     const int side = H * g.NJ;       // per side strip
     const int tb = g.nx * H;         // per top/bottom strip
@@ -53,13 +62,25 @@ __global__ void k_finite(Geo g, const double *__restrict__ a, const double *__re
 
 }  // namespace
 
-int atm_launch_halo(atm_ctx *c, double *a, int nk) {
+int atm_launch_halo_multi(atm_ctx *c, double *const *a, const int *nk, int n) {
     const Geo &g = c->g;
+    if (n < 1 || n > ATM_HALO_MAXF) return atm_fail(c, ATM_ERR_ARGUMENT, "halo fill: bad field count");
+    HaloSet set{};
+    int kmax = 0;
+    for (int i = 0; i < n; i++) {
+        set.a[i] = a[i];
+        set.nk[i] = nk[i];
+        kmax = nk[i] > kmax ? nk[i] : kmax;
+    }
     int per = 2 * ATM_HALO * g.NJ + 2 * g.nx * ATM_HALO;
-    dim3 b(256), gr((per + 255) / 256, nk);
-    hipLaunchKernelGGL(k_halo_fill, gr, b, 0, c->stream, g, a, nk);
+    dim3 b(256), gr((per + 255) / 256, kmax, n);
+    hipLaunchKernelGGL(k_halo_fill, gr, b, 0, c->stream, g, set);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;
+}
+
+int atm_launch_halo(atm_ctx *c, double *a, int nk) {
+    return atm_launch_halo_multi(c, &a, &nk, 1);
 }
 
 int atm_launch_stability(atm_ctx *c) {
