@@ -340,11 +340,13 @@ int atm_rad_run(atm_ctx *c, double time) {
     const Geo &g = c->g;
     RadWork *w = c->rad;
     dim3 b(64, 2);
+    atm_timer_begin(c, ATM_T_RAD);
     hipLaunchKernelGGL(k_rad_profiles, grid3(g.nx + 6, g.ny + 6, 1, b), b, 0, c->stream, g, c->dp, c->pt,
                        c->plev, c->pkap, c->play, c->playkap, w->layer_T, w->level_T);
     ATM_HIP(c, hipMemsetAsync(c->net_flux, 0, sizeof(double) * g.plane() * (g.L + 1), c->stream));
     hipLaunchKernelGGL(k_rad_columns, grid3(g.nx, g.ny, 1, b), b, 0, c->stream, g, w->t, c->prm.gravity, w->scr,
                        c->dp, c->play, w->layer_T, w->level_T, c->ts, c->radius, c->net_flux, c->flag);
+    atm_timer_end(c, ATM_T_RAD);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;
 }

@@ -31,6 +31,13 @@ HBM_PEAK_GBS = 8000.0
 
 # algorithmic HBM bytes per interior cell for one launch of each kernel group
 # (compulsory reads + writes of the fields each kernel consumes/produces)
+# radiation: FP64 operations per layer and g-point of the column solve
+# (k_rad_columns: shortwave optics + two-stream coefficients ~145, longwave
+# ~85, the two adding passes ~44; exp, sqrt and each division count as one)
+RAD_FLOP_PER_LAYER_G = 280
+RAD_G_POINTS = 16
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+
 ALG_BYTES_PER_CELL = {
     "c1": 8 * (6 + 4),   # read dp, pt, ud, vd, playkap, net_flux; write dpc, ptc, uc, vc
     "c2": 8 * (6 + 2),   # read uc, vc, ud, vd, pkap, gz; write uc, vc
@@ -55,6 +62,8 @@ def parse():
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--cpu-nx", type=int, default=256, help="bounded CPU sample: nx = ny")
     ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--radiation-calls", type=int, default=3,
+                    help="also time this many radiation calls on the same grid (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of this many MiB (read + write) a few times, "
                          "to scale HBM counters (FETCH_SIZE/WRITE_SIZE) against a known size")
@@ -91,6 +100,27 @@ def cpu_baseline(args):
     return {"value": n * n * L * steps / sec, "unit": "cell-steps/s", "cores": 1, "kind": "port",
             "sample": f"{n}x{n}x{L}, {steps} steps incl. {steps // args.remap_every} remap, "
                       f"{sec:.1f} s on 1 host core (oracle/atm_oracle.c, -O2, glibc libm)"}
+
+
+def radiation_timing(m, args, time_s):
+    """calculate_radiative_heating on the benchmark state (all interior
+    columns, 16 g-points, shortwave and longwave), timed with HIP events."""
+    m.calculate_radiative_heating(time_s)  # warm-up (first call also checks the tables)
+    m.sync()
+    m.timer_enable(True)
+    for _ in range(args.radiation_calls):
+        m.calculate_radiative_heating(time_s)
+    m.sync()
+    n, ms = m.timer_read("radiation")
+    m.timer_enable(False)
+    per = ms / n
+    cols = args.nx * args.ny
+    flops = RAD_FLOP_PER_LAYER_G * RAD_G_POINTS * args.nlay * cols
+    achieved = flops / (per / 1e3) / 1e12
+    return {"ms_per_call": per, "columns_per_s": cols / (per / 1e3), "calls": n,
+            "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "bound": "valu",
+            "flop_per_call": flops}
 
 
 def main():
@@ -161,6 +191,8 @@ def main():
                      "alg_bytes_per_launch": alg, "avg_launch_ms": avg_s * 1e3},
         "kernels_ms_per_step": {k: v[1] / args.steps for k, v in kernels.items()},
     }
+    if args.radiation_calls > 0:
+        out["radiation"] = radiation_timing(m, args, time_s)
     if args.cpu_baseline == "on":
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)
