@@ -7,9 +7,8 @@
 // overwritten by the halo exchange that follows (mod_model.f90:122-124) and
 // radius is undefined there (mod_input.f90:233-243).
 //
-// One thread per column, looping over the 16 g-points in the reference
-// order so the flux sums accumulate in the same order.  Per-column work
-// arrays live in a device scratch laid out [slot][column].
+// One wavefront per column (see k_rad_columns); the flux sums over the 16
+// g-points accumulate in the reference's order.
 //
 // Table lookups: the reference scans its tables linearly for the first
 // node >= x (lookup_1d 4556-4579, lookup_pT_table 4581-4609).  Every table
@@ -25,11 +24,9 @@ struct Tables {
 };
 
 struct RadWork {
-    double *scr = nullptr;  // [slots][plane]
     double *layer_T = nullptr, *level_T = nullptr;
     float *tab = nullptr;
     Tables t;
-    int slots = 0;
 };
 
 namespace {
@@ -63,62 +60,6 @@ __device__ __forceinline__ double lookup_pT(const Tables &t, double p, double T,
     double t2 = lookup_1d(T, t.T + 6 * (k - 1), table + 6 * (k - 1), 6, err);
     double pk = t.P[k], pm = t.P[k - 1];
     return (t1 * (pm - p) + t2 * (p - pk)) / (pm - pk);
-}
-
-struct RCol {
-    double *s;
-    size_t plane, col;
-    __device__ double &at(int slot, int k) const { return s[(size_t)(slot + k) * plane + col]; }
-};
-
-// scratch slots, L+1 wide each
-struct RSlots {
-    int n;  // L+1
-    __device__ int R() const { return 0; }
-    __device__ int T() const { return n; }
-    __device__ int Su() const { return 2 * n; }
-    __device__ int Sd() const { return 3 * n; }
-    __device__ int B() const { return 4 * n; }   // solar beam (SW) / level Planck (LW)
-    __device__ int alb() const { return 5 * n; }
-    __device__ int emi() const { return 6 * n; }
-    __device__ int bet() const { return 7 * n; }
-    __device__ int swu() const { return 8 * n; }
-    __device__ int swd() const { return 9 * n; }
-    __device__ int lwu() const { return 10 * n; }
-    __device__ int lwd() const { return 11 * n; }
-    static int count(int L) { return 12 * (L + 1); }
-};
-
-// calculate_fluxes (349-381); adds the result into (accU, accD) slots.
-// For the shortwave the direct beam (slot B) is added to the downward flux.
-__device__ void fluxes_accumulate(const RCol &c, const RSlots &S, int L, double toa, double sfc_up,
-                                  double sfc_alb, int accU, int accD, bool add_beam) {
-    c.at(S.alb(), 0) = sfc_alb;
-    c.at(S.emi(), 0) = sfc_up;
-    double a = sfc_alb, e = sfc_up;
-    for (int k = 0; k < L; k++) {
-        double R = c.at(S.R(), k), T = c.at(S.T(), k);
-        double b = 1. / (1. - R * a);
-        double an = R + (T * T) * b * a;
-        double en = c.at(S.Su(), k) + T * b * (e + c.at(S.Sd(), k) * a);
-        c.at(S.bet(), k) = b;
-        c.at(S.alb(), k + 1) = an;
-        c.at(S.emi(), k + 1) = en;
-        a = an;
-        e = en;
-    }
-    double fd = toa;
-    double fu = a * toa + e;
-    c.at(accU, L) = c.at(accU, L) + fu;
-    c.at(accD, L) = c.at(accD, L) + (add_beam ? fd + c.at(S.B(), L) : fd);
-    for (int k = L; k >= 1; k--) {
-        double b = c.at(S.bet(), k - 1);
-        double fdn = b * (c.at(S.Sd(), k - 1) + c.at(S.T(), k - 1) * fd + c.at(S.R(), k - 1) * c.at(S.emi(), k - 1));
-        double fup = c.at(S.alb(), k - 1) * fdn + c.at(S.emi(), k - 1);
-        c.at(accU, k - 1) = c.at(accU, k - 1) + fup;
-        c.at(accD, k - 1) = c.at(accD, k - 1) + (add_beam ? fdn + c.at(S.B(), k - 1) : fdn);
-        fd = fdn;
-    }
 }
 
 // layer and level profiles over is+4..ie-4 (53-101)
@@ -165,131 +106,318 @@ __global__ void k_rad_profiles(Geo g, const double *__restrict__ dp, const doubl
     }
 }
 
-// column solves (117-154)
-__global__ void k_rad_columns(Geo g, Tables t, double gravity, double *__restrict__ scr,
-                              const double *__restrict__ dp, const double *__restrict__ play,
-                              const double *__restrict__ lT, const double *__restrict__ vT,
-                              const double *__restrict__ ts, const double *__restrict__ radius,
-                              double *__restrict__ net_flux, int *flag) {
-    const int I = 7 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int J = 7 + blockIdx.y * blockDim.y + threadIdx.y;
-    if (I > g.NI - 8 || J > g.NJ - 8) return;
-    const int L = g.L;
-    RSlots S{L + 1};
-    RCol c{scr, g.plane(), g.ix2(I, J)};
-    int err = 0;
-    for (int k = 0; k <= L; k++) {
-        c.at(S.swu(), k) = 0.;
-        c.at(S.swd(), k) = 0.;
-        c.at(S.lwu(), k) = 0.;
-        c.at(S.lwd(), k) = 0.;
+// ---------------------------------------------------------------------------
+// Column solves (117-154): one wavefront per column.
+//
+// Per pass over RG g-points:
+//   optics   every lane takes layers (lane, lane+64, ...) for each of the RG
+//            g-points: k_abs, delta-Eddington, two-stream coefficients, R,
+//            T and the shortwave beam factors (or the longwave Planck
+//            sources) into LDS.  The table positions and interpolation
+//            weights depend only on (p, T), so they are found once per
+//            layer and shared by all g-points and both tables;
+//   chains   lanes 0..RG-1, one per g-point, run the recurrences down and
+//            up the column (shortwave beam, the adding method) over LDS;
+//   sums     every lane adds the RG g-points' fluxes of its levels into its
+//            accumulators in g order, the reference's summation order.
+// The pass repeats for the longwave, then for the next RG g-points.
+// ---------------------------------------------------------------------------
+constexpr int RG = 4;       // g-points per pass
+constexpr int RMAXL = 4;    // layers per lane at most: L <= 64 * RMAXL
+constexpr int RARR = 8;     // LDS arrays of L + 1 per g-point
+
+// linear interpolation on one table axis (lookup_1d 4556-4579), shared by
+// every table on that axis
+struct Lin {
+    int k = 0;
+    bool hit = false, oor = false;
+    double d1 = 0., d2 = 0., d3 = 1.;  // xm - x, x - xk, xm - xk
+    __device__ double apply(const float *ys) const {
+        if (oor) return 0.;
+        if (hit) return (double)ys[k];
+        const double yk = ys[k], ym = ys[k - 1];
+        return (yk * d1 + ym * d2) / d3;
     }
+};
+
+__device__ __forceinline__ Lin lin_1d(double x, const float *xs, int n, int *err) {
+    Lin l;
+    if (x < (double)xs[0] || x > (double)xs[n - 1]) {
+        *err |= 1;
+        l.oor = true;
+        return l;
+    }
+    l.k = first_ge(xs, n, x);
+    const double xk = xs[l.k];
+    l.hit = x == xk;
+    if (!l.hit) {
+        const double xm = xs[l.k - 1];
+        l.d1 = xm - x;
+        l.d2 = x - xk;
+        l.d3 = xm - xk;
+    }
+    return l;
+}
+
+// lookup_pT_table (4581-4609) on the (pressure, temperature) axes
+struct LinPT {
+    Lin t1, t2;  // rows k and k - 1
+    int k = 0;
+    bool hit = false, oor = false;
+    double d1 = 0., d2 = 0., d3 = 1.;
+    __device__ double apply(const float *table) const {
+        if (oor) return 0.;
+        if (hit) return t1.apply(table + 6 * k);
+        const double v1 = t1.apply(table + 6 * k);
+        const double v2 = t2.apply(table + 6 * (k - 1));
+        return (v1 * d1 + v2 * d2) / d3;
+    }
+};
+
+__device__ __forceinline__ LinPT lin_pT(const Tables &t, double p, double T, int *err) {
+    LinPT l;
+    if (p < (double)t.P[0] || p > (double)t.P[52]) {
+        *err |= 1;
+        l.oor = true;
+        return l;
+    }
+    l.k = first_ge(t.P, 53, p);
+    l.hit = p == (double)t.P[l.k];
+    l.t1 = lin_1d(T, t.T + 6 * l.k, 6, err);
+    if (!l.hit) {
+        l.t2 = lin_1d(T, t.T + 6 * (l.k - 1), 6, err);
+        const double pk = t.P[l.k], pm = t.P[l.k - 1];
+        l.d1 = pm - p;
+        l.d2 = p - pk;
+        l.d3 = pm - pk;
+    }
+    return l;
+}
+
+// calculate_fluxes (349-381) on LDS arrays of one g-point.  On return
+// alb[k] holds the upward and emi[k] the downward flux of level k (the
+// direct beam B added to the latter when B != nullptr).
+__device__ void adding(int L, const double *R, const double *T, const double *Su, const double *Sd,
+                       const double *B, double *bet, double *alb, double *emi, double toa, double sfc_up,
+                       double sfc_alb) {
+    alb[0] = sfc_alb;
+    emi[0] = sfc_up;
+    double a = sfc_alb, e = sfc_up;
+    for (int k = 0; k < L; k++) {
+        const double Rk = R[k], Tk = T[k];
+        const double b = 1. / (1. - Rk * a);
+        const double an = Rk + (Tk * Tk) * b * a;
+        const double en = Su[k] + Tk * b * (e + Sd[k] * a);
+        bet[k] = b;
+        alb[k + 1] = an;
+        emi[k + 1] = en;
+        a = an;
+        e = en;
+    }
+    double fd = toa;
+    alb[L] = a * toa + e;
+    emi[L] = B ? fd + B[L] : fd;
+    for (int k = L; k >= 1; k--) {
+        const double b = bet[k - 1], em = emi[k - 1];
+        const double fdn = b * (Sd[k - 1] + T[k - 1] * fd + R[k - 1] * em);
+        const double fup = alb[k - 1] * fdn + em;
+        alb[k - 1] = fup;
+        emi[k - 1] = B ? fdn + B[k - 1] : fdn;
+        fd = fdn;
+    }
+}
+
+// NL layers and NL + 1 levels per lane (L <= 64 NL)
+template <int NL>
+__global__ void __launch_bounds__(64) k_rad_columns(Geo g, Tables t, double gravity, const double *__restrict__ dp,
+                                                    const double *__restrict__ play, const double *__restrict__ lT,
+                                                    const double *__restrict__ vT, const double *__restrict__ ts,
+                                                    const double *__restrict__ radius, double *__restrict__ net_flux,
+                                                    int *flag) {
+    extern __shared__ double lds[];
+    const int col = blockIdx.x;
+    const int I = ATM_HALO + col % g.nx, J = ATM_HALO + col / g.nx;
+    const int lane = threadIdx.x;
+    const int L = g.L, N = L + 1;
+    const size_t pl = g.plane(), c0 = g.ix2(I, J);
+    auto arr = [&](int a, int gi) { return lds + (size_t)(a * RG + gi) * N; };
+    // arrays: 0 Tdir -> beam, 1 Rdif -> Su, 2 Tdif -> Sd, 3 R, 4 T, 5 bet, 6 alb -> up, 7 emi -> down
+    int err = 0;
     // solar_irradiance (mod_solar.f90:26-35)
-    double r = radius[g.ix2(I, J)] / (double)100000e3f;
+    const double r = radius[c0] / (double)100000e3f;
     const double irr = 600. * (1. + (double)0.2f / (1. + r * r));
     const double mu0 = cos(0.);
     const double sfc_alb = (double)0.3f, emiss = (double)0.99f;
-    const double tsur = ts[g.ix2(I, J)];
-    for (int gp = 0; gp < 16; gp++) {
-        // ---- shortwave (158-268) ----
-        const float *swt = t.sw + 6 * 53 * gp;
-        const double k_sca = (double)t.ray[gp];
-        c.at(S.B(), L) = mu0 * ((double)t.frac[gp] * irr);
-        for (int K = L - 1; K >= 0; K--) {
-            double dm = dp[g.ix(I, J, K)] / gravity;
-            double k_abs = lookup_pT(t, play[g.ix(I, J, K)], lT[g.ix(I, J, K)], swt, &err);
-            double k_ext = k_abs + k_sca;
-            double tau = k_ext * dm;
-            double ssa = k_sca / k_ext;
-            double asy = 0.;
-            // delta_eddington_rescaling (383-398)
-            {
-                double f = asy * asy;
-                double coef = 1. - ssa * f;
-                tau = tau * coef;
-                ssa = ssa * (1. - f) / coef;
-                asy = asy / (1. + asy);
-            }
-            double asy2 = asy * asy;
-            double back = (2. - 3. * asy * mu0) / 4.;
-            double tmp1 = 4. * (1. - asy2 * (1. - mu0));
-            double gamma3 = ssa * asy2;
-            double gamma4 = 4. * back + 3. * asy;
-            double tmp2 = 3. * asy;
-            double gamma1 = (7. - 3. * asy2 - ssa * (4. + tmp2) + gamma3 * gamma4) / tmp1;
-            double gamma2 = (-1. + asy2 + ssa * (4. - tmp2) + gamma3 * (gamma4 - 4.)) / tmp1;
-            gamma3 = back;
-            gamma4 = 1. - gamma3;
-            double alpha1 = gamma1 * gamma4 + gamma2 * gamma3;
-            double alpha2 = gamma2 * gamma3 + gamma2 * gamma4;
-            double lambda = sqrt(gamma1 * gamma1 - gamma2 * gamma2);
-            double lam_mu = lambda * mu0;
-            double e_minus = exp(-lambda * tau);
-            double e_2minus = e_minus * e_minus;
-            double Tdir = exp(-tau / mu0);
-            tmp1 = ssa / ((1. - lam_mu * lam_mu) * (lambda + gamma1 + (lambda - gamma1) * e_2minus));
-            tmp2 = lambda * gamma3;
-            double Rdif = tmp1 * ((1. - lam_mu) * (alpha2 + tmp2) - (1. + lam_mu) * (alpha2 - tmp2) * e_2minus -
-                                  2. * lambda * (gamma3 - alpha2 * mu0) * e_minus * Tdir);
-            tmp2 = lambda * gamma4;
-            double Tdif = tmp1 * (2. * lambda * (gamma4 + alpha1 * mu0) * e_minus -
-                                  Tdir * ((1. + lam_mu) * (alpha1 + tmp2) - (1. - lam_mu) * (alpha1 - tmp2) * e_2minus));
-            double bk1 = c.at(S.B(), K + 1);
-            c.at(S.B(), K) = Tdir * bk1;
-            c.at(S.Su(), K) = Rdif * bk1;
-            c.at(S.Sd(), K) = Tdif * bk1;
-            back = (4. + asy) / (8. * (1. + asy));
-            gamma1 = 2. * (1. - ssa * (1. - back));
-            gamma2 = 2. * ssa * back;
-            lambda = sqrt(gamma1 * gamma1 - gamma2 * gamma2);
-            e_2minus = e_minus * e_minus;  // Q6: e_minus of the direct-beam lambda
-            tmp1 = 1. / (lambda + gamma1 + (lambda - gamma1) * e_2minus);
-            c.at(S.R(), K) = tmp1 * gamma2 * (1. - e_2minus);
-            c.at(S.T(), K) = 2. * tmp1 * lambda * e_minus;
+    const double tsur = ts[c0];
+    const Lin ls = lin_1d(tsur, t.planckT, 231, &err);
+    // per-lane inputs of its layers (K = lane + 64 v) and accumulators of its levels
+    constexpr int NV = NL + 1;
+    double dmv[NL], swu[NV], swd[NV], lwu[NV], lwd[NV];
+    LinPT lpt[NL];
+    Lin lb0[NL], lb1[NL];  // Planck positions of levels K and K + 1
+#pragma unroll
+    for (int v = 0; v < NV; v++) swu[v] = swd[v] = lwu[v] = lwd[v] = 0.;
+#pragma unroll
+    for (int v = 0; v < NL; v++) {
+        dmv[v] = 0.;
+        const int K = lane + 64 * v;
+        if (K < L) {
+            const size_t x = c0 + (size_t)K * pl;
+            dmv[v] = dp[x] / gravity;
+            lpt[v] = lin_pT(t, play[x], lT[x], &err);
+            lb0[v] = lin_1d(vT[x], t.planckT, 231, &err);
+            lb1[v] = lin_1d(vT[x + pl], t.planckT, 231, &err);
         }
-        fluxes_accumulate(c, S, L, c.at(S.B(), L), 0., sfc_alb, S.swu(), S.swd(), true);
-        // ---- longwave (270-347) ----
-        const float *lwt = t.lw + 6 * 53 * gp;
-        const float *pl = t.planck + 231 * gp;
-        const double Bs = lookup_1d(tsur, t.planckT, pl, 231, &err);
-        double Bk = lookup_1d(vT[g.ix(I, J, 0)], t.planckT, pl, 231, &err);
-        for (int K = 0; K < L; K++) {
-            double dm = dp[g.ix(I, J, K)] / gravity;
-            double k_abs = lookup_pT(t, play[g.ix(I, J, K)], lT[g.ix(I, J, K)], lwt, &err);
-            double k_sca = 0.;
-            double k_ext = k_abs + k_sca;
-            double tau = k_ext * dm;
-            double ssa = k_sca / k_ext;
-            double asy = 0.;
-            {
-                double f = asy * asy;
-                double coef = 1. - ssa * f;
-                tau = tau * coef;
-                ssa = ssa * (1. - f) / coef;
-                asy = asy / (1. + asy);
-            }
-            double back = (4. - 3. * asy) / 8.;
-            double gamma1 = 2. * (1. - ssa * (1. - back));
-            double gamma2 = 2. * ssa * back;
-            double lambda = sqrt(gamma1 * gamma1 - gamma2 * gamma2);
-            double e_minus = exp(-lambda * tau);
-            double e_2minus = e_minus * e_minus;
-            double tmp1 = 1. / (lambda + gamma1 + (lambda - gamma1) * e_2minus);
-            double R = tmp1 * gamma2 * (1. - e_2minus);
-            double T = 2. * tmp1 * lambda * e_minus;
-            c.at(S.R(), K) = R;
-            c.at(S.T(), K) = T;
-            double Bk1 = lookup_1d(vT[g.ix(I, J, K + 1)], t.planckT, pl, 231, &err);
-            double pdiff = (1. + R - T) * (Bk - Bk1) / (tau * (gamma1 + gamma2));
-            tmp1 = 1. - R;
-            c.at(S.Su(), K) = ATM_PI * (tmp1 * Bk1 - T * Bk + pdiff);
-            c.at(S.Sd(), K) = ATM_PI * (tmp1 * Bk - T * Bk1 - pdiff);
-            Bk = Bk1;
-        }
-        fluxes_accumulate(c, S, L, 0., emiss * ATM_PI * Bs, 1. - emiss, S.lwu(), S.lwd(), false);
     }
-    for (int k = 0; k <= L; k++)
-        net_flux[g.ix(I, J, k)] = (c.at(S.swd(), k) + c.at(S.lwd(), k)) - (c.at(S.swu(), k) + c.at(S.lwu(), k));
+    for (int g0 = 0; g0 < 16; g0 += RG) {
+        // ---- shortwave optics (158-260) ----
+#pragma unroll
+        for (int v = 0; v < NL; v++) {
+            const int K = lane + 64 * v;
+            if (K >= L) continue;
+#pragma unroll 1
+            for (int gi = 0; gi < RG; gi++) {
+                const int gp = g0 + gi;
+                const double k_sca = (double)t.ray[gp];
+                double k_abs = lpt[v].apply(t.sw + 6 * 53 * gp);
+                double k_ext = k_abs + k_sca;
+                double tau = k_ext * dmv[v];
+                double ssa = k_sca / k_ext;
+                double asy = 0.;
+                {  // delta_eddington_rescaling (383-398)
+                    double f = asy * asy;
+                    double coef = 1. - ssa * f;
+                    tau = tau * coef;
+                    ssa = ssa * (1. - f) / coef;
+                    asy = asy / (1. + asy);
+                }
+                double asy2 = asy * asy;
+                double back = (2. - 3. * asy * mu0) / 4.;
+                double tmp1 = 4. * (1. - asy2 * (1. - mu0));
+                double gamma3 = ssa * asy2;
+                double gamma4 = 4. * back + 3. * asy;
+                double tmp2 = 3. * asy;
+                double gamma1 = (7. - 3. * asy2 - ssa * (4. + tmp2) + gamma3 * gamma4) / tmp1;
+                double gamma2 = (-1. + asy2 + ssa * (4. - tmp2) + gamma3 * (gamma4 - 4.)) / tmp1;
+                gamma3 = back;
+                gamma4 = 1. - gamma3;
+                double alpha1 = gamma1 * gamma4 + gamma2 * gamma3;
+                double alpha2 = gamma2 * gamma3 + gamma2 * gamma4;
+                double lambda = sqrt(gamma1 * gamma1 - gamma2 * gamma2);
+                double lam_mu = lambda * mu0;
+                double e_minus = exp(-lambda * tau);
+                double e_2minus = e_minus * e_minus;
+                double Tdir = exp(-tau / mu0);
+                tmp1 = ssa / ((1. - lam_mu * lam_mu) * (lambda + gamma1 + (lambda - gamma1) * e_2minus));
+                tmp2 = lambda * gamma3;
+                double Rdif = tmp1 * ((1. - lam_mu) * (alpha2 + tmp2) - (1. + lam_mu) * (alpha2 - tmp2) * e_2minus -
+                                      2. * lambda * (gamma3 - alpha2 * mu0) * e_minus * Tdir);
+                tmp2 = lambda * gamma4;
+                double Tdif = tmp1 * (2. * lambda * (gamma4 + alpha1 * mu0) * e_minus -
+                                      Tdir * ((1. + lam_mu) * (alpha1 + tmp2) - (1. - lam_mu) * (alpha1 - tmp2) * e_2minus));
+                back = (4. + asy) / (8. * (1. + asy));
+                gamma1 = 2. * (1. - ssa * (1. - back));
+                gamma2 = 2. * ssa * back;
+                lambda = sqrt(gamma1 * gamma1 - gamma2 * gamma2);
+                e_2minus = e_minus * e_minus;  // Q6: e_minus of the direct-beam lambda
+                tmp1 = 1. / (lambda + gamma1 + (lambda - gamma1) * e_2minus);
+                arr(0, gi)[K] = Tdir;
+                arr(1, gi)[K] = Rdif;
+                arr(2, gi)[K] = Tdif;
+                arr(3, gi)[K] = tmp1 * gamma2 * (1. - e_2minus);
+                arr(4, gi)[K] = 2. * tmp1 * lambda * e_minus;
+            }
+        }
+        __syncthreads();
+        if (lane < RG) {  // beam down the column (221-249), then the adding method
+            const int gi = lane, gp = g0 + gi;
+            double *B = arr(0, gi), *Su = arr(1, gi), *Sd = arr(2, gi);
+            double bk1 = mu0 * ((double)t.frac[gp] * irr);
+            B[L] = bk1;
+            for (int K = L - 1; K >= 0; K--) {
+                const double bk = B[K] * bk1;
+                Su[K] = Su[K] * bk1;
+                Sd[K] = Sd[K] * bk1;
+                B[K] = bk;
+                bk1 = bk;
+            }
+            adding(L, arr(3, gi), arr(4, gi), Su, Sd, B, arr(5, gi), arr(6, gi), arr(7, gi), B[L], 0., sfc_alb);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const int k = lane + 64 * v;
+            if (k > L) continue;
+            for (int gi = 0; gi < RG; gi++) {
+                swu[v] = swu[v] + arr(6, gi)[k];
+                swd[v] = swd[v] + arr(7, gi)[k];
+            }
+        }
+        __syncthreads();
+        // ---- longwave optics (270-339) ----
+#pragma unroll
+        for (int v = 0; v < NL; v++) {
+            const int K = lane + 64 * v;
+            if (K >= L) continue;
+#pragma unroll 1
+            for (int gi = 0; gi < RG; gi++) {
+                const int gp = g0 + gi;
+                const float *plk = t.planck + 231 * gp;
+                double k_abs = lpt[v].apply(t.lw + 6 * 53 * gp);
+                double k_sca = 0.;
+                double k_ext = k_abs + k_sca;
+                double tau = k_ext * dmv[v];
+                double ssa = k_sca / k_ext;
+                double asy = 0.;
+                {
+                    double f = asy * asy;
+                    double coef = 1. - ssa * f;
+                    tau = tau * coef;
+                    ssa = ssa * (1. - f) / coef;
+                    asy = asy / (1. + asy);
+                }
+                double back = (4. - 3. * asy) / 8.;
+                double gamma1 = 2. * (1. - ssa * (1. - back));
+                double gamma2 = 2. * ssa * back;
+                double lambda = sqrt(gamma1 * gamma1 - gamma2 * gamma2);
+                double e_minus = exp(-lambda * tau);
+                double e_2minus = e_minus * e_minus;
+                double tmp1 = 1. / (lambda + gamma1 + (lambda - gamma1) * e_2minus);
+                double R = tmp1 * gamma2 * (1. - e_2minus);
+                double T = 2. * tmp1 * lambda * e_minus;
+                const double Bk = lb0[v].apply(plk), Bk1 = lb1[v].apply(plk);
+                double pdiff = (1. + R - T) * (Bk - Bk1) / (tau * (gamma1 + gamma2));
+                tmp1 = 1. - R;
+                arr(3, gi)[K] = R;
+                arr(4, gi)[K] = T;
+                arr(1, gi)[K] = ATM_PI * (tmp1 * Bk1 - T * Bk + pdiff);
+                arr(2, gi)[K] = ATM_PI * (tmp1 * Bk - T * Bk1 - pdiff);
+            }
+        }
+        __syncthreads();
+        if (lane < RG) {
+            const int gi = lane, gp = g0 + gi;
+            const double Bs = ls.apply(t.planck + 231 * gp);
+            adding(L, arr(3, gi), arr(4, gi), arr(1, gi), arr(2, gi), nullptr, arr(5, gi), arr(6, gi), arr(7, gi),
+                   0., emiss * ATM_PI * Bs, 1. - emiss);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const int k = lane + 64 * v;
+            if (k > L) continue;
+            for (int gi = 0; gi < RG; gi++) {
+                lwu[v] = lwu[v] + arr(6, gi)[k];
+                lwd[v] = lwd[v] + arr(7, gi)[k];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const int k = lane + 64 * v;
+        if (k <= L) net_flux[c0 + (size_t)k * pl] = (swd[v] + lwd[v]) - (swu[v] + lwu[v]);
+    }
     if (err) atomicOr(flag, 2);
 }
 
@@ -300,9 +428,8 @@ int atm_rad_alloc(atm_ctx *c) {
     if (c->rad) return ATM_OK;
     RadWork *w = new RadWork();
     c->rad = w;
-    w->slots = RSlots::count(g.L);
+    if (g.L > 64 * RMAXL) return atm_fail(c, ATM_ERR_ARGUMENT, "radiation supports nlay <= 256");
     size_t p = g.plane();
-    ATM_HIP(c, hipMalloc(&w->scr, sizeof(double) * p * w->slots));
     ATM_HIP(c, hipMalloc(&w->layer_T, sizeof(double) * p * g.L));
     ATM_HIP(c, hipMalloc(&w->level_T, sizeof(double) * p * (g.L + 1)));
     // gas-optics tables (binary32 values of the reference literals)
@@ -326,7 +453,6 @@ int atm_rad_alloc(atm_ctx *c) {
 
 void atm_rad_free(atm_ctx *c) {
     if (!c->rad) return;
-    (void)hipFree(c->rad->scr);
     (void)hipFree(c->rad->layer_T);
     (void)hipFree(c->rad->level_T);
     (void)hipFree(c->rad->tab);
@@ -344,8 +470,16 @@ int atm_rad_run(atm_ctx *c, double time) {
     hipLaunchKernelGGL(k_rad_profiles, grid3(g.nx + 6, g.ny + 6, 1, b), b, 0, c->stream, g, c->dp, c->pt,
                        c->plev, c->pkap, c->play, c->playkap, w->layer_T, w->level_T);
     ATM_HIP(c, hipMemsetAsync(c->net_flux, 0, sizeof(double) * g.plane() * (g.L + 1), c->stream));
-    hipLaunchKernelGGL(k_rad_columns, grid3(g.nx, g.ny, 1, b), b, 0, c->stream, g, w->t, c->prm.gravity, w->scr,
-                       c->dp, c->play, w->layer_T, w->level_T, c->ts, c->radius, c->net_flux, c->flag);
+    const size_t lds = sizeof(double) * RARR * RG * (g.L + 1);
+    const dim3 grid((unsigned)(g.nx * g.ny)), blk(64);
+#define ATM_RAD_ARGS g, w->t, c->prm.gravity, c->dp, c->play, w->layer_T, w->level_T, c->ts, c->radius, c->net_flux, c->flag
+    if (g.L <= 64)
+        hipLaunchKernelGGL(k_rad_columns<1>, grid, blk, lds, c->stream, ATM_RAD_ARGS);
+    else if (g.L <= 128)
+        hipLaunchKernelGGL(k_rad_columns<2>, grid, blk, lds, c->stream, ATM_RAD_ARGS);
+    else
+        hipLaunchKernelGGL(k_rad_columns<RMAXL>, grid, blk, lds, c->stream, ATM_RAD_ARGS);
+#undef ATM_RAD_ARGS
     atm_timer_end(c, ATM_T_RAD);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;
