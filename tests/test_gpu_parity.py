@@ -184,3 +184,50 @@ def test_driver_run_with_physics(gpu_available, path):
     _print(f"driver physics {path} 64x64x20 x20", **errs)
     for n, e in errs.items():
         assert e <= PARITY, (n, e)
+
+
+def test_radiation_table_range_error(gpu_available):
+    """A temperature outside the gas-optics table (the reference aborts,
+    mod_gas_optics.f90:4562-4592) comes back as ATM_ERR_TABLE_RANGE."""
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import AtmError, Model
+    nx, ny, nlay = 32, 16, 8
+    ic = make_initial_state(nx, ny, nlay, 100e3)
+    ic = dict(ic)
+    ic["pt"] = ic["pt"] * 3.0  # ~900 K: beyond every temperature row
+    m = Model(nx, ny, nlay, 100e3)
+    cfg = Config(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12,
+                 dt_remap=1e12, radiation_on=True, physics_on=False)
+    driver.init_model(m, ic, cfg)
+    with pytest.raises(AtmError) as e:
+        m.calculate_radiative_heating(0.0)
+    assert e.value.code == 3
+
+
+def test_call_order_errors(gpu_available):
+    """The state machine of the boundary: a second D step before the halo
+    fill, and physics before its allocation, are refused (status 5)."""
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import AtmError, Model
+    nx, ny, nlay = 32, 16, 8
+    m = Model(nx, ny, nlay, 100e3)
+    assert m.path == "fused"
+    with pytest.raises(AtmError) as e:
+        m.calculate_physics_tendencies()
+    assert e.value.code == 5
+    ic = make_initial_state(nx, ny, nlay, 100e3)
+    cfg = Config(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12,
+                 dt_remap=1e12, radiation_on=False, physics_on=False)
+    driver.init_model(m, ic, cfg)
+    m.cgrid_dynamics_step(30.0)
+    m.cgrid_halo_exchange()
+    m.dgrid_dynamics_step(60.0)
+    with pytest.raises(AtmError) as e:
+        m.dgrid_dynamics_step(60.0)
+    assert e.value.code == 5
+    m.dgrid_halo_exchange()
+    assert m.check_stability()
