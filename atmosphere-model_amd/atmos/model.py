@@ -27,7 +27,8 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_remap_init", "atm_remap", "atm_radiation_init", "atm_radiation",
            "atm_radiation_halo", "atm_halo_exchange", "atm_check_stability", "atm_sync",
            "atm_build_info", "atm_set_path", "atm_get_path", "atm_timer_enable", "atm_timer_read",
-           "atm_physics_init", "atm_physics_tendencies", "atm_apply_physics", "atm_physics_halo"]
+           "atm_physics_init", "atm_physics_tendencies", "atm_apply_physics", "atm_physics_halo",
+           "atm_output_init", "atm_accumulate_output", "atm_output_mean"]
 PATHS = {"fused": 0, "reference": 1}
 TIMERS = {"c1": 0, "c2": 1, "d1": 2, "d2": 3, "scan": 4, "halo": 5, "remap": 6, "radiation": 7,
           "physics": 8}
@@ -69,7 +70,7 @@ def load_library(path: str = LIB_PATH):
     for n in ["atm_destroy", "atm_init_prognostic_fields", "atm_initial_halo_exchange",
               "atm_init_sw_dyn", "atm_cgrid_halo", "atm_dgrid_halo", "atm_remap_init", "atm_remap",
               "atm_radiation_init", "atm_radiation_halo", "atm_sync", "atm_physics_init",
-              "atm_physics_tendencies", "atm_physics_halo"]:
+              "atm_physics_tendencies", "atm_physics_halo", "atm_output_init"]:
         getattr(L, n).argtypes = [vp]
     L.atm_set_stream.argtypes = [vp, vp]
     L.atm_set_coordinate.argtypes = [vp, dp, dp]
@@ -77,9 +78,10 @@ def load_library(path: str = LIB_PATH):
         getattr(L, n).argtypes = [vp, ip, dp]
     for n in ["atm_put_field_device", "atm_get_field_device"]:
         getattr(L, n).argtypes = [vp, ip, vp]
-    for n in ["atm_cgrid_step", "atm_dgrid_step", "atm_radiation", "atm_apply_physics"]:
+    for n in ["atm_cgrid_step", "atm_dgrid_step", "atm_radiation", "atm_apply_physics", "atm_accumulate_output"]:
         getattr(L, n).argtypes = [vp, C.c_double]
     L.atm_halo_exchange.argtypes = [vp, ip]
+    L.atm_output_mean.argtypes = [vp, dp]
     L.atm_check_stability.argtypes = [vp, C.POINTER(C.c_int)]
     L.atm_set_path.argtypes = [vp, ip]
     L.atm_get_path.argtypes = [vp, C.POINTER(C.c_int)]
@@ -188,6 +190,18 @@ class Model:
 
     def physics_halo_exchange(self):
         self._ck(self._L.atm_physics_halo(self._h))
+
+    def init_output(self):
+        self._ck(self._L.atm_output_init(self._h))
+
+    def accumulate_output(self, dt):
+        self._ck(self._L.atm_accumulate_output(self._h, dt))
+
+    def output_mean(self):
+        """write_output's time means as (6, nlay+1, ny, nx): dp, pt, u, v, ts, nflx."""
+        out = np.zeros((6, self.nlay + 1, self.ny, self.nx))
+        self._ck(self._L.atm_output_mean(self._h, _dptr(out)))
+        return out
 
     def calculate_radiative_heating(self, time):
         self._ck(self._L.atm_radiation(self._h, time))

@@ -72,6 +72,7 @@ static void release(atm_ctx *c) {
     atm_remap_free(c);
     atm_rad_free(c);
     atm_phys_free(c);
+    if (c->outf) { (void)hipFree(c->outf); c->outf = nullptr; }
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -434,6 +435,29 @@ int atm_apply_physics(atm_ctx *c, double dt) {
 int atm_physics_halo(atm_ctx *c) {
     CHECK_CTX(c);
     return atm_launch_halo(c, c->ts, 1);
+}
+
+int atm_output_init(atm_ctx *c) {
+    CHECK_CTX(c);
+    int rc = atm_output_alloc(c);
+    if (rc) return rc;
+    c->out_init = true;
+    return ATM_OK;
+}
+
+int atm_accumulate_output(atm_ctx *c, double dt) {
+    CHECK_CTX(c);
+    if (!c->out_init) return atm_fail(c, ATM_ERR_STATE, "atm_output_init first");
+    if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "accumulate after atm_dgrid_halo");
+    return atm_output_accumulate(c, dt);
+}
+
+int atm_output_mean(atm_ctx *c, double *host) {
+    CHECK_CTX(c);
+    if (!c->out_init) return atm_fail(c, ATM_ERR_STATE, "atm_output_init first");
+    if (!host) return atm_fail(c, ATM_ERR_ARGUMENT, "null output array");
+    if (!(c->acc_time > 0.)) return atm_fail(c, ATM_ERR_STATE, "nothing accumulated");
+    return atm_output_take_mean(c, host);
 }
 
 int atm_halo_exchange(atm_ctx *c, int id) {

@@ -206,6 +206,10 @@ struct atm_ctx {
     RemapWork *remap = nullptr;
     RadWork *rad = nullptr;
     PhysWork *phys = nullptr;
+    // time-mean output accumulators (nx, ny, nlay+1, 6) and their time
+    double *outf = nullptr;
+    double acc_time = 0.;
+    bool out_init = false;
 };
 
 // error helpers
@@ -251,6 +255,9 @@ int atm_phys_alloc(atm_ctx *c);
 void atm_phys_free(atm_ctx *c);
 int atm_phys_tendencies(atm_ctx *c);
 int atm_phys_apply(atm_ctx *c, double dt);
+int atm_output_alloc(atm_ctx *c);
+int atm_output_accumulate(atm_ctx *c, double dt);
+int atm_output_take_mean(atm_ctx *c, double *host);
 int atm_rad_alloc(atm_ctx *c);
 void atm_rad_free(atm_ctx *c);
 int atm_rad_run(atm_ctx *c, double time);

@@ -15,7 +15,8 @@ module atm_c_api
              atm_cgrid_halo, atm_dgrid_step, atm_dgrid_halo, atm_remap_init, &
              atm_remap, atm_radiation_init, atm_radiation, atm_radiation_halo, &
              atm_halo_exchange, atm_check_stability, atm_sync, atm_physics_init, &
-             atm_physics_tendencies, atm_apply_physics, atm_physics_halo
+             atm_physics_tendencies, atm_apply_physics, atm_physics_halo, &
+             atm_output_init, atm_accumulate_output, atm_output_mean
    public :: ATM_DP, ATM_PT, ATM_UD, ATM_VD, ATM_TS, ATM_GZ, ATM_NET_FLUX, &
              ATM_PLEV, ATM_PKAP, ATM_RADIUS
 
@@ -134,6 +135,20 @@ module atm_c_api
       integer(c_int) function atm_physics_halo(ctx) bind(c, name='atm_physics_halo')
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function atm_output_init(ctx) bind(c, name='atm_output_init')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function atm_accumulate_output(ctx, dt) bind(c, name='atm_accumulate_output')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: dt
+      end function
+      integer(c_int) function atm_output_mean(ctx, host) bind(c, name='atm_output_mean')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(inout) :: host(*)
       end function
       integer(c_int) function atm_halo_exchange(ctx, field) bind(c, name='atm_halo_exchange')
          import :: c_int, c_ptr

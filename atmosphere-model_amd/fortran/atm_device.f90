@@ -1,6 +1,7 @@
 ! atm_device.f90 -- the device context behind the drop-in modules
-! (mod_sw_dyn, mod_remap, mod_radiation, mod_sync in this folder) and the
-! coherence of the host's mod_fields arrays with the device copy.
+! (mod_sw_dyn, mod_remap, mod_radiation, mod_physics, mod_sync in this
+! folder), the coherence of the host's mod_fields arrays with the device
+! copy, and the device form of mod_output's time-mean accumulation.
 !
 ! The reference keeps its state in the module-global allocatables of
 ! mod_fields (src/mod_fields.f90:15-37) and its callers read and write them
@@ -37,7 +38,8 @@ module atm_device
    private
    public :: device_ctx, device_start, device_check, device_resident, &
              device_set_resident, device_push, device_pull, device_push_all, &
-             device_pull_all, device_check_stability, device_stop
+             device_pull_all, device_check_stability, device_stop, &
+             device_output_init, device_accumulate_output, device_output_mean
 
    type(c_ptr), save :: ctx = c_null_ptr
    logical, save :: resident = .false.
@@ -188,5 +190,25 @@ contains
       call device_check(atm_check_stability(device_ctx(), stable), 'check_stability')
       device_check_stability = stable /= 0
    end function device_check_stability
+
+   subroutine device_output_init()
+      ! init_output's accumulators (src/mod_output.f90:37-55) on the device
+      call device_check(atm_output_init(device_ctx()), 'init_output')
+   end subroutine device_output_init
+
+   subroutine device_accumulate_output(dt)
+      ! accumulate_output (src/mod_output.f90:94-130) on the device copy,
+      ! after dgrid_halo_exchange
+      real(rk), intent(in) :: dt
+      call device_check(atm_accumulate_output(device_ctx(), dt), 'accumulate_output')
+   end subroutine device_accumulate_output
+
+   subroutine device_output_mean(output_field)
+      ! the time means write_output writes (src/mod_output.f90:144-154):
+      ! output_field(isd:ied, jsd:jed, nlay+1, 6) of dp, pt, u, v, ts, nflx;
+      ! the device accumulators restart from zero
+      real(rk), intent(inout) :: output_field(:, :, :, :)
+      call device_check(atm_output_mean(device_ctx(), output_field), 'write_output')
+   end subroutine device_output_mean
 
 end module atm_device

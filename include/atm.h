@@ -130,6 +130,15 @@ int atm_physics_tendencies(atm_ctx *ctx);          /* calculate_physics_tendenci
 int atm_apply_physics(atm_ctx *ctx, double dt);    /* apply_physics_tendencies 162-185 */
 int atm_physics_halo(atm_ctx *ctx);                /* physics_halo_exchange 187-191 (ts) */
 
+/* mod_output's time-mean accumulation on the device (mod_output.f90:37-160,
+   without the netCDF writes).  atm_output_mean returns what write_output
+   writes -- output_field / accumulation_time as a (nx, ny, nlay+1, 6)
+   Fortran-order array of dp, pt, u, v (A grid), ts, net_flux -- and resets
+   the accumulators.  Blocks. */
+int atm_output_init(atm_ctx *ctx);                 /* init_output 37-80 (accumulators) */
+int atm_accumulate_output(atm_ctx *ctx, double dt); /* accumulate_output 94-130 */
+int atm_output_mean(atm_ctx *ctx, double *host);   /* write_output 144-154 */
+
 /* mod_sync generic halo_exchange (mod_sync.f90:14-17) for one field: a
    periodic fill of the 7-wide halo, edges and corners.  For ATM_GZ only
    levels 1..nlay are filled (the 5-argument specific's view of gz). */

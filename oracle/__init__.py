@@ -49,6 +49,9 @@ def lib():
                      "orc_physics_tendencies", "orc_physics_halo"]:
             getattr(L, name).argtypes = [vp]
         L.orc_apply_physics.argtypes = [vp, C.c_double]
+        L.orc_output_init.argtypes = [vp]
+        L.orc_accumulate_output.argtypes = [vp, C.c_double]
+        L.orc_output_mean.argtypes = [vp, dp]
         L.orc_physics_column_tend.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp]
         L.orc_cgrid_step.argtypes = [vp, C.c_double]
         L.orc_dgrid_step.argtypes = [vp, C.c_double]
@@ -200,6 +203,18 @@ class Oracle:
         u, v, t = (np.zeros(self.nlay) for _ in range(3))
         self._L.orc_physics_column_tend(self._h, i, j, _p(u), _p(v), _p(t))
         return u, v, t
+
+    def init_output(self):
+        self._L.orc_output_init(self._h)
+
+    def accumulate_output(self, dt):
+        self._L.orc_accumulate_output(self._h, dt)
+
+    def output_mean(self):
+        """write_output's time means as (6, nlay+1, ny, nx): dp, pt, u, v, ts, nflx."""
+        out = np.zeros((6, self.nlay + 1, self.ny, self.nx))
+        self._L.orc_output_mean(self._h, _p(out))
+        return out
 
     def halo_exchange(self, fid):
         a = self._views[fid]

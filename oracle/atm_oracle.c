@@ -61,6 +61,8 @@ struct orc {
        are kept over the whole halo'd plane, of which the reference allocates
        isd-2:ied+2 x jsd-2:jed+2 */
     double *dp_tend, *pt_tend, *u_tend, *v_tend, *ts_tend, *pua, *pva;
+    /* mod_output accumulators (src/mod_output.f90:20-22) */
+    double *output_field, accumulation_time;
 };
 
 /* Fortran-layout accessors */
@@ -139,7 +141,7 @@ void orc_destroy(orc_t *o) {
                       &o->v_remap, &o->pt_remap, &o->target_plev, &o->target_pkap,
                       &o->target_playkap, &o->lw_up, &o->lw_dn, &o->sw_up, &o->sw_dn,
                       &o->layer_T, &o->level_T, &o->dp_tend, &o->pt_tend, &o->u_tend,
-                      &o->v_tend, &o->ts_tend, &o->pua, &o->pva};
+                      &o->v_tend, &o->ts_tend, &o->pua, &o->pva, &o->output_field};
     for (size_t i = 0; i < sizeof(all) / sizeof(all[0]); i++) free(*all[i]);
     free(o);
 }
@@ -1417,4 +1419,51 @@ void orc_physics_column_tend(orc_t *o, int i, int j, double *u_tend, double *v_t
         v_tend[k - 1] = F3(o->v_tend, i, j, k);
         pt_tend[k - 1] = F3(o->pt_tend, i, j, k);
     }
+}
+
+/* ------------------------------------------------------------------------
+ * mod_output accumulation (src/mod_output.f90:37-160 without the netCDF
+ * writes): output_field(isd:ied, jsd:jed, nlay+1, 6) with the fields
+ * dp, pt, u, v, ts, nflx.  compensated_sum (132-142) never stores its
+ * compensation term (Q9), so compensation stays 0 and the sum is plain.
+ * ------------------------------------------------------------------------ */
+#define OUT(o, i, j, k, f) \
+    (o)->output_field[(((size_t)(f) * ((o)->nlay + 1) + (size_t)((k) - 1)) * (o)->ny + (size_t)((j) - (o)->jsd)) * (o)->nx + (size_t)((i) - (o)->isd)]
+
+void orc_output_init(orc_t *o) { /* init_output 37-55 */
+    size_t n = (size_t)o->nx * o->ny * (o->nlay + 1) * 6;
+    if (!o->output_field) o->output_field = zalloc(n);
+    memset(o->output_field, 0, n * sizeof(double));
+    o->accumulation_time = 0.;
+}
+
+static void compensated_sum(orc_t *o, double val, int i, int j, int k, int f) { /* 132-142 */
+    double y = val - 0.; /* compensation(i, j, k, idx) is never updated */
+    double sum = OUT(o, i, j, k, f) + y;
+    OUT(o, i, j, k, f) = sum;
+}
+
+void orc_accumulate_output(orc_t *o, double dt) { /* accumulate_output 94-130 */
+    const int L = o->nlay;
+    for (int i = o->isd; i <= o->ied; i++)
+        for (int j = o->jsd; j <= o->jed; j++) {
+            compensated_sum(o, dt * F2(o->ts, i, j), i, j, 1, 4);
+            for (int k = 1; k <= L; k++) {
+                compensated_sum(o, dt * F3(o->dp, i, j, k), i, j, k, 0);
+                compensated_sum(o, dt * F3(o->pt, i, j, k), i, j, k, 1);
+                compensated_sum(o, dt * .5 * (F3(o->ud, i, j, k) + F3(o->ud, i, j + 1, k)), i, j, k, 2);
+                compensated_sum(o, dt * .5 * (F3(o->vd, i, j, k) + F3(o->vd, i + 1, j, k)), i, j, k, 3);
+                compensated_sum(o, dt * F3(o->net_flux, i, j, k), i, j, k, 5);
+            }
+            compensated_sum(o, dt * F3(o->net_flux, i, j, L + 1), i, j, L + 1, 5);
+        }
+    o->accumulation_time = o->accumulation_time + dt;
+}
+
+/* write_output 144-154: the time means that are written, then the reset */
+void orc_output_mean(orc_t *o, double *out) {
+    size_t n = (size_t)o->nx * o->ny * (o->nlay + 1) * 6;
+    for (size_t t = 0; t < n; t++) out[t] = o->output_field[t] / o->accumulation_time;
+    memset(o->output_field, 0, n * sizeof(double));
+    o->accumulation_time = 0.;
 }

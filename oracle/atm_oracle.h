@@ -5,7 +5,8 @@
  * restatement of the reference Fortran model's per-timestep hot path
  * (felix-goddard/atmosphere-model: src/mod_sw_dyn.f90, src/mod_remap.f90,
  * src/mod_radiation.f90, src/mod_gas_optics.f90, src/mod_solar.f90,
- * src/mod_physics.f90, src/mod_sync.f90 on a single image).  It is the checker the HIP library is
+ * src/mod_physics.f90, src/mod_output.f90 accumulation, src/mod_sync.f90 on a
+ * single image).  It is the checker the HIP library is
  * compared against.  Only tests/, __graft_entry__.smoke() and the CPU-baseline
  * leg of bench.py may load it; the product path never does.
  *
@@ -53,6 +54,9 @@ void orc_allocate_physics(orc_t *o);            /* allocate_physics_arrays */
 void orc_physics_tendencies(orc_t *o);          /* calculate_physics_tendencies */
 void orc_apply_physics(orc_t *o, double dt);    /* apply_physics_tendencies */
 void orc_physics_halo(orc_t *o);                /* physics_halo_exchange */
+void orc_output_init(orc_t *o);                 /* init_output (accumulators only) */
+void orc_accumulate_output(orc_t *o, double dt); /* accumulate_output */
+void orc_output_mean(orc_t *o, double *out);    /* write_output's means, (nx, ny, nlay+1, 6), then reset */
 void orc_physics_column_tend(orc_t *o, int i, int j, double *u_tend, double *v_tend,
                              double *pt_tend); /* tendencies of column (i, j), 1-based i, j */
 

@@ -8,7 +8,8 @@
 !   bk(nlay+1); per step float64 time, dt and int32 radiation, physics,
 !   remap flags; then the halo'd arrays dp, pt, ud, vd, ts, gz, plev, pkap,
 !   net_flux, radius.
-!   output: int32 steps completed, int32 stable; dp, pt, ud, vd, net_flux, ts.
+!   output: int32 steps completed, int32 stable; dp, pt, ud, vd, net_flux, ts;
+!   the time means of the device output accumulation (nx, ny, nlay+1, 6).
 program drop_in_host
    use iso_fortran_env, only: int32, real64
    use mod_kinds, only: ik, rk
@@ -25,7 +26,8 @@ program drop_in_host
    use mod_remap, only: allocate_remapping_arrays, perform_vertical_remapping
    use mod_physics, only: allocate_physics_arrays, physics_halo_exchange, &
                           calculate_physics_tendencies, apply_physics_tendencies
-   use atm_device, only: device_set_resident, device_check_stability, device_pull_all, device_stop
+   use atm_device, only: device_set_resident, device_check_stability, device_pull_all, device_stop, &
+                         device_output_init, device_accumulate_output, device_output_mean
    implicit none
 
    character(len=512) :: fin, fout
@@ -35,6 +37,7 @@ program drop_in_host
    integer(int32), allocatable :: rad(:), phys(:), remap(:)
    logical :: stable
    integer :: u
+   real(real64), allocatable :: means(:, :, :, :)
 
    call get_command_argument(1, fin)
    call get_command_argument(2, fout)
@@ -67,6 +70,7 @@ program drop_in_host
    call allocate_physics_arrays()
    call allocate_remapping_arrays()
    if (resident /= 0) call device_set_resident(.true.)
+   call device_output_init()
 
    done = 0
    stable = .true.
@@ -89,8 +93,11 @@ program drop_in_host
          stable = check_stability()
       end if
       if (.not. stable) exit
+      call device_accumulate_output(dt(n))
       done = n
    end do
+   allocate (means(config%nx, config%ny, config%nlay + 1, 6))
+   call device_output_mean(means)
    if (resident /= 0) call device_pull_all()
    call device_stop()
 
@@ -98,6 +105,7 @@ program drop_in_host
    open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
    write (u) done, stable_out
    write (u) dp, pt, ud, vd, net_flux, ts
+   write (u) means
    close (u)
    print '(a,i0,a)', 'drop_in_host: ', done, ' steps'
 end program drop_in_host

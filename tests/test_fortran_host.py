@@ -7,8 +7,8 @@ tests/fortran/drop_in_host runs run_model's call order
 step schedule (dt, radiation, physics and remap steps) the host driver's
 calculate_dt produced, in both coherence modes: mirrored (every call moves
 the host arrays) and resident (the state stays on the device).  Its final
-dp, pt, ud, vd, net_flux and ts must match the oracle's to 1e-12
-norm-relative.
+dp, pt, ud, vd, net_flux and ts, and the time means of the device output
+accumulation, must match the oracle's to 1e-12 norm-relative.
 """
 import os
 import subprocess
@@ -97,9 +97,12 @@ def _run_case(tmp_path, nx, ny, nlay, dx, steps, radiation, physics, resident):
     inp = str(tmp_path / f"in_{resident}.bin")
     snapshot = [np.array(o.field(f), dtype=np.float64) for f in FIELDS_IN]
     rec = _Recorder(o)
+    o.init_output()
     time_s = 0.0
     for _ in range(steps):
-        time_s += driver.step(rec, cfg, st, time_s)
+        dt = driver.step(rec, cfg, st, time_s)
+        time_s += dt
+        o.accumulate_output(dt)
     _write_input(inp, o, dx, cfg, rec, resident, params)
     with open(inp, "ab") as fh:
         for a in snapshot:
@@ -113,6 +116,7 @@ def _run_case(tmp_path, nx, ny, nlay, dx, steps, radiation, physics, resident):
         for f in (O.DP, O.PT, O.UD, O.VD, O.NET_FLUX, O.TS):
             shape = o.field(f).shape
             got[f] = np.fromfile(fh, np.float64, int(np.prod(shape))).reshape(shape)
+        got_means = np.fromfile(fh, np.float64, 6 * (nlay + 1) * ny * nx).reshape(6, nlay + 1, ny, nx)
     assert done == steps and stable == 1
     h = O.HALO
     worst = {}
@@ -120,6 +124,7 @@ def _run_case(tmp_path, nx, ny, nlay, dx, steps, radiation, physics, resident):
         a = got[f][..., h:h + ny, h:h + nx]
         b = o.interior(f)
         worst[f] = norm_rel(a, b)
+    worst["output_means"] = norm_rel(got_means, o.output_mean())
     return worst, rec
 
 

@@ -231,3 +231,36 @@ def test_call_order_errors(gpu_available):
     assert e.value.code == 5
     m.dgrid_halo_exchange()
     assert m.check_stability()
+
+
+def test_output_accumulation(gpu_available):
+    """accumulate_output every step and write_output's means, GPU vs oracle,
+    with radiation on (net_flux) over 6 steps."""
+    import oracle as O
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+    nx, ny, nlay = 32, 16, 12
+    ic = make_initial_state(nx, ny, nlay, 100e3)
+    cfg = dict(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=120.0, dt_physics=1e12,
+               dt_remap=180.0, radiation_on=True, physics_on=False)
+    gpu, orc = Model(nx, ny, nlay, 100e3), O.Oracle(nx, ny, nlay, 100e3, 100e3)
+    means = []
+    for b in (gpu, orc):
+        c = Config(**cfg)
+        st = driver.init_model(b, ic, c)
+        b.init_output()
+        t = 0.0
+        for _ in range(6):
+            dt = driver.step(b, c, st, t)
+            assert b.check_stability()
+            t += dt
+            b.accumulate_output(dt)
+        means.append(b.output_mean())
+    a, o = means
+    errs = {f: float(np.max(np.abs(a[i] - o[i])) / max(np.max(np.abs(o[i])), 1e-300))
+            for i, f in enumerate(("dp", "pt", "u", "v", "ts", "nflx"))}
+    _print("output means x6", **errs)
+    for f, e in errs.items():
+        assert e <= PARITY, (f, e)
