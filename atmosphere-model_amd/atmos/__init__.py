@@ -6,7 +6,9 @@
 * `driver`: init_model / run_model / calculate_dt of src/mod_model.f90.
 * `config`: config.nml namelists and ISO-8601 durations.
 * `ic`: seeded synthetic initial conditions.
+* `ncio`: initial, output and restart files in the netCDF classic formats
+  (libatmos_io.so).
 """
-from . import config, driver, ic, model  # noqa: F401
+from . import config, driver, ic, model, ncio  # noqa: F401
 
-__all__ = ["config", "driver", "ic", "model"]
+__all__ = ["config", "driver", "ic", "model", "ncio"]
