@@ -116,13 +116,14 @@ __global__ void k_rad_profiles(Geo g, const double *__restrict__ dp, const doubl
 //            sources) into LDS.  The table positions and interpolation
 //            weights depend only on (p, T), so they are found once per
 //            layer and shared by all g-points and both tables;
-//   chains   lanes 0..RG-1, one per g-point, run the recurrences down and
-//            up the column (shortwave beam, the adding method) over LDS;
+//   chains   16 lanes per g-point run the recurrences down and up the
+//            column (shortwave beam, the adding method) over LDS, each on
+//            a block of layers (beam_group, adding_group);
 //   sums     every lane adds the RG g-points' fluxes of its levels into its
 //            accumulators in g order, the reference's summation order.
 // The pass repeats for the longwave, then for the next RG g-points.
 // ---------------------------------------------------------------------------
-constexpr int RG = 4;       // g-points per pass
+constexpr int RG = 4;       // g-points per pass (64 lanes = RG groups of 16)
 constexpr int RMAXL = 4;    // layers per lane at most: L <= 64 * RMAXL
 constexpr int RARR = 8;     // LDS arrays of L + 1 per g-point
 
@@ -194,37 +195,143 @@ __device__ __forceinline__ LinPT lin_pT(const Tables &t, double p, double T, int
     return l;
 }
 
-// calculate_fluxes (349-381) on LDS arrays of one g-point.  On return
-// alb[k] holds the upward and emi[k] the downward flux of level k (the
-// direct beam B added to the latter when B != nullptr).
-__device__ void adding(int L, const double *R, const double *T, const double *Su, const double *Sd,
-                       const double *B, double *bet, double *alb, double *emi, double toa, double sfc_up,
-                       double sfc_alb) {
-    alb[0] = sfc_alb;
-    emi[0] = sfc_up;
-    double a = sfc_alb, e = sfc_up;
-    for (int k = 0; k < L; k++) {
+// ---------------------------------------------------------------------------
+// The column recurrences on 16 lanes per g-point.  Lane j of the group owns
+// the layers [lo, hi) of block j (ceil(L/16) layers each).  A block is
+// first reduced to one composite layer, then the composites are combined
+// across the group with 4-step shuffle scans (from the surface up and from
+// the top down); each lane then runs the reference's recurrences over its
+// own layers from the boundary values of its block.  Values agree with the
+// sequential column to rounding (the grouping of the products differs).
+// ---------------------------------------------------------------------------
+constexpr int CG = 16;  // lanes per g-point
+static_assert(RG * CG == 64, "one wavefront holds the RG g-points' lane groups");
+
+// a stack of layers: reflectance for light from above (rt) and from below
+// (rb), transmittance, emission up out of its top and down out of its bottom
+struct Comp {
+    double rt, rb, t, su, sd;
+};
+
+__device__ __forceinline__ Comp comp_identity() { return Comp{0., 0., 1., 0., 0.}; }
+
+// stack u on top of stack w (the adding of two layers, 349-381).  REFL
+// false: no layer reflects light from below (u.rb = 0), so the multiple-
+// reflection factor d is exactly 1.
+template <bool REFL = true>
+__device__ __forceinline__ Comp comp_over(const Comp &u, const Comp &w) {
+    const double d = REFL ? 1. / (1. - u.rb * w.rt) : 1.;
+    Comp c;
+    c.rt = u.rt + u.t * u.t * w.rt * d;
+    c.rb = w.rb + w.t * w.t * u.rb * d;
+    c.t = u.t * w.t * d;
+    c.su = u.su + u.t * (w.su + w.rt * u.sd) * d;
+    c.sd = w.sd + w.t * (u.sd + u.rb * w.su) * d;
+    return c;
+}
+
+__device__ __forceinline__ Comp comp_shfl_up(const Comp &c, unsigned d) {
+    return Comp{__shfl_up(c.rt, d, CG), __shfl_up(c.rb, d, CG), __shfl_up(c.t, d, CG), __shfl_up(c.su, d, CG),
+                __shfl_up(c.sd, d, CG)};
+}
+
+__device__ __forceinline__ Comp comp_shfl_down(const Comp &c, unsigned d) {
+    return Comp{__shfl_down(c.rt, d, CG), __shfl_down(c.rb, d, CG), __shfl_down(c.t, d, CG),
+                __shfl_down(c.su, d, CG), __shfl_down(c.sd, d, CG)};
+}
+
+// calculate_fluxes (349-381) of one g-point on lane j of its group.  On
+// entry R, T, Su, Sd hold the layer properties; on return up[k] and dn[k]
+// hold the upward and downward fluxes of level k (the direct beam B added
+// to the latter when B != nullptr).  bet is scratch of L entries.  REFL
+// false: every R is 0 (the longwave), so each 1 / (1 - R a) is exactly 1.
+template <bool REFL>
+__device__ void adding_group(int L, int j, const double *R, const double *T, const double *Su, const double *Sd,
+                             const double *B, double *bet, double *up, double *dn, double toa, double sfc_up,
+                             double sfc_alb) {
+    const int nb = (L + CG - 1) / CG;
+    const int lo = min(j * nb, L), hi = min(lo + nb, L);
+    // the block as one layer, built from its bottom
+    Comp c = comp_identity();
+    for (int k = lo; k < hi; k++) c = comp_over<REFL>(Comp{R[k], R[k], T[k], Su[k], Sd[k]}, c);
+    // the surface is a layer that transmits nothing
+    const Comp sfc{sfc_alb, 0., 0., sfc_up, 0.};
+    Comp v = j == 0 ? comp_over<REFL>(c, sfc) : c;  // blocks 0..j over the surface
+    Comp u = c;                               // blocks j..15
+#pragma unroll
+    for (unsigned d = 1; d < CG; d <<= 1) {
+        const Comp w = comp_shfl_up(v, d);
+        if (j >= (int)d) v = comp_over<REFL>(v, w);
+        const Comp x = comp_shfl_down(u, d);
+        if (j + (int)d < CG) u = comp_over<REFL>(x, u);
+    }
+    Comp below = comp_shfl_up(v, 1), above = comp_shfl_down(u, 1);
+    if (j == 0) below = sfc;
+    if (j == CG - 1) above = comp_identity();
+    // albedo and emission of everything below, up the block (the
+    // reference's loop body)
+    double a = below.rt, e = below.su;
+    if (lo < hi) {
+        up[lo] = a;
+        dn[lo] = e;
+    }
+    for (int k = lo; k < hi; k++) {
         const double Rk = R[k], Tk = T[k];
-        const double b = 1. / (1. - Rk * a);
+        const double b = REFL ? 1. / (1. - Rk * a) : 1.;
         const double an = Rk + (Tk * Tk) * b * a;
         const double en = Su[k] + Tk * b * (e + Sd[k] * a);
         bet[k] = b;
-        alb[k + 1] = an;
-        emi[k + 1] = en;
+        if (k + 1 < hi) {
+            up[k + 1] = an;
+            dn[k + 1] = en;
+        }
         a = an;
         e = en;
     }
-    double fd = toa;
-    alb[L] = a * toa + e;
-    emi[L] = B ? fd + B[L] : fd;
-    for (int k = L; k >= 1; k--) {
-        const double b = bet[k - 1], em = emi[k - 1];
+    // downward flux at the top of the block from the stack above it (the
+    // top block: toa itself)
+    double fd = above.sd + above.t * toa + above.rb * e;
+    if (REFL) fd = fd / (1. - above.rb * a);
+    if (lo < hi && hi == L) {
+        up[L] = a * toa + e;
+        dn[L] = B ? fd + B[L] : fd;
+    }
+    for (int k = hi; k > lo; k--) {
+        const double b = bet[k - 1], em = dn[k - 1];
         const double fdn = b * (Sd[k - 1] + T[k - 1] * fd + R[k - 1] * em);
-        const double fup = alb[k - 1] * fdn + em;
-        alb[k - 1] = fup;
-        emi[k - 1] = B ? fdn + B[k - 1] : fdn;
+        const double fup = up[k - 1] * fdn + em;
+        up[k - 1] = fup;
+        dn[k - 1] = B ? fdn + B[k - 1] : fdn;
         fd = fdn;
     }
+}
+
+// the direct beam down the column (221-249) on lane j of a g-point's group:
+// on entry B holds the layers' direct transmittance, Su and Sd the direct-
+// to-diffuse reflectance and transmittance; on return B[k] is the beam at
+// level k and Su, Sd the layers' diffuse sources.  Returns B[L].
+__device__ double beam_group(int L, int j, double toa_dir, double *B, double *Su, double *Sd) {
+    const int nb = (L + CG - 1) / CG;
+    const int lo = min(j * nb, L), hi = min(lo + nb, L);
+    double p = 1.;
+    for (int k = hi - 1; k >= lo; k--) p = p * B[k];
+#pragma unroll
+    for (unsigned d = 1; d < CG; d <<= 1) {  // product of blocks j..15
+        const double w = __shfl_down(p, d, CG);
+        if (j + (int)d < CG) p = p * w;
+    }
+    double above = __shfl_down(p, 1, CG);
+    if (j == CG - 1) above = 1.;
+    double bk1 = toa_dir * above;
+    if (lo < hi && hi == L) B[L] = toa_dir;
+    for (int k = hi - 1; k >= lo; k--) {
+        const double bk = B[k] * bk1;
+        Su[k] = Su[k] * bk1;
+        Sd[k] = Sd[k] * bk1;
+        B[k] = bk;
+        bk1 = bk;
+    }
+    return toa_dir;
 }
 
 // NL layers and NL + 1 levels per lane (L <= 64 NL)
@@ -279,69 +386,49 @@ __global__ void __launch_bounds__(64) k_rad_columns(Geo g, Tables t, double grav
             for (int gi = 0; gi < RG; gi++) {
                 const int gp = g0 + gi;
                 const double k_sca = (double)t.ray[gp];
-                double k_abs = lpt[v].apply(t.sw + 6 * 53 * gp);
-                double k_ext = k_abs + k_sca;
-                double tau = k_ext * dmv[v];
-                double ssa = k_sca / k_ext;
-                double asy = 0.;
-                {  // delta_eddington_rescaling (383-398)
-                    double f = asy * asy;
-                    double coef = 1. - ssa * f;
-                    tau = tau * coef;
-                    ssa = ssa * (1. - f) / coef;
-                    asy = asy / (1. + asy);
-                }
-                double asy2 = asy * asy;
-                double back = (2. - 3. * asy * mu0) / 4.;
-                double tmp1 = 4. * (1. - asy2 * (1. - mu0));
-                double gamma3 = ssa * asy2;
-                double gamma4 = 4. * back + 3. * asy;
-                double tmp2 = 3. * asy;
-                double gamma1 = (7. - 3. * asy2 - ssa * (4. + tmp2) + gamma3 * gamma4) / tmp1;
-                double gamma2 = (-1. + asy2 + ssa * (4. - tmp2) + gamma3 * (gamma4 - 4.)) / tmp1;
-                gamma3 = back;
-                gamma4 = 1. - gamma3;
-                double alpha1 = gamma1 * gamma4 + gamma2 * gamma3;
-                double alpha2 = gamma2 * gamma3 + gamma2 * gamma4;
+                const double k_abs = lpt[v].apply(t.sw + 6 * 53 * gp);
+                const double k_ext = k_abs + k_sca;
+                const double tau = k_ext * dmv[v];
+                const double ssa = k_sca / k_ext;
+                // The reference fixes the asymmetry at 0 (186) and the zenith
+                // angle at 0 (107), so mu0 = cos(0) = 1: the delta-Eddington
+                // rescaling (383-398) leaves tau and ssa unchanged, and the
+                // coefficients below are the reference's expressions (195-260)
+                // with those values substituted; each substitution is exact for
+                // finite ssa (x*1, x/1, x+0, x*0 -> the same double).
+                const double gamma1 = (7. - ssa * 4.) * 0.25;   // gamma3 = gamma4 = 1/2
+                const double gamma2 = (-1. + ssa * 4.) * 0.25;
+                const double alpha1 = gamma1 * 0.5 + gamma2 * 0.5;
+                const double alpha2 = gamma2 * 0.5 + gamma2 * 0.5;
                 double lambda = sqrt(gamma1 * gamma1 - gamma2 * gamma2);
-                double lam_mu = lambda * mu0;
-                double e_minus = exp(-lambda * tau);
+                const double e_minus = exp(-lambda * tau);
                 double e_2minus = e_minus * e_minus;
-                double Tdir = exp(-tau / mu0);
-                tmp1 = ssa / ((1. - lam_mu * lam_mu) * (lambda + gamma1 + (lambda - gamma1) * e_2minus));
-                tmp2 = lambda * gamma3;
-                double Rdif = tmp1 * ((1. - lam_mu) * (alpha2 + tmp2) - (1. + lam_mu) * (alpha2 - tmp2) * e_2minus -
-                                      2. * lambda * (gamma3 - alpha2 * mu0) * e_minus * Tdir);
-                tmp2 = lambda * gamma4;
-                double Tdif = tmp1 * (2. * lambda * (gamma4 + alpha1 * mu0) * e_minus -
-                                      Tdir * ((1. + lam_mu) * (alpha1 + tmp2) - (1. - lam_mu) * (alpha1 - tmp2) * e_2minus));
-                back = (4. + asy) / (8. * (1. + asy));
-                gamma1 = 2. * (1. - ssa * (1. - back));
-                gamma2 = 2. * ssa * back;
-                lambda = sqrt(gamma1 * gamma1 - gamma2 * gamma2);
+                const double Tdir = exp(-tau);
+                double tmp1 = ssa / ((1. - lambda * lambda) * (lambda + gamma1 + (lambda - gamma1) * e_2minus));
+                double tmp2 = lambda * 0.5;
+                const double Rdif = tmp1 * ((1. - lambda) * (alpha2 + tmp2) - (1. + lambda) * (alpha2 - tmp2) * e_2minus -
+                                            2. * lambda * (0.5 - alpha2) * e_minus * Tdir);
+                const double Tdif = tmp1 * (2. * lambda * (0.5 + alpha1) * e_minus -
+                                            Tdir * ((1. + lambda) * (alpha1 + tmp2) - (1. - lambda) * (alpha1 - tmp2) * e_2minus));
+                // diffuse: backscatter 1/2, gamma1 = 2 (1 - ssa/2), gamma2 = ssa
+                const double g1 = 2. * (1. - ssa * 0.5), g2 = ssa;
+                lambda = sqrt(g1 * g1 - g2 * g2);
                 e_2minus = e_minus * e_minus;  // Q6: e_minus of the direct-beam lambda
-                tmp1 = 1. / (lambda + gamma1 + (lambda - gamma1) * e_2minus);
+                tmp1 = 1. / (lambda + g1 + (lambda - g1) * e_2minus);
                 arr(0, gi)[K] = Tdir;
                 arr(1, gi)[K] = Rdif;
                 arr(2, gi)[K] = Tdif;
-                arr(3, gi)[K] = tmp1 * gamma2 * (1. - e_2minus);
+                arr(3, gi)[K] = tmp1 * g2 * (1. - e_2minus);
                 arr(4, gi)[K] = 2. * tmp1 * lambda * e_minus;
             }
         }
         __syncthreads();
-        if (lane < RG) {  // beam down the column (221-249), then the adding method
-            const int gi = lane, gp = g0 + gi;
+        {  // beam down the column (221-249), then the adding method: CG lanes per g-point
+            const int gi = lane / CG, j = lane % CG, gp = g0 + gi;
             double *B = arr(0, gi), *Su = arr(1, gi), *Sd = arr(2, gi);
-            double bk1 = mu0 * ((double)t.frac[gp] * irr);
-            B[L] = bk1;
-            for (int K = L - 1; K >= 0; K--) {
-                const double bk = B[K] * bk1;
-                Su[K] = Su[K] * bk1;
-                Sd[K] = Sd[K] * bk1;
-                B[K] = bk;
-                bk1 = bk;
-            }
-            adding(L, arr(3, gi), arr(4, gi), Su, Sd, B, arr(5, gi), arr(6, gi), arr(7, gi), B[L], 0., sfc_alb);
+            const double toa = beam_group(L, j, mu0 * ((double)t.frac[gp] * irr), B, Su, Sd);
+            adding_group<true>(L, j, arr(3, gi), arr(4, gi), Su, Sd, B, arr(5, gi), arr(6, gi), arr(7, gi), toa, 0.,
+                               sfc_alb);
         }
         __syncthreads();
 #pragma unroll
@@ -363,43 +450,29 @@ __global__ void __launch_bounds__(64) k_rad_columns(Geo g, Tables t, double grav
             for (int gi = 0; gi < RG; gi++) {
                 const int gp = g0 + gi;
                 const float *plk = t.planck + 231 * gp;
-                double k_abs = lpt[v].apply(t.lw + 6 * 53 * gp);
-                double k_sca = 0.;
-                double k_ext = k_abs + k_sca;
-                double tau = k_ext * dmv[v];
-                double ssa = k_sca / k_ext;
-                double asy = 0.;
-                {
-                    double f = asy * asy;
-                    double coef = 1. - ssa * f;
-                    tau = tau * coef;
-                    ssa = ssa * (1. - f) / coef;
-                    asy = asy / (1. + asy);
-                }
-                double back = (4. - 3. * asy) / 8.;
-                double gamma1 = 2. * (1. - ssa * (1. - back));
-                double gamma2 = 2. * ssa * back;
-                double lambda = sqrt(gamma1 * gamma1 - gamma2 * gamma2);
-                double e_minus = exp(-lambda * tau);
-                double e_2minus = e_minus * e_minus;
-                double tmp1 = 1. / (lambda + gamma1 + (lambda - gamma1) * e_2minus);
-                double R = tmp1 * gamma2 * (1. - e_2minus);
-                double T = 2. * tmp1 * lambda * e_minus;
+                // No scattering (k_sca = 0, 291) and asymmetry 0: with the
+                // table's k_abs > 0, ssa = 0 exactly, so backscatter = 1/2,
+                // gamma1 = 2, gamma2 = 0, lambda = 2, tmp1 = 1/4 and the
+                // reference's reflectance is 0 and its transmittance e_minus
+                // (the expressions of 298-339 with those values, exact).
+                const double tau = lpt[v].apply(t.lw + 6 * 53 * gp) * dmv[v];
+                const double e_minus = exp(-2. * tau);
+                const double R = 0.;
+                const double T = e_minus;
                 const double Bk = lb0[v].apply(plk), Bk1 = lb1[v].apply(plk);
-                double pdiff = (1. + R - T) * (Bk - Bk1) / (tau * (gamma1 + gamma2));
-                tmp1 = 1. - R;
+                const double pdiff = (1. - T) * (Bk - Bk1) / (tau * 2.);
                 arr(3, gi)[K] = R;
                 arr(4, gi)[K] = T;
-                arr(1, gi)[K] = ATM_PI * (tmp1 * Bk1 - T * Bk + pdiff);
-                arr(2, gi)[K] = ATM_PI * (tmp1 * Bk - T * Bk1 - pdiff);
+                arr(1, gi)[K] = ATM_PI * (Bk1 - T * Bk + pdiff);  // tmp1 = 1 - R = 1
+                arr(2, gi)[K] = ATM_PI * (Bk - T * Bk1 - pdiff);
             }
         }
         __syncthreads();
-        if (lane < RG) {
-            const int gi = lane, gp = g0 + gi;
+        {
+            const int gi = lane / CG, j = lane % CG, gp = g0 + gi;
             const double Bs = ls.apply(t.planck + 231 * gp);
-            adding(L, arr(3, gi), arr(4, gi), arr(1, gi), arr(2, gi), nullptr, arr(5, gi), arr(6, gi), arr(7, gi),
-                   0., emiss * ATM_PI * Bs, 1. - emiss);
+            adding_group<false>(L, j, arr(3, gi), arr(4, gi), arr(1, gi), arr(2, gi), nullptr, arr(5, gi),
+                                arr(6, gi), arr(7, gi), 0., emiss * ATM_PI * Bs, 1. - emiss);
         }
         __syncthreads();
 #pragma unroll
