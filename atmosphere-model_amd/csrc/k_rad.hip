@@ -125,19 +125,30 @@ __global__ void k_rad_profiles(Geo g, const double *__restrict__ dp, const doubl
 // ---------------------------------------------------------------------------
 constexpr int RG = 4;       // g-points per pass (64 lanes = RG groups of 16)
 constexpr int RMAXL = 4;    // layers per lane at most: L <= 64 * RMAXL
-constexpr int RARR = 8;     // LDS arrays of L + 1 per g-point
+constexpr int RARR = 5;     // LDS arrays of L + 1 per g-point
+
+// v / d with z the correctly rounded 1 / d: the quotient q0 = v z is
+// within an ulp of v / d, and one correction with the exact residual
+// v - q0 d gives the correctly rounded quotient (Markstein's theorem, round
+// to nearest, no overflow or underflow), as the division v / d would.  A
+// position's divisor is shared by the 16 g-points and both tables, so its
+// reciprocal is formed once.
+__device__ __forceinline__ double div_by(double v, double d, double z) {
+    const double q0 = v * z;
+    return fma(fma(-q0, d, v), z, q0);
+}
 
 // linear interpolation on one table axis (lookup_1d 4556-4579), shared by
 // every table on that axis
 struct Lin {
     int k = 0;
     bool hit = false, oor = false;
-    double d1 = 0., d2 = 0., d3 = 1.;  // xm - x, x - xk, xm - xk
+    double d1 = 0., d2 = 0., d3 = 1., z = 1.;  // xm - x, x - xk, xm - xk, 1 / (xm - xk)
     __device__ double apply(const float *ys) const {
         if (oor) return 0.;
         if (hit) return (double)ys[k];
         const double yk = ys[k], ym = ys[k - 1];
-        return (yk * d1 + ym * d2) / d3;
+        return div_by(yk * d1 + ym * d2, d3, z);
     }
 };
 
@@ -156,6 +167,7 @@ __device__ __forceinline__ Lin lin_1d(double x, const float *xs, int n, int *err
         l.d1 = xm - x;
         l.d2 = x - xk;
         l.d3 = xm - xk;
+        l.z = 1. / l.d3;
     }
     return l;
 }
@@ -165,13 +177,13 @@ struct LinPT {
     Lin t1, t2;  // rows k and k - 1
     int k = 0;
     bool hit = false, oor = false;
-    double d1 = 0., d2 = 0., d3 = 1.;
+    double d1 = 0., d2 = 0., d3 = 1., z = 1.;
     __device__ double apply(const float *table) const {
         if (oor) return 0.;
         if (hit) return t1.apply(table + 6 * k);
         const double v1 = t1.apply(table + 6 * k);
         const double v2 = t2.apply(table + 6 * (k - 1));
-        return (v1 * d1 + v2 * d2) / d3;
+        return div_by(v1 * d1 + v2 * d2, d3, z);
     }
 };
 
@@ -191,6 +203,7 @@ __device__ __forceinline__ LinPT lin_pT(const Tables &t, double p, double T, int
         l.d1 = pm - p;
         l.d2 = p - pk;
         l.d3 = pm - pk;
+        l.z = 1. / l.d3;
     }
     return l;
 }
@@ -241,68 +254,82 @@ __device__ __forceinline__ Comp comp_shfl_down(const Comp &c, unsigned d) {
 }
 
 // calculate_fluxes (349-381) of one g-point on lane j of its group.  On
-// entry R, T, Su, Sd hold the layer properties; on return up[k] and dn[k]
+// entry R, T, Su, Sd hold the layer properties; on return R[k] and T[k]
 // hold the upward and downward fluxes of level k (the direct beam B added
-// to the latter when B != nullptr).  bet is scratch of L entries.  REFL
-// false: every R is 0 (the longwave), so each 1 / (1 - R a) is exactly 1.
-template <bool REFL>
-__device__ void adding_group(int L, int j, const double *R, const double *T, const double *Su, const double *Sd,
-                             const double *B, double *bet, double *up, double *dn, double toa, double sfc_up,
-                             double sfc_alb) {
+// to the latter when B != nullptr).  A lane reads and writes only the
+// entries of its own block (and the top block entry L); the recurrence
+// values stay in registers (NBMAX >= ceil(L / CG)).  REFL false: every R is
+// 0 (the longwave), so each 1 / (1 - R a) is exactly 1.
+template <bool REFL, int NBMAX>
+__device__ void adding_group(int L, int j, double *R, double *T, const double *Su, const double *Sd,
+                             const double *B, double toa, double sfc_up, double sfc_alb) {
     const int nb = (L + CG - 1) / CG;
-    const int lo = min(j * nb, L), hi = min(lo + nb, L);
+    const int lo = min(j * nb, L), hi = min(lo + nb, L), n = hi - lo;
     // the block as one layer, built from its bottom
     Comp c = comp_identity();
-    for (int k = lo; k < hi; k++) c = comp_over<REFL>(Comp{R[k], R[k], T[k], Su[k], Sd[k]}, c);
+#pragma unroll
+    for (int i = 0; i < NBMAX; i++)
+        if (i < n) {
+            const int k = lo + i;
+            c = comp_over<REFL>(Comp{R[k], R[k], T[k], Su[k], Sd[k]}, c);
+        }
     // the surface is a layer that transmits nothing
     const Comp sfc{sfc_alb, 0., 0., sfc_up, 0.};
     Comp v = j == 0 ? comp_over<REFL>(c, sfc) : c;  // blocks 0..j over the surface
-    Comp u = c;                               // blocks j..15
 #pragma unroll
     for (unsigned d = 1; d < CG; d <<= 1) {
         const Comp w = comp_shfl_up(v, d);
         if (j >= (int)d) v = comp_over<REFL>(v, w);
+    }
+    Comp u = c;  // blocks j..15
+#pragma unroll
+    for (unsigned d = 1; d < CG; d <<= 1) {
         const Comp x = comp_shfl_down(u, d);
         if (j + (int)d < CG) u = comp_over<REFL>(x, u);
     }
     Comp below = comp_shfl_up(v, 1), above = comp_shfl_down(u, 1);
     if (j == 0) below = sfc;
     if (j == CG - 1) above = comp_identity();
-    // albedo and emission of everything below, up the block (the
-    // reference's loop body)
-    double a = below.rt, e = below.su;
-    if (lo < hi) {
-        up[lo] = a;
-        dn[lo] = e;
-    }
-    for (int k = lo; k < hi; k++) {
-        const double Rk = R[k], Tk = T[k];
-        const double b = REFL ? 1. / (1. - Rk * a) : 1.;
-        const double an = Rk + (Tk * Tk) * b * a;
-        const double en = Su[k] + Tk * b * (e + Sd[k] * a);
-        bet[k] = b;
-        if (k + 1 < hi) {
-            up[k + 1] = an;
-            dn[k + 1] = en;
+    // albedo a and emission e of everything below each level of the block,
+    // up the block (the reference's loop body)
+    double a[NBMAX + 1], e[NBMAX + 1], bet[NBMAX];
+    a[0] = below.rt;
+    e[0] = below.su;
+#pragma unroll
+    for (int i = 0; i < NBMAX; i++) {
+        if (i < n) {
+            const int k = lo + i;
+            const double Rk = R[k], Tk = T[k];
+            const double b = REFL ? 1. / (1. - Rk * a[i]) : 1.;
+            bet[i] = b;
+            a[i + 1] = Rk + (Tk * Tk) * b * a[i];
+            e[i + 1] = Su[k] + Tk * b * (e[i] + Sd[k] * a[i]);
         }
-        a = an;
-        e = en;
     }
+    double an = a[0], en = e[0];  // level hi
+#pragma unroll
+    for (int i = 1; i <= NBMAX; i++)
+        if (i == n) {
+            an = a[i];
+            en = e[i];
+        }
     // downward flux at the top of the block from the stack above it (the
     // top block: toa itself)
-    double fd = above.sd + above.t * toa + above.rb * e;
-    if (REFL) fd = fd / (1. - above.rb * a);
-    if (lo < hi && hi == L) {
-        up[L] = a * toa + e;
-        dn[L] = B ? fd + B[L] : fd;
+    double fd = above.sd + above.t * toa + above.rb * en;
+    if (REFL) fd = fd / (1. - above.rb * an);
+    if (n > 0 && hi == L) {
+        R[L] = an * toa + en;
+        T[L] = B ? fd + B[L] : fd;
     }
-    for (int k = hi; k > lo; k--) {
-        const double b = bet[k - 1], em = dn[k - 1];
-        const double fdn = b * (Sd[k - 1] + T[k - 1] * fd + R[k - 1] * em);
-        const double fup = up[k - 1] * fdn + em;
-        up[k - 1] = fup;
-        dn[k - 1] = B ? fdn + B[k - 1] : fdn;
-        fd = fdn;
+#pragma unroll
+    for (int i = NBMAX - 1; i >= 0; i--) {
+        if (i < n) {
+            const int k = lo + i;
+            const double fdn = bet[i] * (Sd[k] + T[k] * fd + R[k] * e[i]);
+            R[k] = a[i] * fdn + e[i];
+            T[k] = B ? fdn + B[k] : fdn;
+            fd = fdn;
+        }
     }
 }
 
@@ -334,9 +361,11 @@ __device__ double beam_group(int L, int j, double toa_dir, double *B, double *Su
     return toa_dir;
 }
 
-// NL layers and NL + 1 levels per lane (L <= 64 NL)
+// NL layers and NL + 1 levels per lane (L <= 64 NL).  With NL = 1 the
+// kernel fits 3 waves per SIMD (LDS 12.4 KB and <= 168 registers at
+// nlay = 64), which the launch bound asks for.
 template <int NL>
-__global__ void __launch_bounds__(64) k_rad_columns(Geo g, Tables t, double gravity, const double *__restrict__ dp,
+__global__ void __launch_bounds__(64, NL == 1 ? 3 : 1) k_rad_columns(Geo g, Tables t, double gravity, const double *__restrict__ dp,
                                                     const double *__restrict__ play, const double *__restrict__ lT,
                                                     const double *__restrict__ vT, const double *__restrict__ ts,
                                                     const double *__restrict__ radius, double *__restrict__ net_flux,
@@ -348,7 +377,7 @@ __global__ void __launch_bounds__(64) k_rad_columns(Geo g, Tables t, double grav
     const int L = g.L, N = L + 1;
     const size_t pl = g.plane(), c0 = g.ix2(I, J);
     auto arr = [&](int a, int gi) { return lds + (size_t)(a * RG + gi) * N; };
-    // arrays: 0 Tdir -> beam, 1 Rdif -> Su, 2 Tdif -> Sd, 3 R, 4 T, 5 bet, 6 alb -> up, 7 emi -> down
+    // arrays: 0 Tdir -> beam, 1 Rdif -> Su, 2 Tdif -> Sd, 3 R -> up, 4 T -> down
     int err = 0;
     // solar_irradiance (mod_solar.f90:26-35)
     const double r = radius[c0] / (double)100000e3f;
@@ -359,11 +388,16 @@ __global__ void __launch_bounds__(64) k_rad_columns(Geo g, Tables t, double grav
     const Lin ls = lin_1d(tsur, t.planckT, 231, &err);
     // per-lane inputs of its layers (K = lane + 64 v) and accumulators of its levels
     constexpr int NV = NL + 1;
-    double dmv[NL], swu[NV], swd[NV], lwu[NV], lwd[NV];
+    double dmv[NL];
+    // flux sums over the g-points, per level: sw up, sw down, lw up, lw down
+    double *acc = lds + (size_t)RARR * RG * N;
     LinPT lpt[NL];
     Lin lb0[NL], lb1[NL];  // Planck positions of levels K and K + 1
 #pragma unroll
-    for (int v = 0; v < NV; v++) swu[v] = swd[v] = lwu[v] = lwd[v] = 0.;
+    for (int v = 0; v < NV; v++) {
+        const int k = lane + 64 * v;
+        if (k <= L) acc[k] = acc[N + k] = acc[2 * N + k] = acc[3 * N + k] = 0.;
+    }
 #pragma unroll
     for (int v = 0; v < NL; v++) {
         dmv[v] = 0.;
@@ -427,18 +461,20 @@ __global__ void __launch_bounds__(64) k_rad_columns(Geo g, Tables t, double grav
             const int gi = lane / CG, j = lane % CG, gp = g0 + gi;
             double *B = arr(0, gi), *Su = arr(1, gi), *Sd = arr(2, gi);
             const double toa = beam_group(L, j, mu0 * ((double)t.frac[gp] * irr), B, Su, Sd);
-            adding_group<true>(L, j, arr(3, gi), arr(4, gi), Su, Sd, B, arr(5, gi), arr(6, gi), arr(7, gi), toa, 0.,
-                               sfc_alb);
+            adding_group<true, 4 * NL>(L, j, arr(3, gi), arr(4, gi), Su, Sd, B, toa, 0., sfc_alb);
         }
         __syncthreads();
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             const int k = lane + 64 * v;
             if (k > L) continue;
+            double su = acc[k], sd = acc[N + k];
             for (int gi = 0; gi < RG; gi++) {
-                swu[v] = swu[v] + arr(6, gi)[k];
-                swd[v] = swd[v] + arr(7, gi)[k];
+                su = su + arr(3, gi)[k];
+                sd = sd + arr(4, gi)[k];
             }
+            acc[k] = su;
+            acc[N + k] = sd;
         }
         __syncthreads();
         // ---- longwave optics (270-339) ----
@@ -471,25 +507,28 @@ __global__ void __launch_bounds__(64) k_rad_columns(Geo g, Tables t, double grav
         {
             const int gi = lane / CG, j = lane % CG, gp = g0 + gi;
             const double Bs = ls.apply(t.planck + 231 * gp);
-            adding_group<false>(L, j, arr(3, gi), arr(4, gi), arr(1, gi), arr(2, gi), nullptr, arr(5, gi),
-                                arr(6, gi), arr(7, gi), 0., emiss * ATM_PI * Bs, 1. - emiss);
+            adding_group<false, 4 * NL>(L, j, arr(3, gi), arr(4, gi), arr(1, gi), arr(2, gi), nullptr, 0.,
+                                        emiss * ATM_PI * Bs, 1. - emiss);
         }
         __syncthreads();
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             const int k = lane + 64 * v;
             if (k > L) continue;
+            double lu = acc[2 * N + k], ld = acc[3 * N + k];
             for (int gi = 0; gi < RG; gi++) {
-                lwu[v] = lwu[v] + arr(6, gi)[k];
-                lwd[v] = lwd[v] + arr(7, gi)[k];
+                lu = lu + arr(3, gi)[k];
+                ld = ld + arr(4, gi)[k];
             }
+            acc[2 * N + k] = lu;
+            acc[3 * N + k] = ld;
         }
         __syncthreads();
     }
 #pragma unroll
     for (int v = 0; v < NV; v++) {
         const int k = lane + 64 * v;
-        if (k <= L) net_flux[c0 + (size_t)k * pl] = (swd[v] + lwd[v]) - (swu[v] + lwu[v]);
+        if (k <= L) net_flux[c0 + (size_t)k * pl] = (acc[N + k] + acc[3 * N + k]) - (acc[k] + acc[2 * N + k]);
     }
     if (err) atomicOr(flag, 2);
 }
@@ -543,7 +582,7 @@ int atm_rad_run(atm_ctx *c, double time) {
     hipLaunchKernelGGL(k_rad_profiles, grid3(g.nx + 6, g.ny + 6, 1, b), b, 0, c->stream, g, c->dp, c->pt,
                        c->plev, c->pkap, c->play, c->playkap, w->layer_T, w->level_T);
     ATM_HIP(c, hipMemsetAsync(c->net_flux, 0, sizeof(double) * g.plane() * (g.L + 1), c->stream));
-    const size_t lds = sizeof(double) * RARR * RG * (g.L + 1);
+    const size_t lds = sizeof(double) * (RARR * RG + 4) * (g.L + 1);
     const dim3 grid((unsigned)(g.nx * g.ny)), blk(64);
 #define ATM_RAD_ARGS g, w->t, c->prm.gravity, c->dp, c->play, w->layer_T, w->level_T, c->ts, c->radius, c->net_flux, c->flag
     if (g.L <= 64)
