@@ -69,6 +69,7 @@ static void release(atm_ctx *c) {
     }
     if (c->flag) (void)hipFree(c->flag);
     if (c->flag_host) (void)hipHostFree(c->flag_host);
+    if (c->nfw) (void)hipFree(c->nfw);
     atm_remap_free(c);
     atm_rad_free(c);
     atm_phys_free(c);
@@ -166,6 +167,8 @@ int atm_create(const atm_grid *grid, const atm_params *params, int precision, at
     }
     atm_point_current(c);
     if (hipMalloc(&c->flag, sizeof(int)) != hipSuccess) return fail(ATM_ERR_HIP);
+    if (hipMalloc(&c->nfw, sizeof(int)) != hipSuccess) return fail(ATM_ERR_HIP);
+    if (hipMemsetAsync(c->nfw, 0, sizeof(int), c->stream) != hipSuccess) return fail(ATM_ERR_HIP);
     if (hipHostMalloc((void **)&c->flag_host, sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(ATM_ERR_HIP);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(ATM_ERR_HIP);
@@ -233,6 +236,7 @@ static int xfer(atm_ctx *c, int id, void *dst, const void *src, bool to_dev, hip
     const Geo &g = c->g;
     size_t bytes = sizeof(double) * g.plane() * nk;
     if (to_dev && id == ATM_DP) c->plk_valid = false;
+    if (to_dev && (id == ATM_DP || id == ATM_PT || id == ATM_UD || id == ATM_VD)) c->nfw_tracked = false;
     if (split_winds(c, id)) {
         double *ring = c->pbuf[id == ATM_UD ? 2 : 3][1 - c->cur];
         if (to_dev) {
@@ -341,6 +345,7 @@ int atm_dgrid_step(atm_ctx *c, double dt) {
         return ATM_OK;
     }
     c->plk_valid = false;
+    c->nfw_tracked = false;  // the reference-structured kernels do not report
     return atm_ref_dgrid(c, dt);
 }
 
@@ -469,14 +474,28 @@ int atm_halo_exchange(atm_ctx *c, int id) {
     return atm_launch_halo(c, d, nk);
 }
 
+// check_stability (mod_fields.f90:69-106): all of interior dp, pt, ud, vd
+// finite.  While the state is tracked (atm_ctx::nfw) a clear tracking word
+// answers; otherwise, or when a reported value may since have been
+// overwritten, the four fields are scanned.
 int atm_check_stability(atm_ctx *c, int *stable) {
     CHECK_CTX(c);
     if (!stable) return atm_fail(c, ATM_ERR_ARGUMENT, "null result pointer");
+    if (c->nfw_tracked) {
+        ATM_HIP(c, hipMemcpyAsync(c->flag_host, c->nfw, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        ATM_HIP(c, hipStreamSynchronize(c->stream));
+        if (!(*c->flag_host & 1)) {
+            *stable = 1;
+            return ATM_OK;
+        }
+    }
     int rc = atm_launch_stability(c);
     if (rc) return rc;
     ATM_HIP(c, hipMemcpyAsync(c->flag_host, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    ATM_HIP(c, hipMemsetAsync(c->nfw, 0, sizeof(int), c->stream));
     ATM_HIP(c, hipStreamSynchronize(c->stream));
     *stable = (*c->flag_host & 1) ? 0 : 1;
+    c->nfw_tracked = *stable != 0;
     return ATM_OK;
 }
 
@@ -485,6 +504,7 @@ int atm_set_path(atm_ctx *c, int path) {
     if (path != ATM_PATH_FUSED && path != ATM_PATH_REFERENCE) return atm_fail(c, ATM_ERR_ARGUMENT, "bad path");
     if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "finish the step (atm_dgrid_halo) first");
     c->plk_valid = false;
+    c->nfw_tracked = false;
     if (path == ATM_PATH_FUSED) {
         if (!atm_fused_supported(c->g) || !c->uc2)
             return atm_fail(c, ATM_ERR_ARGUMENT, "fused path needs nx % " + std::to_string(ATM_TILE_X) + " == 0 and ny % " +

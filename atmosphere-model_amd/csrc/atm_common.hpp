@@ -73,6 +73,11 @@ __device__ __forceinline__ double div3(double x) {
 }
 
 // dqi_mono of the middle cell from (q(m-1), q(m), q(m+1)): ppm 870-875
+// report a non-finite result value into the stability word (see atm_ctx::nfw)
+__device__ __forceinline__ void atm_report_nonfinite(int *nfw, double v) {
+    if (!isfinite(v)) atomicOr(nfw, 1);
+}
+
 __device__ __forceinline__ double dq_mono(double qm, double q0, double qp) {
     double d = (qp - qm) / 4.;
     double dmax = dmax3(qp, q0, qm) - q0;
@@ -203,6 +208,14 @@ struct atm_ctx {
     double *ua = nullptr, *va = nullptr, *ub = nullptr, *vb = nullptr;  // 2-D (last level)
     int *flag = nullptr;        // device status word
     int *flag_host = nullptr;   // pinned mirror
+    // Stability tracking.  Every kernel that writes interior dp, pt, ud or
+    // vd as a step's result (D step of the fused path, remap, physics)
+    // sets bit 0 of nfw when a value it writes is not finite.  While
+    // nfw_tracked holds, the last check found the state finite and every
+    // write since went through such a kernel, so nfw == 0 means the state
+    // is finite; otherwise the check scans the four fields.
+    int *nfw = nullptr;
+    bool nfw_tracked = false;
     RemapWork *remap = nullptr;
     RadWork *rad = nullptr;
     PhysWork *phys = nullptr;

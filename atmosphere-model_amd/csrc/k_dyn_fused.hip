@@ -75,7 +75,7 @@ __global__ __launch_bounds__(NT_PAIR, 6) void k_pair(Geo g, DynCoef c, const dou
                                              const double *__restrict__ plk, const double *__restrict__ nf,
                                              double *__restrict__ dpo, double *__restrict__ pto,
                                              double *__restrict__ uco, double *__restrict__ vco,
-                                             double *__restrict__ ua, double *__restrict__ va) {
+                                             double *__restrict__ ua, double *__restrict__ va, int *nfw) {
     constexpr int TY = TY_PAIR, NT = NT_PAIR;
     using RUC = Reg<0, TX + 1, -3, TY + 3>;
     using RVC = Reg<-3, TX + 3, 0, TY + 1>;
@@ -175,6 +175,7 @@ __global__ __launch_bounds__(NT_PAIR, 6) void k_pair(Geo g, DynCoef c, const dou
             d = d - (fy1 - fy0 + c.dtdy * c.vy * (lyp - 2. * l0 + lym));
             dpcv = d;
             G(dpo, tx, ty) = d;
+            if (!CGRID) atm_report_nonfinite(nfw, d);
             __syncthreads();
             FOR_REG(RQ) Q(x, y) = G(pt, x, y); END_FOR
             __syncthreads();
@@ -196,6 +197,7 @@ __global__ __launch_bounds__(NT_PAIR, 6) void k_pair(Geo g, DynCoef c, const dou
                  c.dt * dpv * hr) /
                 dpcv;
             G(pto, tx, ty) = p;
+            if (!CGRID) atm_report_nonfinite(nfw, p);
         }
     }
     if (CGRID) {
@@ -328,7 +330,7 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
                                            const double *__restrict__ vd, const double *__restrict__ uc,
                                            const double *__restrict__ vc, const double *__restrict__ pkapb,
                                            const double *__restrict__ gzb, double *__restrict__ udo,
-                                           double *__restrict__ vdo) {
+                                           double *__restrict__ vdo, int *nfw) {
     constexpr int TY = TY_WIND, NT = NT_WIND;
     using RU = Reg<-3, TX + 3, -3, TY + 4>;
     using RV = Reg<-3, TX + 4, -3, TY + 3>;
@@ -439,19 +441,23 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
 #define T(m) sTU[RTU::at(tx, ty + (m))]
         const double fy = ppm_face<V_PPM_C>(T(-3), T(-2), T(-1), T(0), T(1), T(2), c.dtdx * vcl);
 #undef T
-        G(udo, tx, ty) = U(tx, ty) +
-                         c.dt * (vcv * fy + c.vy * (sL[RLP::at(tx, ty)] - sL[RLP::at(tx, ty - 1)])) -
-                         c.dtdx * (sKE[RKE::at(tx + 1, ty)] - sKE[RKE::at(tx, ty)] - sD4[RKE::at(tx + 1, ty)] +
-                                   sD4[RKE::at(tx, ty)] - pgfx);
+        const double un = U(tx, ty) +
+                          c.dt * (vcv * fy + c.vy * (sL[RLP::at(tx, ty)] - sL[RLP::at(tx, ty - 1)])) -
+                          c.dtdx * (sKE[RKE::at(tx + 1, ty)] - sKE[RKE::at(tx, ty)] - sD4[RKE::at(tx + 1, ty)] +
+                                    sD4[RKE::at(tx, ty)] - pgfx);
+        G(udo, tx, ty) = un;
+        atm_report_nonfinite(nfw, un);
     }
     {  // vd (755-769)
 #define T(m) sTW[RTW::at(tx + (m), ty)]
         const double fx = ppm_face<V_PPM_C>(T(-3), T(-2), T(-1), T(0), T(1), T(2), c.dtdx * ucl);
 #undef T
-        G(vdo, tx, ty) = V(tx, ty) -
-                         c.dt * (ucv * fx + c.vx * (sL[RLP::at(tx, ty)] - sL[RLP::at(tx - 1, ty)])) -
-                         c.dtdy * (sKE[RKE::at(tx, ty + 1)] - sKE[RKE::at(tx, ty)] - sD4[RKE::at(tx, ty + 1)] +
-                                   sD4[RKE::at(tx, ty)] - pgfy);
+        const double vn = V(tx, ty) -
+                          c.dt * (ucv * fx + c.vx * (sL[RLP::at(tx, ty)] - sL[RLP::at(tx - 1, ty)])) -
+                          c.dtdy * (sKE[RKE::at(tx, ty + 1)] - sKE[RKE::at(tx, ty)] - sD4[RKE::at(tx, ty + 1)] +
+                                    sD4[RKE::at(tx, ty)] - pgfy);
+        G(vdo, tx, ty) = vn;
+        atm_report_nonfinite(nfw, vn);
     }
 #undef G
 #undef U
@@ -646,7 +652,7 @@ int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
     atm_timer_begin(c, ATM_T_C1);
     hipLaunchKernelGGL((k_pair<V_PPM_U, true>), grid, blk, 0, c->stream, g, k, b.dp, b.pt, b.ud, b.vd,
                        (const double *)nullptr, c->plk, c->net_flux,
-                       c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va);
+                       c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va, (int *)nullptr);
     atm_timer_end(c, ATM_T_C1);
     {
         double *fill[] = {c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va};
@@ -692,7 +698,7 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
     hipLaunchKernelGGL((k_pair<V_PPM_NN, false>), grid, blk, 0, c->stream, g, k, cur.dp, cur.pt, c->uc2, c->vc2,
                        c->heatd, (const double *)nullptr, (const double *)nullptr, nxt.dp, nxt.pt,
                        (double *)nullptr, (double *)nullptr, (double *)nullptr,
-                       (double *)nullptr);
+                       (double *)nullptr, c->nfw);
     atm_timer_end(c, ATM_T_D1);
     {
         double *fill[] = {nxt.dp, nxt.pt};
@@ -711,7 +717,7 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
     atm_timer_end(c, ATM_T_SCAN);
     atm_timer_begin(c, ATM_T_D2);
     hipLaunchKernelGGL(k_d2, gridw, blkw, 0, c->stream, g, k, cur.ud, cur.vd, c->uc2, c->vc2, c->pkapb, c->gzb,
-                       nxt.ud, nxt.vd);
+                       nxt.ud, nxt.vd, c->nfw);
     atm_timer_end(c, ATM_T_D2);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;

@@ -126,15 +126,21 @@ __global__ void __launch_bounds__(64) k_phys_tend(Geo g, const double *__restric
 __global__ void k_phys_apply(Geo g, double dt, const double *__restrict__ u_tend,
                              const double *__restrict__ v_tend, const double *__restrict__ pt_tend,
                              const double *__restrict__ ts_tend, double *__restrict__ pt, double *__restrict__ ud,
-                             double *__restrict__ vd, double *__restrict__ ts) {
+                             double *__restrict__ vd, double *__restrict__ ts, int *nfw) {
     const int I = ATM_HALO + blockIdx.x * blockDim.x + threadIdx.x;
     const int J = ATM_HALO + blockIdx.y;
     const int K = blockIdx.z;
     if (I >= ATM_HALO + g.nx) return;
     const size_t x = g.ix(I, J, K), sJ = g.NI;
-    pt[x] = pt[x] + dt * pt_tend[x];
-    ud[x] = ud[x] + dt * (ATM_W7_12 * (u_tend[x] + u_tend[x - sJ]) - ATM_W1_12 * (u_tend[x + sJ] + u_tend[x - 2 * sJ]));
-    vd[x] = vd[x] + dt * (ATM_W7_12 * (v_tend[x] + v_tend[x - 1]) - ATM_W1_12 * (v_tend[x + 1] + v_tend[x - 2]));
+    const double p = pt[x] + dt * pt_tend[x];
+    const double u = ud[x] + dt * (ATM_W7_12 * (u_tend[x] + u_tend[x - sJ]) - ATM_W1_12 * (u_tend[x + sJ] + u_tend[x - 2 * sJ]));
+    const double v = vd[x] + dt * (ATM_W7_12 * (v_tend[x] + v_tend[x - 1]) - ATM_W1_12 * (v_tend[x + 1] + v_tend[x - 2]));
+    pt[x] = p;
+    ud[x] = u;
+    vd[x] = v;
+    atm_report_nonfinite(nfw, p);
+    atm_report_nonfinite(nfw, u);
+    atm_report_nonfinite(nfw, v);
     if (K == 0) {
         const size_t c = g.ix2(I, J);
         ts[c] = ts[c] + dt * ts_tend[c];
@@ -187,7 +193,7 @@ int atm_phys_apply(atm_ctx *c, double dt) {
     PhysWork *w = c->phys;
     atm_timer_begin(c, ATM_T_PHYS);
     hipLaunchKernelGGL(k_phys_apply, dim3((g.nx + 63) / 64, g.ny, g.L), dim3(64), 0, c->stream, g, dt, w->u_tend,
-                       w->v_tend, w->pt_tend, w->ts_tend, c->pt, c->ud, c->vd, c->ts);
+                       w->v_tend, w->pt_tend, w->ts_tend, c->pt, c->ud, c->vd, c->ts, c->nfw);
     atm_timer_end(c, ATM_T_PHYS);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;

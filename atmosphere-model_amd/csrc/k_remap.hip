@@ -203,7 +203,8 @@ __global__ void __launch_bounds__(256) k_remap_energy(
     const double *__restrict__ playkap, const double *__restrict__ gz, const double *pt,
     const double *__restrict__ ud, const double *__restrict__ vd, const double *__restrict__ udr,
     const double *__restrict__ vdr, const double *__restrict__ ur, const double *__restrict__ vr, double *pt_out,
-    double *__restrict__ dp_out, double *__restrict__ ud_out, double *__restrict__ vd_out, int write_winds) {
+    double *__restrict__ dp_out, double *__restrict__ ud_out, double *__restrict__ vd_out, int write_winds,
+    int *nfw) {
     const int W = g.nx;
     const size_t t = (size_t)xcd_block1d() * blockDim.x + threadIdx.x;
     if (t >= ncol) return;
@@ -243,10 +244,15 @@ __global__ void __launch_bounds__(256) k_remap_energy(
         double p = (e - gzr - .5 * (ua * ua + va * va)) / (ATM_CV_DRY * tplk[x] + .5 * ATM_CP_DRY * (tk - tk1));
         pt_out[x] = p;
         gzr = gzr + ATM_CP_DRY * p * (tk - tk1);
-        dp_out[x] = dm0 - dm1;  // target_plev(k) - target_plev(k+1)
+        const double d = dm0 - dm1;  // target_plev(k) - target_plev(k+1)
+        dp_out[x] = d;
+        atm_report_nonfinite(nfw, p);
+        atm_report_nonfinite(nfw, d);
         if (write_winds) {
             ud_out[x] = ur[x];
             vd_out[x] = vr[x];
+            atm_report_nonfinite(nfw, ur[x]);
+            atm_report_nonfinite(nfw, vr[x]);
         }
     };
     ppm_merge(c, L, s.lv(), s.gm(), ncol, out);
@@ -254,14 +260,17 @@ __global__ void __launch_bounds__(256) k_remap_energy(
 
 // interior wind write-back when the output winds are the input buffers
 __global__ void k_remap_write_winds(Geo g, const double *__restrict__ ur, const double *__restrict__ vr,
-                                    double *__restrict__ ud, double *__restrict__ vd) {
+                                    double *__restrict__ ud, double *__restrict__ vd, int *nfw) {
     const int I = 7 + blockIdx.x * blockDim.x + threadIdx.x;
     const int J = 7 + blockIdx.y * blockDim.y + threadIdx.y;
     const int K = blockIdx.z;
     if (I > g.NI - 8 || J > g.NJ - 8) return;
     size_t x = g.ix(I, J, K);
-    ud[x] = ur[x];
-    vd[x] = vr[x];
+    const double u = ur[x], v = vr[x];
+    ud[x] = u;
+    vd[x] = v;
+    atm_report_nonfinite(nfw, u);
+    atm_report_nonfinite(nfw, v);
 }
 
 __global__ void k_fill_plane(double *a, size_t n, double v) {
@@ -334,11 +343,11 @@ int atm_remap_run_split(atm_ctx *c, const double *dp, const double *pt, const do
     const int write_winds = (ud_out != ud_in && vd_out != vd_in) ? 1 : 0;
     hipLaunchKernelGGL(k_remap_energy, dim3(blocks8(ne, bs)), dim3(bs), 0, c->stream, g, w->scr, ne, c->plev,
                        w->tplev, w->tpkap, w->tplk, c->playkap, c->gz, pt, ud_in, vd_in, ud_ring, vd_ring,
-                       w->uremap, w->vremap, pt_out, dp_out, ud_out, vd_out, write_winds);
+                       w->uremap, w->vremap, pt_out, dp_out, ud_out, vd_out, write_winds, c->nfw);
     if (!write_winds) {
         dim3 bw(64, 4);
         hipLaunchKernelGGL(k_remap_write_winds, grid3(g.nx, g.ny, g.L, bw), bw, 0, c->stream, g, w->uremap,
-                           w->vremap, ud_out, vd_out);
+                           w->vremap, ud_out, vd_out, c->nfw);
     }
     atm_timer_end(c, ATM_T_REMAP);
     ATM_HIP(c, hipGetLastError());

@@ -264,3 +264,27 @@ def test_output_accumulation(gpu_available):
     _print("output means x6", **errs)
     for f, e in errs.items():
         assert e <= PARITY, (f, e)
+
+
+@pytest.mark.parametrize("route", ["net_flux", "gz"])
+def test_stability_check_sees_nonfinite_from_a_step(gpu_available, route):
+    """A NaN entering through a field the check does not scan (net_flux:
+    the C-step heating rate -> pt; surface gz: the pressure gradient -> ud,
+    vd) reaches the prognostic state in the next step, and the check finds
+    it while it answers from the D-step kernels' tracking word."""
+    from atmos import driver
+    gpu, orc, cfg, sg, so, ic = make_pair(64, 32, 8)
+    assert gpu.path == "fused"
+    t = 0.0
+    for _ in range(2):  # the first check scans the fields, the second reads the word
+        t += driver.step(gpu, cfg, sg, t)
+        assert gpu.check_stability()
+    a = gpu.get(route)
+    a[0 if route == "gz" else 3, 10, 20] = np.nan
+    gpu.put(route, a)
+    t += driver.step(gpu, cfg, sg, t)
+    bad = [n for n in ("dp", "pt", "ud", "vd") if not np.all(np.isfinite(gpu.get(n)))]
+    print(f"{route}: non-finite after the step in {bad}", flush=True)
+    assert bad
+    assert not gpu.check_stability()
+    assert not gpu.check_stability()
