@@ -12,8 +12,9 @@ Prints one JSON line: value = grid-cell-steps per second
 (nx * ny * nlay * steps / seconds) with all state resident in HBM, a
 roofline entry for the dominant kernel (algorithmic bytes per launch over
 its HIP-event-measured average duration, against 8 TB/s) and a CPU
-baseline: the oracle (a single-core C restatement of the reference
-algorithm) timed on a bounded sample of the same workload.
+baseline: the oracle (a C restatement of the reference algorithm, threaded
+over the host cores with OpenMP) timed on a bounded sample of the same
+workload, with a single-core figure beside it.
 """
 import argparse
 import json
@@ -60,8 +61,10 @@ def parse():
     ap.add_argument("--remap-every", type=int, default=5)
     ap.add_argument("--path", choices=["fused", "reference"], default="fused")
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
-    ap.add_argument("--cpu-nx", type=int, default=256, help="bounded CPU sample: nx = ny")
+    ap.add_argument("--cpu-nx", type=int, default=512, help="bounded CPU sample: nx = ny")
     ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (0: OMP_NUM_THREADS or all host cores)")
     ap.add_argument("--radiation-calls", type=int, default=3,
                     help="also time this many radiation calls on the same grid (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -82,24 +85,39 @@ def calibrate_copy(mib):
 
 
 def cpu_baseline(args):
-    """Single-core oracle (C restatement of the reference) on a bounded sample."""
+    """The oracle (C restatement of the reference) on a bounded sample of the
+    same workload, threaded over the host's cores (OpenMP: OMP_NUM_THREADS or
+    all cores), and on one core for reference."""
     import oracle as O
     from atmos import driver
     from atmos.config import Config
     from atmos.ic import make_initial_state
 
+    def run(n, steps):
+        L = args.nlay
+        ic = make_initial_state(n, n, L, args.dx)
+        cfg = Config(dt_max=args.dt, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12,
+                     dt_remap=args.remap_every * args.dt, radiation_on=False, physics_on=False)
+        o = O.Oracle(n, n, L, args.dx, args.dx)
+        st = driver.init_model(o, ic, cfg)
+        t0 = time.perf_counter()
+        _, done, _ = driver.run_model(o, cfg, st, max_steps=steps)
+        sec = time.perf_counter() - t0
+        return n * n * L * done / sec, done, sec
+
+    default = O.set_threads(0)
+    cores = O.set_threads(args.cpu_threads) if args.cpu_threads > 0 else default
     n, L = args.cpu_nx, args.nlay
-    ic = make_initial_state(n, n, L, args.dx)
-    cfg = Config(dt_max=args.dt, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12,
-                 dt_remap=args.remap_every * args.dt, radiation_on=False, physics_on=False)
-    o = O.Oracle(n, n, L, args.dx, args.dx)
-    st = driver.init_model(o, ic, cfg)
-    t0 = time.perf_counter()
-    _, steps, _ = driver.run_model(o, cfg, st, max_steps=args.cpu_steps)
-    sec = time.perf_counter() - t0
-    return {"value": n * n * L * steps / sec, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n}x{n}x{L}, {steps} steps incl. {steps // args.remap_every} remap, "
-                      f"{sec:.1f} s on 1 host core (oracle/atm_oracle.c, -O2, glibc libm)"}
+    v, steps, sec = run(n, args.cpu_steps)
+    O.set_threads(1)
+    n1 = args.cpu_nx // 2
+    v1, steps1, sec1 = run(n1, max(1, args.cpu_steps // 2))
+    O.set_threads(default)
+    return {"value": v, "unit": "cell-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{n}x{n}x{L}, {steps} steps incl. {steps // args.remap_every} remap, {sec:.1f} s on "
+                      f"{cores} host threads (oracle/atm_oracle.c, -O2, OpenMP over levels/columns, glibc libm)",
+            "single_core": {"value": v1, "cores": 1,
+                            "sample": f"{n1}x{n1}x{L}, {steps1} steps, {sec1:.1f} s on 1 host core"}}
 
 
 def radiation_timing(m, args, time_s):

@@ -69,6 +69,8 @@ def lib():
         L.orc_solar_irradiance.restype = C.c_double
         L.orc_solar_irradiance.argtypes = [C.c_double]
         L.orc_constants.argtypes = [dp]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = C.c_int
         L.orc_flux_pencil.argtypes = [dp, dp, dp, C.c_int, C.c_int]
         L.orc_vertical_ppm.argtypes = [C.c_int, dp, dp, dp, dp, dp, vp]
         L.orc_remap_column.argtypes = [C.c_int, dp, dp, dp, dp]
@@ -81,6 +83,12 @@ def lib():
         L.orc_lw_column.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, C.c_double, dp, dp]
         _LIB = L
     return _LIB
+
+
+def set_threads(n: int = 0) -> int:
+    """Threads of the oracle's parallel loops (n <= 0: leave as is);
+    returns the count in use.  Results do not depend on it."""
+    return lib().orc_set_threads(int(n))
 
 
 def _p(a):

@@ -68,6 +68,7 @@ double orc_planck(int g, double T, int *err);
 double orc_solar_fraction(int g);
 double orc_solar_irradiance(double radius);
 void orc_constants(double *out5); /* pi, R_dry, cp_dry, cv_dry, kappa */
+int orc_set_threads(int n);       /* threads of the parallel loops (n <= 0: query); returns the count */
 void orc_flux_pencil(double *f, const double *q, const double *c, int n,
                      int variant);
 void orc_vertical_ppm(int nlay, const double *var, const double *coord,

@@ -303,3 +303,30 @@ def test_translation_and_stale_wind_ring():
     ring[[0, -1], :] = True
     ring[:, [0, -1]] = True
     assert d.max() > 0 and not np.any(d[~(ring | roll(ring[None])[0])] > 0)
+
+
+def test_oracle_threads_do_not_change_results():
+    """The oracle's threaded level and column loops give the same bits with
+    one thread and with several (C, D, remap over 6 steps)."""
+    import oracle as O
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    ic = make_initial_state(32, 24, 10, 100e3)
+    out = []
+    n0 = O.set_threads(0)
+    try:
+        for n in (1, 4):
+            O.set_threads(n)
+            o = O.Oracle(32, 24, 10, 100e3, 100e3)
+            cfg = Config(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12,
+                         dt_remap=120.0, radiation_on=False, physics_on=False)
+            st = driver.init_model(o, ic, cfg)
+            t = 0.0
+            for _ in range(6):
+                t += driver.step(o, cfg, st, t)
+            out.append([o.interior(f).copy() for f in (O.DP, O.PT, O.UD, O.VD)])
+    finally:
+        O.set_threads(n0)
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
