@@ -30,6 +30,7 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_physics_init", "atm_physics_tendencies", "atm_apply_physics", "atm_physics_halo",
            "atm_output_init", "atm_accumulate_output", "atm_output_mean"]
 PATHS = {"fused": 0, "reference": 1}
+PRECISIONS = {"fp64": 0, "mixed": 1}
 TIMERS = {"c1": 0, "c2": 1, "d1": 2, "d2": 3, "scan": 4, "halo": 5, "remap": 6, "radiation": 7,
           "physics": 8}
 
@@ -100,16 +101,17 @@ class Model:
     """One model instance on the current GPU (FP64 drop-in path)."""
 
     def __init__(self, nx, ny, nlay, dx, dy=None, gravity=9.807, coriolis=1e-5,
-                 divergence_damping=0.15, vorticity_damping=0.005):
+                 divergence_damping=0.15, vorticity_damping=0.005, precision="fp64"):
         self.nx, self.ny, self.nlay = nx, ny, nlay
+        self.precision = precision
         self.NI, self.NJ = nx + 2 * HALO, ny + 2 * HALO
         self._L = load_library()
         h = C.c_void_p()
         grid = Grid(nx, ny, nlay, dx, dx if dy is None else dy)
         prm = Params(gravity, coriolis, divergence_damping, vorticity_damping)
-        rc = self._L.atm_create(C.byref(grid), C.byref(prm), 0, C.byref(h))
+        rc = self._L.atm_create(C.byref(grid), C.byref(prm), PRECISIONS[precision], C.byref(h))
         if rc != 0:
-            raise AtmError(rc, "atm_create failed (no GPU or bad sizes)")
+            raise AtmError(rc, f"atm_create failed (no GPU, bad sizes, or {precision} not available at this size)")
         self._h = h
         self._ak = self._bk = None
         self._gzs = None

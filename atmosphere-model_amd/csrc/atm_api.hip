@@ -116,7 +116,7 @@ extern "C" {
 int atm_create(const atm_grid *grid, const atm_params *params, int precision, atm_ctx **out) {
     if (!grid || !params || !out) return ATM_ERR_ARGUMENT;
     *out = nullptr;
-    if (precision != ATM_PRECISION_FP64) return ATM_ERR_ARGUMENT;
+    if (precision != ATM_PRECISION_FP64 && precision != ATM_PRECISION_MIXED) return ATM_ERR_ARGUMENT;
     // the halo is 7 wide and vertical_ppm needs at least 4 layers
     if (grid->nx < ATM_HALO || grid->ny < ATM_HALO || grid->nlay < 4) return ATM_ERR_ARGUMENT;
     if (!(grid->dx > 0) || !(grid->dy > 0)) return ATM_ERR_ARGUMENT;
@@ -143,6 +143,8 @@ int atm_create(const atm_grid *grid, const atm_params *params, int precision, at
     const size_t p = c->g.plane(), L = c->g.L;
     const char *path = getenv("ATM_PATH");
     c->fused = atm_fused_supported(c->g) && !(path && std::string(path) == "reference");
+    c->mixed = precision == ATM_PRECISION_MIXED;
+    if (c->mixed && !c->fused) return fail(ATM_ERR_ARGUMENT);  // the mixed form is on the fused path only
     struct { double **ptr; size_t n; } bufs[] = {
         {&c->pbuf[0][0], p * L}, {&c->pbuf[1][0], p * L}, {&c->pbuf[2][0], p * L}, {&c->pbuf[3][0], p * L},
         {&c->ak, L + 1}, {&c->bk, L + 1}, {&c->ts, p}, {&c->gz, p * (L + 1)}, {&c->net_flux, p * (L + 1)},
@@ -511,6 +513,7 @@ int atm_set_path(atm_ctx *c, int path) {
                                                       std::to_string(ATM_TILE_Y) + " == 0");
         c->fused = true;
     } else {
+        if (c->mixed) return atm_fail(c, ATM_ERR_ARGUMENT, "the mixed-precision form runs on the fused path only");
         c->fused = false;
     }
     return ATM_OK;

@@ -50,67 +50,83 @@ struct DynCoef {
 // ---------------------------------------------------------------------------
 // Device numerics shared by all dynamics kernels (mod_sw_dyn.f90:781-919).
 // ---------------------------------------------------------------------------
-// Fortran MAX/MIN on finite values: one v_max_f64 / v_min_f64 each (the
+// Fortran MAX/MIN on finite values: one v_max / v_min each (the
 // compare-and-select form costs three instructions).  They differ from a
 // compare-and-select only for NaN operands and in the sign of an exact zero.
 __host__ __device__ __forceinline__ double dmax2(double a, double b) { return __builtin_fmax(a, b); }
 __host__ __device__ __forceinline__ double dmin2(double a, double b) { return __builtin_fmin(a, b); }
-__host__ __device__ __forceinline__ double dmax3(double a, double b, double c) { return dmax2(dmax2(a, b), c); }
-__host__ __device__ __forceinline__ double dmin3(double a, double b, double c) { return dmin2(dmin2(a, b), c); }
+__host__ __device__ __forceinline__ float dmax2(float a, float b) { return __builtin_fmaxf(a, b); }
+__host__ __device__ __forceinline__ float dmin2(float a, float b) { return __builtin_fminf(a, b); }
+template <class T>
+__host__ __device__ __forceinline__ T dmax3(T a, T b, T c) { return dmax2(dmax2(a, b), c); }
+template <class T>
+__host__ __device__ __forceinline__ T dmin3(T a, T b, T c) { return dmin2(dmin2(a, b), c); }
 // Fortran SIGN(a, b)
 __device__ __forceinline__ double fsign(double a, double b) { return copysign(fabs(a), b); }
+__device__ __forceinline__ float fsign(float a, float b) { return copysignf(fabsf(a), b); }
 
-// x / 3 rounded to nearest, without the FP64 division sequence: the product
+// x / 3 rounded to nearest, without the division sequence: the product
 // with RN(1/3) is corrected by one FMA step using the exact remainder
 // x - 3q.  For every finite x away from the subnormal and overflow ranges
 // this equals the IEEE quotient (checked on 1.6e9 random doubles spanning
-// 2^-300..2^300 against x / 3.); a zero product keeps the sign of x.
+// 2^-300..2^300 against x / 3.; in binary32 the same argument holds: x / 3
+// is never a rounding midpoint); a zero product keeps the sign of x.
 __device__ __forceinline__ double div3(double x) {
     const double r3 = 1.0 / 3.0;
     const double q = x * r3;
     if (q == 0.) return q;
     return fma(fma(-q, 3.0, x), r3, q);
 }
+__device__ __forceinline__ float div3(float x) {
+    const float r3 = 1.0f / 3.0f;
+    const float q = x * r3;
+    if (q == 0.f) return q;
+    return fmaf(fmaf(-q, 3.0f, x), r3, q);
+}
 
-// dqi_mono of the middle cell from (q(m-1), q(m), q(m+1)): ppm 870-875
 // report a non-finite result value into the stability word (see atm_ctx::nfw)
 __device__ __forceinline__ void atm_report_nonfinite(int *nfw, double v) {
     if (!isfinite(v)) atomicOr(nfw, 1);
 }
 
-__device__ __forceinline__ double dq_mono(double qm, double q0, double qp) {
-    double d = (qp - qm) / 4.;
-    double dmax = dmax3(qp, q0, qm) - q0;
-    double dmin = q0 - dmin3(qp, q0, qm);
-    return fsign(dmin3(fabs(d), dmin, dmax), d);
+// The PPM operators below are templates on the arithmetic type T: double
+// for the FP64 path (the reference's operations and rounding), float for
+// the mixed-precision path's reconstruction and fluxes.
+
+// dqi_mono of the middle cell from (q(m-1), q(m), q(m+1)): ppm 870-875
+template <class T>
+__device__ __forceinline__ T dq_mono(T qm, T q0, T qp) {
+    T d = (qp - qm) / T(4);
+    T dmax = dmax3(qp, q0, qm) - q0;
+    T dmin = q0 - dmin3(qp, q0, qm);
+    return fsign(dmin3(T(fabs(d)), dmin, dmax), d);
 }
 
 // Edge anomalies of the middle cell from q(m-2..m+2): ppm 877-911.
-template <int V>
-__device__ __forceinline__ void ppm_edges(double qm2, double qm1, double q0, double qp1, double qp2,
-                                          double &eL, double &eR) {
-    double dmm = dq_mono(qm2, qm1, q0);
-    double dm0 = dq_mono(qm1, q0, qp1);
-    double dmp = dq_mono(q0, qp1, qp2);
-    eL = (qm1 - q0) / 2. + div3(dmm - dm0);
-    eR = (qp1 - q0) / 2. + div3(dm0 - dmp);
+template <int V, class T>
+__device__ __forceinline__ void ppm_edges(T qm2, T qm1, T q0, T qp1, T qp2, T &eL, T &eR) {
+    T dmm = dq_mono(qm2, qm1, q0);
+    T dm0 = dq_mono(qm1, q0, qp1);
+    T dmp = dq_mono(q0, qp1, qp2);
+    eL = (qm1 - q0) / T(2) + div3(dmm - dm0);
+    eR = (qp1 - q0) / T(2) + div3(dm0 - dmp);
     if (V == V_PPM_C || V == V_PPM_NN) {
-        double d = (qp1 - qm1) / 4.;
-        eL = -fsign(dmin2(2. * fabs(d), fabs(eL)), d);
-        eR = +fsign(dmin2(2. * fabs(d), fabs(eR)), d);
+        T d = (qp1 - qm1) / T(4);
+        eL = -fsign(dmin2(T(2) * T(fabs(d)), T(fabs(eL))), d);
+        eR = +fsign(dmin2(T(2) * T(fabs(d)), T(fabs(eR))), d);
         if (V == V_PPM_NN) {
             eL = dmax2(eL, -q0);
             eR = dmax2(eR, -q0);
-            double a = eR - eL;
-            double b = -3. * (eR + eL);
-            if ((fabs(a) < -b) && (q0 + a * a / (4. * b) + b / 12. < 0.)) {
-                if (eL * eR > 0.) {
-                    eL = 0.;
-                    eR = 0.;
+            T a = eR - eL;
+            T b = T(-3) * (eR + eL);
+            if ((T(fabs(a)) < -b) && (q0 + a * a / (T(4) * b) + b / T(12) < T(0))) {
+                if (eL * eR > T(0)) {
+                    eL = T(0);
+                    eR = T(0);
                 } else if (eL > eR) {
-                    eL = 2. * fabs(eR);
+                    eL = T(2) * T(fabs(eR));
                 } else {
-                    eR = 2. * fabs(eL);
+                    eR = T(2) * T(fabs(eL));
                 }
             }
         }
@@ -119,26 +135,25 @@ __device__ __forceinline__ void ppm_edges(double qm2, double qm1, double q0, dou
 
 // Upwind PPM value at the face between cells i-1 and i, from
 // q(i-3..i+2) and the Courant number c(i): flux 802-829.
-template <int V>
-__device__ __forceinline__ double ppm_face(double qm3, double qm2, double qm1, double q0, double qp1,
-                                           double qp2, double c) {
-    double s = fabs(c);
-    double k1 = (1. - 2. * s) * (1. - s);
-    double k2 = s * (s - 1.);
+template <int V, class T>
+__device__ __forceinline__ T ppm_face(T qm3, T qm2, T qm1, T q0, T qp1, T qp2, T c) {
+    T s = T(fabs(c));
+    T k1 = (T(1) - T(2) * s) * (T(1) - s);
+    T k2 = s * (s - T(1));
     if (V == V_PPM_C) {
-        double eLm, eRm, eL0, eR0;
+        T eLm, eRm, eL0, eR0;
         ppm_edges<V>(qm3, qm2, qm1, q0, qp1, eLm, eRm);
         ppm_edges<V>(qm2, qm1, q0, qp1, qp2, eL0, eR0);
-        if ((3. * fabs(eL0 + eR0) > fabs(eL0 - eR0)) && (3. * fabs(eLm + eRm) > fabs(eLm - eRm)))
-            return c > 0. ? qm1 : q0;
-        if (c > 0.) return qm1 + k1 * eRm + k2 * eLm;
+        if ((T(3) * T(fabs(eL0 + eR0)) > T(fabs(eL0 - eR0))) && (T(3) * T(fabs(eLm + eRm)) > T(fabs(eLm - eRm))))
+            return c > T(0) ? qm1 : q0;
+        if (c > T(0)) return qm1 + k1 * eRm + k2 * eLm;
         return q0 + k1 * eL0 + k2 * eR0;
     } else {
         // only the upwind cell's edges are needed
-        bool up = c > 0.;
-        double a2 = up ? qm3 : qm2, a1 = up ? qm2 : qm1, a0 = up ? qm1 : q0;
-        double b1 = up ? q0 : qp1, b2 = up ? qp1 : qp2;
-        double eL, eR;
+        bool up = c > T(0);
+        T a2 = up ? qm3 : qm2, a1 = up ? qm2 : qm1, a0 = up ? qm1 : q0;
+        T b1 = up ? q0 : qp1, b2 = up ? qp1 : qp2;
+        T eL, eR;
         ppm_edges<V>(a2, a1, a0, b1, b2, eL, eR);
         if (up) return a0 + k1 * eR + k2 * eL;
         return a0 + k1 * eL + k2 * eR;
@@ -146,7 +161,8 @@ __device__ __forceinline__ double ppm_face(double qm3, double qm2, double qm1, d
 }
 
 // Piecewise-constant upwind value at the face between cells i-1 and i: flux 832-838.
-__device__ __forceinline__ double pcm_face(double qm1, double q0, double c) { return c > 0. ? qm1 : q0; }
+template <class T>
+__device__ __forceinline__ T pcm_face(T qm1, T q0, T c) { return c > T(0) ? qm1 : q0; }
 
 // ---------------------------------------------------------------------------
 // Context
@@ -180,6 +196,9 @@ struct atm_ctx {
     // execution path: fused LDS-tiled kernels (default where the grid tiles
     // evenly) or the reference-structured kernels of k_dyn_ref.hip
     bool fused = false;
+    // mixed precision (fused path only): the tile kernels reconstruct and
+    // form fluxes in float, update the fields in double
+    bool mixed = false;
     // prognostic fields: two copies each for the fused path; dp/pt/ud/vd
     // below point at the current copy, prv at the previous one
     double *pbuf[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};

@@ -22,10 +22,34 @@
 //
 // Tile coordinates: x = I - I0, y = J - J0 with (I0, J0) the tile origin in
 // the halo'd index space; LDS arrays cover [x0, x1) x [y0, y1) regions.
+#include <type_traits>
+
 #include "atm_common.hpp"
 #include "atm_fused.hpp"
 
 namespace {
+
+// Arithmetic type R of the tile kernels: double on the FP64 path; float on
+// the mixed-precision path for the reconstruction, the fluxes and the other
+// intermediates in LDS.  The final updates of dp, pt, uc, vc, ud, vd, the
+// heating rate and the pressure gradient stay in double (each face flux is
+// one value used by both cells it separates, so dp changes by a telescoping
+// sum: mass is conserved to FP64 rounding).  Every R expression below is
+// the FP64 path's expression when R is double.
+template <class R>
+struct CoefR {
+    R dt, dtdx, dtdy, rdx, rdy, rdx2, rdy2, rdx2rdy2, rA, vx, vy, divdamp, coriolis;
+    __device__ explicit CoefR(const DynCoef &c)
+        : dt(R(c.dt)), dtdx(R(c.dtdx)), dtdy(R(c.dtdy)), rdx(R(c.rdx)), rdy(R(c.rdy)), rdx2(R(c.rdx2)),
+          rdy2(R(c.rdy2)), rdx2rdy2(R(c.rdx2rdy2)), rA(R(c.rA)), vx(R(c.vx)), vy(R(c.vy)), divdamp(R(c.divdamp)),
+          coriolis(R(c.coriolis)) {}
+};
+
+// 1 / x: the IEEE quotient in double; v_rcp_f32 (1 ulp) in float
+__device__ __forceinline__ double rcp(double x) { return 1. / x; }
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+#define D(x) static_cast<double>(x)
 
 constexpr int TX = ATM_TILE_X;
 // Tile rows per kernel: the dp/pt kernels (the heaviest in registers) run
@@ -68,8 +92,13 @@ struct Reg {
 // CGRID: the C winds come from the D winds (109-126) and are also written
 // out; otherwise uc/vc of the C step are read (the D step).
 // ---------------------------------------------------------------------------
-template <int VAR, bool CGRID>
-__global__ __launch_bounds__(NT_PAIR, 6) void k_pair(Geo g, DynCoef c, const double *__restrict__ dp,
+// waves per SIMD of the dp/pt kernels: 6; 8 for the float D step, whose
+// registers fit without spills (the float C step's do not)
+template <class R, bool CGRID>
+constexpr int pair_waves() { return std::is_same<R, double>::value || CGRID ? 6 : 8; }
+
+template <int VAR, bool CGRID, class R>
+__global__ __launch_bounds__(NT_PAIR, pair_waves<R, CGRID>()) void k_pair(Geo g, DynCoef c, const double *__restrict__ dp,
                                              const double *__restrict__ pt, const double *__restrict__ uw,
                                              const double *__restrict__ vw, const double *__restrict__ heat,
                                              const double *__restrict__ plk, const double *__restrict__ nf,
@@ -77,6 +106,8 @@ __global__ __launch_bounds__(NT_PAIR, 6) void k_pair(Geo g, DynCoef c, const dou
                                              double *__restrict__ uco, double *__restrict__ vco,
                                              double *__restrict__ ua, double *__restrict__ va, int *nfw) {
     constexpr int TY = TY_PAIR, NT = NT_PAIR;
+    constexpr bool FP64 = std::is_same<R, double>::value;
+    const CoefR<R> r(c);
     using RUC = Reg<0, TX + 1, -3, TY + 3>;
     using RVC = Reg<-3, TX + 3, 0, TY + 1>;
     using RDX = Reg<0, TX, -3, TY + 3>;  // tmp of the inner x step (denominators recomputed per pass)
@@ -85,11 +116,11 @@ __global__ __launch_bounds__(NT_PAIR, 6) void k_pair(Geo g, DynCoef c, const dou
     using RL = Reg<-1, TX + 1, -1, TY + 1>;
     using RFX = Reg<0, TX + 1, 0, TY>;
     using RFY = Reg<0, TX, 0, TY + 1>;
-    __shared__ double sUC[RUC::n], sVC[RVC::n], sQ[RQ::n];
-    __shared__ double sTX[RDY::n], sTY[RDX::n], sL[RL::n];
+    __shared__ R sUC[RUC::n], sVC[RVC::n], sQ[RQ::n];
+    __shared__ R sTX[RDY::n], sTY[RDX::n], sL[RL::n];
     // face values: the dp mass fluxes in the dp pass (each thread keeps the
     // four around its cell in registers), the pt face values in the pt pass
-    __shared__ double sFX[RFX::n], sFY[RFY::n];
+    __shared__ R sFX[RFX::n], sFY[RFY::n];
 
     int bx, by, K;
     xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
@@ -123,56 +154,59 @@ __global__ __launch_bounds__(NT_PAIR, 6) void k_pair(Geo g, DynCoef c, const dou
 
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     double dpv = 0., dpcv = 0.;
-    double fx0 = 0., fx1 = 0., fy0 = 0., fy1 = 0.;  // dp mass fluxes on the cell's faces
+    R fx0 = 0., fx1 = 0., fy0 = 0., fy1 = 0.;  // dp mass fluxes on the cell's faces
     for (int pass = 0; pass < 2; pass++) {
         // inner y step (tmp of the outer x step), with deny; inner x step with denx; Laplacian
         FOR_REG(RDY)
-        const double v0 = VC(x, y), v1 = VC(x, y + 1);
-        const double dny = 1. / (1. - c.dtdy * (v1 - v0));
-        const double q0 = Q(x, y);
-        const double f0 = pcm_face(Q(x, y - 1), q0, c.dtdx * v0);  // Q2: cy = dt/dx * vc
-        const double f1 = pcm_face(q0, Q(x, y + 1), c.dtdx * v1);
-        sTX[RDY::at(x, y)] = .5 * (q0 + dny * (q0 - c.rA * (f1 - f0)));
+        const R v0 = VC(x, y), v1 = VC(x, y + 1);
+        const R dny = rcp(R(1) - r.dtdy * (v1 - v0));
+        const R q0 = Q(x, y);
+        const R f0 = pcm_face(Q(x, y - 1), q0, r.dtdx * v0);  // Q2: cy = dt/dx * vc
+        const R f1 = pcm_face(q0, Q(x, y + 1), r.dtdx * v1);
+        sTX[RDY::at(x, y)] = R(.5) * (q0 + dny * (q0 - r.rA * (f1 - f0)));
         END_FOR
         FOR_REG(RDX)
-        const double u0 = UC(x, y), u1 = UC(x + 1, y);
-        const double dnx = 1. / (1. - c.dtdx * (u1 - u0));
-        const double q0 = Q(x, y);
-        const double f0 = pcm_face(Q(x - 1, y), q0, c.dtdx * u0);
-        const double f1 = pcm_face(q0, Q(x + 1, y), c.dtdx * u1);
-        sTY[RDX::at(x, y)] = .5 * (q0 + dnx * (q0 - c.rA * (f1 - f0)));
+        const R u0 = UC(x, y), u1 = UC(x + 1, y);
+        const R dnx = rcp(R(1) - r.dtdx * (u1 - u0));
+        const R q0 = Q(x, y);
+        const R f0 = pcm_face(Q(x - 1, y), q0, r.dtdx * u0);
+        const R f1 = pcm_face(q0, Q(x + 1, y), r.dtdx * u1);
+        sTY[RDX::at(x, y)] = R(.5) * (q0 + dnx * (q0 - r.rA * (f1 - f0)));
         END_FOR
         FOR_REG(RL)
-        LP(x, y) = -2. * Q(x, y) * c.rdx2rdy2 + (Q(x + 1, y) + Q(x - 1, y)) * c.rdx2 +
-                   (Q(x, y + 1) + Q(x, y - 1)) * c.rdy2;
+        LP(x, y) = R(-2) * Q(x, y) * r.rdx2rdy2 + (Q(x + 1, y) + Q(x - 1, y)) * r.rdx2 +
+                   (Q(x, y + 1) + Q(x, y - 1)) * r.rdy2;
         END_FOR
         __syncthreads();
         // outer steps: PPM fluxes on faces
         FOR_REG(RFX)
 #define T(m) sTX[RDY::at(x + (m), y)]
-        const double cx = c.dtdx * UC(x, y);
-        const double f = ppm_face<VAR>(T(-3), T(-2), T(-1), T(0), T(1), T(2), cx);
+        const R cx = r.dtdx * UC(x, y);
+        const R f = ppm_face<VAR>(T(-3), T(-2), T(-1), T(0), T(1), T(2), cx);
 #undef T
         sFX[RFX::at(x, y)] = pass == 0 ? f * cx : f;
         END_FOR
         FOR_REG(RFY)
 #define T(m) sTY[RDX::at(x, y + (m))]
-        const double cy = c.dtdx * VC(x, y);  // Q2
-        const double f = ppm_face<VAR>(T(-3), T(-2), T(-1), T(0), T(1), T(2), cy);
+        const R cy = r.dtdx * VC(x, y);  // Q2
+        const R f = ppm_face<VAR>(T(-3), T(-2), T(-1), T(0), T(1), T(2), cy);
 #undef T
         sFY[RFY::at(x, y)] = pass == 0 ? f * cy : f;
         END_FOR
         __syncthreads();
-        const double l0 = LP(tx, ty);
-        const double lxp = LP(tx + 1, ty), lxm = LP(tx - 1, ty), lyp = LP(tx, ty + 1), lym = LP(tx, ty - 1);
+        const R l0 = LP(tx, ty);
+        const R lxp = LP(tx + 1, ty), lxm = LP(tx - 1, ty), lyp = LP(tx, ty + 1), lym = LP(tx, ty - 1);
         if (pass == 0) {
             fx0 = sFX[RFX::at(tx, ty)];
             fx1 = sFX[RFX::at(tx + 1, ty)];
             fy0 = sFY[RFY::at(tx, ty)];
             fy1 = sFY[RFY::at(tx, ty + 1)];
-            dpv = Q(tx, ty);
-            double d = dpv - (fx1 - fx0 + c.dtdx * c.vx * (lxp - 2. * l0 + lxm));
-            d = d - (fy1 - fy0 + c.dtdy * c.vy * (lyp - 2. * l0 + lym));
+            if constexpr (FP64)
+                dpv = Q(tx, ty);
+            else
+                dpv = G(dp, tx, ty);
+            double d = dpv - (D(fx1) - D(fx0) + c.dtdx * c.vx * (D(lxp) - 2. * D(l0) + D(lxm)));
+            d = d - (D(fy1) - D(fy0) + c.dtdy * c.vy * (D(lyp) - 2. * D(l0) + D(lym)));
             dpcv = d;
             G(dpo, tx, ty) = d;
             if (!CGRID) atm_report_nonfinite(nfw, d);
@@ -180,7 +214,11 @@ __global__ __launch_bounds__(NT_PAIR, 6) void k_pair(Geo g, DynCoef c, const dou
             FOR_REG(RQ) Q(x, y) = G(pt, x, y); END_FOR
             __syncthreads();
         } else {
-            const double ptv = Q(tx, ty);
+            double ptv;
+            if constexpr (FP64)
+                ptv = Q(tx, ty);
+            else
+                ptv = G(pt, tx, ty);
             // heating rate: the C step forms it from dp and playkap(dp)
             // (cgrid 99-106); the D step reads the C step's (dgrid 457-465)
             double hr;
@@ -190,10 +228,10 @@ __global__ __launch_bounds__(NT_PAIR, 6) void k_pair(Geo g, DynCoef c, const dou
             } else {
                 hr = G(heat, tx, ty);
             }
-            double p = dpv * ptv - (sFX[RFX::at(tx + 1, ty)] * fx1 - sFX[RFX::at(tx, ty)] * fx0 +
-                                    c.vx * ((lxp - l0) * fx1 - (l0 - lxm) * fx0));
-            p = (p - (sFY[RFY::at(tx, ty + 1)] * fy1 - sFY[RFY::at(tx, ty)] * fy0 +
-                      c.vy * ((lyp - l0) * fy1 - (l0 - lym) * fy0)) +
+            double p = dpv * ptv - (D(sFX[RFX::at(tx + 1, ty)] * fx1) - D(sFX[RFX::at(tx, ty)] * fx0) +
+                                    c.vx * (D((lxp - l0) * fx1) - D((l0 - lxm) * fx0)));
+            p = (p - (D(sFY[RFY::at(tx, ty + 1)] * fy1) - D(sFY[RFY::at(tx, ty)] * fy0) +
+                      c.vy * (D((lyp - l0) * fy1) - D((l0 - lym) * fy0))) +
                  c.dt * dpv * hr) /
                 dpcv;
             G(pto, tx, ty) = p;
@@ -201,7 +239,16 @@ __global__ __launch_bounds__(NT_PAIR, 6) void k_pair(Geo g, DynCoef c, const dou
         }
     }
     if (CGRID) {
-        const double u = UC(tx, ty), v = VC(tx, ty);
+        double u, v;
+        if constexpr (FP64) {
+            u = UC(tx, ty);
+            v = VC(tx, ty);
+        } else {  // the C winds of the cell in double (109-117)
+            const double ub = .5 * (G(uw, tx, ty) + G(uw, tx - 1, ty));
+            u = .5 * (ub + .5 * (G(uw, tx, ty + 1) + G(uw, tx - 1, ty + 1)));
+            const double vb = .5 * (G(vw, tx, ty) + G(vw, tx, ty - 1));
+            v = .5 * (vb + .5 * (G(vw, tx + 1, ty) + G(vw, tx + 1, ty - 1)));
+        }
         G(uco, tx, ty) = u;
         G(vco, tx, ty) = v;
         if (K == g.L - 1) {  // A winds of the last level (119-125), read by the KE pass
@@ -221,6 +268,7 @@ __global__ __launch_bounds__(NT_PAIR, 6) void k_pair(Geo g, DynCoef c, const dou
 // C2: kinetic energy, vorticity, pressure gradient and the uc/vc update of
 // one level (cgrid 294-414).
 // ---------------------------------------------------------------------------
+template <class R>
 __global__ __launch_bounds__(NT_WIND) void k_c2(Geo g, DynCoef c, const double *__restrict__ uc1,
                                            const double *__restrict__ vc1, const double *__restrict__ ud,
                                            const double *__restrict__ vd, const double *__restrict__ pkap,
@@ -228,13 +276,15 @@ __global__ __launch_bounds__(NT_WIND) void k_c2(Geo g, DynCoef c, const double *
                                            const double *__restrict__ va, double *__restrict__ uco,
                                            double *__restrict__ vco) {
     constexpr int TY = TY_WIND, NT = NT_WIND;
+    constexpr bool FP64 = std::is_same<R, double>::value;
+    const CoefR<R> r(c);
     using RB = Reg<-3, TX + 3, -3, TY + 3>;
     using RW = Reg<-2, TX + 3, -2, TY + 3>;
     using RK = Reg<-1, TX, -1, TY>;
     using RTV = Reg<0, TX, -2, TY + 3>;
     using RTW = Reg<-2, TX + 3, 0, TY>;
     using RLV = Reg<0, TX + 1, 0, TY + 1>;
-    __shared__ double sU[RB::n], sV[RB::n], sW[RW::n], sK[RK::n], sTV[RTV::n], sTW[RTW::n], sL[RLV::n];
+    __shared__ R sU[RB::n], sV[RB::n], sW[RW::n], sK[RK::n], sTV[RTV::n], sTW[RTW::n], sL[RLV::n];
     int bx, by, K;
     xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
     const int I0 = ATM_HALO + bx * TX, J0 = ATM_HALO + by * TY;
@@ -249,38 +299,38 @@ __global__ __launch_bounds__(NT_WIND) void k_c2(Geo g, DynCoef c, const double *
     END_FOR
     __syncthreads();
     FOR_REG(RW)  // vorticity (319-323)
-    W(x, y) = c.coriolis - (U(x, y) - U(x, y - 1)) * c.rdy + (V(x, y) - V(x - 1, y)) * c.rdx;
+    W(x, y) = r.coriolis - (U(x, y) - U(x, y - 1)) * r.rdy + (V(x, y) - V(x - 1, y)) * r.rdx;
     END_FOR
     FOR_REG(RK)  // kinetic energy (299-317), last-level A winds, Q4 shifts
     const size_t p2 = g.ix2(I0 + x, J0 + y);
-    const double vaij = va[p2], uaij = ua[p2];
-    const double fy = ppm_face<V_PPM_U>(V(x, y - 2), V(x, y - 1), V(x, y), V(x, y + 1), V(x, y + 2), V(x, y + 3),
-                                        c.dtdy * vaij);
-    const double fx = ppm_face<V_PPM_U>(U(x - 2, y), U(x - 1, y), U(x, y), U(x + 1, y), U(x + 2, y), U(x + 3, y),
-                                        c.dtdx * uaij);
-    const double k = vaij * fy;
-    sK[RK::at(x, y)] = .5 * (k + uaij * fx);
+    const R vaij = va[p2], uaij = ua[p2];
+    const R fy = ppm_face<V_PPM_U>(V(x, y - 2), V(x, y - 1), V(x, y), V(x, y + 1), V(x, y + 2), V(x, y + 3),
+                                   r.dtdy * vaij);
+    const R fx = ppm_face<V_PPM_U>(U(x - 2, y), U(x - 1, y), U(x, y), U(x + 1, y), U(x + 2, y), U(x + 3, y),
+                                   r.dtdx * uaij);
+    const R k = vaij * fy;
+    sK[RK::at(x, y)] = R(.5) * (k + uaij * fx);
     END_FOR
     __syncthreads();
     FOR_REG(RTV)  // inner x step of vorticity, Courant cx shifted by one (353-367, Q4)
-    const double w0 = W(x, y);
-    const double um = G(ud, x - 1, y), u0 = G(ud, x, y);
-    const double dnx = 1. / (1. - c.dtdx * (u0 - um));
-    const double f0 = pcm_face(W(x - 1, y), w0, c.dtdx * um);
-    const double f1 = pcm_face(w0, W(x + 1, y), c.dtdx * u0);
-    sTV[RTV::at(x, y)] = .5 * (w0 + dnx * (w0 - c.rA * (f1 - f0)));
+    const R w0 = W(x, y);
+    const R um = G(ud, x - 1, y), u0 = G(ud, x, y);
+    const R dnx = rcp(R(1) - r.dtdx * (u0 - um));
+    const R f0 = pcm_face(W(x - 1, y), w0, r.dtdx * um);
+    const R f1 = pcm_face(w0, W(x + 1, y), r.dtdx * u0);
+    sTV[RTV::at(x, y)] = R(.5) * (w0 + dnx * (w0 - r.rA * (f1 - f0)));
     END_FOR
     FOR_REG(RTW)  // inner y step of vorticity, Courant cy shifted by one (388-397, Q4)
-    const double w0 = W(x, y);
-    const double vm = G(vd, x, y - 1), v0 = G(vd, x, y);
-    const double dny = 1. / (1. - c.dtdy * (v0 - vm));
-    const double f0 = pcm_face(W(x, y - 1), w0, c.dtdx * vm);
-    const double f1 = pcm_face(w0, W(x, y + 1), c.dtdx * v0);
-    sTW[RTW::at(x, y)] = .5 * (w0 + dny * (w0 - c.rA * (f1 - f0)));
+    const R w0 = W(x, y);
+    const R vm = G(vd, x, y - 1), v0 = G(vd, x, y);
+    const R dny = rcp(R(1) - r.dtdy * (v0 - vm));
+    const R f0 = pcm_face(W(x, y - 1), w0, r.dtdx * vm);
+    const R f1 = pcm_face(w0, W(x, y + 1), r.dtdx * v0);
+    sTW[RTW::at(x, y)] = R(.5) * (w0 + dny * (w0 - r.rA * (f1 - f0)));
     END_FOR
     FOR_REG(RLV)
-    sL[RLV::at(x, y)] = -2. * W(x, y) * c.rdx2rdy2 + (W(x + 1, y) + W(x - 1, y)) * c.rdx2 +
-                        (W(x, y + 1) + W(x, y - 1)) * c.rdy2;
+    sL[RLV::at(x, y)] = R(-2) * W(x, y) * r.rdx2rdy2 + (W(x + 1, y) + W(x - 1, y)) * r.rdx2 +
+                        (W(x, y + 1) + W(x, y - 1)) * r.rdy2;
     END_FOR
     __syncthreads();
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
@@ -300,21 +350,29 @@ __global__ __launch_bounds__(NT_WIND) void k_c2(Geo g, DynCoef c, const double *
                (a - b + cc - d);
     }
     const double udv = G(ud, tx, ty), vdv = G(vd, tx, ty);
+    double ucv, vcv;  // this cell's C winds in double
+    if constexpr (FP64) {
+        ucv = U(tx, ty);
+        vcv = V(tx, ty);
+    } else {
+        ucv = G(uc1, tx, ty);
+        vcv = G(vc1, tx, ty);
+    }
     // uc (369-382)
     {
 #define T(m) sTV[RTV::at(tx, ty + 1 + (m))]
-        const double fy = ppm_face<V_PPM_U>(T(-3), T(-2), T(-1), T(0), T(1), T(2), c.dtdx * vdv);
+        const R fy = ppm_face<V_PPM_U>(T(-3), T(-2), T(-1), T(0), T(1), T(2), r.dtdx * R(vdv));
 #undef T
-        G(uco, tx, ty) = U(tx, ty) + c.dt * (vdv * fy + c.vy * (sL[RLV::at(tx, ty + 1)] - sL[RLV::at(tx, ty)])) -
-                         c.dtdx * (sK[RK::at(tx, ty)] - sK[RK::at(tx - 1, ty)] - pgfx);
+        G(uco, tx, ty) = ucv + c.dt * (vdv * D(fy) + c.vy * (D(sL[RLV::at(tx, ty + 1)]) - D(sL[RLV::at(tx, ty)]))) -
+                         c.dtdx * (D(sK[RK::at(tx, ty)]) - D(sK[RK::at(tx - 1, ty)]) - pgfx);
     }
     // vc (399-412)
     {
 #define T(m) sTW[RTW::at(tx + 1 + (m), ty)]
-        const double fx = ppm_face<V_PPM_U>(T(-3), T(-2), T(-1), T(0), T(1), T(2), c.dtdx * udv);
+        const R fx = ppm_face<V_PPM_U>(T(-3), T(-2), T(-1), T(0), T(1), T(2), r.dtdx * R(udv));
 #undef T
-        G(vco, tx, ty) = V(tx, ty) - c.dt * (udv * fx + c.vx * (sL[RLV::at(tx + 1, ty)] - sL[RLV::at(tx, ty)])) -
-                         c.dtdy * (sK[RK::at(tx, ty)] - sK[RK::at(tx, ty - 1)] - pgfy);
+        G(vco, tx, ty) = vcv - c.dt * (udv * D(fx) + c.vx * (D(sL[RLV::at(tx + 1, ty)]) - D(sL[RLV::at(tx, ty)]))) -
+                         c.dtdy * (D(sK[RK::at(tx, ty)]) - D(sK[RK::at(tx, ty - 1)]) - pgfy);
     }
 #undef G
 #undef U
@@ -326,12 +384,15 @@ __global__ __launch_bounds__(NT_WIND) void k_c2(Geo g, DynCoef c, const double *
 // D2: kinetic energy, vorticity, divergence damping, B-grid pressure
 // gradient and the ud/vd update of one level (dgrid 606-771).
 // ---------------------------------------------------------------------------
+template <class R>
 __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *__restrict__ ud,
                                            const double *__restrict__ vd, const double *__restrict__ uc,
                                            const double *__restrict__ vc, const double *__restrict__ pkapb,
                                            const double *__restrict__ gzb, double *__restrict__ udo,
                                            double *__restrict__ vdo, int *nfw) {
     constexpr int TY = TY_WIND, NT = NT_WIND;
+    constexpr bool FP64 = std::is_same<R, double>::value;
+    const CoefR<R> r(c);
     using RU = Reg<-3, TX + 3, -3, TY + 4>;
     using RV = Reg<-3, TX + 4, -3, TY + 3>;
     using RKE = Reg<0, TX + 1, 0, TY + 1>;
@@ -341,11 +402,11 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
     using RTU = Reg<0, TX, -3, TY + 3>;
     using RTW = Reg<-3, TX + 3, 0, TY>;
     using RLP = Reg<-1, TX, -1, TY>;
-    __shared__ double sU[RU::n], sV[RV::n], sKE[RKE::n], sW[RW::n], sDV[RDV::n], sD2[RD2::n];
+    __shared__ R sU[RU::n], sV[RV::n], sKE[RKE::n], sW[RW::n], sDV[RDV::n], sD2[RD2::n];
     // del^4 divergence reuses the divergence's storage (dead once del^2 is formed)
     static_assert(RKE::n <= RDV::n, "sD4 fits in sDV");
-    double *const sD4 = sDV;
-    __shared__ double sTU[RTU::n], sTW[RTW::n], sL[RLP::n];
+    R *const sD4 = sDV;
+    __shared__ R sTU[RTU::n], sTW[RTW::n], sL[RLP::n];
     int bx, by, K;
     xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
     const int I0 = ATM_HALO + bx * TX, J0 = ATM_HALO + by * TY;
@@ -362,25 +423,25 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
     FOR_REG(RKE)  // kinetic energy (639-657), PPM_CONSTRAINED, last-level B winds
     const int Lm = g.L - 1;
     const double *ucl = uc + TBASE(Lm), *vcl = vc + TBASE(Lm);
-    const double ubij = .5 * (ucl[TOFF(x, y)] + ucl[TOFF(x, y - 1)]);
-    const double vbij = .5 * (vcl[TOFF(x, y)] + vcl[TOFF(x - 1, y)]);
-    const double fy = ppm_face<V_PPM_C>(V(x, y - 3), V(x, y - 2), V(x, y - 1), V(x, y), V(x, y + 1), V(x, y + 2),
-                                        c.dtdy * vbij);
-    const double fx = ppm_face<V_PPM_C>(U(x - 3, y), U(x - 2, y), U(x - 1, y), U(x, y), U(x + 1, y), U(x + 2, y),
-                                        c.dtdx * ubij);
-    const double k = vbij * fy;
-    sKE[RKE::at(x, y)] = .5 * (k + ubij * fx);
+    const R ubij = R(.5 * (ucl[TOFF(x, y)] + ucl[TOFF(x, y - 1)]));
+    const R vbij = R(.5 * (vcl[TOFF(x, y)] + vcl[TOFF(x - 1, y)]));
+    const R fy = ppm_face<V_PPM_C>(V(x, y - 3), V(x, y - 2), V(x, y - 1), V(x, y), V(x, y + 1), V(x, y + 2),
+                                   r.dtdy * vbij);
+    const R fx = ppm_face<V_PPM_C>(U(x - 3, y), U(x - 2, y), U(x - 1, y), U(x, y), U(x + 1, y), U(x + 2, y),
+                                   r.dtdx * ubij);
+    const R k = vbij * fy;
+    sKE[RKE::at(x, y)] = R(.5) * (k + ubij * fx);
     END_FOR
     FOR_REG(RW)  // vorticity (659-663)
-    W(x, y) = c.coriolis - (U(x, y + 1) - U(x, y)) * c.rdy + (V(x + 1, y) - V(x, y)) * c.rdx;
+    W(x, y) = r.coriolis - (U(x, y + 1) - U(x, y)) * r.rdy + (V(x + 1, y) - V(x, y)) * r.rdx;
     END_FOR
     FOR_REG(RDV)  // divergence (682-685)
-    DV(x, y) = c.rdx * (U(x, y) - U(x - 1, y)) + c.rdy * (V(x, y) - V(x, y - 1));
+    DV(x, y) = r.rdx * (U(x, y) - U(x - 1, y)) + r.rdy * (V(x, y) - V(x, y - 1));
     END_FOR
     __syncthreads();
     FOR_REG(RD2)  // del^2 divergence (688-692)
-    D2(x, y) = -2. * DV(x, y) * c.rdx2rdy2 + (DV(x + 1, y) + DV(x - 1, y)) * c.rdx2 +
-               (DV(x, y + 1) + DV(x, y - 1)) * c.rdy2;
+    D2(x, y) = R(-2) * DV(x, y) * r.rdx2rdy2 + (DV(x + 1, y) + DV(x - 1, y)) * r.rdx2 +
+               (DV(x, y + 1) + DV(x, y - 1)) * r.rdy2;
     END_FOR
     // The reference's second level loop (634-771) reuses the 2-D arrays cx,
     // cy, denom_x, denom_y (and ub, vb) that its first loop (441-454) left
@@ -390,29 +451,29 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
     const size_t kbL = TBASE(Lm);
 #define GL(a, x, y) (a + kbL)[TOFF(x, y)]
     FOR_REG(RTU)  // inner x step of vorticity for ud (708-716)
-    const double w0 = W(x, y);
-    const double u0 = GL(uc, x, y), u1 = GL(uc, x + 1, y);
-    const double dnx = 1. / (1. - c.dtdx * (u1 - u0));
-    const double f0 = pcm_face(W(x - 1, y), w0, c.dtdx * u0);
-    const double f1 = pcm_face(w0, W(x + 1, y), c.dtdx * u1);
-    sTU[RTU::at(x, y)] = .5 * (w0 + dnx * (w0 - c.rA * (f1 - f0)));
+    const R w0 = W(x, y);
+    const R u0 = GL(uc, x, y), u1 = GL(uc, x + 1, y);
+    const R dnx = rcp(R(1) - r.dtdx * (u1 - u0));
+    const R f0 = pcm_face(W(x - 1, y), w0, r.dtdx * u0);
+    const R f1 = pcm_face(w0, W(x + 1, y), r.dtdx * u1);
+    sTU[RTU::at(x, y)] = R(.5) * (w0 + dnx * (w0 - r.rA * (f1 - f0)));
     END_FOR
     FOR_REG(RTW)  // inner y step of vorticity for vd (744-752)
-    const double w0 = W(x, y);
-    const double v0 = GL(vc, x, y), v1 = GL(vc, x, y + 1);
-    const double dny = 1. / (1. - c.dtdy * (v1 - v0));
-    const double f0 = pcm_face(W(x, y - 1), w0, c.dtdx * v0);  // Q2
-    const double f1 = pcm_face(w0, W(x, y + 1), c.dtdx * v1);
-    sTW[RTW::at(x, y)] = .5 * (w0 + dny * (w0 - c.rA * (f1 - f0)));
+    const R w0 = W(x, y);
+    const R v0 = GL(vc, x, y), v1 = GL(vc, x, y + 1);
+    const R dny = rcp(R(1) - r.dtdy * (v1 - v0));
+    const R f0 = pcm_face(W(x, y - 1), w0, r.dtdx * v0);  // Q2
+    const R f1 = pcm_face(w0, W(x, y + 1), r.dtdx * v1);
+    sTW[RTW::at(x, y)] = R(.5) * (w0 + dny * (w0 - r.rA * (f1 - f0)));
     END_FOR
     FOR_REG(RLP)  // Laplacian of vorticity (717-720)
-    sL[RLP::at(x, y)] = -2. * W(x, y) * c.rdx2rdy2 + (W(x + 1, y) + W(x - 1, y)) * c.rdx2 +
-                        (W(x, y + 1) + W(x, y - 1)) * c.rdy2;
+    sL[RLP::at(x, y)] = R(-2) * W(x, y) * r.rdx2rdy2 + (W(x + 1, y) + W(x - 1, y)) * r.rdx2 +
+                        (W(x, y + 1) + W(x, y - 1)) * r.rdy2;
     END_FOR
     __syncthreads();
     FOR_REG(RKE)  // del^4 divergence times the damping coefficient (695-702)
-    sD4[RKE::at(x, y)] = c.divdamp * (-2. * D2(x, y) * c.rdx2rdy2 + (D2(x + 1, y) + D2(x - 1, y)) * c.rdx2 +
-                                      (D2(x, y + 1) + D2(x, y - 1)) * c.rdy2);
+    sD4[RKE::at(x, y)] = r.divdamp * (R(-2) * D2(x, y) * r.rdx2rdy2 + (D2(x + 1, y) + D2(x - 1, y)) * r.rdx2 +
+                                      (D2(x, y + 1) + D2(x, y - 1)) * r.rdy2);
     END_FOR
     __syncthreads();
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
@@ -437,25 +498,33 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
     const double ucv = G(uc, tx, ty), vcv = G(vc, tx, ty);
     const double ucl = GL(uc, tx, ty), vcl = GL(vc, tx, ty);
 #undef GL
+    double udv, vdv;  // this cell's D winds in double
+    if constexpr (FP64) {
+        udv = U(tx, ty);
+        vdv = V(tx, ty);
+    } else {
+        udv = G(ud, tx, ty);
+        vdv = G(vd, tx, ty);
+    }
     {  // ud (724-738)
 #define T(m) sTU[RTU::at(tx, ty + (m))]
-        const double fy = ppm_face<V_PPM_C>(T(-3), T(-2), T(-1), T(0), T(1), T(2), c.dtdx * vcl);
+        const R fy = ppm_face<V_PPM_C>(T(-3), T(-2), T(-1), T(0), T(1), T(2), r.dtdx * R(vcl));
 #undef T
-        const double un = U(tx, ty) +
-                          c.dt * (vcv * fy + c.vy * (sL[RLP::at(tx, ty)] - sL[RLP::at(tx, ty - 1)])) -
-                          c.dtdx * (sKE[RKE::at(tx + 1, ty)] - sKE[RKE::at(tx, ty)] - sD4[RKE::at(tx + 1, ty)] +
-                                    sD4[RKE::at(tx, ty)] - pgfx);
+        const double un = udv +
+                          c.dt * (vcv * D(fy) + c.vy * (D(sL[RLP::at(tx, ty)]) - D(sL[RLP::at(tx, ty - 1)]))) -
+                          c.dtdx * (D(sKE[RKE::at(tx + 1, ty)]) - D(sKE[RKE::at(tx, ty)]) -
+                                    D(sD4[RKE::at(tx + 1, ty)]) + D(sD4[RKE::at(tx, ty)]) - pgfx);
         G(udo, tx, ty) = un;
         atm_report_nonfinite(nfw, un);
     }
     {  // vd (755-769)
 #define T(m) sTW[RTW::at(tx + (m), ty)]
-        const double fx = ppm_face<V_PPM_C>(T(-3), T(-2), T(-1), T(0), T(1), T(2), c.dtdx * ucl);
+        const R fx = ppm_face<V_PPM_C>(T(-3), T(-2), T(-1), T(0), T(1), T(2), r.dtdx * R(ucl));
 #undef T
-        const double vn = V(tx, ty) -
-                          c.dt * (ucv * fx + c.vx * (sL[RLP::at(tx, ty)] - sL[RLP::at(tx - 1, ty)])) -
-                          c.dtdy * (sKE[RKE::at(tx, ty + 1)] - sKE[RKE::at(tx, ty)] - sD4[RKE::at(tx, ty + 1)] +
-                                    sD4[RKE::at(tx, ty)] - pgfy);
+        const double vn = vdv -
+                          c.dt * (ucv * D(fx) + c.vx * (D(sL[RLP::at(tx, ty)]) - D(sL[RLP::at(tx - 1, ty)]))) -
+                          c.dtdy * (D(sKE[RKE::at(tx, ty + 1)]) - D(sKE[RKE::at(tx, ty)]) -
+                                    D(sD4[RKE::at(tx, ty + 1)]) + D(sD4[RKE::at(tx, ty)]) - pgfy);
         G(vdo, tx, ty) = vn;
         atm_report_nonfinite(nfw, vn);
     }
@@ -625,6 +694,8 @@ __global__ void __launch_bounds__(64) k_scan_pg(Geo g, double ptop, double pktop
     }
 }
 
+#undef D
+
 }  // namespace
 
 bool atm_fused_supported(const Geo &g) { return g.nx % TX == 0 && g.ny % TY_PAIR == 0 && g.ny % TY_WIND == 0; }
@@ -650,9 +721,14 @@ int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
     dim3 grid(g.nx / TX, g.ny / TY_PAIR, g.L), blk(NT_PAIR);
     dim3 gridw(g.nx / TX, g.ny / TY_WIND, g.L), blkw(NT_WIND);
     atm_timer_begin(c, ATM_T_C1);
-    hipLaunchKernelGGL((k_pair<V_PPM_U, true>), grid, blk, 0, c->stream, g, k, b.dp, b.pt, b.ud, b.vd,
-                       (const double *)nullptr, c->plk, c->net_flux,
-                       c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va, (int *)nullptr);
+    if (c->mixed)
+        hipLaunchKernelGGL((k_pair<V_PPM_U, true, float>), grid, blk, 0, c->stream, g, k, b.dp, b.pt, b.ud, b.vd,
+                           (const double *)nullptr, c->plk, c->net_flux, c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va,
+                           (int *)nullptr);
+    else
+        hipLaunchKernelGGL((k_pair<V_PPM_U, true, double>), grid, blk, 0, c->stream, g, k, b.dp, b.pt, b.ud, b.vd,
+                           (const double *)nullptr, c->plk, c->net_flux, c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va,
+                           (int *)nullptr);
     atm_timer_end(c, ATM_T_C1);
     {
         double *fill[] = {c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va};
@@ -668,8 +744,12 @@ int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
                        (double *)nullptr);
     atm_timer_end(c, ATM_T_SCAN);
     atm_timer_begin(c, ATM_T_C2);
-    hipLaunchKernelGGL(k_c2, gridw, blkw, 0, c->stream, g, k, c->uc, c->vc, b.ud, b.vd, c->pkap, c->gz, c->ua, c->va,
-                       c->uc2, c->vc2);
+    if (c->mixed)
+        hipLaunchKernelGGL(k_c2<float>, gridw, blkw, 0, c->stream, g, k, c->uc, c->vc, b.ud, b.vd, c->pkap, c->gz, c->ua,
+                           c->va, c->uc2, c->vc2);
+    else
+        hipLaunchKernelGGL(k_c2<double>, gridw, blkw, 0, c->stream, g, k, c->uc, c->vc, b.ud, b.vd, c->pkap, c->gz,
+                           c->ua, c->va, c->uc2, c->vc2);
     atm_timer_end(c, ATM_T_C2);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;
@@ -695,10 +775,14 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
     dim3 grid(g.nx / TX, g.ny / TY_PAIR, g.L), blk(NT_PAIR);
     dim3 gridw(g.nx / TX, g.ny / TY_WIND, g.L), blkw(NT_WIND);
     atm_timer_begin(c, ATM_T_D1);
-    hipLaunchKernelGGL((k_pair<V_PPM_NN, false>), grid, blk, 0, c->stream, g, k, cur.dp, cur.pt, c->uc2, c->vc2,
-                       c->heatd, (const double *)nullptr, (const double *)nullptr, nxt.dp, nxt.pt,
-                       (double *)nullptr, (double *)nullptr, (double *)nullptr,
-                       (double *)nullptr, c->nfw);
+    if (c->mixed)
+        hipLaunchKernelGGL((k_pair<V_PPM_NN, false, float>), grid, blk, 0, c->stream, g, k, cur.dp, cur.pt, c->uc2,
+                           c->vc2, c->heatd, (const double *)nullptr, (const double *)nullptr, nxt.dp, nxt.pt,
+                           (double *)nullptr, (double *)nullptr, (double *)nullptr, (double *)nullptr, c->nfw);
+    else
+        hipLaunchKernelGGL((k_pair<V_PPM_NN, false, double>), grid, blk, 0, c->stream, g, k, cur.dp, cur.pt, c->uc2,
+                           c->vc2, c->heatd, (const double *)nullptr, (const double *)nullptr, nxt.dp, nxt.pt,
+                           (double *)nullptr, (double *)nullptr, (double *)nullptr, (double *)nullptr, c->nfw);
     atm_timer_end(c, ATM_T_D1);
     {
         double *fill[] = {nxt.dp, nxt.pt};
@@ -716,8 +800,12 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
                        c->gz, c->pkapb, c->gzb);
     atm_timer_end(c, ATM_T_SCAN);
     atm_timer_begin(c, ATM_T_D2);
-    hipLaunchKernelGGL(k_d2, gridw, blkw, 0, c->stream, g, k, cur.ud, cur.vd, c->uc2, c->vc2, c->pkapb, c->gzb,
-                       nxt.ud, nxt.vd, c->nfw);
+    if (c->mixed)
+        hipLaunchKernelGGL(k_d2<float>, gridw, blkw, 0, c->stream, g, k, cur.ud, cur.vd, c->uc2, c->vc2, c->pkapb,
+                           c->gzb, nxt.ud, nxt.vd, c->nfw);
+    else
+        hipLaunchKernelGGL(k_d2<double>, gridw, blkw, 0, c->stream, g, k, cur.ud, cur.vd, c->uc2, c->vc2, c->pkapb,
+                           c->gzb, nxt.ud, nxt.vd, c->nfw);
     atm_timer_end(c, ATM_T_D2);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;

@@ -65,6 +65,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads of the CPU baseline (0: OMP_NUM_THREADS or all host cores)")
+    ap.add_argument("--mixed", choices=["on", "off"], default="on",
+                    help="also time the mixed-precision form and its error after --mixed-error-steps steps")
+    ap.add_argument("--mixed-error-steps", type=int, default=100)
     ap.add_argument("--radiation-calls", type=int, default=3,
                     help="also time this many radiation calls on the same grid (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -120,6 +123,79 @@ def cpu_baseline(args):
                             "sample": f"{n1}x{n1}x{L}, {steps1} steps, {sec1:.1f} s on 1 host core"}}
 
 
+def surface_pressure_and_temperature(m, ptop):
+    """ps = p_top + sum(dp) and layer temperature T = pt * playkap (the
+    dynamics' playkap form, mod_sw_dyn.f90:99-101) of a model's state."""
+    import numpy as np
+    kappa = float(np.float32(287.052874)) / 1005.0
+    dp, pt = m.get("dp"), m.get("pt")
+    plev = np.concatenate([np.cumsum(dp[::-1], axis=0)[::-1] + ptop, np.full((1,) + dp.shape[1:], ptop)])
+    pk = plev ** kappa
+    return plev[0], pt * ((pk[1:] - pk[:-1]) / np.log(pk[1:] / pk[:-1]))
+
+
+def mixed_timing(args, ic, cfg_kw):
+    """The mixed-precision form (ATM_PRECISION_MIXED) on the same workload:
+    throughput over the same steps, and after --mixed-error-steps steps from
+    the initial state its relative RMS and max error against the FP64 path
+    in surface pressure, temperature and the prognostic fields, with the
+    drift of the total mass."""
+    import numpy as np
+
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.model import Model
+    nx, ny, L = args.nx, args.ny, args.nlay
+    m = Model(nx, ny, L, args.dx, precision="mixed")
+    st = driver.init_model(m, ic, Config(**cfg_kw))
+    cfg = Config(**cfg_kw)
+    t = 0.0
+    for _ in range(args.warmup):
+        t += driver.step(m, cfg, st, t)
+        assert m.check_stability()
+    m.sync()
+    m.timer_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t += driver.step(m, cfg, st, t)
+        if not m.check_stability():
+            raise SystemExit("mixed: unstable")
+    m.sync()
+    sec = time.perf_counter() - t0
+    kernels = {}
+    for name in ("c1", "c2", "d1", "d2", "scan", "halo", "remap"):
+        n, ms = m.timer_read(name)
+        if n:
+            kernels[name] = ms / args.steps
+    m.timer_enable(False)
+    del m
+    # error after N steps from the same initial state
+    res = {}
+    for prec in ("fp64", "mixed"):
+        mm = Model(nx, ny, L, args.dx, precision=prec)
+        c = Config(**cfg_kw)
+        s2 = driver.init_model(mm, ic, c)
+        mass0 = float(mm.get("dp").sum())
+        tt = 0.0
+        for _ in range(args.mixed_error_steps):
+            tt += driver.step(mm, c, s2, tt)
+        ps, T = surface_pressure_and_temperature(mm, float(ic["ak"][0]))
+        res[prec] = {"ps": ps, "T": T, "mass_drift": (float(mm.get("dp").sum()) - mass0) / mass0,
+                     **{k: mm.get(k) for k in ("dp", "pt", "ud", "vd")}}
+        del mm
+
+    def rel(a, b):
+        return {"rel_rms": float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2))),
+                "rel_max": float(np.max(np.abs(a - b)) / np.max(np.abs(b)))}
+    a, b = res["mixed"], res["fp64"]
+    err = {k: rel(a[k], b[k]) for k in ("ps", "T", "dp", "pt", "ud", "vd")}
+    err["mass_drift"] = {"mixed": a["mass_drift"], "fp64": b["mass_drift"]}
+    value = nx * ny * L * args.steps / sec
+    return {"value": value, "unit": "cell-steps/s", "ms_per_step": sec / args.steps * 1e3,
+            "dtype": "f32 reconstruction and fluxes, f64 updates, scans, pressure gradient, remap",
+            "kernels_ms_per_step": kernels, "error_vs_fp64": {"steps": args.mixed_error_steps, **err}}
+
+
 def radiation_timing(m, args, time_s):
     """calculate_radiative_heating on the benchmark state (all interior
     columns, 16 g-points, shortwave and longwave), timed with HIP events."""
@@ -159,8 +235,9 @@ def main():
 
     nx, ny, L = args.nx, args.ny, args.nlay
     ic = make_initial_state(nx, ny, L, args.dx)
-    cfg = Config(dt_max=args.dt, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12,
-                 dt_remap=args.remap_every * args.dt, radiation_on=False, physics_on=False)
+    cfg_kw = dict(dt_max=args.dt, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12,
+                  dt_remap=args.remap_every * args.dt, radiation_on=False, physics_on=False)
+    cfg = Config(**cfg_kw)
     m = Model(nx, ny, L, args.dx)
     if args.path != m.path:
         m.set_path(args.path)
@@ -211,6 +288,9 @@ def main():
     }
     if args.radiation_calls > 0:
         out["radiation"] = radiation_timing(m, args, time_s)
+    if args.mixed == "on" and args.path == "fused":
+        del m
+        out["mixed"] = mixed_timing(args, ic, cfg_kw)
     if args.cpu_baseline == "on":
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)

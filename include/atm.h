@@ -70,7 +70,15 @@ typedef struct atm_params {
     double vorticity_damping_coefficient;
 } atm_params;
 
-enum { ATM_PRECISION_FP64 = 0 };
+/* Arithmetic of the dynamics.  FP64: the drop-in, the reference's
+   operations in double.  MIXED: the C- and D-step tile kernels form the PPM
+   reconstructions, fluxes, kinetic energy, vorticity and damping terms in
+   float and update dp, pt and the winds in double; the column scans,
+   pressure gradient, heating rate, remap, radiation and physics stay FP64.
+   Each face flux is one value shared by the two cells it separates, so mass
+   is conserved to FP64 rounding.  MIXED needs the fused path (nx % 32 == 0,
+   ny % 16 == 0); it has no counterpart in the reference. */
+enum { ATM_PRECISION_FP64 = 0, ATM_PRECISION_MIXED = 1 };
 
 /* Context lifetime.  Allocates every device buffer for one model instance.
    Replaces allocate_fields (mod_fields.f90:41-67) and the module

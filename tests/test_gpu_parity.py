@@ -288,3 +288,50 @@ def test_stability_check_sees_nonfinite_from_a_step(gpu_available, route):
     assert bad
     assert not gpu.check_stability()
     assert not gpu.check_stability()
+
+
+def _ps_T(m, ptop):
+    kappa = float(np.float32(287.052874)) / 1005.0
+    dp, pt = m.get("dp"), m.get("pt")
+    plev = np.concatenate([np.cumsum(dp[::-1], axis=0)[::-1] + ptop, np.full((1,) + dp.shape[1:], ptop)])
+    pk = plev ** kappa
+    return plev[0], pt * ((pk[1:] - pk[:-1]) / np.log(pk[1:] / pk[:-1]))
+
+
+def test_mixed_precision_error_and_mass(gpu_available):
+    """ATM_PRECISION_MIXED against the FP64 path: 30 steps with remaps every
+    5 at 64x64x20 from the same state.  Surface pressure and temperature
+    agree to float-level error, and the total mass (sum of dp) changes only
+    by FP64 rounding in both forms (the float fluxes are shared by the two
+    cells of each face)."""
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import AtmError, Model
+    nx, ny, nlay = 64, 64, 20
+    ic = make_initial_state(nx, ny, nlay, 100e3)
+    kw = dict(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12, dt_remap=300.0,
+              radiation_on=False, physics_on=False)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        m = Model(nx, ny, nlay, 100e3, precision=prec)
+        cfg = Config(**kw)
+        st = driver.init_model(m, ic, cfg)
+        mass0 = m.get("dp").sum()
+        t = 0.0
+        for _ in range(30):
+            t += driver.step(m, cfg, st, t)
+            assert m.check_stability()
+        ps, T = _ps_T(m, ic["ak"][0])
+        out[prec] = dict(ps=ps, T=T, ud=m.get("ud"), vd=m.get("vd"), drift=(m.get("dp").sum() - mass0) / mass0)
+        if prec == "mixed":
+            with pytest.raises(AtmError):
+                m.set_path("reference")
+    a, b = out["mixed"], out["fp64"]
+    rr = {k: float(np.sqrt(np.mean((a[k] - b[k]) ** 2)) / np.sqrt(np.mean(b[k] ** 2))) for k in ("ps", "T", "ud", "vd")}
+    _print("mixed vs fp64 x30 rel rms", **rr, drift_mixed=abs(a["drift"]), drift_fp64=abs(b["drift"]))
+    assert rr["ps"] <= 1e-6 and rr["T"] <= 1e-6, rr
+    assert rr["ud"] <= 1e-4 and rr["vd"] <= 1e-4, rr
+    assert abs(a["drift"]) <= 1e-13 and abs(b["drift"]) <= 1e-13
+    with pytest.raises(AtmError):  # the mixed form needs the fused path's tiling
+        Model(40, 24, 8, 100e3, precision="mixed")
