@@ -98,7 +98,7 @@ template <class R, bool CGRID>
 constexpr int pair_waves() { return std::is_same<R, double>::value || CGRID ? 6 : 8; }
 
 template <int VAR, bool CGRID, class R>
-__global__ __launch_bounds__(NT_PAIR, pair_waves<R, CGRID>()) void k_pair(Geo g, DynCoef c, const double *__restrict__ dp,
+__global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo g, DynCoef c, const double *__restrict__ dp,
                                              const double *__restrict__ pt, const double *__restrict__ uw,
                                              const double *__restrict__ vw, const double *__restrict__ heat,
                                              const double *__restrict__ plk, const double *__restrict__ nf,
