@@ -335,3 +335,35 @@ def test_mixed_precision_error_and_mass(gpu_available):
     assert abs(a["drift"]) <= 1e-13 and abs(b["drift"]) <= 1e-13
     with pytest.raises(AtmError):  # the mixed form needs the fused path's tiling
         Model(40, 24, 8, 100e3, precision="mixed")
+
+
+def test_full_size_parity(gpu_available):
+    """The benchmark workload itself: 512x512x64, dx = 25 km, dt = 20 s,
+    remap every 5 steps, 6 steps (one remap) from the seeded state, GPU
+    fused path against the (threaded) oracle, 1e-12 norm-relative on the
+    prognostic fields, and the total mass change of both at FP64 rounding."""
+    import oracle as O
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+    nx, ny, nlay, dx = 512, 512, 64, 25e3
+    print(f"full size {nx}x{ny}x{nlay}", flush=True)
+    ic = make_initial_state(nx, ny, nlay, dx)
+    kw = dict(dt_max=20.0, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12, dt_remap=100.0,
+              radiation_on=False, physics_on=False)
+    gpu, orc = Model(nx, ny, nlay, dx), O.Oracle(nx, ny, nlay, dx, dx)
+    assert gpu.path == "fused"
+    mass = {}
+    for b in (gpu, orc):
+        cfg = Config(**kw)
+        st = driver.init_model(b, ic, cfg)
+        t = 0.0
+        for _ in range(6):
+            t += driver.step(b, cfg, st, t)
+    errs = {n: compare(gpu, orc, n) for n in ("dp", "pt", "ud", "vd")}
+    _print("full size x6", **errs)
+    for n, e in errs.items():
+        assert e <= PARITY, (n, e)
+    m0 = ic["dp"].sum()
+    assert abs(gpu.get("dp").sum() - m0) / m0 <= 1e-13
