@@ -14,6 +14,7 @@
 // node >= x (lookup_1d 4556-4579, lookup_pT_table 4581-4609).  Every table
 // axis is strictly increasing, so a binary search returns the same node and
 // the interpolation is then computed with the reference's formula.
+#include <type_traits>
 #include <vector>
 
 #include "atm_common.hpp"
@@ -144,7 +145,7 @@ struct Lin {
     int k = 0;
     bool hit = false, oor = false;
     double d1 = 0., d2 = 0., d3 = 1., z = 1.;  // xm - x, x - xk, xm - xk, 1 / (xm - xk)
-    __device__ double apply(const float *ys) const {
+    __device__ __forceinline__ double apply(const float *ys) const {
         if (oor) return 0.;
         if (hit) return (double)ys[k];
         const double yk = ys[k], ym = ys[k - 1];
@@ -178,7 +179,7 @@ struct LinPT {
     int k = 0;
     bool hit = false, oor = false;
     double d1 = 0., d2 = 0., d3 = 1., z = 1.;
-    __device__ double apply(const float *table) const {
+    __device__ __forceinline__ double apply(const float *table) const {
         if (oor) return 0.;
         if (hit) return t1.apply(table + 6 * k);
         const double v1 = t1.apply(table + 6 * k);
@@ -261,7 +262,7 @@ __device__ __forceinline__ Comp comp_shfl_down(const Comp &c, unsigned d) {
 // values stay in registers (NBMAX >= ceil(L / CG)).  REFL false: every R is
 // 0 (the longwave), so each 1 / (1 - R a) is exactly 1.
 template <bool REFL, int NBMAX>
-__device__ void adding_group(int L, int j, double *R, double *T, const double *Su, const double *Sd,
+__device__ __forceinline__ void adding_group(int L, int j, double *R, double *T, const double *Su, const double *Sd,
                              const double *B, double toa, double sfc_up, double sfc_alb) {
     const int nb = (L + CG - 1) / CG;
     const int lo = min(j * nb, L), hi = min(lo + nb, L), n = hi - lo;
@@ -337,7 +338,7 @@ __device__ void adding_group(int L, int j, double *R, double *T, const double *S
 // on entry B holds the layers' direct transmittance, Su and Sd the direct-
 // to-diffuse reflectance and transmittance; on return B[k] is the beam at
 // level k and Su, Sd the layers' diffuse sources.  Returns B[L].
-__device__ double beam_group(int L, int j, double toa_dir, double *B, double *Su, double *Sd) {
+__device__ __forceinline__ double beam_group(int L, int j, double toa_dir, double *B, double *Su, double *Sd) {
     const int nb = (L + CG - 1) / CG;
     const int lo = min(j * nb, L), hi = min(lo + nb, L);
     double p = 1.;
@@ -361,10 +362,17 @@ __device__ double beam_group(int L, int j, double toa_dir, double *B, double *Su
     return toa_dir;
 }
 
+// Optics arithmetic type R: double on the FP64 path; float on the
+// mixed-precision path (the g-point optics: extinction, two-stream
+// coefficients, reflectance, transmittance and sources), whose results
+// enter the LDS arrays and the adding method in double.
+__device__ __forceinline__ double odiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float odiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // NL layers and NL + 1 levels per lane (L <= 64 NL).  With NL = 1 the
 // kernel fits 3 waves per SIMD (LDS 12.4 KB and <= 168 registers at
 // nlay = 64), which the launch bound asks for.
-template <int NL>
+template <int NL, class R>
 __global__ void __launch_bounds__(64, NL == 1 ? 3 : 1) k_rad_columns(Geo g, Tables t, double gravity, const double *__restrict__ dp,
                                                     const double *__restrict__ play, const double *__restrict__ lT,
                                                     const double *__restrict__ vT, const double *__restrict__ ts,
@@ -385,7 +393,9 @@ __global__ void __launch_bounds__(64, NL == 1 ? 3 : 1) k_rad_columns(Geo g, Tabl
     const double mu0 = cos(0.);
     const double sfc_alb = (double)0.3f, emiss = (double)0.99f;
     const double tsur = ts[c0];
-    const Lin ls = lin_1d(tsur, t.planckT, 231, &err);
+    // the surface Planck position is found again in each longwave pass
+    // (fewer live registers); its range error is reported here once
+    (void)lin_1d(tsur, t.planckT, 231, &err);
     // per-lane inputs of its layers (K = lane + 64 v) and accumulators of its levels
     constexpr int NV = NL + 1;
     double dmv[NL];
@@ -419,41 +429,41 @@ __global__ void __launch_bounds__(64, NL == 1 ? 3 : 1) k_rad_columns(Geo g, Tabl
 #pragma unroll 1
             for (int gi = 0; gi < RG; gi++) {
                 const int gp = g0 + gi;
-                const double k_sca = (double)t.ray[gp];
-                const double k_abs = lpt[v].apply(t.sw + 6 * 53 * gp);
-                const double k_ext = k_abs + k_sca;
-                const double tau = k_ext * dmv[v];
-                const double ssa = k_sca / k_ext;
+                const R k_sca = R((double)t.ray[gp]);
+                const R k_abs = R(lpt[v].apply(t.sw + 6 * 53 * gp));
+                const R k_ext = k_abs + k_sca;
+                const R tau = k_ext * R(dmv[v]);
+                const R ssa = odiv(k_sca, k_ext);
                 // The reference fixes the asymmetry at 0 (186) and the zenith
                 // angle at 0 (107), so mu0 = cos(0) = 1: the delta-Eddington
                 // rescaling (383-398) leaves tau and ssa unchanged, and the
                 // coefficients below are the reference's expressions (195-260)
                 // with those values substituted; each substitution is exact for
                 // finite ssa (x*1, x/1, x+0, x*0 -> the same double).
-                const double gamma1 = (7. - ssa * 4.) * 0.25;   // gamma3 = gamma4 = 1/2
-                const double gamma2 = (-1. + ssa * 4.) * 0.25;
-                const double alpha1 = gamma1 * 0.5 + gamma2 * 0.5;
-                const double alpha2 = gamma2 * 0.5 + gamma2 * 0.5;
-                double lambda = sqrt(gamma1 * gamma1 - gamma2 * gamma2);
-                const double e_minus = exp(-lambda * tau);
-                double e_2minus = e_minus * e_minus;
-                const double Tdir = exp(-tau);
-                double tmp1 = ssa / ((1. - lambda * lambda) * (lambda + gamma1 + (lambda - gamma1) * e_2minus));
-                double tmp2 = lambda * 0.5;
-                const double Rdif = tmp1 * ((1. - lambda) * (alpha2 + tmp2) - (1. + lambda) * (alpha2 - tmp2) * e_2minus -
-                                            2. * lambda * (0.5 - alpha2) * e_minus * Tdir);
-                const double Tdif = tmp1 * (2. * lambda * (0.5 + alpha1) * e_minus -
-                                            Tdir * ((1. + lambda) * (alpha1 + tmp2) - (1. - lambda) * (alpha1 - tmp2) * e_2minus));
+                const R gamma1 = (R(7) - ssa * R(4)) * R(0.25);   // gamma3 = gamma4 = 1/2
+                const R gamma2 = (R(-1) + ssa * R(4)) * R(0.25);
+                const R alpha1 = gamma1 * R(0.5) + gamma2 * R(0.5);
+                const R alpha2 = gamma2 * R(0.5) + gamma2 * R(0.5);
+                R lambda = sqrt(gamma1 * gamma1 - gamma2 * gamma2);
+                const R e_minus = exp(-lambda * tau);
+                R e_2minus = e_minus * e_minus;
+                const R Tdir = exp(-tau);
+                R tmp1 = odiv(ssa, (R(1) - lambda * lambda) * (lambda + gamma1 + (lambda - gamma1) * e_2minus));
+                R tmp2 = lambda * R(0.5);
+                const R Rdif = tmp1 * ((R(1) - lambda) * (alpha2 + tmp2) - (R(1) + lambda) * (alpha2 - tmp2) * e_2minus -
+                                       R(2) * lambda * (R(0.5) - alpha2) * e_minus * Tdir);
+                const R Tdif = tmp1 * (R(2) * lambda * (R(0.5) + alpha1) * e_minus -
+                                       Tdir * ((R(1) + lambda) * (alpha1 + tmp2) - (R(1) - lambda) * (alpha1 - tmp2) * e_2minus));
                 // diffuse: backscatter 1/2, gamma1 = 2 (1 - ssa/2), gamma2 = ssa
-                const double g1 = 2. * (1. - ssa * 0.5), g2 = ssa;
+                const R g1 = R(2) * (R(1) - ssa * R(0.5)), g2 = ssa;
                 lambda = sqrt(g1 * g1 - g2 * g2);
                 e_2minus = e_minus * e_minus;  // Q6: e_minus of the direct-beam lambda
-                tmp1 = 1. / (lambda + g1 + (lambda - g1) * e_2minus);
+                tmp1 = odiv(R(1), lambda + g1 + (lambda - g1) * e_2minus);
                 arr(0, gi)[K] = Tdir;
                 arr(1, gi)[K] = Rdif;
                 arr(2, gi)[K] = Tdif;
-                arr(3, gi)[K] = tmp1 * g2 * (1. - e_2minus);
-                arr(4, gi)[K] = 2. * tmp1 * lambda * e_minus;
+                arr(3, gi)[K] = tmp1 * g2 * (R(1) - e_2minus);
+                arr(4, gi)[K] = R(2) * tmp1 * lambda * e_minus;
             }
         }
         __syncthreads();
@@ -491,22 +501,22 @@ __global__ void __launch_bounds__(64, NL == 1 ? 3 : 1) k_rad_columns(Geo g, Tabl
                 // gamma1 = 2, gamma2 = 0, lambda = 2, tmp1 = 1/4 and the
                 // reference's reflectance is 0 and its transmittance e_minus
                 // (the expressions of 298-339 with those values, exact).
-                const double tau = lpt[v].apply(t.lw + 6 * 53 * gp) * dmv[v];
-                const double e_minus = exp(-2. * tau);
-                const double R = 0.;
-                const double T = e_minus;
-                const double Bk = lb0[v].apply(plk), Bk1 = lb1[v].apply(plk);
-                const double pdiff = (1. - T) * (Bk - Bk1) / (tau * 2.);
-                arr(3, gi)[K] = R;
+                const R tau = R(lpt[v].apply(t.lw + 6 * 53 * gp)) * R(dmv[v]);
+                const R e_minus = exp(R(-2) * tau);
+                const R T = e_minus;
+                const R Bk = R(lb0[v].apply(plk)), Bk1 = R(lb1[v].apply(plk));
+                const R pdiff = odiv((R(1) - T) * (Bk - Bk1), tau * R(2));
+                arr(3, gi)[K] = 0.;  // reflectance
                 arr(4, gi)[K] = T;
-                arr(1, gi)[K] = ATM_PI * (Bk1 - T * Bk + pdiff);  // tmp1 = 1 - R = 1
-                arr(2, gi)[K] = ATM_PI * (Bk - T * Bk1 - pdiff);
+                arr(1, gi)[K] = R(ATM_PI) * (Bk1 - T * Bk + pdiff);  // tmp1 = 1 - R = 1
+                arr(2, gi)[K] = R(ATM_PI) * (Bk - T * Bk1 - pdiff);
             }
         }
         __syncthreads();
         {
             const int gi = lane / CG, j = lane % CG, gp = g0 + gi;
-            const double Bs = ls.apply(t.planck + 231 * gp);
+            int e2 = 0;
+            const double Bs = lin_1d(tsur, t.planckT, 231, &e2).apply(t.planck + 231 * gp);
             adding_group<false, 4 * NL>(L, j, arr(3, gi), arr(4, gi), arr(1, gi), arr(2, gi), nullptr, 0.,
                                         emiss * ATM_PI * Bs, 1. - emiss);
         }
@@ -586,11 +596,14 @@ int atm_rad_run(atm_ctx *c, double time) {
     const dim3 grid((unsigned)(g.nx * g.ny)), blk(64);
 #define ATM_RAD_ARGS g, w->t, c->prm.gravity, c->dp, c->play, w->layer_T, w->level_T, c->ts, c->radius, c->net_flux, c->flag
     if (g.L <= 64)
-        hipLaunchKernelGGL(k_rad_columns<1>, grid, blk, lds, c->stream, ATM_RAD_ARGS);
+        if (c->mixed)
+            hipLaunchKernelGGL((k_rad_columns<1, float>), grid, blk, lds, c->stream, ATM_RAD_ARGS);
+        else
+            hipLaunchKernelGGL((k_rad_columns<1, double>), grid, blk, lds, c->stream, ATM_RAD_ARGS);
     else if (g.L <= 128)
-        hipLaunchKernelGGL(k_rad_columns<2>, grid, blk, lds, c->stream, ATM_RAD_ARGS);
+        hipLaunchKernelGGL((k_rad_columns<2, double>), grid, blk, lds, c->stream, ATM_RAD_ARGS);
     else
-        hipLaunchKernelGGL(k_rad_columns<RMAXL>, grid, blk, lds, c->stream, ATM_RAD_ARGS);
+        hipLaunchKernelGGL((k_rad_columns<RMAXL, double>), grid, blk, lds, c->stream, ATM_RAD_ARGS);
 #undef ATM_RAD_ARGS
     atm_timer_end(c, ATM_T_RAD);
     ATM_HIP(c, hipGetLastError());

@@ -70,11 +70,12 @@ typedef struct atm_params {
     double vorticity_damping_coefficient;
 } atm_params;
 
-/* Arithmetic of the dynamics.  FP64: the drop-in, the reference's
-   operations in double.  MIXED: the C- and D-step tile kernels form the PPM
-   reconstructions, fluxes, kinetic energy, vorticity and damping terms in
-   float and update dp, pt and the winds in double; the column scans,
-   pressure gradient, heating rate, remap, radiation and physics stay FP64.
+/* Arithmetic.  FP64: the drop-in, the reference's operations in double.
+   MIXED: the C- and D-step tile kernels form the PPM reconstructions,
+   fluxes, kinetic energy, vorticity and damping terms in float and update
+   dp, pt and the winds in double; radiation forms its g-point optics in
+   float and runs the adding method in double; the column scans, pressure
+   gradient, heating rate, remap and physics stay FP64.
    Each face flux is one value shared by the two cells it separates, so mass
    is conserved to FP64 rounding.  MIXED needs the fused path (nx % 32 == 0,
    ny % 16 == 0); it has no counterpart in the reference. */

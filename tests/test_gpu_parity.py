@@ -367,3 +367,27 @@ def test_full_size_parity(gpu_available):
         assert e <= PARITY, (n, e)
     m0 = ic["dp"].sum()
     assert abs(gpu.get("dp").sum() - m0) / m0 <= 1e-13
+
+
+def test_mixed_precision_radiation(gpu_available):
+    """Radiation of the mixed form (float g-point optics, double adding
+    method and sums) against the FP64 path on the same state: net_flux to
+    float-level error (1e-4 norm-relative; the two-stream coefficients lose
+    digits where 1 - lambda^2 is small)."""
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+    n, L = 64, 20
+    ic = make_initial_state(n, n, L, 25e3)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        m = Model(n, n, L, 25e3, precision=prec)
+        driver.init_model(m, ic, Config(dt_max=20.0, t_final=1e12, dt_output=1e12, dt_radiation=1e12,
+                                        dt_physics=1e12, dt_remap=100.0, radiation_on=False, physics_on=False))
+        m.calculate_radiative_heating(0.0)
+        out[prec] = m.get("net_flux")
+    a, b = out["mixed"], out["fp64"]
+    e = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    _print("mixed radiation", net_flux=e)
+    assert e <= 1e-4
