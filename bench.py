@@ -168,6 +168,7 @@ def mixed_timing(args, ic, cfg_kw):
         if n:
             kernels[name] = ms / args.steps
     m.timer_enable(False)
+    rad = radiation_timing(m, args, t) if args.radiation_calls > 0 else None
     del m
     # error after N steps from the same initial state
     res = {}
@@ -193,7 +194,8 @@ def mixed_timing(args, ic, cfg_kw):
     value = nx * ny * L * args.steps / sec
     return {"value": value, "unit": "cell-steps/s", "ms_per_step": sec / args.steps * 1e3,
             "dtype": "f32 reconstruction and fluxes, f64 updates, scans, pressure gradient, remap",
-            "kernels_ms_per_step": kernels, "error_vs_fp64": {"steps": args.mixed_error_steps, **err}}
+            "kernels_ms_per_step": kernels, "error_vs_fp64": {"steps": args.mixed_error_steps, **err},
+            "radiation_ms_per_call": rad["ms_per_call"] if rad else None}
 
 
 def radiation_timing(m, args, time_s):
