@@ -244,14 +244,50 @@ __device__ __forceinline__ Comp comp_over(const Comp &u, const Comp &w) {
     return c;
 }
 
-__device__ __forceinline__ Comp comp_shfl_up(const Comp &c, unsigned d) {
-    return Comp{__shfl_up(c.rt, d, CG), __shfl_up(c.rb, d, CG), __shfl_up(c.t, d, CG), __shfl_up(c.su, d, CG),
-                __shfl_up(c.sd, d, CG)};
+// Moves within a row of 16 lanes (one g-point's group) by the DPP modifier
+// of a VALU move, two 32-bit halves per double: row_up<D> gives lane j the
+// value of lane j - D (row_shr), row_down<D> that of lane j + D (row_shl).
+// A lane whose source lies outside its row keeps its own value.
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+    const int rlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+    const int rhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)rhi << 32) | (unsigned)rlo));
+}
+template <int D>
+__device__ __forceinline__ double row_up(double x) {
+    static_assert(D >= 1 && D < CG, "row shift");
+    return dpp_row<0x110 + D>(x);
+}
+template <int D>
+__device__ __forceinline__ double row_down(double x) {
+    static_assert(D >= 1 && D < CG, "row shift");
+    return dpp_row<0x100 + D>(x);
 }
 
-__device__ __forceinline__ Comp comp_shfl_down(const Comp &c, unsigned d) {
-    return Comp{__shfl_down(c.rt, d, CG), __shfl_down(c.rb, d, CG), __shfl_down(c.t, d, CG),
-                __shfl_down(c.su, d, CG), __shfl_down(c.sd, d, CG)};
+template <int D>
+__device__ __forceinline__ Comp comp_row_up(const Comp &c) {
+    return Comp{row_up<D>(c.rt), row_up<D>(c.rb), row_up<D>(c.t), row_up<D>(c.su), row_up<D>(c.sd)};
+}
+
+template <int D>
+__device__ __forceinline__ Comp comp_row_down(const Comp &c) {
+    return Comp{row_down<D>(c.rt), row_down<D>(c.rb), row_down<D>(c.t), row_down<D>(c.su), row_down<D>(c.sd)};
+}
+
+// one step of the inclusive scans over the group: v = blocks 0..j over the
+// surface (from below), u = blocks j..15 (from above)
+template <bool REFL, int D>
+__device__ __forceinline__ void scan_up(Comp &v, int j) {
+    const Comp w = comp_row_up<D>(v);
+    if (j >= D) v = comp_over<REFL>(v, w);
+}
+template <bool REFL, int D>
+__device__ __forceinline__ void scan_down(Comp &u, int j) {
+    const Comp x = comp_row_down<D>(u);
+    if (j + D < CG) u = comp_over<REFL>(x, u);
 }
 
 // calculate_fluxes (349-381) of one g-point on lane j of its group.  On
@@ -277,18 +313,17 @@ __device__ __forceinline__ void adding_group(int L, int j, double *R, double *T,
     // the surface is a layer that transmits nothing
     const Comp sfc{sfc_alb, 0., 0., sfc_up, 0.};
     Comp v = j == 0 ? comp_over<REFL>(c, sfc) : c;  // blocks 0..j over the surface
-#pragma unroll
-    for (unsigned d = 1; d < CG; d <<= 1) {
-        const Comp w = comp_shfl_up(v, d);
-        if (j >= (int)d) v = comp_over<REFL>(v, w);
-    }
+    static_assert(CG == 16, "four scan steps");
+    scan_up<REFL, 1>(v, j);
+    scan_up<REFL, 2>(v, j);
+    scan_up<REFL, 4>(v, j);
+    scan_up<REFL, 8>(v, j);
     Comp u = c;  // blocks j..15
-#pragma unroll
-    for (unsigned d = 1; d < CG; d <<= 1) {
-        const Comp x = comp_shfl_down(u, d);
-        if (j + (int)d < CG) u = comp_over<REFL>(x, u);
-    }
-    Comp below = comp_shfl_up(v, 1), above = comp_shfl_down(u, 1);
+    scan_down<REFL, 1>(u, j);
+    scan_down<REFL, 2>(u, j);
+    scan_down<REFL, 4>(u, j);
+    scan_down<REFL, 8>(u, j);
+    Comp below = comp_row_up<1>(v), above = comp_row_down<1>(u);
     if (j == 0) below = sfc;
     if (j == CG - 1) above = comp_identity();
     // albedo a and emission e of everything below each level of the block,
@@ -343,12 +378,17 @@ __device__ __forceinline__ double beam_group(int L, int j, double toa_dir, doubl
     const int lo = min(j * nb, L), hi = min(lo + nb, L);
     double p = 1.;
     for (int k = hi - 1; k >= lo; k--) p = p * B[k];
-#pragma unroll
-    for (unsigned d = 1; d < CG; d <<= 1) {  // product of blocks j..15
-        const double w = __shfl_down(p, d, CG);
-        if (j + (int)d < CG) p = p * w;
+    {  // product of blocks j..15
+        double w = row_down<1>(p);
+        if (j + 1 < CG) p = p * w;
+        w = row_down<2>(p);
+        if (j + 2 < CG) p = p * w;
+        w = row_down<4>(p);
+        if (j + 4 < CG) p = p * w;
+        w = row_down<8>(p);
+        if (j + 8 < CG) p = p * w;
     }
-    double above = __shfl_down(p, 1, CG);
+    double above = row_down<1>(p);
     if (j == CG - 1) above = 1.;
     double bk1 = toa_dir * above;
     if (lo < hi && hi == L) B[L] = toa_dir;
