@@ -32,35 +32,38 @@ struct RadWork {
 
 namespace {
 
-// first index k in [0, n) with x <= xs[k]; caller guarantees xs[0] <= x <= xs[n-1]
-__device__ __forceinline__ int first_ge(const float *xs, int n, double x) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (x > (double)xs[mid]) lo = mid + 1; else hi = mid;
+// The first index k in [0, n) with x <= xs[k], for xs increasing and
+// xs[0] <= x <= xs[n - 1] (the reference's search from k = 1 upward,
+// 4571-4574).  It starts from a guess g of k and moves down while the entry
+// below still reaches x, then up while its own entry is below x: the result
+// is the same for any guess, and a good guess costs one or two table reads
+// where a search costs log2(n) dependent ones.
+__device__ __forceinline__ int first_ge_from(const float *xs, int n, double x, int g) {
+    g = min(max(g, 0), n - 1);
+    while (g > 0 && (double)xs[g - 1] >= x) g--;
+    while ((double)xs[g] < x) g++;  // stops at n - 1 at the latest: x <= xs[n - 1]
+    return g;
+}
+
+// the guess of first_ge_from on an axis spaced evenly in x (LOG false) or
+// in log x (LOG true), from its end points; float arithmetic, as a guess
+// needs no more
+template <bool LOG>
+__device__ __forceinline__ int axis_guess(const float *xs, int n, double x) {
+    float a = xs[0], b = xs[n - 1], y = (float)x;
+    if (LOG) {
+        a = __logf(a);
+        b = __logf(b);
+        y = __logf(y);
     }
-    return lo;
+    return (int)ceilf((y - a) * ((float)(n - 1) / (b - a)));
 }
 
-// lookup_1d (4556-4579); *err |= 1 outside the table
-__device__ __forceinline__ double lookup_1d(double x, const float *xs, const float *ys, int n, int *err) {
-    if (x < (double)xs[0] || x > (double)xs[n - 1]) { *err |= 1; return 0.; }
-    int k = first_ge(xs, n, x);
-    double xk = xs[k];
-    if (x == xk) return (double)ys[k];
-    double xm = xs[k - 1], yk = ys[k], ym = ys[k - 1];
-    return (yk * (xm - x) + ym * (x - xk)) / (xm - xk);
-}
-
-// lookup_pT_table (4581-4609); table is the (6, 53) slab of one g-point
-__device__ __forceinline__ double lookup_pT(const Tables &t, double p, double T, const float *table, int *err) {
-    if (p < (double)t.P[0] || p > (double)t.P[52]) { *err |= 1; return 0.; }
-    int k = first_ge(t.P, 53, p);
-    if (p == (double)t.P[k]) return lookup_1d(T, t.T + 6 * k, table + 6 * k, 6, err);
-    double t1 = lookup_1d(T, t.T + 6 * k, table + 6 * k, 6, err);
-    double t2 = lookup_1d(T, t.T + 6 * (k - 1), table + 6 * (k - 1), 6, err);
-    double pk = t.P[k], pm = t.P[k - 1];
-    return (t1 * (pm - p) + t2 * (p - pk)) / (pm - pk);
+// x within [xs[0], xs[n - 1]]; false for NaN, which the reference's range
+// test (4562) lets through to a read before the table: here it is an error
+// of the table range like any other
+__device__ __forceinline__ bool in_range(double x, const float *xs, int n) {
+    return x >= (double)xs[0] && x <= (double)xs[n - 1];
 }
 
 // layer and level profiles over is+4..ie-4 (53-101)
@@ -155,12 +158,12 @@ struct Lin {
 
 __device__ __forceinline__ Lin lin_1d(double x, const float *xs, int n, int *err) {
     Lin l;
-    if (x < (double)xs[0] || x > (double)xs[n - 1]) {
+    if (!in_range(x, xs, n)) {
         *err |= 1;
         l.oor = true;
         return l;
     }
-    l.k = first_ge(xs, n, x);
+    l.k = first_ge_from(xs, n, x, axis_guess<false>(xs, n, x));
     const double xk = xs[l.k];
     l.hit = x == xk;
     if (!l.hit) {
@@ -190,12 +193,12 @@ struct LinPT {
 
 __device__ __forceinline__ LinPT lin_pT(const Tables &t, double p, double T, int *err) {
     LinPT l;
-    if (p < (double)t.P[0] || p > (double)t.P[52]) {
+    if (!in_range(p, t.P, 53)) {
         *err |= 1;
         l.oor = true;
         return l;
     }
-    l.k = first_ge(t.P, 53, p);
+    l.k = first_ge_from(t.P, 53, p, axis_guess<true>(t.P, 53, p));
     l.hit = p == (double)t.P[l.k];
     l.t1 = lin_1d(T, t.T + 6 * l.k, 6, err);
     if (!l.hit) {

@@ -1137,7 +1137,9 @@ void orc_remap(orc_t *o) {
  * ======================================================================== */
 /* lookup_1d over xs[0..n-1] (stride xsstr) / ys (stride ysstr): 4556-4579 */
 static double lookup_1d(double x, const float *xs, int xstr, const float *ys, int ystr, int n, int *err) {
-    if (x < (double)xs[0] || x > (double)xs[(n - 1) * xstr]) { *err = 1; return 0.; }
+    /* NaN fails the test as well (the reference's test lets it through to
+     * a read before the table) */
+    if (!(x >= (double)xs[0] && x <= (double)xs[(n - 1) * xstr])) { *err = 1; return 0.; }
     int k = 0;
     while (x > (double)xs[k * xstr]) k++;
     if (x == (double)xs[k * xstr]) return (double)ys[k * ystr];
@@ -1148,7 +1150,7 @@ static double lookup_1d(double x, const float *xs, int xstr, const float *ys, in
 /* lookup_pT_table: 4581-4609.  table is (6, 53) for one g-point. */
 static double lookup_pT(double p, double T, const float *table, int *err) {
     const float *P = GO_PRESSURES, *TT = GO_TEMPERATURES;
-    if (p < (double)P[0] || p > (double)P[52]) { *err = 2; return 0.; }
+    if (!(p >= (double)P[0] && p <= (double)P[52])) { *err = 2; return 0.; }
     int k = 0;
     while (p > (double)P[k]) k++;
     if (p == (double)P[k]) return lookup_1d(T, TT + 6 * k, 1, table + 6 * k, 1, 6, err);

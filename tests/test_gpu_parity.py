@@ -206,6 +206,53 @@ def test_radiation_table_range_error(gpu_available):
     assert e.value.code == 3
 
 
+def test_radiation_nan_is_range_error(gpu_available):
+    """A NaN surface temperature fails the table's range test (the
+    reference's test lets NaN through to a read before the table): the call
+    reports ATM_ERR_TABLE_RANGE instead of reading outside the table."""
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import AtmError, Model
+    nx, ny, nlay = 32, 16, 8
+    ic = dict(make_initial_state(nx, ny, nlay, 100e3))
+    ic["ts"] = ic["ts"].copy()
+    ic["ts"].flat[37] = np.nan
+    m = Model(nx, ny, nlay, 100e3)
+    cfg = Config(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12,
+                 dt_remap=1e12, radiation_on=True, physics_on=False)
+    driver.init_model(m, ic, cfg)
+    with pytest.raises(AtmError) as e:
+        m.calculate_radiative_heating(0.0)
+    assert e.value.code == 3
+
+
+def test_radiation_on_table_points(gpu_available):
+    """Surface temperatures on the Planck table's points, both ends among
+    them, take the lookup's exact-hit branch (mod_gas_optics.f90:4576-4577)
+    and the edges of the table search; fluxes as the oracle's."""
+    import oracle as O
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+    nx, ny, nlay = 24, 20, 20
+    ic = dict(make_initial_state(nx, ny, nlay, 100e3, seed=11))
+    ic["ts"] = ic["ts"].copy()
+    pts = [120.0, 121.0, 200.0, 287.0, 349.0, 350.0, 120.5, 349.5, 300.0 + 2.0 ** -40]
+    for c, v in enumerate(pts):
+        ic["ts"].flat[c * 7] = v
+    cfg = Config(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12,
+                 dt_remap=1e12, radiation_on=False, physics_on=False)
+    gpu, orc = Model(nx, ny, nlay, 100e3), O.Oracle(nx, ny, nlay, 100e3, 100e3)
+    for b in (gpu, orc):
+        driver.init_model(b, ic, Config(**cfg.__dict__))
+        b.calculate_radiative_heating(0.0)
+    e = compare(gpu, orc, "net_flux")
+    _print("radiation on table points", net_flux=e)
+    assert e <= PARITY
+
+
 def test_call_order_errors(gpu_available):
     """The state machine of the boundary: a second D step before the halo
     fill, and physics before its allocation, are refused (status 5)."""

@@ -70,6 +70,12 @@ def test_table_range_errors():
     assert O.lw_abs(1, 1.2e5, 280.0)[1] != 0        # above the highest pressure
     assert O.sw_abs(1, 1e5, 100.0)[1] != 0          # temperature below the row
     assert O.planck(1, 119.0)[1] != 0 and O.planck(1, 351.0)[1] != 0
+    # NaN is out of range too (the reference's test lets it through)
+    assert O.planck(1, float("nan"))[1] != 0
+    assert O.sw_abs(1, float("nan"), 250.0)[1] != 0
+    assert O.lw_abs(1, 5e4, float("nan"))[1] != 0
+    # the table's end points themselves are inside
+    assert O.planck(1, 120.0)[1] == 0 and O.planck(1, 350.0)[1] == 0
 
 
 def test_halo_closed_form():
