@@ -128,6 +128,7 @@ int atm_create(const atm_grid *grid, const atm_params *params, int precision, at
     c->g.L = grid->nlay;
     c->g.NI = grid->nx + 2 * ATM_HALO;
     c->g.NJ = grid->ny + 2 * ATM_HALO;
+    c->g.SI = (c->g.NI + 15) / 16 * 16;
     c->dx = grid->dx;
     c->dy = grid->dy;
     c->prm = *params;
@@ -236,27 +237,36 @@ static int xfer(atm_ctx *c, int id, void *dst, const void *src, bool to_dev, hip
     if (!d) return atm_fail(c, ATM_ERR_ARGUMENT, "unknown field id " + std::to_string(id));
     if (!(to_dev ? src : dst)) return atm_fail(c, ATM_ERR_ARGUMENT, "null array");
     const Geo &g = c->g;
-    size_t bytes = sizeof(double) * g.plane() * nk;
+    // the caller's array has rows of NI values back to back (the reference's
+    // layout); the device's rows have the stride SI: one 2-D copy of the
+    // NJ * nk rows converts
+    const size_t hp = sizeof(double) * g.NI, dpch = sizeof(double) * g.SI, w = sizeof(double) * g.NI;
+    const size_t rows = (size_t)g.NJ * nk;
+    auto to_device = [&](double *dev, const void *from) {
+        return hipMemcpy2DAsync(dev, dpch, from, hp, w, rows, kind, c->stream);
+    };
+    auto from_device = [&](void *to, const double *dev) {
+        return hipMemcpy2DAsync(to, hp, dev, dpch, w, rows, kind, c->stream);
+    };
     if (to_dev && id == ATM_DP) c->plk_valid = false;
     if (to_dev && (id == ATM_DP || id == ATM_PT || id == ATM_UD || id == ATM_VD)) c->nfw_tracked = false;
     if (split_winds(c, id)) {
         double *ring = c->pbuf[id == ATM_UD ? 2 : 3][1 - c->cur];
         if (to_dev) {
-            ATM_HIP(c, hipMemcpyAsync(d, src, bytes, kind, c->stream));
-            ATM_HIP(c, hipMemcpyAsync(ring, src, bytes, kind, c->stream));
+            ATM_HIP(c, to_device(d, src));
+            ATM_HIP(c, to_device(ring, src));
         } else {
-            ATM_HIP(c, hipMemcpyAsync(dst, ring, bytes, kind, c->stream));
-            const size_t pitch = sizeof(double) * g.NI;
+            ATM_HIP(c, from_device(dst, ring));
             for (int K = 0; K < nk; K++) {
-                const size_t o = g.ix(ATM_HALO, ATM_HALO, K);
-                ATM_HIP(c, hipMemcpy2DAsync((double *)dst + o, pitch, d + o, pitch, sizeof(double) * g.nx, g.ny,
-                                            kind, c->stream));
+                const size_t ho = ((size_t)K * g.NJ + ATM_HALO) * g.NI + ATM_HALO;
+                ATM_HIP(c, hipMemcpy2DAsync((double *)dst + ho, hp, d + g.ix(ATM_HALO, ATM_HALO, K), dpch,
+                                            sizeof(double) * g.nx, g.ny, kind, c->stream));
             }
         }
     } else if (to_dev) {
-        ATM_HIP(c, hipMemcpyAsync(d, src, bytes, kind, c->stream));
+        ATM_HIP(c, to_device(d, src));
     } else {
-        ATM_HIP(c, hipMemcpyAsync(dst, d, bytes, kind, c->stream));
+        ATM_HIP(c, from_device(dst, d));
     }
     if (block) ATM_HIP(c, hipStreamSynchronize(c->stream));
     return ATM_OK;

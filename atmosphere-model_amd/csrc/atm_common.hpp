@@ -1,10 +1,13 @@
 // atm_common.hpp -- shared definitions of the gfx950 HIP library.
 //
-// Layout: every field is FP64 in the reference's halo'd Fortran layout,
+// Layout: every field is FP64 over the reference's halo'd index space,
 // index (I, J, K) with I = i - is in [0, NI), J = j - js in [0, NJ),
-// K = k - 1, NI = nx + 14, NJ = ny + 14, element ((K*NJ)+J)*NI + I.
-// Reference index "is+a" is I = a; "ie-b" is I = NI-1-b; the interior
-// isd..ied is I = 7 .. 7+nx-1.
+// K = k - 1, NI = nx + 14, NJ = ny + 14.  On the device a row of NI values
+// is stored with the stride SI = NI rounded up to 16 values (128 bytes), so
+// that every row and plane starts on a cache line: element
+// ((K*NJ)+J)*SI + I.  Host arrays at the boundary use SI = NI (the
+// reference's own layout); the transfers convert.  Reference index "is+a"
+// is I = a; "ie-b" is I = NI-1-b; the interior isd..ied is I = 7 .. 7+nx-1.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,11 +36,12 @@ enum { V_PCM = 0, V_PPM_U = 1, V_PPM_C = 2, V_PPM_NN = 3 };  // mod_sw_dyn.f90:7
 
 struct Geo {
     int NI, NJ, L, nx, ny;
-    __host__ __device__ size_t plane() const { return (size_t)NI * NJ; }
+    int SI;  // row stride on the device, NI rounded up to a multiple of 16
+    __host__ __device__ size_t plane() const { return (size_t)SI * NJ; }
     __host__ __device__ size_t ix(int I, int J, int K) const {
-        return ((size_t)K * NJ + (size_t)J) * NI + (size_t)I;
+        return ((size_t)K * NJ + (size_t)J) * SI + (size_t)I;
     }
-    __host__ __device__ size_t ix2(int I, int J) const { return (size_t)J * NI + (size_t)I; }
+    __host__ __device__ size_t ix2(int I, int J) const { return (size_t)J * SI + (size_t)I; }
 };
 
 // Per-step scalars of mod_sw_dyn (init_sw_dyn 958-1023 and dt-derived).

@@ -84,8 +84,8 @@ struct Reg {
 // workgroup (level k, tile origin moved back by 4 rows and columns) plus a
 // non-negative 32-bit per-lane offset, which the loads and stores take as
 // a scalar base and a 32-bit vector offset.  Offsets x, y >= -4.
-#define TBASE(k) ((size_t)(k) * g.plane() + (size_t)(J0 - 4) * g.NI + (size_t)(I0 - 4))
-#define TOFF(x, y) ((unsigned)(((y) + 4) * g.NI + (x) + 4))
+#define TBASE(k) ((size_t)(k) * g.plane() + (size_t)(J0 - 4) * g.SI + (size_t)(I0 - 4))
+#define TOFF(x, y) ((unsigned)(((y) + 4) * g.SI + (x) + 4))
 
 // ---------------------------------------------------------------------------
 // C1 / D1: the dp and pt updates of one level (cgrid 94-268, dgrid 439-598).
@@ -694,6 +694,134 @@ __global__ void __launch_bounds__(64) k_scan_pg(Geo g, double ptop, double pktop
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same scans with a wavefront over 16 columns and 4 chunks of levels
+// (nlay <= 4 CMAX): lane (c, q) = (lane % 16, lane / 16) holds the levels
+// [q C, q C + n) of column c, C = ceil(nlay / 4).  The two serial sums (dp
+// down from the top, gz up from the surface) pass from chunk to chunk in
+// the order of the loops above, so every partial sum is the sequential one
+// and every output the same double; the pow, log and divisions of the
+// levels run on all 64 lanes, four times the columns' parallelism.  Outputs
+// are optional (nullptr: not written); th == nullptr skips gz.
+// ---------------------------------------------------------------------------
+constexpr int SC_COLS = 16, SC_PARTS = 4;
+
+template <int CMAX>
+__global__ void __launch_bounds__(64, 3) k_scan_cols(Geo g, int i0, int j0, int ni, double ptop, double pktop,
+                                                  double gravity, const double *__restrict__ src,
+                                                  const double *__restrict__ th, const double *__restrict__ nf,
+                                                  double *__restrict__ plev, double *__restrict__ pkap,
+                                                  double *__restrict__ gz, double *__restrict__ heat,
+                                                  double *__restrict__ plk_out) {
+    static_assert(SC_COLS * SC_PARTS == 64, "one wavefront");
+    const int lane = threadIdx.x, q = lane / SC_COLS;
+    const int I = i0 + blockIdx.x * SC_COLS + lane % SC_COLS, J = j0 + blockIdx.y;
+    const bool col = I < i0 + ni;  // lanes past the last column still pass the sums on
+    const int C = (g.L + SC_PARTS - 1) / SC_PARTS;
+    const int k0 = min(q * C, g.L), n = min(C, g.L - k0);
+    const size_t pl = g.plane(), c0 = g.ix2(col ? I : i0, J);
+    const bool interior = col && I >= ATM_HALO && I < ATM_HALO + g.nx && J >= ATM_HALO && J < ATM_HALO + g.ny;
+    const bool want_heat = heat && interior, want_plk = plk_out && interior;
+    double s[CMAX], pk[CMAX];
+#pragma unroll
+    for (int u = 0; u < CMAX; u++) {
+        s[u] = 0.;
+        pk[u] = 1.;
+        if (u < n && col) s[u] = src[c0 + (size_t)(k0 + u) * pl];
+    }
+    // p = ptop + the dp above and at each level, from the top chunk down
+    double p = ptop;
+#pragma unroll
+    for (int ph = SC_PARTS - 1; ph >= 0; ph--) {
+        if (q == ph) {
+#pragma unroll
+            for (int u = CMAX - 1; u >= 0; u--)
+                if (u < n) {
+                    p = p + s[u];
+                    pk[u] = p;
+                }
+        }
+        if (ph > 0) {
+            const double above = __shfl_down(p, SC_COLS);
+            if (q == ph - 1) p = above;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < CMAX; u++)
+        if (u < n) {
+            const size_t x = c0 + (size_t)(k0 + u) * pl;
+            if (plev && col) plev[x] = pk[u];
+            pk[u] = pow(pk[u], ATM_KAPPA);
+            if (pkap && col) pkap[x] = pk[u];
+        }
+    // pkap of the level above the chunk: the next chunk's first, or the top
+    double pk_up = __shfl_down(pk[0], SC_COLS);
+    if (k0 + n >= g.L) pk_up = pktop;
+    if (want_heat || want_plk) {
+        double nfv[CMAX + 1];
+#pragma unroll
+        for (int u = 0; u <= CMAX; u++) nfv[u] = (want_heat && u <= n) ? nf[c0 + (size_t)(k0 + u) * pl] : 0.;
+#pragma unroll
+        for (int u = 0; u < CMAX; u++)
+            if (u < n) {
+                const size_t x = c0 + (size_t)(k0 + u) * pl;
+                const double pk1 = u + 1 < n ? pk[u + 1] : pk_up;
+                const double plk = (pk1 - pk[u]) / log(pk1 / pk[u]);
+                if (want_heat) heat[x] = (nfv[u + 1] - nfv[u]) / (ATM_CP_DRY * s[u] / gravity) / plk;
+                if (want_plk) plk_out[x] = plk;
+            }
+    }
+    if (!th) return;
+    // gz: the terms of the levels, then their sum from the surface chunk up
+    double tz[CMAX];
+#pragma unroll
+    for (int u = 0; u < CMAX; u++) {
+        tz[u] = 0.;
+        if (u < n && col) {
+            const double pkn = u + 1 < n ? pk[u + 1] : pk_up;
+            tz[u] = ATM_CP_DRY * th[c0 + (size_t)(k0 + u) * pl] * (pk[u] - pkn);
+        }
+    }
+    double z = col ? gz[c0] : 0.;
+#pragma unroll
+    for (int ph = 0; ph < SC_PARTS; ph++) {
+        if (q == ph) {
+#pragma unroll
+            for (int u = 0; u < CMAX; u++)
+                if (u < n) {
+                    z = z + tz[u];
+                    tz[u] = z;
+                }
+        }
+        if (ph + 1 < SC_PARTS) {
+            const double below = __shfl_up(z, SC_COLS);
+            if (q == ph + 1) z = below;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < CMAX; u++)
+        if (u < n && col) gz[c0 + (size_t)(k0 + u + 1) * pl] = tz[u];
+}
+
+constexpr int SC_CMAX = 16;  // k_scan_cols for nlay <= 64
+
+// the scans of the columns (i0.., j0..) x ni x nj: k_scan_cols when nlay
+// fits, else one thread per column (k_scan_pg / k_scan_plk)
+void launch_scans(atm_ctx *c, int i0, int j0, int ni, int nj, double gravity, const double *src, const double *th,
+                  const double *nf, double *plev, double *pkap, double *gz, double *heat, double *plk_out) {
+    const Geo &g = c->g;
+    if (g.L <= SC_PARTS * SC_CMAX) {
+        hipLaunchKernelGGL(k_scan_cols<SC_CMAX>, dim3((ni + SC_COLS - 1) / SC_COLS, nj), dim3(64), 0, c->stream, g,
+                           i0, j0, ni, c->ak0, c->pktop, gravity, src, th, nf, plev, pkap, gz, heat, plk_out);
+    } else if (th) {
+        hipLaunchKernelGGL(k_scan_pg, dim3((g.NI + 63) / 64, g.NJ), dim3(64), 0, c->stream, g, c->ak0, c->pktop,
+                           gravity, src, th, nf, plev, pkap, gz, heat, plk_out);
+    } else {
+        hipLaunchKernelGGL(k_scan_plk, dim3((g.nx + 63) / 64, g.ny), dim3(64), 0, c->stream, g, c->ak0, c->pktop, src,
+                           plk_out);
+    }
+}
+
 #undef D
 
 }  // namespace
@@ -710,11 +838,10 @@ int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
     c->coef.dtdx = dt * c->coef.rdx;
     c->coef.dtdy = dt * c->coef.rdy;
     const DynCoef &k = c->coef;
-    const double pktop = c->pktop;
     if (!c->plk_valid) {
         atm_timer_begin(c, ATM_T_SCAN);
-        hipLaunchKernelGGL(k_scan_plk, dim3((g.nx + 63) / 64, g.ny), dim3(64), 0, c->stream, g, c->ak0, pktop, b.dp,
-                           c->plk);
+        launch_scans(c, ATM_HALO, ATM_HALO, g.nx, g.ny, k.gravity, b.dp, nullptr, nullptr, nullptr, nullptr, nullptr,
+                     nullptr, c->plk);
         atm_timer_end(c, ATM_T_SCAN);
         c->plk_valid = true;
     }
@@ -739,9 +866,8 @@ int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
         if (rc) return rc;
     }
     atm_timer_begin(c, ATM_T_SCAN);
-    hipLaunchKernelGGL(k_scan_pg, dim3((g.NI + 63) / 64, g.NJ), dim3(64), 0, c->stream, g, c->ak0, pktop, k.gravity,
-                       c->dpc, c->ptc, c->net_flux, (double *)nullptr, c->pkap, c->gz, c->heatd,
-                       (double *)nullptr);
+    launch_scans(c, 0, 0, g.NI, g.NJ, k.gravity, c->dpc, c->ptc, c->net_flux, nullptr, c->pkap, c->gz, c->heatd,
+                 nullptr);
     atm_timer_end(c, ATM_T_SCAN);
     atm_timer_begin(c, ATM_T_C2);
     if (c->mixed)
@@ -793,9 +919,8 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
         if (rc) return rc;
     }
     atm_timer_begin(c, ATM_T_SCAN);
-    hipLaunchKernelGGL(k_scan_pg, dim3((g.NI + 63) / 64, g.NJ), dim3(64), 0, c->stream, g, c->ak0, c->pktop,
-                       k.gravity, nxt.dp, nxt.pt, c->net_flux, c->plev, c->pkap, c->gz, (double *)nullptr,
-                       c->plk);
+    launch_scans(c, 0, 0, g.NI, g.NJ, k.gravity, nxt.dp, nxt.pt, c->net_flux, c->plev, c->pkap, c->gz, nullptr,
+                 c->plk);
     hipLaunchKernelGGL(k_bcorners, dim3((g.nx + TX) / TX, (g.ny + TY_WIND) / TY_WIND, g.L), blkw, 0, c->stream, g, c->pkap,
                        c->gz, c->pkapb, c->gzb);
     atm_timer_end(c, ATM_T_SCAN);

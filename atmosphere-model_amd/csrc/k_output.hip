@@ -31,7 +31,7 @@ __global__ void k_accumulate(Geo g, double dt, const double *__restrict__ dp, co
         const size_t x = g.ix(I, J, k);
         add(0, dt * dp[x]);
         add(1, dt * pt[x]);
-        add(2, dt * .5 * (ud[x] + ud[x + g.NI]));
+        add(2, dt * .5 * (ud[x] + ud[x + g.SI]));
         add(3, dt * .5 * (vd[x] + vd[x + 1]));
     }
     add(5, dt * nf[g.ix(I, J, k)]);

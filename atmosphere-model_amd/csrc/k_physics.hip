@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(64) k_phys_tend(Geo g, const double *__restric
     if (I < ATM_HALO - 2 || I > ATM_HALO + g.nx + 1 || J < ATM_HALO - 2 || J > ATM_HALO + g.ny + 1) return;
     const double min_v2 = (double)1e-4f;  // default-real literal
     const double crit = .25, timescale = 600;
-    const size_t sJ = g.NI;
+    const size_t sJ = g.SI;
     // A-grid winds of the column (35-45)
     auto ua_at = [&](size_t x) {
         return (.25 * (ud[x] + ud[x + sJ]) + .125 * (ud[x - 1] + ud[x - 1 + sJ] + ud[x + 1] + ud[x + 1 + sJ]));
@@ -131,7 +131,7 @@ __global__ void k_phys_apply(Geo g, double dt, const double *__restrict__ u_tend
     const int J = ATM_HALO + blockIdx.y;
     const int K = blockIdx.z;
     if (I >= ATM_HALO + g.nx) return;
-    const size_t x = g.ix(I, J, K), sJ = g.NI;
+    const size_t x = g.ix(I, J, K), sJ = g.SI;
     const double p = pt[x] + dt * pt_tend[x];
     const double u = ud[x] + dt * (ATM_W7_12 * (u_tend[x] + u_tend[x - sJ]) - ATM_W1_12 * (u_tend[x + sJ] + u_tend[x - 2 * sJ]));
     const double v = vd[x] + dt * (ATM_W7_12 * (v_tend[x] + v_tend[x - 1]) - ATM_W1_12 * (v_tend[x + 1] + v_tend[x - 2]));

@@ -180,7 +180,7 @@ __global__ void __launch_bounds__(256) k_remap_winds(Geo g, double *__restrict__
     const size_t base = g.ix2(I, J), plane = g.plane();
     Scr s{scr, ncol, t, L};
     {
-        WindCol c{plev, tplev, inside ? ud : udr, base, (size_t)g.NI, plane};
+        WindCol c{plev, tplev, inside ? ud : udr, base, (size_t)g.SI, plane};
         ppm_forward(c, L, s.lv(), s.gm(), ncol);
         auto out = [&](int m, double acc, double, double) { ur[base + (size_t)(m - 1) * plane] = acc; };
         ppm_merge(c, L, s.lv(), s.gm(), ncol, out);
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(256) k_remap_energy(
     const int I = 7 + (int)(t % W), J = 7 + (int)(t / W);
     const int L = g.L;
     const size_t plane = g.plane(), base = g.ix2(I, J);
-    const size_t sN = g.NI;  // +1 in J
+    const size_t sN = g.SI;  // +1 in J
     Scr s{scr, ncol, t, L};
     // wind source at (I+a, J+b): interior points from ud / vd, the ring from udr / vdr
     auto wsrc = [&](const double *in, const double *ring, int a, int b) {
