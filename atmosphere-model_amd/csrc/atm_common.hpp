@@ -123,7 +123,9 @@ __device__ __forceinline__ void ppm_edges(T qm2, T qm1, T q0, T qp1, T qp2, T &e
             eR = dmax2(eR, -q0);
             T a = eR - eL;
             T b = T(-3) * (eR + eL);
-            if ((T(fabs(a)) < -b) && (q0 + a * a / (T(4) * b) + b / T(12) < T(0))) {
+            // b / 12 as div3(b / 4): the quarter is exact, so this is the
+            // correctly rounded b / 12 without a division sequence
+            if ((T(fabs(a)) < -b) && (q0 + a * a / (T(4) * b) + div3(b * T(0.25)) < T(0))) {
                 if (eL * eR > T(0)) {
                     eL = T(0);
                     eR = T(0);

@@ -24,7 +24,7 @@
 #include "remap_column.hpp"
 
 struct RemapWork {
-    double *scr = nullptr;      // [3 * (L+2)][ncol]: eliminated edges, gam, energy
+    double *scr = nullptr;      // [4 * (L+2)][ncol]: four column slots (see Scr)
     double *tplev = nullptr;    // target_plev  (NI x NJ x L+1), used on isd-3..ied+3
     double *tpkap = nullptr;    // target_pkap
     double *tplk = nullptr;     // target_playkap (L)
@@ -34,14 +34,15 @@ struct RemapWork {
 
 namespace {
 
-// Column scratch: slot s, 1-based level k -> scr[(s*(L+2) + k) * ncol + t]
+// Column scratch: slot s, 1-based level k -> scr[(s*(L+2) + k) * ncol + t].
+// The wind remap uses slots 0-1 (u: eliminated edges, gam) and 2-3 (v); the
+// energy remap slots 0-1 (edges, gam), 2 (energy) and 3 (kinetic energy of
+// the remapped winds).
 struct Scr {
     double *s;
     size_t ncol, t;
     int L;
-    __device__ double *lv() const { return s + t; }
-    __device__ double *gm() const { return s + (size_t)(L + 2) * ncol + t; }
-    __device__ double *var() const { return s + (size_t)2 * (L + 2) * ncol + t; }
+    __device__ double *slot(int q) const { return s + (size_t)q * (L + 2) * ncol + t; }
 };
 
 // Column sources.  X(k): source face coordinate, D(k): destination face
@@ -160,10 +161,12 @@ __global__ void k_remap_grid(Geo g, double top, const double *__restrict__ ak, c
     }
 }
 
-// winds on faces over isd-1..ied+1 (80-105).  The ring of the D-grid winds
-// holds the halo values the previous step's halo exchange left there (the
-// D-grid step updates the interior only): a column outside the interior
-// reads its winds from `ring`, an interior column from `in`.
+// winds on faces over isd-1..ied+1 (80-105): u for blockIdx.y = 0, v for
+// blockIdx.y = 1, so the two columns of a point run on separate threads.
+// The ring of the D-grid winds holds the halo values the previous step's
+// halo exchange left there (the D-grid step updates the interior only): a
+// column outside the interior reads its winds from `ring`, an interior
+// column from `in`.
 __global__ void __launch_bounds__(256) k_remap_winds(Geo g, double *__restrict__ scr, size_t ncol,
                                                      const double *__restrict__ plev,
                                                      const double *__restrict__ tplev,
@@ -178,31 +181,67 @@ __global__ void __launch_bounds__(256) k_remap_winds(Geo g, double *__restrict__
     const int L = g.L;
     const bool inside = I >= ATM_HALO && I < ATM_HALO + g.nx && J >= ATM_HALO && J < ATM_HALO + g.ny;
     const size_t base = g.ix2(I, J), plane = g.plane();
-    Scr s{scr, ncol, t, L};
+    const Scr s{scr, ncol, t, L};
+    const bool is_v = blockIdx.y == 1;
+    // u: faces averaged along J (stride SI); v: along I (stride 1)
+    const WindCol c{plev, tplev, is_v ? (inside ? vd : vdr) : (inside ? ud : udr), base,
+                    is_v ? (size_t)1 : (size_t)g.SI, plane};
+    double *lv = s.slot(is_v ? 2 : 0), *gm = s.slot(is_v ? 3 : 1);
+    double *o = is_v ? vr : ur;
+    ppm_forward(c, L, lv, gm, ncol);
+    auto out = [&](int m, double acc, double, double) { o[base + (size_t)(m - 1) * plane] = acc; };
+    ppm_merge(c, L, lv, gm, ncol, out);
+}
+
+// The per-level inputs of the energy remap, over the interior columns and
+// all levels at once (one thread per cell; column t = (J-7) nx + (I-7)):
+// the total energy of each layer (111-128) into slot 2 and the kinetic
+// energy of the remapped winds (the term of 137-142) into slot 3, each the
+// reference's expression, so the column pass reads one value per level for
+// each.  The winds of a point outside the interior come from the ring.
+__global__ void k_remap_energy_in(Geo g, double *__restrict__ scr, size_t ncol, const double *__restrict__ playkap,
+                                  const double *__restrict__ gz, const double *__restrict__ pt,
+                                  const double *__restrict__ ud, const double *__restrict__ vd,
+                                  const double *__restrict__ udr, const double *__restrict__ vdr,
+                                  const double *__restrict__ ur, const double *__restrict__ vr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int I = ATM_HALO + i, J = ATM_HALO + j, L = g.L;
+    const size_t t = (size_t)j * g.nx + i, plane = g.plane(), x = g.ix(I, J, K);
+    const Scr s{scr, ncol, t, L};
+    auto w = [&](const double *in, const double *ring, int a, int b) {
+        const int ii = I + a, jj = J + b;
+        const bool inside = ii >= ATM_HALO && ii < ATM_HALO + g.nx && jj >= ATM_HALO && jj < ATM_HALO + g.ny;
+        return (inside ? in : ring)[g.ix(ii, jj, K)];
+    };
     {
-        WindCol c{plev, tplev, inside ? ud : udr, base, (size_t)g.SI, plane};
-        ppm_forward(c, L, s.lv(), s.gm(), ncol);
-        auto out = [&](int m, double acc, double, double) { ur[base + (size_t)(m - 1) * plane] = acc; };
-        ppm_merge(c, L, s.lv(), s.gm(), ncol, out);
+        const double ua = (.25 * (w(ud, udr, 0, 0) + w(ud, udr, 0, 1)) +
+                           .125 * (w(ud, udr, -1, 0) + w(ud, udr, -1, 1) + w(ud, udr, 1, 0) + w(ud, udr, 1, 1)));
+        const double va = (.25 * (w(vd, vdr, 0, 0) + w(vd, vdr, 1, 0)) +
+                           .125 * (w(vd, vdr, 0, -1) + w(vd, vdr, 1, -1) + w(vd, vdr, 0, 1) + w(vd, vdr, 1, 1)));
+        s.slot(2)[(size_t)(K + 1) * ncol] =
+            ATM_CV_DRY * pt[x] * playkap[x] + .5 * (gz[x + plane] + gz[x] + ua * ua + va * va);
     }
     {
-        WindCol c{plev, tplev, inside ? vd : vdr, base, (size_t)1, plane};
-        ppm_forward(c, L, s.lv(), s.gm(), ncol);
-        auto out = [&](int m, double acc, double, double) { vr[base + (size_t)(m - 1) * plane] = acc; };
-        ppm_merge(c, L, s.lv(), s.gm(), ncol, out);
+        const size_t sN = g.SI;  // +1 in J
+        const double ua =
+            (.25 * (ur[x] + ur[x + sN]) + .125 * (ur[x - 1] + ur[x - 1 + sN] + ur[x + 1] + ur[x + 1 + sN]));
+        const double va =
+            (.25 * (vr[x] + vr[x + 1]) + .125 * (vr[x - sN] + vr[x + 1 - sN] + vr[x + sN] + vr[x + 1 + sN]));
+        s.slot(3)[(size_t)(K + 1) * ncol] = .5 * (ua * ua + va * va);
     }
 }
 
 // Total energy remap, pt reconstruction (111-155) and the write-back of
-// the interior (157-161).  With write_winds the remapped winds are copied
-// into ud_out / vd_out here (only when those are not the buffers the
-// energy of neighbouring columns is formed from).
+// the interior (157-161), from the energies k_remap_energy_in left in the
+// scratch.  With write_winds the remapped winds are copied into ud_out /
+// vd_out here (only when those are not the buffers the energy of
+// neighbouring columns is formed from).
 __global__ void __launch_bounds__(256) k_remap_energy(
     Geo g, double *__restrict__ scr, size_t ncol, const double *__restrict__ plev,
     const double *__restrict__ tplev, const double *__restrict__ tpkap, const double *__restrict__ tplk,
-    const double *__restrict__ playkap, const double *__restrict__ gz, const double *pt,
-    const double *__restrict__ ud, const double *__restrict__ vd, const double *__restrict__ udr,
-    const double *__restrict__ vdr, const double *__restrict__ ur, const double *__restrict__ vr, double *pt_out,
+    const double *__restrict__ gz, const double *__restrict__ ur, const double *__restrict__ vr, double *pt_out,
     double *__restrict__ dp_out, double *__restrict__ ud_out, double *__restrict__ vd_out, int write_winds,
     int *nfw) {
     const int W = g.nx;
@@ -211,37 +250,15 @@ __global__ void __launch_bounds__(256) k_remap_energy(
     const int I = 7 + (int)(t % W), J = 7 + (int)(t / W);
     const int L = g.L;
     const size_t plane = g.plane(), base = g.ix2(I, J);
-    const size_t sN = g.SI;  // +1 in J
-    Scr s{scr, ncol, t, L};
-    // wind source at (I+a, J+b): interior points from ud / vd, the ring from udr / vdr
-    auto wsrc = [&](const double *in, const double *ring, int a, int b) {
-        const int i = I + a, j = J + b;
-        const bool inside = i >= ATM_HALO && i < ATM_HALO + g.nx && j >= ATM_HALO && j < ATM_HALO + g.ny;
-        return (inside ? in : ring) + g.ix2(i, j);
-    };
-    {
-        const double *u00 = wsrc(ud, udr, 0, 0), *u01 = wsrc(ud, udr, 0, 1), *um0 = wsrc(ud, udr, -1, 0),
-                     *um1 = wsrc(ud, udr, -1, 1), *up0 = wsrc(ud, udr, 1, 0), *up1 = wsrc(ud, udr, 1, 1);
-        const double *v00 = wsrc(vd, vdr, 0, 0), *v10 = wsrc(vd, vdr, 1, 0), *v0m = wsrc(vd, vdr, 0, -1),
-                     *v1m = wsrc(vd, vdr, 1, -1), *v01 = wsrc(vd, vdr, 0, 1), *v11 = wsrc(vd, vdr, 1, 1);
-        double *var = s.var();
-        for (int K = 0; K < L; K++) {
-            const size_t o = (size_t)K * plane;
-            double ua = (.25 * (u00[o] + u01[o]) + .125 * (um0[o] + um1[o] + up0[o] + up1[o]));
-            double va = (.25 * (v00[o] + v10[o]) + .125 * (v0m[o] + v1m[o] + v01[o] + v11[o]));
-            var[(size_t)(K + 1) * ncol] = ATM_CV_DRY * pt[base + o] * playkap[base + o] +
-                                          .5 * (gz[base + o + plane] + gz[base + o] + ua * ua + va * va);
-        }
-    }
-    EnergyCol c{plev, tplev, s.var(), base, plane, ncol};
-    ppm_forward(c, L, s.lv(), s.gm(), ncol);
+    const Scr s{scr, ncol, t, L};
+    const double *ke = s.slot(3);
+    const EnergyCol c{plev, tplev, s.slot(2), base, plane, ncol};
+    ppm_forward(c, L, s.slot(0), s.slot(1), ncol);
     double gzr = gz[base];
     auto out = [&](int m, double e, double dm0, double dm1) {
         const size_t x = base + (size_t)(m - 1) * plane;
-        double ua = (.25 * (ur[x] + ur[x + sN]) + .125 * (ur[x - 1] + ur[x - 1 + sN] + ur[x + 1] + ur[x + 1 + sN]));
-        double va = (.25 * (vr[x] + vr[x + 1]) + .125 * (vr[x - sN] + vr[x + 1 - sN] + vr[x + sN] + vr[x + 1 + sN]));
-        double tk = tpkap[x], tk1 = tpkap[x + plane];
-        double p = (e - gzr - .5 * (ua * ua + va * va)) / (ATM_CV_DRY * tplk[x] + .5 * ATM_CP_DRY * (tk - tk1));
+        const double tk = tpkap[x], tk1 = tpkap[x + plane];
+        const double p = (e - gzr - ke[(size_t)m * ncol]) / (ATM_CV_DRY * tplk[x] + .5 * ATM_CP_DRY * (tk - tk1));
         pt_out[x] = p;
         gzr = gzr + ATM_CP_DRY * p * (tk - tk1);
         const double d = dm0 - dm1;  // target_plev(k) - target_plev(k+1)
@@ -255,7 +272,7 @@ __global__ void __launch_bounds__(256) k_remap_energy(
             atm_report_nonfinite(nfw, vr[x]);
         }
     };
-    ppm_merge(c, L, s.lv(), s.gm(), ncol, out);
+    ppm_merge(c, L, s.slot(0), s.slot(1), ncol, out);
 }
 
 // interior wind write-back when the output winds are the input buffers
@@ -292,7 +309,7 @@ int atm_remap_alloc(atm_ctx *c) {
         c->remap = w;
         w->ncol = (size_t)(g.nx + 2) * (g.ny + 2);
         size_t p = g.plane();
-        ATM_HIP(c, hipMalloc(&w->scr, sizeof(double) * w->ncol * 3 * (g.L + 2)));
+        ATM_HIP(c, hipMalloc(&w->scr, sizeof(double) * w->ncol * 4 * (g.L + 2)));
         ATM_HIP(c, hipMalloc(&w->tplev, sizeof(double) * p * (g.L + 1)));
         ATM_HIP(c, hipMalloc(&w->tpkap, sizeof(double) * p * (g.L + 1)));
         ATM_HIP(c, hipMalloc(&w->tplk, sizeof(double) * p * g.L));
@@ -334,16 +351,21 @@ int atm_remap_run_split(atm_ctx *c, const double *dp, const double *pt, const do
                        dp, pt, c->plev, c->pkap, c->playkap, c->gz, w->tplev, w->tpkap, w->tplk);
     const size_t nw = (size_t)(g.nx + 2) * (g.ny + 2), ne = (size_t)g.nx * g.ny;
     const unsigned bs = 256;
-    hipLaunchKernelGGL(k_remap_winds, dim3(blocks8(nw, bs)), dim3(bs), 0, c->stream, g, w->scr, nw, c->plev,
+    hipLaunchKernelGGL(k_remap_winds, dim3(blocks8(nw, bs), 2), dim3(bs), 0, c->stream, g, w->scr, nw, c->plev,
                        w->tplev, ud_in, vd_in, ud_ring, vd_ring, w->uremap, w->vremap);
+    {
+        const dim3 be(64, 4);
+        hipLaunchKernelGGL(k_remap_energy_in, grid3(g.nx, g.ny, g.L, be), be, 0, c->stream, g, w->scr, ne,
+                           c->playkap, c->gz, pt, ud_in, vd_in, ud_ring, vd_ring, w->uremap, w->vremap);
+    }
     // the remapped winds go to the output buffers from the energy kernel
     // unless those are the input winds, whose interior neighbouring columns
     // still read there (a ring-only source buffer is read outside the
     // interior only)
     const int write_winds = (ud_out != ud_in && vd_out != vd_in) ? 1 : 0;
     hipLaunchKernelGGL(k_remap_energy, dim3(blocks8(ne, bs)), dim3(bs), 0, c->stream, g, w->scr, ne, c->plev,
-                       w->tplev, w->tpkap, w->tplk, c->playkap, c->gz, pt, ud_in, vd_in, ud_ring, vd_ring,
-                       w->uremap, w->vremap, pt_out, dp_out, ud_out, vd_out, write_winds, c->nfw);
+                       w->tplev, w->tpkap, w->tplk, c->gz, w->uremap, w->vremap, pt_out, dp_out, ud_out, vd_out,
+                       write_winds, c->nfw);
     if (!write_winds) {
         dim3 bw(64, 4);
         hipLaunchKernelGGL(k_remap_write_winds, grid3(g.nx, g.ny, g.L, bw), bw, 0, c->stream, g, w->uremap,
