@@ -88,6 +88,87 @@ __device__ __forceinline__ float div3(float x) {
     return fmaf(fmaf(-q, 3.0f, x), r3, q);
 }
 
+// x**kappa, the reference's pressure-to-the-kappa (a libm pow of the
+// double kappa = R_dry / cp_dry), for the column scans.  Written for this
+// exponent from the textbook reduction (e.g. Muller, "Elementary
+// Functions", ch. 11 and 13):
+//   ln x = e ln2 + ln m,  x = m 2^e with m in [1/sqrt2, sqrt2),
+//   ln m = 2 atanh(f) = 2f + f^3 (2/3 + 2/5 f^2 + ... + 2/23 f^20),
+//          f = (m-1)/(m+1), |f| < 0.1716 (truncation below 2^-60),
+//   y = kappa ln x as a double-double (ln2 in two parts, Fast2Sum),
+//   e^y = 2^n (1 + p), n = rint(y / ln2), p = e^r - 1 by the Taylor
+//          series to degree 13 in r = y - n ln2 (|r| <= ln2/2, Cody-Waite).
+// The result is within one ulp of the exact power (checked on the host
+// against long-double powl over the pressures' range, where it also
+// agrees with the libm pow in all but a small fraction of cases, each one
+// ulp apart), in about a quarter of the general pow's instructions.
+// x = 0 gives 0, x = +inf gives +inf, x < 0 and NaN give NaN.
+// N values at once, each step over all N before the next: a coefficient
+// is brought into registers once per step instead of once per value.
+// Each result is pow_kappa's, operation for operation.
+template <int N>
+__host__ __device__ inline void pow_kappa_n(double (&x)[N]) {
+    constexpr double kappa = ATM_KAPPA;
+    constexpr double ln2_hi = 6.93147180369123816490e-01;  // 0x3fe62e42fee00000: e ln2_hi exact for |e| < 2^11
+    constexpr double ln2_lo = 1.90821492927058770002e-10;
+    constexpr double inv_ln2 = 1.44269504088896338700e+00;
+    constexpr double cl[10] = {2. / 21., 2. / 19., 2. / 17., 2. / 15., 2. / 13., 2. / 11.,
+                               2. / 9.,  2. / 7.,  2. / 5.,  2. / 3.};
+    constexpr double ce[11] = {1. / 479001600., 1. / 39916800., 1. / 3628800., 1. / 362880.,
+                               1. / 40320.,     1. / 5040.,     1. / 720.,     1. / 120.,
+                               1. / 24.,        1. / 6.,        .5};
+    double f[N], f2[N], q[N], v[N];
+    int e[N];
+    for (int i = 0; i < N; i++) {
+        double m = frexp(x[i], &e[i]);  // [0.5, 1)
+        if (m < 0.70710678118654752440) {
+            m = m + m;
+            e[i] = e[i] - 1;
+        }
+        f[i] = (m - 1.) / (m + 1.);  // m - 1 exact
+        f2[i] = f[i] * f[i];
+        q[i] = 2. / 23.;
+    }
+    for (int c = 0; c < 10; c++)
+        for (int i = 0; i < N; i++) q[i] = fma(q[i], f2[i], cl[c]);
+    double yh[N], yl[N];
+    for (int i = 0; i < N; i++) {
+        const double lnm = fma(f[i] * f2[i], q[i], f[i] + f[i]);
+        // ln x = s + lo
+        const double ed = (double)e[i];
+        const double a = ed * ln2_hi;
+        const double s = a + lnm;
+        const double lo = (lnm - (s - a)) + ed * ln2_lo;  // Fast2Sum error (|a| >= |lnm| or a = 0)
+        yh[i] = kappa * s;
+        yl[i] = fma(kappa, s, -yh[i]) + kappa * lo;
+    }
+    // e^(yh + yl) = 2^n e^(rh + rl): rh exact (Cody-Waite), 1 + rh exact
+    // as h + l (Fast2Sum), the rest added to l, one rounding at the end
+    double n[N], rh[N];
+    for (int i = 0; i < N; i++) {
+        n[i] = rint(yh[i] * inv_ln2);
+        rh[i] = fma(-n[i], ln2_hi, yh[i]);
+        q[i] = 1. / 6227020800.;  // 1/13!
+    }
+    for (int c = 0; c < 11; c++)
+        for (int i = 0; i < N; i++) q[i] = fma(q[i], rh[i], ce[c]);
+    for (int i = 0; i < N; i++) {
+        const double rl = yl[i] - n[i] * ln2_lo;
+        const double h = 1. + rh[i];
+        const double l = (1. - h) + rh[i];
+        const double t = rh[i] * rh[i] * q[i];  // e^rh = h + l + t
+        v[i] = ldexp(h + (t + fma(rl, h + t, l)), (int)n[i]);
+        const double xi = x[i];
+        x[i] = (xi > 0. && xi < __builtin_inf()) ? v[i] : xi == 0. ? 0. : xi == __builtin_inf() ? xi : __builtin_nan("");
+    }
+}
+
+__host__ __device__ inline double pow_kappa(double x) {
+    double a[1] = {x};
+    pow_kappa_n<1>(a);
+    return a[0];
+}
+
 // report a non-finite result value into the stability word (see atm_ctx::nfw)
 __device__ __forceinline__ void atm_report_nonfinite(int *nfw, double v) {
     if (!isfinite(v)) atomicOr(nfw, 1);

@@ -622,13 +622,13 @@ __global__ void __launch_bounds__(64) k_scan_plk(Geo g, double ptop, double pkto
             pv[u] = p;
         }
 #pragma unroll
-        for (int u = 0; u < SCAN_CH; u++) pk[u] = pow(pv[u], ATM_KAPPA);
+        for (int u = 0; u < SCAN_CH; u++) pk[u] = pow_kappa(pv[u]);
 #pragma unroll
         for (int u = 0; u < SCAN_CH; u++) level(K - u, pk[u]);
     }
     for (; K >= 0; K--) {
         p = p + src[c0 + (size_t)K * pl];
-        level(K, pow(p, ATM_KAPPA));
+        level(K, pow_kappa(p));
     }
 }
 
@@ -675,14 +675,14 @@ __global__ void __launch_bounds__(64) k_scan_pg(Geo g, double ptop, double pktop
             pv[u] = p;
         }
 #pragma unroll
-        for (int u = 0; u < SCAN_CH; u++) pk[u] = pow(pv[u], ATM_KAPPA);
+        for (int u = 0; u < SCAN_CH; u++) pk[u] = pow_kappa(pv[u]);
 #pragma unroll
         for (int u = 0; u < SCAN_CH; u++) level(K - u, s[u], pv[u], pk[u]);
     }
     for (; K >= 0; K--) {
         const double s = src[c0 + (size_t)K * pl];
         p = p + s;
-        level(K, s, p, pow(p, ATM_KAPPA));
+        level(K, s, p, pow_kappa(p));
     }
     double z = gz[c0];
     double pk = pkap[c0];
@@ -748,12 +748,18 @@ __global__ void __launch_bounds__(64, 3) k_scan_cols(Geo g, int i0, int j0, int 
     }
 #pragma unroll
     for (int u = 0; u < CMAX; u++)
-        if (u < n) {
-            const size_t x = c0 + (size_t)(k0 + u) * pl;
-            if (plev && col) plev[x] = pk[u];
-            pk[u] = pow(pk[u], ATM_KAPPA);
-            if (pkap && col) pkap[x] = pk[u];
-        }
+        if (u < n && plev && col) plev[c0 + (size_t)(k0 + u) * pl] = pk[u];
+    static_assert(CMAX % 4 == 0, "pow_kappa in groups of 4");
+#pragma unroll
+    for (int u0 = 0; u0 < CMAX; u0 += 4) {  // levels past n hold 1 and stay unused
+        double v[4] = {pk[u0], pk[u0 + 1], pk[u0 + 2], pk[u0 + 3]};
+        pow_kappa_n<4>(v);
+#pragma unroll
+        for (int i = 0; i < 4; i++) pk[u0 + i] = v[i];
+    }
+#pragma unroll
+    for (int u = 0; u < CMAX; u++)
+        if (u < n && pkap && col) pkap[c0 + (size_t)(k0 + u) * pl] = pk[u];
     // pkap of the level above the chunk: the next chunk's first, or the top
     double pk_up = __shfl_down(pk[0], SC_COLS);
     if (k0 + n >= g.L) pk_up = pktop;

@@ -45,7 +45,7 @@ __global__ void k_scan_pkap(Geo g, Rng r, const double *__restrict__ src, double
     for (int K = g.L - 1; K >= 0; K--) {
         p = p + src[g.ix(I, J, K)];
         plev[g.ix(I, J, K)] = p;
-        pkap[g.ix(I, J, K)] = pow(p, ATM_KAPPA);
+        pkap[g.ix(I, J, K)] = pow_kappa(p);
     }
 }
 
@@ -61,7 +61,7 @@ __global__ void k_scan_pkap_gz(Geo g, Rng r, const double *__restrict__ src,
     for (int K = g.L - 1; K >= 0; K--) {
         p = p + src[g.ix(I, J, K)];
         plev[g.ix(I, J, K)] = p;
-        pkap[g.ix(I, J, K)] = pow(p, ATM_KAPPA);
+        pkap[g.ix(I, J, K)] = pow_kappa(p);
     }
     double z = gz[g.ix(I, J, 0)];
     double pk = pkap[g.ix(I, J, 0)];

@@ -37,7 +37,7 @@ __global__ void __launch_bounds__(64) k_phys_tend(Geo g, const double *__restric
     for (int K = L - 1; K >= 0; K--) {
         p = p + dp[c0 + (size_t)K * pl];
         plev[c0 + (size_t)K * pl] = p;
-        pkap[c0 + (size_t)K * pl] = pow(p, ATM_KAPPA);
+        pkap[c0 + (size_t)K * pl] = pow_kappa(p);
     }
     // geopotential and playkap from the surface (57-70)
     {

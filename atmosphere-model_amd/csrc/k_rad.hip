@@ -80,7 +80,7 @@ __global__ void k_rad_profiles(Geo g, const double *__restrict__ dp, const doubl
     for (int K = L - 1; K >= 0; K--) {
         double p = p1v + dp[g.ix(I, J, K)];
         plev[g.ix(I, J, K)] = p;
-        double pk = pow(p, ATM_KAPPA);
+        double pk = pow_kappa(p);
         pkap[g.ix(I, J, K)] = pk;
         double plog = log(p / p1v);
         double pl = dp[g.ix(I, J, K)] / plog;

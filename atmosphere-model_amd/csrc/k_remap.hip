@@ -114,13 +114,13 @@ __global__ void k_remap_grid(Geo g, double top, const double *__restrict__ ak, c
                 pv[u] = p;
             }
 #pragma unroll
-            for (int u = 0; u < CH; u++) pk[u] = pow(pv[u], ATM_KAPPA);
+            for (int u = 0; u < CH; u++) pk[u] = pow_kappa(pv[u]);
 #pragma unroll
             for (int u = 0; u < CH; u++) level(K - u, pv[u], pk[u]);
         }
         for (; K >= 0; K--) {
             p = p + dp[c0 + (size_t)K * pl];
-            level(K, p, pow(p, ATM_KAPPA));
+            level(K, p, pow_kappa(p));
         }
     }
     // target grid from the surface pressure (60-70)
@@ -141,13 +141,13 @@ __global__ void k_remap_grid(Geo g, double top, const double *__restrict__ ak, c
 #pragma unroll
             for (int u = 0; u < CH; u++) tp[u] = ak[K - u] + bk[K - u] * (ps - top);
 #pragma unroll
-            for (int u = 0; u < CH; u++) tk[u] = pow(tp[u], ATM_KAPPA);
+            for (int u = 0; u < CH; u++) tk[u] = pow_kappa(tp[u]);
 #pragma unroll
             for (int u = 0; u < CH; u++) level(K - u, tp[u], tk[u]);
         }
         for (; K >= 0; K--) {
             const double tp = ak[K] + bk[K] * (ps - top);
-            level(K, tp, pow(tp, ATM_KAPPA));
+            level(K, tp, pow_kappa(tp));
         }
     }
     // geopotential (72-78)

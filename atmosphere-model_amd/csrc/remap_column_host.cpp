@@ -1,5 +1,6 @@
-// remap_column_host.cpp -- host build of the device remap column
-// (remap_column.hpp) for CPU-side checks; not used by the GPU library.
+// remap_column_host.cpp -- host build of device numerics for CPU-side
+// checks; not used by the GPU library: the remap column (remap_column.hpp)
+// and pow_kappa (atm_common.hpp).
 //
 // atmc_remap_column has the argument convention of remap_column
 // (src/mod_remap.f90:165): var(1:nlay) on source faces coord(1:nlay+1)
@@ -24,4 +25,17 @@ extern "C" void atmc_remap_column(int nlay, const double *src_var, double *dst_v
     ppm_forward(c, nlay, lv.data(), gm.data(), 1);
     auto out = [&](int m, double acc, double, double) { dst_var[m - 1] = acc; };
     ppm_merge(c, nlay, lv.data(), gm.data(), 1, out);
+}
+
+// pow_kappa on n values (atm_common.hpp), for the CPU check of its accuracy;
+// width 4 is the grouped form the column scans use
+extern "C" void atmc_pow_kappa(const double *x, double *y, long n, int width) {
+    long i = 0;
+    if (width == 4)
+        for (; i + 4 <= n; i += 4) {
+            double v[4] = {x[i], x[i + 1], x[i + 2], x[i + 3]};
+            pow_kappa_n<4>(v);
+            for (int k = 0; k < 4; k++) y[i + k] = v[k];
+        }
+    for (; i < n; i++) y[i] = pow_kappa(x[i]);
 }
