@@ -250,13 +250,14 @@ __device__ __forceinline__ Comp comp_over(const Comp &u, const Comp &w) {
 // Moves within a row of 16 lanes (one g-point's group) by the DPP modifier
 // of a VALU move, two 32-bit halves per double: row_up<D> gives lane j the
 // value of lane j - D (row_shr), row_down<D> that of lane j + D (row_shl).
-// A lane whose source lies outside its row keeps its own value.
+// A lane whose source lies outside its row receives 0 (bound_ctrl); every
+// caller uses the moved value only where the source is inside the row.
 template <int CTRL>
 __device__ __forceinline__ double dpp_row(double x) {
     const unsigned long long b = (unsigned long long)__double_as_longlong(x);
     const int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
-    const int rlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
-    const int rhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+    const int rlo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, true);
+    const int rhi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, true);
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)rhi << 32) | (unsigned)rlo));
 }
 template <int D>
