@@ -187,6 +187,34 @@ def test_driver_run_with_physics(gpu_available, path):
         assert e <= PARITY, (n, e)
 
 
+@pytest.mark.parametrize("path", ["fused", "reference"])
+def test_unequal_spacing(gpu_available, path):
+    """dx != dy, the one case where Q2 shows: the y Courant numbers use
+    dt/dx (mod_sw_dyn.f90:133, 349, 448) while the denominators, the
+    Laplacians and the pressure gradient use dy.  12 steps with two remaps
+    and radiation at 100 km x 75 km."""
+    import oracle as O
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+    nx, ny, nlay, dx, dy = 64, 32, 16, 100e3, 75e3
+    ic = make_initial_state(nx, ny, nlay, dx, dy, seed=5)
+    cfg = dict(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=600.0, dt_physics=1e12,
+               dt_remap=300.0, radiation_on=True, physics_on=False)
+    gpu, orc = Model(nx, ny, nlay, dx, dy), O.Oracle(nx, ny, nlay, dx, dy)
+    if path != gpu.path:
+        gpu.set_path(path)
+    for b in (gpu, orc):
+        st = driver.init_model(b, ic, Config(**cfg))
+        t, n, stable = driver.run_model(b, Config(**cfg), st, max_steps=12)
+        assert stable and n == 12
+    errs = {n: compare(gpu, orc, n) for n in ("dp", "pt", "ud", "vd", "net_flux")}
+    _print(f"dx != dy {path} {nx}x{ny}x{nlay} x12", **errs)
+    for n, e in errs.items():
+        assert e <= PARITY, (n, e)
+
+
 def test_radiation_table_range_error(gpu_available):
     """A temperature outside the gas-optics table (the reference aborts,
     mod_gas_optics.f90:4562-4592) comes back as ATM_ERR_TABLE_RANGE."""
@@ -346,23 +374,24 @@ def _ps_T(m, ptop):
     return plev[0], pt * ((pk[1:] - pk[:-1]) / np.log(pk[1:] / pk[:-1]))
 
 
-def test_mixed_precision_error_and_mass(gpu_available):
+@pytest.mark.parametrize("dy", [100e3, 75e3])
+def test_mixed_precision_error_and_mass(gpu_available, dy):
     """ATM_PRECISION_MIXED against the FP64 path: 30 steps with remaps every
-    5 at 64x64x20 from the same state.  Surface pressure and temperature
-    agree to float-level error, and the total mass (sum of dp) changes only
-    by FP64 rounding in both forms (the float fluxes are shared by the two
-    cells of each face)."""
+    5 at 64x64x20 from the same state, at dx = dy and dx != dy.  Surface
+    pressure and temperature agree to float-level error, and the total mass
+    (sum of dp) changes only by FP64 rounding in both forms (the float
+    fluxes are shared by the two cells of each face)."""
     from atmos import driver
     from atmos.config import Config
     from atmos.ic import make_initial_state
     from atmos.model import AtmError, Model
     nx, ny, nlay = 64, 64, 20
-    ic = make_initial_state(nx, ny, nlay, 100e3)
+    ic = make_initial_state(nx, ny, nlay, 100e3, dy)
     kw = dict(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12, dt_remap=300.0,
               radiation_on=False, physics_on=False)
     out = {}
     for prec in ("fp64", "mixed"):
-        m = Model(nx, ny, nlay, 100e3, precision=prec)
+        m = Model(nx, ny, nlay, 100e3, dy, precision=prec)
         cfg = Config(**kw)
         st = driver.init_model(m, ic, cfg)
         mass0 = m.get("dp").sum()
@@ -377,7 +406,7 @@ def test_mixed_precision_error_and_mass(gpu_available):
                 m.set_path("reference")
     a, b = out["mixed"], out["fp64"]
     rr = {k: float(np.sqrt(np.mean((a[k] - b[k]) ** 2)) / np.sqrt(np.mean(b[k] ** 2))) for k in ("ps", "T", "ud", "vd")}
-    _print("mixed vs fp64 x30 rel rms", **rr, drift_mixed=abs(a["drift"]), drift_fp64=abs(b["drift"]))
+    _print(f"mixed vs fp64 x30 dy={dy:.0f} rel rms", **rr, drift_mixed=abs(a["drift"]), drift_fp64=abs(b["drift"]))
     assert rr["ps"] <= 1e-6 and rr["T"] <= 1e-6, rr
     assert rr["ud"] <= 1e-4 and rr["vd"] <= 1e-4, rr
     assert abs(a["drift"]) <= 1e-13 and abs(b["drift"]) <= 1e-13
