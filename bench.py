@@ -287,6 +287,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": alg, "avg_launch_ms": avg_s * 1e3},
         "kernels_ms_per_step": {k: v[1] / args.steps for k, v in kernels.items()},
+        # the metric's third part: no tangent-linear or adjoint kernels exist
+        # in this library, so there is no ratio to report
+        "adjoint_forward_cost_ratio": None,
     }
     if args.radiation_calls > 0:
         out["radiation"] = radiation_timing(m, args, time_s)
