@@ -707,7 +707,7 @@ __global__ void __launch_bounds__(64) k_scan_pg(Geo g, double ptop, double pktop
 constexpr int SC_COLS = 16, SC_PARTS = 4;
 
 template <int CMAX>
-__global__ void __launch_bounds__(64, 3) k_scan_cols(Geo g, int i0, int j0, int ni, double ptop, double pktop,
+__global__ void __launch_bounds__(64, 4) k_scan_cols(Geo g, int i0, int j0, int ni, double ptop, double pktop,
                                                   double gravity, const double *__restrict__ src,
                                                   const double *__restrict__ th, const double *__restrict__ nf,
                                                   double *__restrict__ plev, double *__restrict__ pkap,
