@@ -92,10 +92,11 @@ struct Reg {
 // CGRID: the C winds come from the D winds (109-126) and are also written
 // out; otherwise uc/vc of the C step are read (the D step).
 // ---------------------------------------------------------------------------
-// waves per SIMD of the dp/pt kernels: 6; 8 for the float D step, whose
-// registers fit without spills (the float C step's do not)
+// waves per SIMD of the dp/pt kernels (measured): 6 in double (5 and 7
+// are slower); in float 7 for the C step (one spilled register) and 8 for
+// the D step, whose registers fit without spills
 template <class R, bool CGRID>
-constexpr int pair_waves() { return std::is_same<R, double>::value || CGRID ? 6 : 8; }
+constexpr int pair_waves() { return std::is_same<R, double>::value ? 6 : CGRID ? 7 : 8; }
 
 template <int VAR, bool CGRID, class R>
 __global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo g, DynCoef c, const double *__restrict__ dp,
