@@ -216,8 +216,8 @@ __device__ __forceinline__ LinPT lin_pT(const Tables &t, double p, double T, int
 // The column recurrences on 16 lanes per g-point.  Lane j of the group owns
 // the layers [lo, hi) of block j (ceil(L/16) layers each).  A block is
 // first reduced to one composite layer, then the composites are combined
-// across the group with 4-step shuffle scans (from the surface up and from
-// the top down); each lane then runs the reference's recurrences over its
+// across the group with 4-step scans over DPP row moves (from the surface
+// up and from the top down); each lane then runs the reference's recurrences over its
 // own layers from the boundary values of its block.  Values agree with the
 // sequential column to rounding (the grouping of the products differs).
 // ---------------------------------------------------------------------------

@@ -226,7 +226,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              f"(the path shards by 2-D tiles with halo exchange, see DESIGN.md)", file=sys.stderr)
+              f"(the path would shard by 2-D tiles with 7-wide halo exchanges)", file=sys.stderr)
         sys.exit(2)
     import numpy as np  # noqa: F401
 

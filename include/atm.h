@@ -6,7 +6,8 @@
  * FFI: its hot path is a set of Fortran module procedures that share module
  * arrays (src/mod_fields.f90:15-37).  Each entry point below replaces one
  * of those procedures; a Fortran host binds them through ISO_C_BINDING
- * (see INTEGRATION.md).  All functions return an atm_status (0 = success);
+ * (the modules under atmosphere-model_amd/fortran).  All functions return
+ * an atm_status (0 = success);
  * atm_last_error() gives the message of the last failure on a context.
  *
  * Data layout.  Every field crossing this boundary is FP64 in the
