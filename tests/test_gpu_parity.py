@@ -79,7 +79,8 @@ def test_radiation_stage(gpu_available):
 
 @pytest.mark.parametrize("shape,steps,rad", [((32, 32, 20), 10, True), ((64, 64, 20), 100, False),
                                              ((32, 32, 72), 6, True), ((8, 8, 4), 6, True),
-                                             ((37, 23, 10), 6, True)])
+                                             ((37, 23, 10), 6, True), ((32, 16, 6), 6, True),
+                                             ((32, 32, 13), 6, True)])
 def test_driver_run(gpu_available, shape, steps, rad):
     from atmos import driver
     gpu, orc, cfg, sg, so, ic = make_pair(*shape)
