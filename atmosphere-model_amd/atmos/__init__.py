@@ -8,7 +8,8 @@
 * `ic`: seeded synthetic initial conditions.
 * `ncio`: initial, output and restart files in the netCDF classic formats
   (libatmos_io.so).
+* `ensemble`: independent members of uneven grid sizes stepped together.
 """
-from . import config, driver, ic, model, ncio  # noqa: F401
+from . import config, driver, ensemble, ic, model, ncio  # noqa: F401
 
-__all__ = ["config", "driver", "ic", "model", "ncio"]
+__all__ = ["config", "driver", "ensemble", "ic", "model", "ncio"]

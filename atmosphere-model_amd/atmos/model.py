@@ -28,7 +28,8 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_radiation_halo", "atm_halo_exchange", "atm_check_stability", "atm_sync",
            "atm_build_info", "atm_set_path", "atm_get_path", "atm_timer_enable", "atm_timer_read",
            "atm_physics_init", "atm_physics_tendencies", "atm_apply_physics", "atm_physics_halo",
-           "atm_output_init", "atm_accumulate_output", "atm_output_mean"]
+           "atm_output_init", "atm_accumulate_output", "atm_output_mean", "atm_stream_create",
+           "atm_stream_destroy"]
 PATHS = {"fused": 0, "reference": 1}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 TIMERS = {"c1": 0, "c2": 1, "d1": 2, "d2": 3, "scan": 4, "halo": 5, "remap": 6, "radiation": 7,
@@ -74,6 +75,8 @@ def load_library(path: str = LIB_PATH):
               "atm_physics_tendencies", "atm_physics_halo", "atm_output_init"]:
         getattr(L, n).argtypes = [vp]
     L.atm_set_stream.argtypes = [vp, vp]
+    L.atm_stream_create.argtypes = [C.POINTER(vp)]
+    L.atm_stream_destroy.argtypes = [vp]
     L.atm_set_coordinate.argtypes = [vp, dp, dp]
     for n in ["atm_put_field", "atm_get_field"]:
         getattr(L, n).argtypes = [vp, ip, dp]
@@ -90,6 +93,22 @@ def load_library(path: str = LIB_PATH):
     L.atm_timer_read.argtypes = [vp, ip, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     _LIB = L
     return L
+
+
+def stream_create() -> int:
+    """A HIP stream of default priority from the library's runtime (handle as int)."""
+    L = load_library()
+    s = C.c_void_p()
+    rc = L.atm_stream_create(C.byref(s))
+    if rc:
+        raise AtmError(rc, "atm_stream_create")
+    return s.value
+
+
+def stream_destroy(handle: int) -> None:
+    rc = load_library().atm_stream_destroy(C.c_void_p(handle))
+    if rc:
+        raise AtmError(rc, "atm_stream_destroy")
 
 
 def _dptr(a):

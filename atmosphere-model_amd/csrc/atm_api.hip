@@ -189,6 +189,19 @@ int atm_destroy(atm_ctx *c) {
 
 const char *atm_last_error(const atm_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
+int atm_stream_create(void **s) {
+    if (!s) return ATM_ERR_ARGUMENT;
+    hipStream_t h = nullptr;
+    if (hipStreamCreateWithFlags(&h, hipStreamNonBlocking) != hipSuccess) return ATM_ERR_HIP;
+    *s = (void *)h;
+    return ATM_OK;
+}
+
+int atm_stream_destroy(void *s) {
+    if (!s) return ATM_ERR_ARGUMENT;
+    return hipStreamDestroy((hipStream_t)s) == hipSuccess ? ATM_OK : ATM_ERR_HIP;
+}
+
 int atm_set_stream(atm_ctx *c, void *s) {
     CHECK_CTX(c);
     ATM_HIP(c, hipStreamSynchronize(c->stream));

@@ -70,6 +70,11 @@ def parse():
     ap.add_argument("--mixed-error-steps", type=int, default=100)
     ap.add_argument("--radiation-calls", type=int, default=3,
                     help="also time this many radiation calls on the same grid (0: skip)")
+    ap.add_argument("--ensemble", choices=["on", "off"], default="on",
+                    help="also time the uneven ensemble: 24 x 192x192x48 + 6 x 256x256x48 + 2 x 384x384x64, "
+                         "mixed precision, radiation every 10 steps")
+    ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--ensemble-streams", type=int, default=4)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of this many MiB (read + write) a few times, "
                          "to scale HBM counters (FETCH_SIZE/WRITE_SIZE) against a known size")
@@ -85,6 +90,38 @@ def calibrate_copy(mib):
         torch.mul(x, 1.0, out=y)
     torch.cuda.synchronize()
     print(json.dumps({"calibrate_copy_mib": mib, "bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8}))
+
+
+ENSEMBLE_SPECS = [(24, 192, 192, 48), (6, 256, 256, 48), (2, 384, 384, 64)]
+
+
+def ensemble_timing(args):
+    """The uneven ensemble: 32 members of three grid sizes, each from its own
+    seeded state (k^-3 temperature noise of 0.5 K), mixed precision, dx and
+    dt of the main workload, remap every 5 steps and radiation every 10.
+    Whole steps of all members (dynamics, remap, radiation, the stability
+    check of every member), timed after 5 warm-up steps."""
+    from atmos.ensemble import Ensemble
+    ens = Ensemble(ENSEMBLE_SPECS, args.dx, args.dt, remap_every=5, radiation_every=10, precision="mixed",
+                   nstreams=args.ensemble_streams)
+    for _ in range(5):
+        if not ens.step():
+            raise RuntimeError("ensemble member unstable in warm-up")
+    ens.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.ensemble_steps):
+        if not ens.step():
+            raise RuntimeError("ensemble member unstable")
+    ens.sync()
+    sec = time.perf_counter() - t0
+    cells = ens.cells
+    out = {"members": " + ".join(f"{c} x {nx}x{ny}x{L}" for c, nx, ny, L in ENSEMBLE_SPECS), "cells": cells,
+           "precision": "mixed", "steps": args.ensemble_steps, "remap_every": 5, "radiation_every": 10,
+           "value": cells * args.ensemble_steps / sec, "unit": "cell-steps/s",
+           "ms_per_step": sec / args.ensemble_steps * 1e3,
+           "launch": f"each member's own launches, members dealt to {args.ensemble_streams} HIP streams"}
+    ens.close()
+    return out
 
 
 def cpu_baseline(args):
@@ -296,6 +333,8 @@ def main():
     if args.mixed == "on" and args.path == "fused":
         del m
         out["mixed"] = mixed_timing(args, ic, cfg_kw)
+    if args.ensemble == "on" and args.path == "fused":
+        out["ensemble"] = ensemble_timing(args)
     if args.cpu_baseline == "on":
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)

@@ -92,6 +92,11 @@ const char *atm_last_error(const atm_ctx *ctx);
 /* Run on an existing HIP stream (hipStream_t passed as void*); NULL = the
    context's own stream. */
 int atm_set_stream(atm_ctx *ctx, void *hip_stream);
+/* A HIP stream of default priority from the runtime this library uses, for
+   hosts that run several contexts on a few streams (an ensemble), and its
+   release.  No reference counterpart (plumbing of this library). */
+int atm_stream_create(void **hip_stream);
+int atm_stream_destroy(void *hip_stream);
 
 /* Hybrid coordinate coefficients Ak, Bk (nlay+1 each) -- the reference's
    coord_Ak/coord_Bk read by init_prognostic_fields (mod_input.f90:161-168). */
