@@ -29,7 +29,7 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_build_info", "atm_set_path", "atm_get_path", "atm_timer_enable", "atm_timer_read",
            "atm_physics_init", "atm_physics_tendencies", "atm_apply_physics", "atm_physics_halo",
            "atm_output_init", "atm_accumulate_output", "atm_output_mean", "atm_stream_create",
-           "atm_stream_destroy"]
+           "atm_stream_destroy", "atm_graph_begin", "atm_graph_end", "atm_graph_launch", "atm_graph_free"]
 PATHS = {"fused": 0, "reference": 1}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 TIMERS = {"c1": 0, "c2": 1, "d1": 2, "d2": 3, "scan": 4, "halo": 5, "remap": 6, "radiation": 7,
@@ -77,6 +77,9 @@ def load_library(path: str = LIB_PATH):
     L.atm_set_stream.argtypes = [vp, vp]
     L.atm_stream_create.argtypes = [C.POINTER(vp)]
     L.atm_stream_destroy.argtypes = [vp]
+    for n in ["atm_graph_begin", "atm_graph_end", "atm_graph_free"]:
+        getattr(L, n).argtypes = [vp]
+    L.atm_graph_launch.argtypes = [vp, ip]
     L.atm_set_coordinate.argtypes = [vp, dp, dp]
     for n in ["atm_put_field", "atm_get_field"]:
         getattr(L, n).argtypes = [vp, ip, dp]
@@ -273,6 +276,19 @@ class Model:
         n, ms = C.c_int64(0), C.c_double(0)
         self._ck(self._L.atm_timer_read(self._h, TIMERS[name], C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    def graph_begin(self):
+        """Record the following step calls into a HIP graph instead of running them."""
+        self._ck(self._L.atm_graph_begin(self._h))
+
+    def graph_end(self):
+        self._ck(self._L.atm_graph_end(self._h))
+
+    def graph_launch(self, times=1):
+        self._ck(self._L.atm_graph_launch(self._h, int(times)))
+
+    def graph_free(self):
+        self._ck(self._L.atm_graph_free(self._h))
 
     def set_stream(self, stream_handle):
         self._ck(self._L.atm_set_stream(self._h, C.c_void_p(stream_handle)))

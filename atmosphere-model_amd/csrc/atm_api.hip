@@ -19,6 +19,11 @@ int atm_fail(atm_ctx *c, int code, const std::string &msg) {
 #define CHECK_CTX(c) \
     if (!(c)) return ATM_ERR_ARGUMENT
 
+// entry points that wait on the device, copy between host and device or
+// allocate cannot be recorded into a graph
+#define CHECK_NOT_CAPTURING(c) \
+    if ((c)->capturing) return atm_fail(c, ATM_ERR_STATE, "not allowed while recording a graph")
+
 static double *field_ptr(atm_ctx *c, int id, int *nk) {
     const int L = c->g.L;
     switch (id) {
@@ -47,6 +52,8 @@ static int dalloc(atm_ctx *c, double **p, size_t n) {
 }
 
 static void release(atm_ctx *c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
     c->dp = c->pt = c->ud = c->vd = nullptr;  // owned through pbuf
     double **all[] = {&c->ak, &c->bk, &c->ts, &c->gz, &c->net_flux,
                       &c->plev, &c->pkap, &c->play, &c->playkap, &c->radius, &c->dpc, &c->ptc,
@@ -181,6 +188,11 @@ int atm_create(const atm_grid *grid, const atm_params *params, int precision, at
 
 int atm_destroy(atm_ctx *c) {
     CHECK_CTX(c);
+    if (c->capturing) {  // discard an unfinished recording
+        hipGraph_t gr = nullptr;
+        if (hipStreamEndCapture(c->stream, &gr) == hipSuccess && gr) (void)hipGraphDestroy(gr);
+        c->capturing = false;
+    }
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     release(c);
     delete c;
@@ -204,6 +216,7 @@ int atm_stream_destroy(void *s) {
 
 int atm_set_stream(atm_ctx *c, void *s) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     ATM_HIP(c, hipStreamSynchronize(c->stream));
     if (c->own_stream) {
         if (s == nullptr) return ATM_OK;
@@ -223,6 +236,7 @@ int atm_set_stream(atm_ctx *c, void *s) {
 
 int atm_set_coordinate(atm_ctx *c, const double *ak, const double *bk) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     if (!ak || !bk) return atm_fail(c, ATM_ERR_ARGUMENT, "null coordinate array");
     const size_t n = c->g.L + 1;
     ATM_HIP(c, hipMemcpyAsync(c->ak, ak, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
@@ -245,6 +259,7 @@ static bool split_winds(const atm_ctx *c, int id) {
 
 static int xfer(atm_ctx *c, int id, void *dst, const void *src, bool to_dev, hipMemcpyKind kind, bool block) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     int nk = 0;
     double *d = field_ptr(c, id, &nk);
     if (!d) return atm_fail(c, ATM_ERR_ARGUMENT, "unknown field id " + std::to_string(id));
@@ -301,6 +316,7 @@ int atm_get_field_device(atm_ctx *c, int id, double *d) {
 // init_prognostic_fields (mod_input.f90:229-231)
 int atm_init_prognostic_fields(atm_ctx *c) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     if (!c->have_coord) return atm_fail(c, ATM_ERR_STATE, "atm_set_coordinate first");
     const size_t p = c->g.plane();
     int rc = atm_launch_fill(c, c->plev + p * c->g.L, p, c->ak0);
@@ -311,6 +327,7 @@ int atm_init_prognostic_fields(atm_ctx *c) {
 // initial_halo_exchange (mod_input.f90:247-252)
 int atm_initial_halo_exchange(atm_ctx *c) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     const int L = c->g.L;
     double *f[] = {c->dp, c->pt, c->gz, c->ud, c->vd};
     for (double *a : f) {
@@ -322,6 +339,7 @@ int atm_initial_halo_exchange(atm_ctx *c) {
 
 int atm_init_sw_dyn(atm_ctx *c) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     if (!c->have_coord) return atm_fail(c, ATM_ERR_STATE, "atm_set_coordinate first");
     int rc = atm_ref_init_sw_dyn(c);
     if (rc) return rc;
@@ -388,6 +406,7 @@ int atm_dgrid_halo(atm_ctx *c) {
 
 int atm_remap_init(atm_ctx *c) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     if (!c->have_coord) return atm_fail(c, ATM_ERR_STATE, "atm_set_coordinate first");
     int rc = atm_remap_alloc(c);
     if (rc) return rc;
@@ -416,6 +435,7 @@ int atm_remap(atm_ctx *c) {
 
 int atm_radiation_init(atm_ctx *c) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     int rc = atm_rad_alloc(c);
     if (rc) return rc;
     c->rad_init = true;
@@ -424,6 +444,7 @@ int atm_radiation_init(atm_ctx *c) {
 
 int atm_radiation(atm_ctx *c, double time) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     if (!c->rad_init) return atm_fail(c, ATM_ERR_STATE, "atm_radiation_init first");
     ATM_HIP(c, hipMemsetAsync(c->flag, 0, sizeof(int), c->stream));
     int rc = atm_rad_run(c, time);
@@ -443,6 +464,7 @@ int atm_radiation_halo(atm_ctx *c) {
 
 int atm_physics_init(atm_ctx *c) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     int rc = atm_phys_alloc(c);
     if (rc) return rc;
     c->phys_init = true;
@@ -469,6 +491,7 @@ int atm_physics_halo(atm_ctx *c) {
 
 int atm_output_init(atm_ctx *c) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     int rc = atm_output_alloc(c);
     if (rc) return rc;
     c->out_init = true;
@@ -484,6 +507,7 @@ int atm_accumulate_output(atm_ctx *c, double dt) {
 
 int atm_output_mean(atm_ctx *c, double *host) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     if (!c->out_init) return atm_fail(c, ATM_ERR_STATE, "atm_output_init first");
     if (!host) return atm_fail(c, ATM_ERR_ARGUMENT, "null output array");
     if (!(c->acc_time > 0.)) return atm_fail(c, ATM_ERR_STATE, "nothing accumulated");
@@ -505,6 +529,7 @@ int atm_halo_exchange(atm_ctx *c, int id) {
 // overwritten, the four fields are scanned.
 int atm_check_stability(atm_ctx *c, int *stable) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     if (!stable) return atm_fail(c, ATM_ERR_ARGUMENT, "null result pointer");
     if (c->nfw_tracked) {
         ATM_HIP(c, hipMemcpyAsync(c->flag_host, c->nfw, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -526,6 +551,7 @@ int atm_check_stability(atm_ctx *c, int *stable) {
 
 int atm_set_path(atm_ctx *c, int path) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     if (path != ATM_PATH_FUSED && path != ATM_PATH_REFERENCE) return atm_fail(c, ATM_ERR_ARGUMENT, "bad path");
     if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "finish the step (atm_dgrid_halo) first");
     c->plk_valid = false;
@@ -551,6 +577,7 @@ int atm_get_path(atm_ctx *c, int *path) {
 
 int atm_timer_enable(atm_ctx *c, int on) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     ATM_HIP(c, hipStreamSynchronize(c->stream));
     for (auto &t : c->timers) t.used = 0;
     c->timing = on != 0;
@@ -559,6 +586,7 @@ int atm_timer_enable(atm_ctx *c, int on) {
 
 int atm_timer_read(atm_ctx *c, int kernel, int64_t *launches, double *total_ms) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     if (kernel < 0 || kernel >= ATM_T_COUNT || !launches || !total_ms) return ATM_ERR_ARGUMENT;
     ATM_HIP(c, hipStreamSynchronize(c->stream));
     AtmTimer &t = c->timers[kernel];
@@ -575,7 +603,89 @@ int atm_timer_read(atm_ctx *c, int kernel, int64_t *launches, double *total_ms) 
 
 int atm_sync(atm_ctx *c) {
     CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
     ATM_HIP(c, hipStreamSynchronize(c->stream));
+    return ATM_OK;
+}
+
+// Graphs of whole steps.  Between atm_graph_begin and atm_graph_end the
+// step entry points (C and D steps, their halo fills, remap, physics
+// tendencies and their application, halo fills, output accumulation) are
+// recorded on the context's stream instead of run; atm_graph_launch runs
+// the recording.  The host-side state those calls advance (which copy of
+// the double-buffered fields is current, the pending D-step halo) must be
+// back where it started when the recording ends, so that every launch
+// finds the buffers the recording used: whole steps, and on the fused path
+// an even number of buffer switches (each D step and each remap switch).
+int atm_graph_begin(atm_ctx *c) {
+    CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
+    if (!c->sw_init) return atm_fail(c, ATM_ERR_STATE, "atm_init_sw_dyn first");
+    if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "a graph starts at a step boundary");
+    ATM_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->gexec) {
+        ATM_HIP(c, hipGraphExecDestroy(c->gexec));
+        c->gexec = nullptr;
+    }
+    c->cap_cur = c->cur;
+    c->cap_plk_valid = c->plk_valid;
+    c->cap_timing = c->timing;
+    c->timing = false;
+    ATM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    return ATM_OK;
+}
+
+int atm_graph_end(atm_ctx *c) {
+    CHECK_CTX(c);
+    if (!c->capturing) return atm_fail(c, ATM_ERR_STATE, "atm_graph_begin first");
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+    c->capturing = false;
+    c->timing = c->cap_timing;
+    // nothing recorded has run: a refused recording leaves the context as
+    // it was at atm_graph_begin; after an accepted one, playkap counts as
+    // valid only if it was so both before and after the recorded steps
+    const bool whole = !c->after_d && c->cur == c->cap_cur;
+    const bool plk_end = c->plk_valid;
+    c->after_d = false;
+    if (c->cur != c->cap_cur) {
+        c->cur = c->cap_cur;
+        atm_point_current(c);
+    }
+    c->plk_valid = c->cap_plk_valid && (whole ? plk_end : true);
+    if (e != hipSuccess || !gr) return atm_fail(c, ATM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    if (!whole) {
+        (void)hipGraphDestroy(gr);
+        return atm_fail(c, ATM_ERR_STATE,
+                        "a graph must hold whole steps with an even number of buffer switches (D steps + remaps)");
+    }
+    const hipError_t ei = hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (ei != hipSuccess) {
+        c->gexec = nullptr;
+        return atm_fail(c, ATM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    return ATM_OK;
+}
+
+int atm_graph_launch(atm_ctx *c, int times) {
+    CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
+    if (!c->gexec) return atm_fail(c, ATM_ERR_STATE, "no recorded graph");
+    if (times < 0) return ATM_ERR_ARGUMENT;
+    for (int i = 0; i < times; i++) ATM_HIP(c, hipGraphLaunch(c->gexec, c->stream));
+    return ATM_OK;
+}
+
+int atm_graph_free(atm_ctx *c) {
+    CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
+    if (c->gexec) {
+        ATM_HIP(c, hipStreamSynchronize(c->stream));
+        ATM_HIP(c, hipGraphExecDestroy(c->gexec));
+        c->gexec = nullptr;
+    }
     return ATM_OK;
 }
 

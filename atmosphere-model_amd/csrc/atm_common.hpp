@@ -325,6 +325,13 @@ struct atm_ctx {
     RemapWork *remap = nullptr;
     RadWork *rad = nullptr;
     PhysWork *phys = nullptr;
+    // HIP graph of a recorded step sequence (atm_graph_begin / end): while
+    // capturing, the entry points that wait on the device or write memory
+    // from the host refuse; cap_cur is the buffer parity at the start
+    bool capturing = false;
+    int cap_cur = 0;
+    bool cap_timing = false, cap_plk_valid = false;
+    hipGraphExec_t gexec = nullptr;
     // time-mean output accumulators (nx, ny, nlay+1, 6) and their time
     double *outf = nullptr;
     double acc_time = 0.;

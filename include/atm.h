@@ -184,6 +184,24 @@ int atm_timer_read(atm_ctx *ctx, int kernel, int64_t *launches, double *total_ms
 /* Wait for all work queued on the context. */
 int atm_sync(atm_ctx *ctx);
 
+/* Graphs of whole steps (no reference counterpart; for hosts whose steps
+   are short enough that launch overhead shows).  Between atm_graph_begin
+   and atm_graph_end the calls atm_cgrid_step, atm_cgrid_halo,
+   atm_dgrid_step, atm_dgrid_halo, atm_remap, atm_physics_tendencies,
+   atm_apply_physics, atm_physics_halo, atm_halo_exchange,
+   atm_radiation_halo and atm_accumulate_output are recorded instead of run;
+   every call that waits on the device or copies host memory returns
+   ATM_ERR_STATE meanwhile.  The recording must end where it started: at a
+   step boundary and, on the fused path, with the same current copy of the
+   double-buffered fields (an even number of D steps + remaps), else
+   atm_graph_end refuses it.  atm_graph_launch runs the recording `times`
+   times on the context's stream; the dt values are those recorded.  One
+   recording per context; a new atm_graph_begin replaces it. */
+int atm_graph_begin(atm_ctx *ctx);
+int atm_graph_end(atm_ctx *ctx);
+int atm_graph_launch(atm_ctx *ctx, int times);
+int atm_graph_free(atm_ctx *ctx);
+
 /* Library build information (kernel path, compile flags), for logs. */
 const char *atm_build_info(void);
 

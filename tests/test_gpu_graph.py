@@ -1,0 +1,89 @@
+"""Graphs of whole steps (atm_graph_begin / end / launch): every launch of a
+recorded 10-step cycle (two remaps) leaves the state an eager run of the
+same calls reaches, bit for bit, with eager work (checks, reads) between
+the launches; calls that wait on the device are refused while recording,
+and a recording that does not end where it started is refused."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DT = 60.0
+
+
+def _model(path, precision):
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+    nx, ny, nlay = 64, 32, 12
+    ic = make_initial_state(nx, ny, nlay, 100e3, seed=21)
+    m = Model(nx, ny, nlay, 100e3, precision=precision)
+    if m.path != path:
+        m.set_path(path)
+    cfg = Config(dt_max=DT, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12, dt_remap=1e12,
+                 radiation_on=False, physics_on=False)
+    driver.init_model(m, ic, cfg)
+    return m
+
+
+def _step(m, k):
+    m.cgrid_dynamics_step(DT / 2)
+    m.cgrid_halo_exchange()
+    m.dgrid_dynamics_step(DT)
+    if k % 5 == 4:
+        m.perform_vertical_remapping()
+    m.dgrid_halo_exchange()
+
+
+@pytest.mark.parametrize("path,precision", [("fused", "fp64"), ("fused", "mixed"), ("reference", "fp64")])
+def test_graph_launches_match_eager_steps(gpu_available, path, precision):
+    from atmos.model import AtmError
+    print(f"graph {path} {precision}", flush=True)
+    eager, rec = _model(path, precision), _model(path, precision)
+    rec.graph_begin()
+    for k in range(10):
+        _step(rec, k)
+    with pytest.raises(AtmError) as e:
+        rec.check_stability()
+    assert e.value.code == 5
+    rec.graph_end()
+    # nothing ran yet: the recorded model still holds its initial state
+    assert np.array_equal(rec.get("dp"), eager.get("dp"))
+    for r in range(3):
+        rec.graph_launch(1)
+        for k in range(10):
+            _step(eager, k)
+        assert rec.check_stability() and eager.check_stability()
+        for name in ("dp", "pt", "ud", "vd"):
+            assert np.array_equal(rec.get(name), eager.get(name)), (r, name)
+    # two launches in a row, then an eager step on the recorded model
+    rec.graph_launch(2)
+    for _ in range(2):
+        for k in range(10):
+            _step(eager, k)
+    _step(rec, 0)
+    _step(eager, 0)
+    for name in ("dp", "pt", "ud", "vd"):
+        assert np.array_equal(rec.get(name), eager.get(name)), name
+    rec.graph_free()
+
+
+def test_graph_must_end_where_it_started(gpu_available):
+    from atmos.model import AtmError
+    m = _model("fused", "fp64")
+    m.graph_begin()
+    _step(m, 0)  # one D step: the other copy is current
+    with pytest.raises(AtmError) as e:
+        m.graph_end()
+    assert e.value.code == 5
+    with pytest.raises(AtmError):
+        m.graph_launch(1)  # nothing recorded
+    # the refused recording had no effect: the context steps on from where
+    # it was
+    ref = _model("fused", "fp64")
+    _step(m, 1)
+    _step(ref, 1)
+    assert m.check_stability()
+    for name in ("dp", "pt", "ud", "vd"):
+        assert np.array_equal(m.get(name), ref.get(name)), name
