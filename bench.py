@@ -75,6 +75,9 @@ def parse():
                          "mixed precision, radiation every 10 steps")
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--ensemble-streams", type=int, default=4)
+    ap.add_argument("--graph", choices=["on", "off"], default="on",
+                    help="also time a recorded 10-step graph against the same steps issued one call at a "
+                         "time, at 64x64x20 (the CPU configuration's grid) and at the main grid")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of this many MiB (read + write) a few times, "
                          "to scale HBM counters (FETCH_SIZE/WRITE_SIZE) against a known size")
@@ -122,6 +125,56 @@ def ensemble_timing(args):
            "launch": f"each member's own launches, members dealt to {args.ensemble_streams} HIP streams"}
     ens.close()
     return out
+
+
+def graph_timing(args):
+    """Steps recorded as one graph (atm_graph_begin / end, 10 steps with two
+    remaps) against the same calls issued one at a time, FP64, fused path,
+    no per-step stability check in either (one check per 10 steps)."""
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+    dt = args.dt
+
+    def steps(m):
+        for k in range(10):
+            m.cgrid_dynamics_step(dt / 2)
+            m.cgrid_halo_exchange()
+            m.dgrid_dynamics_step(dt)
+            if k % 5 == 4:
+                m.perform_vertical_remapping()
+            m.dgrid_halo_exchange()
+
+    res = {}
+    for nx, ny, L, cycles in ((64, 64, 20, 100), (args.nx, args.ny, args.nlay, 10)):
+        ic = make_initial_state(nx, ny, L, args.dx)
+        out = {}
+        for mode in ("eager", "graph"):
+            m = Model(nx, ny, L, args.dx)
+            cfg = Config(dt_max=dt, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12,
+                         dt_remap=1e12, radiation_on=False, physics_on=False)
+            driver.init_model(m, ic, cfg)
+            if mode == "graph":
+                m.graph_begin()
+                steps(m)
+                m.graph_end()
+                run = lambda: m.graph_launch(1)  # noqa: E731
+            else:
+                run = lambda: steps(m)  # noqa: E731
+            run()
+            if not m.check_stability():
+                raise RuntimeError("graph timing: unstable")
+            t0 = time.perf_counter()
+            for _ in range(cycles):
+                run()
+                if not m.check_stability():
+                    raise RuntimeError("graph timing: unstable")
+            sec = time.perf_counter() - t0
+            out[f"{mode}_ms_per_step"] = sec / (10 * cycles) * 1e3
+            m.close()
+        res[f"{nx}x{ny}x{L}"] = out
+    return res
 
 
 def cpu_baseline(args):
@@ -335,6 +388,8 @@ def main():
         out["mixed"] = mixed_timing(args, ic, cfg_kw)
     if args.ensemble == "on" and args.path == "fused":
         out["ensemble"] = ensemble_timing(args)
+    if args.graph == "on" and args.path == "fused":
+        out["graph"] = graph_timing(args)
     if args.cpu_baseline == "on":
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)
