@@ -359,6 +359,12 @@ __device__ __forceinline__ void xcd_tile(int gx, int gy, int gz, int &bx, int &b
     b /= gx;
     by = b % gy;
     bz = b / gy;
+    // uniform over the workgroup: scalar registers for the tile origin and
+    // the level offset derived from them (the divisions above run on the
+    // vector unit, whose results the compiler cannot otherwise prove uniform)
+    bx = __builtin_amdgcn_readfirstlane(bx);
+    by = __builtin_amdgcn_readfirstlane(by);
+    bz = __builtin_amdgcn_readfirstlane(bz);
 }
 
 // launch helpers: 2-D tiles over I x J, z = level

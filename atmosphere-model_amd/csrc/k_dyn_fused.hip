@@ -87,6 +87,17 @@ struct Reg {
 #define TBASE(k) ((size_t)(k) * g.plane() + (size_t)(J0 - 4) * g.SI + (size_t)(I0 - 4))
 #define TOFF(x, y) ((unsigned)(((y) + 4) * g.SI + (x) + 4))
 
+// This thread's offset TOFF(tx, ty), formed afresh from an opaque copy of
+// the thread index: the compiler would otherwise hold the 64-bit offset of
+// the first use in registers through every later stencil stage, and the C
+// step's dp/pt kernel (the tightest in registers) spilled it.
+__device__ __forceinline__ unsigned own_off(const Geo &g, int tx_n) {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    const int tx = t % tx_n, ty = t / tx_n;
+    return (unsigned)((ty + 4) * g.SI + tx + 4);
+}
+
 // ---------------------------------------------------------------------------
 // C1 / D1: the dp and pt updates of one level (cgrid 94-268, dgrid 439-598).
 // CGRID: the C winds come from the D winds (109-126) and are also written
@@ -209,7 +220,7 @@ __global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo 
             double d = dpv - (D(fx1) - D(fx0) + c.dtdx * c.vx * (D(lxp) - 2. * D(l0) + D(lxm)));
             d = d - (D(fy1) - D(fy0) + c.dtdy * c.vy * (D(lyp) - 2. * D(l0) + D(lym)));
             dpcv = d;
-            G(dpo, tx, ty) = d;
+            (dpo + kb0)[CGRID ? own_off(g, TX) : TOFF(tx, ty)] = d;
             if (!CGRID) atm_report_nonfinite(nfw, d);
             __syncthreads();
             FOR_REG(RQ) Q(x, y) = G(pt, x, y); END_FOR
@@ -235,7 +246,7 @@ __global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo 
                       c.vy * (D((lyp - l0) * fy1) - D((l0 - lym) * fy0))) +
                  c.dt * dpv * hr) /
                 dpcv;
-            G(pto, tx, ty) = p;
+            (pto + kb0)[CGRID ? own_off(g, TX) : TOFF(tx, ty)] = p;
             if (!CGRID) atm_report_nonfinite(nfw, p);
         }
     }
@@ -250,8 +261,9 @@ __global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo 
             const double vb = .5 * (G(vw, tx, ty) + G(vw, tx, ty - 1));
             v = .5 * (vb + .5 * (G(vw, tx + 1, ty) + G(vw, tx + 1, ty - 1)));
         }
-        G(uco, tx, ty) = u;
-        G(vco, tx, ty) = v;
+        const unsigned o = own_off(g, TX);
+        (uco + kb0)[o] = u;
+        (vco + kb0)[o] = v;
         if (K == g.L - 1) {  // A winds of the last level (119-125), read by the KE pass
             const size_t p2 = g.ix2(I0 + tx, J0 + ty);
             ua[p2] = .5 * (u + .25 * (G(uw, tx + 1, ty) + G(uw, tx, ty) + G(uw, tx + 1, ty + 1) + G(uw, tx, ty + 1)));
