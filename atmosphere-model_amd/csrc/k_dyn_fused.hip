@@ -733,6 +733,8 @@ __global__ void __launch_bounds__(64, 4) k_scan_cols(Geo g, int i0, int j0, int 
     const int C = (g.L + SC_PARTS - 1) / SC_PARTS;
     const int k0 = min(q * C, g.L), n = min(C, g.L - k0);
     const size_t pl = g.plane(), c0 = g.ix2(col ? I : i0, J);
+    // the lane's first level; level k0 + u is u planes on (u pl is uniform)
+    const size_t xb = c0 + (size_t)k0 * pl;
     const bool interior = col && I >= ATM_HALO && I < ATM_HALO + g.nx && J >= ATM_HALO && J < ATM_HALO + g.ny;
     const bool want_heat = heat && interior, want_plk = plk_out && interior;
     double s[CMAX], pk[CMAX];
@@ -740,7 +742,7 @@ __global__ void __launch_bounds__(64, 4) k_scan_cols(Geo g, int i0, int j0, int 
     for (int u = 0; u < CMAX; u++) {
         s[u] = 0.;
         pk[u] = 1.;
-        if (u < n && col) s[u] = src[c0 + (size_t)(k0 + u) * pl];
+        if (u < n && col) s[u] = src[xb + (size_t)u * pl];
     }
     // p = ptop + the dp above and at each level, from the top chunk down
     double p = ptop;
@@ -761,7 +763,7 @@ __global__ void __launch_bounds__(64, 4) k_scan_cols(Geo g, int i0, int j0, int 
     }
 #pragma unroll
     for (int u = 0; u < CMAX; u++)
-        if (u < n && plev && col) plev[c0 + (size_t)(k0 + u) * pl] = pk[u];
+        if (u < n && plev && col) plev[xb + (size_t)u * pl] = pk[u];
     static_assert(CMAX % 4 == 0, "pow_kappa in groups of 4");
 #pragma unroll
     for (int u0 = 0; u0 < CMAX; u0 += 4) {  // levels past n hold 1 and stay unused
@@ -772,18 +774,18 @@ __global__ void __launch_bounds__(64, 4) k_scan_cols(Geo g, int i0, int j0, int 
     }
 #pragma unroll
     for (int u = 0; u < CMAX; u++)
-        if (u < n && pkap && col) pkap[c0 + (size_t)(k0 + u) * pl] = pk[u];
+        if (u < n && pkap && col) pkap[xb + (size_t)u * pl] = pk[u];
     // pkap of the level above the chunk: the next chunk's first, or the top
     double pk_up = __shfl_down(pk[0], SC_COLS);
     if (k0 + n >= g.L) pk_up = pktop;
     if (want_heat || want_plk) {
         double nfv[CMAX + 1];
 #pragma unroll
-        for (int u = 0; u <= CMAX; u++) nfv[u] = (want_heat && u <= n) ? nf[c0 + (size_t)(k0 + u) * pl] : 0.;
+        for (int u = 0; u <= CMAX; u++) nfv[u] = (want_heat && u <= n) ? nf[xb + (size_t)u * pl] : 0.;
 #pragma unroll
         for (int u = 0; u < CMAX; u++)
             if (u < n) {
-                const size_t x = c0 + (size_t)(k0 + u) * pl;
+                const size_t x = xb + (size_t)u * pl;
                 const double pk1 = u + 1 < n ? pk[u + 1] : pk_up;
                 const double plk = (pk1 - pk[u]) / log(pk1 / pk[u]);
                 if (want_heat) heat[x] = (nfv[u + 1] - nfv[u]) / (ATM_CP_DRY * s[u] / gravity) / plk;
@@ -798,7 +800,7 @@ __global__ void __launch_bounds__(64, 4) k_scan_cols(Geo g, int i0, int j0, int 
         tz[u] = 0.;
         if (u < n && col) {
             const double pkn = u + 1 < n ? pk[u + 1] : pk_up;
-            tz[u] = ATM_CP_DRY * th[c0 + (size_t)(k0 + u) * pl] * (pk[u] - pkn);
+            tz[u] = ATM_CP_DRY * th[xb + (size_t)u * pl] * (pk[u] - pkn);
         }
     }
     double z = col ? gz[c0] : 0.;
@@ -819,7 +821,7 @@ __global__ void __launch_bounds__(64, 4) k_scan_cols(Geo g, int i0, int j0, int 
     }
 #pragma unroll
     for (int u = 0; u < CMAX; u++)
-        if (u < n && col) gz[c0 + (size_t)(k0 + u + 1) * pl] = tz[u];
+        if (u < n && col) gz[xb + (size_t)(u + 1) * pl] = tz[u];
 }
 
 constexpr int SC_CMAX = 16;  // k_scan_cols for nlay <= 64
