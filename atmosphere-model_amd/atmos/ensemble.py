@@ -36,20 +36,26 @@ class Ensemble:
                  precision: str = "fp64", nstreams: int = 4, seed: int = 20261205, t_noise: float = 0.5):
         if nstreams < 1:
             raise ValueError("nstreams must be >= 1")
-        self.streams = [stream_create() for _ in range(nstreams)]
+        self.streams: list[int] = []
         self.members: list[Member] = []
-        i = 0
-        for count, nx, ny, nlay in specs:
-            for _ in range(count):
-                ic = make_initial_state(nx, ny, nlay, dx, seed=seed + i, t_noise=t_noise)
-                m = Model(nx, ny, nlay, dx, precision=precision)
-                m.set_stream(self.streams[i % nstreams])
-                cfg = Config(dt_max=dt, t_final=1e12, dt_output=1e12, dt_radiation=radiation_every * dt,
-                             dt_physics=1e12, dt_remap=remap_every * dt,
-                             radiation_on=radiation_every > 0, physics_on=False)
-                st = driver.init_model(m, ic, cfg)
-                self.members.append(Member(m, st, cfg, cfg.t_initial, nx * ny * nlay))
-                i += 1
+        try:
+            self.streams = [stream_create() for _ in range(nstreams)]
+            i = 0
+            for count, nx, ny, nlay in specs:
+                for _ in range(count):
+                    ic = make_initial_state(nx, ny, nlay, dx, seed=seed + i, t_noise=t_noise)
+                    m = Model(nx, ny, nlay, dx, precision=precision)
+                    self.members.append(Member(m, None, None, 0.0, nx * ny * nlay))
+                    m.set_stream(self.streams[i % nstreams])
+                    cfg = Config(dt_max=dt, t_final=1e12, dt_output=1e12, dt_radiation=radiation_every * dt,
+                                 dt_physics=1e12, dt_remap=remap_every * dt,
+                                 radiation_on=radiation_every > 0, physics_on=False)
+                    st = driver.init_model(m, ic, cfg)
+                    self.members[-1] = Member(m, st, cfg, cfg.t_initial, nx * ny * nlay)
+                    i += 1
+        except BaseException:
+            self.close()
+            raise
 
     @property
     def cells(self) -> int:
@@ -70,10 +76,10 @@ class Ensemble:
             mb.model.sync()
 
     def close(self):
-        """Release the members' contexts, then the streams they ran on."""
-        self.sync()
+        """Release the members' contexts, then the streams they ran on
+        (closing twice is harmless)."""
         for mb in self.members:
-            mb.model.close()
+            mb.model.close()  # waits for the member's stream
         self.members.clear()
         for s in self.streams:
             stream_destroy(s)

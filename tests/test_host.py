@@ -40,6 +40,18 @@ def test_no_gpu_create_fails_loudly():
         Model(16, 16, 8, 1e5)
 
 
+def test_no_gpu_ensemble_fails_loudly():
+    """Without a device the ensemble's first stream already fails; nothing is
+    left half built."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from atmos.ensemble import Ensemble
+    from atmos.model import AtmError
+    with pytest.raises(AtmError):
+        Ensemble([(2, 32, 32, 8)], 100e3, 60.0)
+
+
 @pytest.mark.parametrize("text,sec", [("PT20S", 20), ("PT2M", 120), ("PT3H", 10800), ("P5D", 432000),
                                       ("PT100S", 100), ("P1Y2M3DT4H5M6S", 31536000 + 2 * 2592000 + 3 * 86400 + 4 * 3600 + 5 * 60 + 6),
                                       ("P1W", 604800), ("PT1.5H", 5400)])
