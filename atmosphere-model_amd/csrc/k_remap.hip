@@ -1,7 +1,9 @@
 // k_remap.hip -- vertical Lagrangian -> hybrid sigma-p remap (gfx950).
 //
-// src/mod_remap.f90:27-163.  One thread per column, columns flattened so
-// every wavefront is full; each remap_column (165-285) runs in two passes:
+// src/mod_remap.f90:27-163.  One thread per column (the u and v columns of
+// a point on separate threads), columns flattened so every wavefront is
+// full; the per-level energy inputs come from a kernel with one thread per
+// cell.  Each remap_column (165-285) runs in two passes:
 //
 //   forward   the elimination sweep of vertical_ppm (287-356) from k = nlay
 //             down to 2, the only part that runs against the final order;
@@ -63,7 +65,7 @@ struct WindCol {
 };
 
 // Total energy (111-150): coordinates are plev / target_plev of the column,
-// the layer values are formed once into the scratch's third slot.
+// the layer values are those k_remap_energy_in left in the scratch's slot 2.
 struct EnergyCol {
     const double *plev, *tplev, *var;
     size_t base, plane, ncol;
