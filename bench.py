@@ -281,11 +281,22 @@ def mixed_timing(args, ic, cfg_kw):
     a, b = res["mixed"], res["fp64"]
     err = {k: rel(a[k], b[k]) for k in ("ps", "T", "dp", "pt", "ud", "vd")}
     err["mass_drift"] = {"mixed": a["mass_drift"], "fp64": b["mass_drift"]}
+    # radiation alone on the same (initial) state: net flux of the mixed
+    # form (float g-point optics) against the FP64 solve
+    nf = {}
+    for prec in ("fp64", "mixed"):
+        mm = Model(nx, ny, L, args.dx, precision=prec)
+        driver.init_model(mm, ic, Config(**cfg_kw))
+        mm.calculate_radiative_heating(0.0)
+        nf[prec] = mm.get("net_flux")
+        del mm
+    rad_err = rel(nf["mixed"], nf["fp64"])
     value = nx * ny * L * args.steps / sec
     return {"value": value, "unit": "cell-steps/s", "ms_per_step": sec / args.steps * 1e3,
             "dtype": "f32 reconstruction and fluxes, f64 updates, scans, pressure gradient, remap",
             "kernels_ms_per_step": kernels, "error_vs_fp64": {"steps": args.mixed_error_steps, **err},
-            "radiation_ms_per_call": rad["ms_per_call"] if rad else None}
+            "radiation_ms_per_call": rad["ms_per_call"] if rad else None,
+            "radiation_net_flux_error_vs_fp64": rad_err}
 
 
 def radiation_timing(m, args, time_s):
