@@ -556,7 +556,11 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
 // interpolate_to_corners (921-956) is separable: each corner combines four
 // x-direction 4-point averages (rows j-2..j+1) and four y-direction ones
 // (columns i-2..i+1).  A TX x TY tile stages the field, forms each average
-// once in LDS and combines them in the reference's order.
+// once in LDS and combines them in the reference's order.  A block walks
+// BC_LEVELS levels and loads the next level's tile into registers while the
+// current one is combined, so each CU keeps loads in flight through the
+// LDS stages.
+#define BC_LEVELS 4
 __global__ void __launch_bounds__(NT_WIND) k_bcorners(Geo g, const double *__restrict__ pkap,
                                                  const double *__restrict__ gz, double *__restrict__ pkapb,
                                                  double *__restrict__ gzb) {
@@ -564,44 +568,68 @@ __global__ void __launch_bounds__(NT_WIND) k_bcorners(Geo g, const double *__res
     using RQ = Reg<-2, TX + 1, -2, TY + 1>;  // field
     using RAX = Reg<0, TX, -2, TY + 1>;      // x averages
     using RAY = Reg<-2, TX + 1, 0, TY>;      // y averages
+    constexpr int NR = (RQ::n + NT - 1) / NT;  // staged points per thread and field
     __shared__ double sQ[2][RQ::n], sAX[2][RAX::n], sAY[2][RAY::n];
-    const int I0 = ATM_HALO + blockIdx.x * TX, J0 = ATM_HALO + blockIdx.y * TY, K = blockIdx.z;
-    const double *src[2] = {pkap + g.ix(0, 0, K), gz + g.ix(0, 0, K + 1)};
-    // the last tile row / column holds one valid output row / column; loads
-    // beyond the halo'd plane are not used by it and are skipped
-    for (int f = 0; f < 2; f++) {
-        FOR_REG(RQ)
-        const int I = I0 + x, J = J0 + y;
-        sQ[f][RQ::at(x, y)] = (I < g.NI && J < g.NJ) ? src[f][g.ix2(I, J)] : 0.;
-        END_FOR
+    const int I0 = ATM_HALO + blockIdx.x * TX, J0 = ATM_HALO + blockIdx.y * TY;
+    const int K0 = blockIdx.z * BC_LEVELS, K1 = min(K0 + BC_LEVELS, g.L);
+    // this thread's staged points: plane offsets, or -1 past the halo'd
+    // plane (the last tile row / column holds one valid output row / column
+    // and does not use them)
+    long off[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int t = threadIdx.x + r * NT;
+        const int I = I0 + RQ::x0 + t % RQ::w, J = J0 + RQ::y0 + t / RQ::w;
+        off[r] = (t < RQ::n && I < g.NI && J < g.NJ) ? (long)g.ix2(I, J) : -1;
     }
-    __syncthreads();
-#define Q(f, x, y) sQ[f][RQ::at(x, y)]
-    for (int f = 0; f < 2; f++) {
-        FOR_REG(RAX)
-        sAX[f][RAX::at(x, y)] = ATM_W7_12 * (Q(f, x, y) + Q(f, x - 1, y)) - ATM_W1_12 * (Q(f, x + 1, y) + Q(f, x - 2, y));
-        END_FOR
-        FOR_REG(RAY)
-        sAY[f][RAY::at(x, y)] = ATM_W7_12 * (Q(f, x, y) + Q(f, x, y - 1)) - ATM_W1_12 * (Q(f, x, y + 1) + Q(f, x, y - 2));
-        END_FOR
-    }
-#undef Q
-    __syncthreads();
+    double q[2][NR];
+    auto load = [&](int K) {
+        const double *src[2] = {pkap + g.ix(0, 0, K), gz + g.ix(0, 0, K + 1)};
+#pragma unroll
+        for (int f = 0; f < 2; f++)
+#pragma unroll
+            for (int r = 0; r < NR; r++) q[f][r] = off[r] >= 0 ? src[f][off[r]] : 0.;
+    };
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     const int I = I0 + tx, J = J0 + ty;
-    if (I > ATM_HALO + g.nx || J > ATM_HALO + g.ny) return;
-    double out[2];
-    for (int f = 0; f < 2; f++) {
+    const bool writes = I <= ATM_HALO + g.nx && J <= ATM_HALO + g.ny;
+    load(K0);
+    for (int K = K0; K < K1; K++) {
+#pragma unroll
+        for (int f = 0; f < 2; f++)
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int t = threadIdx.x + r * NT;
+                if (t < RQ::n) sQ[f][t] = q[f][r];
+            }
+        __syncthreads();
+        if (K + 1 < K1) load(K + 1);
+#define Q(f, x, y) sQ[f][RQ::at(x, y)]
+        for (int f = 0; f < 2; f++) {
+            FOR_REG(RAX)
+            sAX[f][RAX::at(x, y)] = ATM_W7_12 * (Q(f, x, y) + Q(f, x - 1, y)) - ATM_W1_12 * (Q(f, x + 1, y) + Q(f, x - 2, y));
+            END_FOR
+            FOR_REG(RAY)
+            sAY[f][RAY::at(x, y)] = ATM_W7_12 * (Q(f, x, y) + Q(f, x, y - 1)) - ATM_W1_12 * (Q(f, x, y + 1) + Q(f, x, y - 2));
+            END_FOR
+        }
+#undef Q
+        __syncthreads();
+        if (writes) {
+            double out[2];
+            for (int f = 0; f < 2; f++) {
 #define AX(b) sAX[f][RAX::at(tx, ty + (b))]
 #define AY(a) sAY[f][RAY::at(tx + (a), ty)]
-        double o = (9. / 32.) * (AX(0) + AX(-1)) - (1. / 32.) * (AX(1) + AX(-2));
-        o = o + (9. / 32.) * (AY(0) + AY(-1)) - (1. / 32.) * (AY(1) + AY(-2));
-        out[f] = o;
+                double o = (9. / 32.) * (AX(0) + AX(-1)) - (1. / 32.) * (AX(1) + AX(-2));
+                o = o + (9. / 32.) * (AY(0) + AY(-1)) - (1. / 32.) * (AY(1) + AY(-2));
+                out[f] = o;
 #undef AX
 #undef AY
+            }
+            pkapb[g.ix(I, J, K)] = out[0];
+            gzb[g.ix(I, J, K + 1)] = out[1];
+        }
     }
-    pkapb[g.ix(I, J, K)] = out[0];
-    gzb[g.ix(I, J, K + 1)] = out[1];
 }
 
 // ---------------------------------------------------------------------------
@@ -942,7 +970,8 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
     atm_timer_begin(c, ATM_T_SCAN);
     launch_scans(c, 0, 0, g.NI, g.NJ, k.gravity, nxt.dp, nxt.pt, c->net_flux, c->plev, c->pkap, c->gz, nullptr,
                  c->plk);
-    hipLaunchKernelGGL(k_bcorners, dim3((g.nx + TX) / TX, (g.ny + TY_WIND) / TY_WIND, g.L), blkw, 0, c->stream, g, c->pkap,
+    hipLaunchKernelGGL(k_bcorners,
+                       dim3((g.nx + TX) / TX, (g.ny + TY_WIND) / TY_WIND, (g.L + BC_LEVELS - 1) / BC_LEVELS), blkw, 0, c->stream, g, c->pkap,
                        c->gz, c->pkapb, c->gzb);
     atm_timer_end(c, ATM_T_SCAN);
     atm_timer_begin(c, ATM_T_D2);
