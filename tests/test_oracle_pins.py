@@ -5,6 +5,8 @@
   reference is present);
 * halo fill: the periodic closed form;
 * adding method: brute-force solution of the two-stream linear system;
+* longwave column: the formal solution of the two-stream equations for B
+  linear in optical depth, integrated by quadrature;
 * vertical PPM: the tridiagonal stage against scipy's banded solver and its
   exactness for linear profiles on non-uniform grids;
 * remap_column: identity, exact overlap integrals (40-digit) with the
@@ -112,6 +114,68 @@ def test_adding_method_brute_force():
         x = np.linalg.solve(A, b)
         np.testing.assert_allclose(fu, x[:n], rtol=1e-12, atol=1e-12)
         np.testing.assert_allclose(fd, x[n:], rtol=1e-12, atol=1e-12)
+
+
+def _lw_formal_solution(g, play, dm, Tl, Tlev, ts, pi=float(np.float32(np.pi)), linear=True):
+    """Longwave fluxes of a non-scattering column from the formal solution
+    of the hemispheric two-stream equations (diffusivity 2), dF/dt = -+2 (F -
+    pi B), with B linear in optical depth across each layer between its
+    level values, each layer's source integral evaluated by quadrature.
+    Surface: emissivity and albedo 1 - emissivity at the reference's binary32
+    0.99 (mod_radiation.f90:291-347); level 0 is the surface."""
+    from scipy.integrate import quad
+    eps = float(np.float32(0.99))
+    L = len(play)
+    tau = np.array([O.lw_abs(g, play[k], Tl[k])[0] * dm[k] for k in range(L)])
+    B = np.array([O.planck(g, t)[0] for t in Tlev])
+    Bs = O.planck(g, ts)[0]
+
+    def layer(k, toward):  # source integral; s = depth below the top of the layer
+        d, Bt, Bb = tau[k], B[k + 1], B[k]
+        Bf = (lambda s: Bt + (Bb - Bt) * s / d) if linear else (lambda s: 0.5 * (Bt + Bb))
+        w = (lambda s: np.exp(-2 * (d - s))) if toward == "down" else (lambda s: np.exp(-2 * s))
+        return quad(lambda s: 2 * pi * Bf(s) * w(s), 0.0, d, epsabs=0, epsrel=1e-13, limit=200)[0]
+
+    fd, fu = np.zeros(L + 1), np.zeros(L + 1)
+    for k in range(L - 1, -1, -1):
+        fd[k] = fd[k + 1] * np.exp(-2 * tau[k]) + layer(k, "down")
+    fu[0] = eps * pi * Bs + (1 - eps) * fd[0]
+    for k in range(L):
+        fu[k + 1] = fu[k] * np.exp(-2 * tau[k]) + layer(k, "up")
+    return fu, fd
+
+
+def test_longwave_column_vs_formal_solution():
+    """solve_longwave_column (mod_radiation.f90:270-347) end to end: with no
+    scattering its layer R = 0, T = e^-2tau and its linear-in-tau sources are
+    the exact solution of the two-stream equations for B linear in tau, so the
+    column's fluxes equal the formal-solution integrals, to rounding, for any
+    in-range column (gas-optics lookups and Planck values are pinned above).
+    The tolerance is 1e-12 norm-relative; measured 3.5e-14 over these cases.
+    pi at full precision instead of the binary32 pi (Q1), or B constant
+    across each layer, misses it by orders of magnitude."""
+    rng = np.random.default_rng(5)
+    L = 24
+    plev = np.exp(np.linspace(np.log(1e5), np.log(2e3), L + 1))
+    play = np.sqrt(plev[:-1] * plev[1:])
+    dm = (plev[:-1] - plev[1:]) / 9.80665
+    tmean = lambda p: np.maximum(280.0 * (p / 1e5) ** 0.19, 208.5)
+    for trial in range(2):
+        Tlev = tmean(plev) + rng.normal(0, 2.0, L + 1)
+        Tl = 0.5 * (Tlev[:-1] + Tlev[1:]) + rng.normal(0, 1.0, L)
+        ts = Tlev[0] + 1.5
+        for g in range(1, 17):
+            fu, fd, err = O.lw_column(g, play, dm, Tl, Tlev, ts)
+            assert err == 0, (trial, g)
+            iu, idn = _lw_formal_solution(g, play, dm, Tl, Tlev, ts)
+            scale = max(np.max(np.abs(iu)), np.max(np.abs(idn)))
+            rel = max(np.max(np.abs(fu - iu)), np.max(np.abs(fd - idn))) / scale
+            assert rel < 1e-12, (trial, g, rel)
+            if trial == 0 and g in (1, 16):  # thin (tau ~1e-5..1e-2) and thick (tau ~2..6) g-points
+                for kw in (dict(pi=np.pi), dict(linear=False)):
+                    wu, wd = _lw_formal_solution(g, play, dm, Tl, Tlev, ts, **kw)
+                    wrong = max(np.max(np.abs(fu - wu)), np.max(np.abs(fd - wd))) / scale
+                    assert wrong > 1e-9, (g, kw, wrong)
 
 
 def _random_grid(rng, L):
