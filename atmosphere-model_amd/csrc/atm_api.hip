@@ -627,11 +627,11 @@ int atm_graph_begin(atm_ctx *c) {
         ATM_HIP(c, hipGraphExecDestroy(c->gexec));
         c->gexec = nullptr;
     }
+    ATM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->cap_cur = c->cur;
     c->cap_plk_valid = c->plk_valid;
     c->cap_timing = c->timing;
-    c->timing = false;
-    ATM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->timing = false;  // timer events are not recorded into the graph
     c->capturing = true;
     return ATM_OK;
 }
@@ -673,7 +673,7 @@ int atm_graph_launch(atm_ctx *c, int times) {
     CHECK_CTX(c);
     CHECK_NOT_CAPTURING(c);
     if (!c->gexec) return atm_fail(c, ATM_ERR_STATE, "no recorded graph");
-    if (times < 0) return ATM_ERR_ARGUMENT;
+    if (times < 0) return atm_fail(c, ATM_ERR_ARGUMENT, "times must be >= 0");
     for (int i = 0; i < times; i++) ATM_HIP(c, hipGraphLaunch(c->gexec, c->stream));
     return ATM_OK;
 }

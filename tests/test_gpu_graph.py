@@ -58,6 +58,10 @@ def test_graph_launches_match_eager_steps(gpu_available, path, precision):
         for name in ("dp", "pt", "ud", "vd"):
             assert np.array_equal(rec.get(name), eager.get(name)), (r, name)
     # two launches in a row, then an eager step on the recorded model
+    with pytest.raises(AtmError) as e:
+        rec.graph_launch(-1)
+    assert e.value.code == 1 and "times" in str(e.value)
+    rec.graph_launch(0)  # nothing runs
     rec.graph_launch(2)
     for _ in range(2):
         for k in range(10):
