@@ -204,9 +204,16 @@ __device__ __forceinline__ void ppm_edges(T qm2, T qm1, T q0, T qp1, T qp2, T &e
             eR = dmax2(eR, -q0);
             T a = eR - eL;
             T b = T(-3) * (eR + eL);
+            // The test below holds only where q0 <= |b| / 3: |a| < -b makes
+            // a^2 / (4b) > b / 4.  Where q0 > 0.34 |b| the computed sum is
+            // >= 0 in either precision (0.34 - 1/4 - 1/12 leaves far more
+            // than the rounding of three operations), so those cells (almost
+            // all of a smooth dp or pt field) skip the division; a NaN fails
+            // this pre-test and takes the full test as before.
             // b / 12 as div3(b / 4): the quarter is exact, so this is the
             // correctly rounded b / 12 without a division sequence
-            if ((T(fabs(a)) < -b) && (q0 + a * a / (T(4) * b) + div3(b * T(0.25)) < T(0))) {
+            if (!(q0 > T(0.34) * T(fabs(b))) && (T(fabs(a)) < -b) &&
+                (q0 + a * a / (T(4) * b) + div3(b * T(0.25)) < T(0))) {
                 if (eL * eR > T(0)) {
                     eL = T(0);
                     eR = T(0);
