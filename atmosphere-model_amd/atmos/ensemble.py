@@ -30,10 +30,13 @@ class Member:
 
 
 class Ensemble:
-    """specs: (count, nx, ny, nlay) groups.  Member i is seeded with seed + i."""
+    """specs: (count, nx, ny, nlay) groups.  Member i is seeded with seed + i.
+    autotune: each member times the fused path's launch shapes on its own
+    grid at construction (results do not depend on the choice)."""
 
     def __init__(self, specs, dx: float, dt: float, remap_every: int = 5, radiation_every: int = 10,
-                 precision: str = "fp64", nstreams: int = 4, seed: int = 20261205, t_noise: float = 0.5):
+                 precision: str = "fp64", nstreams: int = 4, seed: int = 20261205, t_noise: float = 0.5,
+                 autotune: bool = False):
         if nstreams < 1:
             raise ValueError("nstreams must be >= 1")
         self.streams: list[int] = []
@@ -51,6 +54,8 @@ class Ensemble:
                                  dt_physics=1e12, dt_remap=remap_every * dt,
                                  radiation_on=radiation_every > 0, physics_on=False)
                     st = driver.init_model(m, ic, cfg)
+                    if autotune:  # launch shapes for this member's own grid
+                        m.autotune()
                     self.members[-1] = Member(m, st, cfg, cfg.t_initial, nx * ny * nlay)
                     i += 1
         except BaseException:

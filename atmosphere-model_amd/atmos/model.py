@@ -29,7 +29,8 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_build_info", "atm_set_path", "atm_get_path", "atm_timer_enable", "atm_timer_read",
            "atm_physics_init", "atm_physics_tendencies", "atm_apply_physics", "atm_physics_halo",
            "atm_output_init", "atm_accumulate_output", "atm_output_mean", "atm_stream_create",
-           "atm_stream_destroy", "atm_graph_begin", "atm_graph_end", "atm_graph_launch", "atm_graph_free"]
+           "atm_stream_destroy", "atm_graph_begin", "atm_graph_end", "atm_graph_launch", "atm_graph_free",
+           "atm_autotune", "atm_get_tuning", "atm_set_tuning"]
 PATHS = {"fused": 0, "reference": 1}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 TIMERS = {"c1": 0, "c2": 1, "d1": 2, "d2": 3, "scan": 4, "halo": 5, "remap": 6, "radiation": 7,
@@ -94,6 +95,9 @@ def load_library(path: str = LIB_PATH):
     L.atm_get_path.argtypes = [vp, C.POINTER(C.c_int)]
     L.atm_timer_enable.argtypes = [vp, ip]
     L.atm_timer_read.argtypes = [vp, ip, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    L.atm_autotune.argtypes = [vp, ip]
+    L.atm_get_tuning.argtypes = [vp, C.POINTER(C.c_int)]
+    L.atm_set_tuning.argtypes = [vp, ip]
     _LIB = L
     return L
 
@@ -267,6 +271,22 @@ class Model:
 
     def set_path(self, name):
         self._ck(self._L.atm_set_path(self._h, PATHS[name]))
+
+    def autotune(self, reps=5):
+        """Time the fused path's candidate launch shapes on this grid and
+        keep the fastest (results do not depend on the choice); returns the
+        chosen corner-kernel levels per workgroup."""
+        self._ck(self._L.atm_autotune(self._h, int(reps)))
+        return self.tuning
+
+    @property
+    def tuning(self):
+        v = C.c_int(0)
+        self._ck(self._L.atm_get_tuning(self._h, C.byref(v)))
+        return v.value
+
+    def set_tuning(self, corner_levels):
+        self._ck(self._L.atm_set_tuning(self._h, int(corner_levels)))
 
     def timer_enable(self, on=True):
         self._ck(self._L.atm_timer_enable(self._h, int(bool(on))))

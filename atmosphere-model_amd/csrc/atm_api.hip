@@ -575,6 +575,30 @@ int atm_get_path(atm_ctx *c, int *path) {
     return ATM_OK;
 }
 
+int atm_autotune(atm_ctx *c, int reps) {
+    CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
+    if (reps < 1) return atm_fail(c, ATM_ERR_ARGUMENT, "reps must be >= 1");
+    if (!c->fused) return ATM_OK;
+    return atm_fused_autotune(c, reps);
+}
+
+int atm_get_tuning(atm_ctx *c, int *corner_levels) {
+    CHECK_CTX(c);
+    if (!corner_levels) return atm_fail(c, ATM_ERR_ARGUMENT, "null output");
+    *corner_levels = c->bc_levels;
+    return ATM_OK;
+}
+
+int atm_set_tuning(atm_ctx *c, int corner_levels) {
+    CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
+    if (corner_levels != 1 && corner_levels != 2 && corner_levels != 4 && corner_levels != 8)
+        return atm_fail(c, ATM_ERR_ARGUMENT, "corner levels per workgroup: 1, 2, 4 or 8");
+    c->bc_levels = corner_levels;
+    return ATM_OK;
+}
+
 int atm_timer_enable(atm_ctx *c, int on) {
     CHECK_CTX(c);
     CHECK_NOT_CAPTURING(c);

@@ -299,6 +299,7 @@ struct atm_ctx {
     int cur = 0;
     bool after_d = false;  // a fused D step ran and its halo fill is pending
     double *uc2 = nullptr, *vc2 = nullptr, *heatd = nullptr;  // fused-path C-step outputs
+    int bc_levels = 4;  // fused path: levels per block of the corner kernel (atm_autotune)
     // fused path: playkap of the current dp on the interior (the C step's
     // heating rate needs it); left by the D step's scan, recomputed by the
     // C step when dp changed since (remap, put)

@@ -17,3 +17,4 @@ bool atm_fused_supported(const Geo &g);
 int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt);
 int atm_fused_cgrid_halo(atm_ctx *c);
 int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt);
+int atm_fused_autotune(atm_ctx *c, int reps);

@@ -557,10 +557,11 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
 // x-direction 4-point averages (rows j-2..j+1) and four y-direction ones
 // (columns i-2..i+1).  A TX x TY tile stages the field, forms each average
 // once in LDS and combines them in the reference's order.  A block walks
-// BC_LEVELS levels and loads the next level's tile into registers while the
+// NLEV levels and loads the next level's tile into registers while the
 // current one is combined, so each CU keeps loads in flight through the
-// LDS stages.
-#define BC_LEVELS 4
+// LDS stages; fewer levels per block give small grids more blocks.  NLEV
+// is the context's tuning (atm_autotune); every choice gives the same values.
+template <int NLEV>
 __global__ void __launch_bounds__(NT_WIND) k_bcorners(Geo g, const double *__restrict__ pkap,
                                                  const double *__restrict__ gz, double *__restrict__ pkapb,
                                                  double *__restrict__ gzb) {
@@ -571,7 +572,7 @@ __global__ void __launch_bounds__(NT_WIND) k_bcorners(Geo g, const double *__res
     constexpr int NR = (RQ::n + NT - 1) / NT;  // staged points per thread and field
     __shared__ double sQ[2][RQ::n], sAX[2][RAX::n], sAY[2][RAY::n];
     const int I0 = ATM_HALO + blockIdx.x * TX, J0 = ATM_HALO + blockIdx.y * TY;
-    const int K0 = blockIdx.z * BC_LEVELS, K1 = min(K0 + BC_LEVELS, g.L);
+    const int K0 = blockIdx.z * NLEV, K1 = min(K0 + NLEV, g.L);
     // this thread's staged points: plane offsets, or -1 past the halo'd
     // plane (the last tile row / column holds one valid output row / column
     // and does not use them)
@@ -629,6 +630,21 @@ __global__ void __launch_bounds__(NT_WIND) k_bcorners(Geo g, const double *__res
             pkapb[g.ix(I, J, K)] = out[0];
             gzb[g.ix(I, J, K + 1)] = out[1];
         }
+    }
+}
+
+// k_bcorners with n levels per block (1, 2, 4 or 8)
+void launch_bcorners(atm_ctx *c, int n) {
+    const Geo &g = c->g;
+    const dim3 blk(NT_WIND), grid((g.nx + TX) / TX, (g.ny + TY_WIND) / TY_WIND, (g.L + n - 1) / n);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, grid, blk, 0, c->stream, g, c->pkap, c->gz, c->pkapb, c->gzb);
+    };
+    switch (n) {
+    case 1: go(k_bcorners<1>); break;
+    case 2: go(k_bcorners<2>); break;
+    case 8: go(k_bcorners<8>); break;
+    default: go(k_bcorners<4>); break;
     }
 }
 
@@ -875,6 +891,46 @@ void launch_scans(atm_ctx *c, int i0, int j0, int ni, int nj, double gravity, co
 
 }  // namespace
 
+// The tuning of the fused path: each candidate of a kernel's launch shape
+// is timed on this context's own grid and stream (one warm-up launch, then
+// `reps` launches between two events), and the fastest is kept.  Only
+// shapes that give the same values are candidates, so tuning never changes
+// results.  Today: the corner kernel's levels per block (1, 2, 4, 8).
+int atm_fused_autotune(atm_ctx *c, int reps) {
+    hipEvent_t e0, e1;
+    ATM_HIP(c, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return atm_fail(c, ATM_ERR_HIP, "hipEventCreate");
+    }
+    int best = c->bc_levels, rc = ATM_OK;
+    float best_ms = 0.f;
+    for (int n : {1, 2, 4, 8}) {
+        launch_bcorners(c, n);
+        hipError_t e = hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps && e == hipSuccess; r++) {
+            launch_bcorners(c, n);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+            rc = atm_fail(c, ATM_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+            break;
+        }
+        if (n == 1 || ms < best_ms) {
+            best = n;
+            best_ms = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == ATM_OK) c->bc_levels = best;
+    return rc;
+}
+
 bool atm_fused_supported(const Geo &g) { return g.nx % TX == 0 && g.ny % TY_PAIR == 0 && g.ny % TY_WIND == 0; }
 
 // ---------------------------------------------------------------------------
@@ -970,9 +1026,7 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
     atm_timer_begin(c, ATM_T_SCAN);
     launch_scans(c, 0, 0, g.NI, g.NJ, k.gravity, nxt.dp, nxt.pt, c->net_flux, c->plev, c->pkap, c->gz, nullptr,
                  c->plk);
-    hipLaunchKernelGGL(k_bcorners,
-                       dim3((g.nx + TX) / TX, (g.ny + TY_WIND) / TY_WIND, (g.L + BC_LEVELS - 1) / BC_LEVELS), blkw, 0, c->stream, g, c->pkap,
-                       c->gz, c->pkapb, c->gzb);
+    launch_bcorners(c, c->bc_levels);
     atm_timer_end(c, ATM_T_SCAN);
     atm_timer_begin(c, ATM_T_D2);
     if (c->mixed)

@@ -75,6 +75,9 @@ def parse():
                          "mixed precision, radiation every 10 steps")
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--ensemble-streams", type=int, default=4)
+    ap.add_argument("--autotune", choices=["on", "off"], default="off",
+                    help="time the fused path's launch-shape candidates on each grid first (results unchanged; "
+                         "off by default: isolated timings have picked shapes slower inside a step)")
     ap.add_argument("--graph", choices=["on", "off"], default="on",
                     help="also time a recorded 10-step graph against the same steps issued one call at a "
                          "time, at 64x64x20 (the CPU configuration's grid) and at the main grid")
@@ -106,7 +109,7 @@ def ensemble_timing(args):
     check of every member), timed after 5 warm-up steps."""
     from atmos.ensemble import Ensemble
     ens = Ensemble(ENSEMBLE_SPECS, args.dx, args.dt, remap_every=5, radiation_every=10, precision="mixed",
-                   nstreams=args.ensemble_streams)
+                   nstreams=args.ensemble_streams, autotune=args.autotune == "on")
     for _ in range(5):
         if not ens.step():
             raise RuntimeError("ensemble member unstable in warm-up")
@@ -122,7 +125,10 @@ def ensemble_timing(args):
            "precision": "mixed", "steps": args.ensemble_steps, "remap_every": 5, "radiation_every": 10,
            "value": cells * args.ensemble_steps / sec, "unit": "cell-steps/s",
            "ms_per_step": sec / args.ensemble_steps * 1e3,
-           "launch": f"each member's own launches, members dealt to {args.ensemble_streams} HIP streams"}
+           "launch": f"each member's own launches, members dealt to {args.ensemble_streams} HIP streams",
+           "corner_levels_per_workgroup": {f"{nx}x{ny}x{L}": sorted({mb.model.tuning for mb in ens.members
+                                                                     if mb.cells == nx * ny * L})
+                                           for _, nx, ny, L in ENSEMBLE_SPECS}}
     ens.close()
     return out
 
@@ -238,6 +244,8 @@ def mixed_timing(args, ic, cfg_kw):
     nx, ny, L = args.nx, args.ny, args.nlay
     m = Model(nx, ny, L, args.dx, precision="mixed")
     st = driver.init_model(m, ic, Config(**cfg_kw))
+    if args.autotune == "on":
+        m.autotune()
     cfg = Config(**cfg_kw)
     t = 0.0
     for _ in range(args.warmup):
@@ -348,6 +356,7 @@ def main():
     if args.path != m.path:
         m.set_path(args.path)
     st = driver.init_model(m, ic, cfg)
+    tuning = m.autotune() if args.autotune == "on" else m.tuning
     time_s = 0.0
     for _ in range(args.warmup):
         time_s += driver.step(m, cfg, st, time_s)
@@ -394,6 +403,7 @@ def main():
         # the metric's third part: no tangent-linear or adjoint kernels exist
         # in this library, so there is no ratio to report
         "adjoint_forward_cost_ratio": None,
+        "tuning": {"corner_levels_per_workgroup": tuning, "autotune": args.autotune}
     }
     if args.radiation_calls > 0:
         out["radiation"] = radiation_timing(m, args, time_s)

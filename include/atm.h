@@ -181,6 +181,21 @@ int atm_get_path(atm_ctx *ctx, int *path);
 int atm_timer_enable(atm_ctx *ctx, int on);
 int atm_timer_read(atm_ctx *ctx, int kernel, int64_t *launches, double *total_ms);
 
+/* Launch-shape tuning (no counterpart in the reference): atm_autotune times
+   each candidate launch shape of the fused path's tunable kernels on this
+   context's grid and stream (one warm-up launch, then `reps` timed ones)
+   and keeps the fastest; it blocks.  Candidates differ only in how the work
+   is divided, never in the values computed.  Today one knob: the corner
+   kernel's levels per workgroup, 1, 2, 4 or 8 (default 4), which
+   atm_get_tuning reads and atm_set_tuning sets directly.  On the
+   reference-structured path these return ATM_OK and change nothing.
+   The candidates are timed in isolation (the same launch repeated), which
+   can rank them differently from their cost inside a step: at 512x512x64
+   it picks 1 level, while a whole-step comparison favours the default 4. */
+int atm_autotune(atm_ctx *ctx, int reps);
+int atm_get_tuning(atm_ctx *ctx, int *corner_levels);
+int atm_set_tuning(atm_ctx *ctx, int corner_levels);
+
 /* Wait for all work queued on the context. */
 int atm_sync(atm_ctx *ctx);
 

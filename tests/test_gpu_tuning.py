@@ -1,0 +1,51 @@
+"""Launch-shape tuning (atm_autotune / atm_get_tuning / atm_set_tuning): every
+candidate shape of the corner kernel (levels per workgroup 1, 2, 4, 8) gives
+the same bits after steps with remaps, in both precisions and at a level
+count that no group size divides; the autotuner picks one of them; bad
+arguments are refused."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DT = 60.0
+
+
+def _run(precision, corner_levels, steps=7):
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+    nx, ny, nlay = 64, 32, 13
+    ic = make_initial_state(nx, ny, nlay, 100e3, seed=5)
+    m = Model(nx, ny, nlay, 100e3, precision=precision)
+    assert m.path == "fused"
+    m.set_tuning(corner_levels)
+    assert m.tuning == corner_levels
+    cfg = Config(dt_max=DT, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12, dt_remap=3 * DT,
+                 radiation_on=False, physics_on=False)
+    st = driver.init_model(m, ic, cfg)
+    _, n, stable = driver.run_model(m, cfg, st, max_steps=steps)
+    assert stable and n == steps
+    return {name: m.get_full(name) for name in ("dp", "pt", "ud", "vd")}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_corner_levels_do_not_change_results(gpu_available, precision):
+    print(f"tuning {precision}", flush=True)
+    ref = _run(precision, 4)
+    for n in (1, 2, 8):
+        got = _run(precision, n)
+        for name in ref:
+            assert np.array_equal(got[name], ref[name]), (n, name)
+
+
+def test_autotune_choice_and_arguments(gpu_available):
+    from atmos.model import AtmError, Model
+    m = Model(64, 32, 13, 100e3)
+    assert m.tuning == 4  # default
+    assert m.autotune(reps=2) in (1, 2, 4, 8)
+    for bad in (lambda: m.autotune(reps=0), lambda: m.set_tuning(3)):
+        with pytest.raises(AtmError) as e:
+            bad()
+        assert e.value.code == 1
