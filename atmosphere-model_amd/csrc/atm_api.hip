@@ -653,9 +653,19 @@ int atm_graph_begin(atm_ctx *c) {
     }
     ATM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->cap_cur = c->cur;
-    c->cap_plk_valid = c->plk_valid;
     c->cap_timing = c->timing;
     c->timing = false;  // timer events are not recorded into the graph
+    // A launch may follow any eager work (a D step, a remap, a put of dp),
+    // so the recording cannot assume playkap valid: its first C step forms
+    // it afresh.
+    c->cap_plk_valid = c->plk_valid;
+    c->plk_valid = false;
+    // set until a recorded call writes dp, pt, ud or vd without reporting
+    // into the stability word (a reference-path D step)
+    c->cap_nfw_tracked = c->nfw_tracked;
+    c->nfw_tracked = true;
+    c->cap_dts.clear();
+    c->graph_dts.clear();
     c->capturing = true;
     return ATM_OK;
 }
@@ -667,17 +677,21 @@ int atm_graph_end(atm_ctx *c) {
     const hipError_t e = hipStreamEndCapture(c->stream, &gr);
     c->capturing = false;
     c->timing = c->cap_timing;
-    // nothing recorded has run: a refused recording leaves the context as
-    // it was at atm_graph_begin; after an accepted one, playkap counts as
-    // valid only if it was so both before and after the recorded steps
+    // Nothing recorded has run: the context goes back to its state at
+    // atm_graph_begin, and what the recorded steps change on the host side
+    // is kept for atm_graph_launch to apply.
     const bool whole = !c->after_d && c->cur == c->cap_cur;
-    const bool plk_end = c->plk_valid;
+    c->graph_plk_end = c->plk_valid;
+    c->graph_tracks = c->nfw_tracked;
     c->after_d = false;
     if (c->cur != c->cap_cur) {
         c->cur = c->cap_cur;
         atm_point_current(c);
     }
-    c->plk_valid = c->cap_plk_valid && (whole ? plk_end : true);
+    c->plk_valid = c->cap_plk_valid;
+    c->nfw_tracked = c->cap_nfw_tracked;
+    std::vector<double> dts;
+    dts.swap(c->cap_dts);
     if (e != hipSuccess || !gr) return atm_fail(c, ATM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     if (!whole) {
         (void)hipGraphDestroy(gr);
@@ -690,6 +704,7 @@ int atm_graph_end(atm_ctx *c) {
         c->gexec = nullptr;
         return atm_fail(c, ATM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
     }
+    c->graph_dts.swap(dts);
     return ATM_OK;
 }
 
@@ -698,7 +713,22 @@ int atm_graph_launch(atm_ctx *c, int times) {
     CHECK_NOT_CAPTURING(c);
     if (!c->gexec) return atm_fail(c, ATM_ERR_STATE, "no recorded graph");
     if (times < 0) return atm_fail(c, ATM_ERR_ARGUMENT, "times must be >= 0");
-    for (int i = 0; i < times; i++) ATM_HIP(c, hipGraphLaunch(c->gexec, c->stream));
+    // the recording reads and writes the buffers that were current when it
+    // began, at a step boundary
+    if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "a graph launches at a step boundary");
+    if (c->cur != c->cap_cur)
+        return atm_fail(c, ATM_ERR_STATE,
+                        "the current copy of the fields is not the one the graph was recorded on "
+                        "(an odd number of eager D steps + remaps since)");
+    for (int i = 0; i < times; i++) {
+        ATM_HIP(c, hipGraphLaunch(c->gexec, c->stream));
+        // the host-side effects of the recorded steps, as if run eagerly:
+        // accumulation time in the recorded order, playkap's validity, and
+        // the stability tracking (lost when a recorded write does not report)
+        for (double dt : c->graph_dts) c->acc_time = c->acc_time + dt;
+        c->plk_valid = c->graph_plk_end;
+        if (!c->graph_tracks) c->nfw_tracked = false;
+    }
     return ATM_OK;
 }
 

@@ -15,6 +15,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/atm.h"
 
@@ -335,10 +336,17 @@ struct atm_ctx {
     PhysWork *phys = nullptr;
     // HIP graph of a recorded step sequence (atm_graph_begin / end): while
     // capturing, the entry points that wait on the device or write memory
-    // from the host refuse; cap_cur is the buffer parity at the start
+    // from the host refuse; cap_cur is the buffer parity at the start.
+    // Host-side state that recorded calls would advance is kept aside while
+    // recording and applied by each launch instead: the accumulation time
+    // steps (cap_dts -> graph_dts), playkap's validity after the recorded
+    // steps (graph_plk_end) and whether every recorded write of dp, pt, ud,
+    // vd reports into the stability word (graph_tracks).
     bool capturing = false;
     int cap_cur = 0;
-    bool cap_timing = false, cap_plk_valid = false;
+    bool cap_timing = false, cap_plk_valid = false, cap_nfw_tracked = false;
+    std::vector<double> cap_dts, graph_dts;
+    bool graph_plk_end = false, graph_tracks = false;
     hipGraphExec_t gexec = nullptr;
     // time-mean output accumulators (nx, ny, nlay+1, 6) and their time
     double *outf = nullptr;

@@ -66,7 +66,11 @@ int atm_output_accumulate(atm_ctx *c, double dt) {
     hipLaunchKernelGGL(k_accumulate, dim3((g.nx + 63) / 64, g.ny, g.L + 1), dim3(64), 0, c->stream, g, dt, c->dp,
                        c->pt, c->ud, c->vd, c->ts, c->net_flux, c->outf);
     ATM_HIP(c, hipGetLastError());
-    c->acc_time = c->acc_time + dt;
+    // while recording a graph the time is added by each launch of it
+    if (c->capturing)
+        c->cap_dts.push_back(dt);
+    else
+        c->acc_time = c->acc_time + dt;
     return ATM_OK;
 }
 

@@ -149,30 +149,40 @@ __global__ void k_phys_apply(Geo g, double dt, const double *__restrict__ u_tend
 
 }  // namespace
 
-int atm_phys_alloc(atm_ctx *c) {
-    if (c->phys) return ATM_OK;
-    const Geo &g = c->g;
-    PhysWork *w = new PhysWork();
-    c->phys = w;
-    const size_t p = g.plane();
-    double **bufs[] = {&w->u_tend, &w->v_tend, &w->pt_tend};
-    for (double **b : bufs) {
-        ATM_HIP(c, hipMalloc(b, sizeof(double) * p * g.L));
-        ATM_HIP(c, hipMemsetAsync(*b, 0, sizeof(double) * p * g.L, c->stream));
-    }
-    ATM_HIP(c, hipMalloc(&w->ts_tend, sizeof(double) * p));
-    ATM_HIP(c, hipMemsetAsync(w->ts_tend, 0, sizeof(double) * p, c->stream));
-    return ATM_OK;
-}
-
-void atm_phys_free(atm_ctx *c) {
-    if (!c->phys) return;
-    PhysWork *w = c->phys;
+// Frees what a PhysWork holds; also for one whose allocation stopped part way.
+static void phys_release(PhysWork *w) {
     (void)hipFree(w->u_tend);
     (void)hipFree(w->v_tend);
     (void)hipFree(w->pt_tend);
     (void)hipFree(w->ts_tend);
     delete w;
+}
+
+// The context's PhysWork is published only once every buffer exists and is
+// cleared, so a failed initialisation leaves physics uninitialised and a
+// retry allocates afresh.
+int atm_phys_alloc(atm_ctx *c) {
+    if (c->phys) return ATM_OK;
+    const Geo &g = c->g;
+    PhysWork *w = new PhysWork();
+    const size_t p = g.plane();
+    double **bufs[] = {&w->u_tend, &w->v_tend, &w->pt_tend, &w->ts_tend};
+    const size_t n[] = {p * g.L, p * g.L, p * g.L, p};
+    for (int i = 0; i < 4; i++) {
+        hipError_t e = hipMalloc(bufs[i], sizeof(double) * n[i]);
+        if (e == hipSuccess) e = hipMemsetAsync(*bufs[i], 0, sizeof(double) * n[i], c->stream);
+        if (e != hipSuccess) {
+            phys_release(w);
+            return atm_fail(c, ATM_ERR_HIP, std::string("physics buffers: ") + hipGetErrorString(e));
+        }
+    }
+    c->phys = w;
+    return ATM_OK;
+}
+
+void atm_phys_free(atm_ctx *c) {
+    if (!c->phys) return;
+    phys_release(c->phys);
     c->phys = nullptr;
 }
 

@@ -589,15 +589,33 @@ __global__ void __launch_bounds__(64, NL == 1 ? 3 : 1) k_rad_columns(Geo g, Tabl
 
 }  // namespace
 
+// Frees what a RadWork holds; also for one whose allocation stopped part way.
+static void rad_release(RadWork *w) {
+    (void)hipFree(w->layer_T);
+    (void)hipFree(w->level_T);
+    (void)hipFree(w->tab);
+    delete w;
+}
+
+// The context's RadWork is published only once every buffer exists, so a
+// refused or failed initialisation leaves radiation uninitialised and a
+// retry allocates afresh.
 int atm_rad_alloc(atm_ctx *c) {
     const Geo &g = c->g;
     if (c->rad) return ATM_OK;
-    RadWork *w = new RadWork();
-    c->rad = w;
     if (g.L > 64 * RMAXL) return atm_fail(c, ATM_ERR_ARGUMENT, "radiation supports nlay <= 256");
+    RadWork *w = new RadWork();
+#define RAD_HIP(call)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess) {                                                                        \
+            rad_release(w);                                                                            \
+            return atm_fail(c, ATM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
+        }                                                                                              \
+    } while (0)
     size_t p = g.plane();
-    ATM_HIP(c, hipMalloc(&w->layer_T, sizeof(double) * p * g.L));
-    ATM_HIP(c, hipMalloc(&w->level_T, sizeof(double) * p * (g.L + 1)));
+    RAD_HIP(hipMalloc(&w->layer_T, sizeof(double) * p * g.L));
+    RAD_HIP(hipMalloc(&w->level_T, sizeof(double) * p * (g.L + 1)));
     // gas-optics tables (binary32 values of the reference literals)
     std::vector<float> h;
     size_t off[8];
@@ -610,19 +628,18 @@ int atm_rad_alloc(atm_ctx *c) {
     add(GO_PLANCK_T, GO_PLANCK_T_N, 5);
     add(GO_PLANCK, GO_PLANCK_N, 6);
     add(GO_SOLAR_FRACTION, GO_SOLAR_FRACTION_N, 7);
-    ATM_HIP(c, hipMalloc(&w->tab, sizeof(float) * h.size()));
-    ATM_HIP(c, hipMemcpy(w->tab, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+    RAD_HIP(hipMalloc(&w->tab, sizeof(float) * h.size()));
+    RAD_HIP(hipMemcpy(w->tab, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+#undef RAD_HIP
     w->t = Tables{w->tab + off[0], w->tab + off[1], w->tab + off[2], w->tab + off[3],
                   w->tab + off[4], w->tab + off[5], w->tab + off[6], w->tab + off[7]};
+    c->rad = w;
     return ATM_OK;
 }
 
 void atm_rad_free(atm_ctx *c) {
     if (!c->rad) return;
-    (void)hipFree(c->rad->layer_T);
-    (void)hipFree(c->rad->level_T);
-    (void)hipFree(c->rad->tab);
-    delete c->rad;
+    rad_release(c->rad);
     c->rad = nullptr;
 }
 

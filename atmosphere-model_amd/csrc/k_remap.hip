@@ -304,19 +304,36 @@ unsigned blocks8(size_t n, unsigned bs) {
 
 }  // namespace
 
+// Frees what a RemapWork holds; also for one whose allocation stopped part way.
+static void remap_release(RemapWork *w) {
+    (void)hipFree(w->scr);
+    (void)hipFree(w->tplev);
+    (void)hipFree(w->tpkap);
+    (void)hipFree(w->tplk);
+    (void)hipFree(w->uremap);
+    (void)hipFree(w->vremap);
+    delete w;
+}
+
+// The context's RemapWork is published only once every buffer exists, so a
+// failed initialisation leaves the remap uninitialised and a retry
+// allocates afresh.
 int atm_remap_alloc(atm_ctx *c) {
     const Geo &g = c->g;
     if (!c->remap) {
         RemapWork *w = new RemapWork();
-        c->remap = w;
         w->ncol = (size_t)(g.nx + 2) * (g.ny + 2);
         size_t p = g.plane();
-        ATM_HIP(c, hipMalloc(&w->scr, sizeof(double) * w->ncol * 4 * (g.L + 2)));
-        ATM_HIP(c, hipMalloc(&w->tplev, sizeof(double) * p * (g.L + 1)));
-        ATM_HIP(c, hipMalloc(&w->tpkap, sizeof(double) * p * (g.L + 1)));
-        ATM_HIP(c, hipMalloc(&w->tplk, sizeof(double) * p * g.L));
-        ATM_HIP(c, hipMalloc(&w->uremap, sizeof(double) * p * g.L));
-        ATM_HIP(c, hipMalloc(&w->vremap, sizeof(double) * p * g.L));
+        double **bufs[] = {&w->scr, &w->tplev, &w->tpkap, &w->tplk, &w->uremap, &w->vremap};
+        const size_t n[] = {w->ncol * 4 * (g.L + 2), p * (g.L + 1), p * (g.L + 1), p * g.L, p * g.L, p * g.L};
+        for (int i = 0; i < 6; i++) {
+            const hipError_t e = hipMalloc(bufs[i], sizeof(double) * n[i]);
+            if (e != hipSuccess) {
+                remap_release(w);
+                return atm_fail(c, ATM_ERR_HIP, std::string("hipMalloc (remap): ") + hipGetErrorString(e));
+            }
+        }
+        c->remap = w;
     }
     // target_plev(:,:,nlay+1) = top; target_pkap(:,:,nlay+1) = top**kappa (381-382)
     RemapWork *w = c->remap;
@@ -331,14 +348,7 @@ int atm_remap_alloc(atm_ctx *c) {
 
 void atm_remap_free(atm_ctx *c) {
     if (!c->remap) return;
-    RemapWork *w = c->remap;
-    (void)hipFree(w->scr);
-    (void)hipFree(w->tplev);
-    (void)hipFree(w->tpkap);
-    (void)hipFree(w->tplk);
-    (void)hipFree(w->uremap);
-    (void)hipFree(w->vremap);
-    delete w;
+    remap_release(c->remap);
     c->remap = nullptr;
 }
 

@@ -210,8 +210,12 @@ int atm_sync(atm_ctx *ctx);
    step boundary and, on the fused path, with the same current copy of the
    double-buffered fields (an even number of D steps + remaps), else
    atm_graph_end refuses it.  atm_graph_launch runs the recording `times`
-   times on the context's stream; the dt values are those recorded.  One
-   recording per context; a new atm_graph_begin replaces it. */
+   times on the context's stream; the dt values are those recorded, and
+   each launch advances the output accumulation time by the recorded dt
+   values.  A launch must find the copy of the fields that was current when
+   the recording began (an even number of eager D steps + remaps since),
+   else it returns ATM_ERR_STATE.  One recording per context; a new
+   atm_graph_begin replaces it. */
 int atm_graph_begin(atm_ctx *ctx);
 int atm_graph_end(atm_ctx *ctx);
 int atm_graph_launch(atm_ctx *ctx, int times);
