@@ -41,6 +41,10 @@ static double *field_ptr(atm_ctx *c, int id, int *nk) {
     case ATM_PTC: *nk = L; return c->ptc;
     case ATM_UC: *nk = L; return c->fused ? c->uc2 : c->uc;
     case ATM_VC: *nk = L; return c->fused ? c->vc2 : c->vc;
+    case ATM_TL_DP: case ATM_TL_PT: case ATM_TL_UD: case ATM_TL_VD:
+    case ATM_AD_DP: case ATM_AD_PT: case ATM_AD_UD: case ATM_AD_VD:
+        *nk = L;
+        return atm_lin_field(c, id);
     default: *nk = 0; return nullptr;
     }
 }
@@ -80,6 +84,7 @@ static void release(atm_ctx *c) {
     atm_remap_free(c);
     atm_rad_free(c);
     atm_phys_free(c);
+    atm_lin_free(c);
     if (c->outf) { (void)hipFree(c->outf); c->outf = nullptr; }
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -262,6 +267,7 @@ static int xfer(atm_ctx *c, int id, void *dst, const void *src, bool to_dev, hip
     CHECK_NOT_CAPTURING(c);
     int nk = 0;
     double *d = field_ptr(c, id, &nk);
+    if (!d && id >= ATM_TL_DP && id < ATM_NFIELDS) return atm_fail(c, ATM_ERR_STATE, "atm_lin_init first");
     if (!d) return atm_fail(c, ATM_ERR_ARGUMENT, "unknown field id " + std::to_string(id));
     if (!(to_dev ? src : dst)) return atm_fail(c, ATM_ERR_ARGUMENT, "null array");
     const Geo &g = c->g;
@@ -741,6 +747,39 @@ int atm_graph_free(atm_ctx *c) {
         c->gexec = nullptr;
     }
     return ATM_OK;
+}
+
+int atm_lin_init(atm_ctx *c) {
+    CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
+    if (!c->sw_init) return atm_fail(c, ATM_ERR_STATE, "atm_init_sw_dyn first");
+    return atm_lin_alloc(c);
+}
+
+static int lin_check(atm_ctx *c, double dt, int remap) {
+    CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
+    if (!c->lin) return atm_fail(c, ATM_ERR_STATE, "atm_lin_init first");
+    if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "a linearised step starts at a step boundary");
+    if (!(dt > 0.)) return atm_fail(c, ATM_ERR_ARGUMENT, "dt must be positive");
+    if (remap && !c->remap_init) return atm_fail(c, ATM_ERR_STATE, "atm_remap_init first");
+    return ATM_OK;
+}
+
+int atm_tl_step(atm_ctx *c, double dt, int remap) {
+    int rc = lin_check(c, dt, remap);
+    if (rc) return rc;
+    rc = atm_lin_tl_step(c, dt, remap);
+    // the state changed outside the tracked kernels
+    c->plk_valid = false;
+    c->nfw_tracked = false;
+    return rc;
+}
+
+int atm_ad_step(atm_ctx *c, double dt, int remap) {
+    int rc = lin_check(c, dt, remap);
+    if (rc) return rc;
+    return atm_lin_ad_step(c, dt, remap);
 }
 
 const char *atm_build_info(void) {

@@ -67,8 +67,8 @@ __host__ __device__ __forceinline__ T dmax3(T a, T b, T c) { return dmax2(dmax2(
 template <class T>
 __host__ __device__ __forceinline__ T dmin3(T a, T b, T c) { return dmin2(dmin2(a, b), c); }
 // Fortran SIGN(a, b)
-__device__ __forceinline__ double fsign(double a, double b) { return copysign(fabs(a), b); }
-__device__ __forceinline__ float fsign(float a, float b) { return copysignf(fabsf(a), b); }
+__host__ __device__ __forceinline__ double fsign(double a, double b) { return copysign(fabs(a), b); }
+__host__ __device__ __forceinline__ float fsign(float a, float b) { return copysignf(fabsf(a), b); }
 
 // x / 3 rounded to nearest, without the division sequence: the product
 // with RN(1/3) is corrected by one FMA step using the exact remainder
@@ -76,13 +76,13 @@ __device__ __forceinline__ float fsign(float a, float b) { return copysignf(fabs
 // this equals the IEEE quotient (checked on 1.6e9 random doubles spanning
 // 2^-300..2^300 against x / 3.; in binary32 the same argument holds: x / 3
 // is never a rounding midpoint); a zero product keeps the sign of x.
-__device__ __forceinline__ double div3(double x) {
+__host__ __device__ __forceinline__ double div3(double x) {
     const double r3 = 1.0 / 3.0;
     const double q = x * r3;
     if (q == 0.) return q;
     return fma(fma(-q, 3.0, x), r3, q);
 }
-__device__ __forceinline__ float div3(float x) {
+__host__ __device__ __forceinline__ float div3(float x) {
     const float r3 = 1.0f / 3.0f;
     const float q = x * r3;
     if (q == 0.f) return q;
@@ -181,7 +181,7 @@ __device__ __forceinline__ void atm_report_nonfinite(int *nfw, double v) {
 
 // dqi_mono of the middle cell from (q(m-1), q(m), q(m+1)): ppm 870-875
 template <class T>
-__device__ __forceinline__ T dq_mono(T qm, T q0, T qp) {
+__host__ __device__ __forceinline__ T dq_mono(T qm, T q0, T qp) {
     T d = (qp - qm) / T(4);
     T dmax = dmax3(qp, q0, qm) - q0;
     T dmin = q0 - dmin3(qp, q0, qm);
@@ -190,7 +190,7 @@ __device__ __forceinline__ T dq_mono(T qm, T q0, T qp) {
 
 // Edge anomalies of the middle cell from q(m-2..m+2): ppm 877-911.
 template <int V, class T>
-__device__ __forceinline__ void ppm_edges(T qm2, T qm1, T q0, T qp1, T qp2, T &eL, T &eR) {
+__host__ __device__ __forceinline__ void ppm_edges(T qm2, T qm1, T q0, T qp1, T qp2, T &eL, T &eR) {
     T dmm = dq_mono(qm2, qm1, q0);
     T dm0 = dq_mono(qm1, q0, qp1);
     T dmp = dq_mono(q0, qp1, qp2);
@@ -231,7 +231,7 @@ __device__ __forceinline__ void ppm_edges(T qm2, T qm1, T q0, T qp1, T qp2, T &e
 // Upwind PPM value at the face between cells i-1 and i, from
 // q(i-3..i+2) and the Courant number c(i): flux 802-829.
 template <int V, class T>
-__device__ __forceinline__ T ppm_face(T qm3, T qm2, T qm1, T q0, T qp1, T qp2, T c) {
+__host__ __device__ __forceinline__ T ppm_face(T qm3, T qm2, T qm1, T q0, T qp1, T qp2, T c) {
     T s = T(fabs(c));
     T k1 = (T(1) - T(2) * s) * (T(1) - s);
     T k2 = s * (s - T(1));
@@ -257,7 +257,7 @@ __device__ __forceinline__ T ppm_face(T qm3, T qm2, T qm1, T q0, T qp1, T qp2, T
 
 // Piecewise-constant upwind value at the face between cells i-1 and i: flux 832-838.
 template <class T>
-__device__ __forceinline__ T pcm_face(T qm1, T q0, T c) { return c > T(0) ? qm1 : q0; }
+__host__ __device__ __forceinline__ T pcm_face(T qm1, T q0, T c) { return c > T(0) ? qm1 : q0; }
 
 // ---------------------------------------------------------------------------
 // Context
@@ -265,6 +265,7 @@ __device__ __forceinline__ T pcm_face(T qm1, T q0, T c) { return c > T(0) ? qm1 
 struct RemapWork;  // k_remap.hip
 struct RadWork;    // k_rad.hip
 struct PhysWork;   // k_physics.hip
+struct LinWork;    // k_lin.hip
 
 // kernels timed with HIP events when timing is enabled (atm_timer_*)
 enum {
@@ -334,6 +335,7 @@ struct atm_ctx {
     RemapWork *remap = nullptr;
     RadWork *rad = nullptr;
     PhysWork *phys = nullptr;
+    LinWork *lin = nullptr;  // tangent-linear / adjoint step (atm_lin_init)
     // HIP graph of a recorded step sequence (atm_graph_begin / end): while
     // capturing, the entry points that wait on the device or write memory
     // from the host refuse; cap_cur is the buffer parity at the start.
@@ -406,6 +408,11 @@ int atm_phys_apply(atm_ctx *c, double dt);
 int atm_output_alloc(atm_ctx *c);
 int atm_output_accumulate(atm_ctx *c, double dt);
 int atm_output_take_mean(atm_ctx *c, double *host);
+int atm_lin_alloc(atm_ctx *c);
+void atm_lin_free(atm_ctx *c);
+double *atm_lin_field(atm_ctx *c, int id);
+int atm_lin_tl_step(atm_ctx *c, double dt, int remap);
+int atm_lin_ad_step(atm_ctx *c, double dt, int remap);
 int atm_rad_alloc(atm_ctx *c);
 void atm_rad_free(atm_ctx *c);
 int atm_rad_run(atm_ctx *c, double time);

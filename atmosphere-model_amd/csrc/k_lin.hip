@@ -1,0 +1,280 @@
+// k_lin.hip -- tangent-linear and adjoint of one model step on the GPU
+// (gfx950).  The step program and its per-point bodies are in
+// lin_step.hpp / lin_ops.hpp; this file runs them with HIP kernels:
+//   forward   one thread per point of an operation's box, the body in double;
+//   tangent   the same body on Jac<1> (value and tangent read, both written);
+//   adjoint   the same body on Jac<NR>: the output's adjoint times each
+//             partial is added to the adjoint of the value read, with
+//             global FP64 atomic adds (neighbouring points read the same
+//             values); outputs whose adjoint is zero return at once, so an
+//             adjoint confined to a small region costs little of the grid.
+// Column scans run one thread per column, the periodic halo fills one
+// thread per point (the adjoint folds each halo point's adjoint onto its
+// interior source with an atomic add).
+#include <cstring>
+
+#include "atm_common.hpp"
+#include "lin_step.hpp"
+
+namespace {
+
+struct DevAdd {
+    __device__ static void add(double *p, double v) { unsafeAtomicAdd(p, v); }
+};
+
+#define LIN_POINT                                                     \
+    const int I = r.I0 + blockIdx.x * blockDim.x + threadIdx.x;       \
+    const int J = r.J0 + blockIdx.y * blockDim.y + threadIdx.y;       \
+    const int K = k0 + blockIdx.z;                                    \
+    if (I > r.I1 || J > r.J1) return;
+
+template <class F>
+__global__ void __launch_bounds__(256) k_lin_fwd(LinArgs x, Rng r, int k0) {
+    LIN_POINT
+    AccFwd a{x};
+    F::run(a, x, I, J, K);
+}
+
+template <class F>
+__global__ void __launch_bounds__(256) k_lin_tan(LinArgs x, Rng r, int k0) {
+    LIN_POINT
+    AccTan a{x};
+    F::run(a, x, I, J, K);
+}
+
+template <class F>
+__global__ void __launch_bounds__(256) k_lin_adj(LinArgs x, Rng r, int k0) {
+    LIN_POINT
+    AccAdj<F::NR, DevAdd> a(x);
+    F::run(a, x, I, J, K);
+}
+#undef LIN_POINT
+
+template <bool GZ>
+__global__ void k_lin_scan(int mode, Geo g, Rng r, const double *sv, double *sd, const double *tv, double *td,
+                           double *pv, double *pd, double *kv, double *kd, double *gv, double *gd) {
+    const int I = r.I0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = r.J0 + blockIdx.y * blockDim.y + threadIdx.y;
+    if (I > r.I1 || J > r.J1) return;
+    const size_t c0 = g.ix2(I, J);
+    if (mode == LIN_FWD)
+        scan_column<double, GZ>(g, c0, sv, nullptr, tv, nullptr, pv, nullptr, kv, nullptr, gv, nullptr);
+    else if (mode == LIN_TAN)
+        scan_column<Jac<1>, GZ>(g, c0, sv, sd, tv, td, pv, pd, kv, kd, gv, gd);
+    else
+        scan_column_adj<GZ>(g, c0, sd, tv, td, pv, pd, kv, kd, gd);
+}
+
+// dst = periodic fill of src's interior over the whole halo'd plane
+__global__ void k_lin_halo(Geo g, int nk, const double *__restrict__ sv, const double *__restrict__ st,
+                           double *__restrict__ dv, double *__restrict__ dt) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I >= g.NI || J >= g.NJ || K >= nk) return;
+    const size_t d = g.ix(I, J, K), s = g.ix(wrap_index(I, g.nx), wrap_index(J, g.ny), K);
+    dv[d] = sv[s];
+    if (dt) dt[d] = st[s];
+}
+
+// adjoint of the fill: each point's adjoint onto its interior source
+__global__ void k_lin_halo_adj(Geo g, int nk, double *__restrict__ sd, double *__restrict__ dd) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I >= g.NI || J >= g.NJ || K >= nk) return;
+    const size_t d = g.ix(I, J, K);
+    const double a = dd[d];
+    if (a == 0.) return;
+    dd[d] = 0.;
+    unsafeAtomicAdd(sd + g.ix(wrap_index(I, g.nx), wrap_index(J, g.ny), K), a);
+}
+
+__global__ void k_lin_fill(double *a, size_t n, double v) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = v;
+}
+
+// interior of src (nk levels) into dst
+__global__ void k_lin_copy_interior(Geo g, int nk, const double *__restrict__ src, double *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, K = blockIdx.z;
+    if (i >= g.nx) return;
+    const size_t x = g.ix(ATM_HALO + i, ATM_HALO + j, K);
+    dst[x] = src[x];
+}
+
+const dim3 LBLK(64, 4, 1);
+
+inline dim3 lin_grid(const Rng &r, int nk) {
+    return dim3((unsigned)((r.I1 - r.I0 + LBLK.x) / LBLK.x), (unsigned)((r.J1 - r.J0 + LBLK.y) / LBLK.y),
+                (unsigned)nk);
+}
+
+}  // namespace
+
+struct GpuBackend {
+    hipStream_t s = nullptr;
+    bool failed = false;  // an allocation failed
+
+    double *alloc(size_t n) {
+        double *p = nullptr;
+        if (hipMalloc(&p, sizeof(double) * (n ? n : 1)) != hipSuccess) {
+            failed = true;
+            return nullptr;
+        }
+        fill(p, n, 0.);
+        return p;
+    }
+    void release(double *p) {
+        if (p) (void)hipFree(p);
+    }
+    void copy(double *d, const double *src, size_t n) {
+        (void)hipMemcpyAsync(d, src, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+    }
+    void fill(double *d, size_t n, double v) {
+        if (n) hipLaunchKernelGGL(k_lin_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, n, v);
+    }
+    template <class F>
+    void fwd(const LinArgs &x, Rng r, int k0, int nk) {
+        hipLaunchKernelGGL(k_lin_fwd<F>, lin_grid(r, nk), LBLK, 0, s, x, r, k0);
+    }
+    template <class F>
+    void tan(const LinArgs &x, Rng r, int k0, int nk) {
+        hipLaunchKernelGGL(k_lin_tan<F>, lin_grid(r, nk), LBLK, 0, s, x, r, k0);
+    }
+    template <class F>
+    void adj(const LinArgs &x, Rng r, int k0, int nk) {
+        hipLaunchKernelGGL(k_lin_adj<F>, lin_grid(r, nk), LBLK, 0, s, x, r, k0);
+    }
+    void scan(int mode, bool gz, const Geo &g, Rng r, const double *sv, double *sd, const double *tv, double *td,
+              double *pv, double *pd, double *kv, double *kd, double *gv, double *gd) {
+        const dim3 grid = lin_grid(r, 1);
+        if (gz)
+            hipLaunchKernelGGL(k_lin_scan<true>, grid, LBLK, 0, s, mode, g, r, sv, sd, tv, td, pv, pd, kv, kd, gv, gd);
+        else
+            hipLaunchKernelGGL(k_lin_scan<false>, grid, LBLK, 0, s, mode, g, r, sv, sd, tv, td, pv, pd, kv, kd, gv,
+                               gd);
+    }
+    void halo_copy(int mode, const Geo &g, int nk, const double *sv, double *sd, double *dv, double *dd) {
+        const dim3 grid((unsigned)((g.NI + LBLK.x - 1) / LBLK.x), (unsigned)((g.NJ + LBLK.y - 1) / LBLK.y),
+                        (unsigned)nk);
+        if (mode == LIN_ADJ)
+            hipLaunchKernelGGL(k_lin_halo_adj, grid, LBLK, 0, s, g, nk, sd, dd);
+        else
+            hipLaunchKernelGGL(k_lin_halo, grid, LBLK, 0, s, g, nk, sv, mode == LIN_TAN ? sd : nullptr, dv,
+                               mode == LIN_TAN ? dd : nullptr);
+    }
+    void copy_interior(const Geo &g, int nk, const double *src, double *dst) {
+        hipLaunchKernelGGL(k_lin_copy_interior, dim3((unsigned)((g.nx + 63) / 64), (unsigned)g.ny, (unsigned)nk),
+                           dim3(64), 0, s, g, nk, src, dst);
+    }
+};
+
+struct LinWork {
+    GpuBackend be;
+    LinCore<GpuBackend> core;
+    double *tl[4] = {}, *ad[4] = {};  // tangent and adjoint of dp, pt, ud, vd (halo'd layout)
+    int last = -1;                    // sweep kind the derivative arrays last held
+    explicit LinWork(const Geo &g) : core(be, g) {}
+};
+
+double *atm_lin_field(atm_ctx *c, int id) {
+    if (!c->lin) return nullptr;
+    if (id >= ATM_TL_DP && id <= ATM_TL_VD) return c->lin->tl[id - ATM_TL_DP];
+    if (id >= ATM_AD_DP && id <= ATM_AD_VD) return c->lin->ad[id - ATM_AD_DP];
+    return nullptr;
+}
+
+void atm_lin_free(atm_ctx *c) {
+    LinWork *w = c->lin;
+    if (!w) return;
+    w->core.release();
+    for (int i = 0; i < 4; i++) {
+        w->be.release(w->tl[i]);
+        w->be.release(w->ad[i]);
+    }
+    delete w;
+    c->lin = nullptr;
+}
+
+int atm_lin_alloc(atm_ctx *c) {
+    if (c->lin) return ATM_OK;
+    LinWork *w = new LinWork(c->g);
+    w->be.s = c->stream;
+    const size_t n = c->g.plane() * c->g.L;
+    bool ok = w->core.allocate();
+    for (int i = 0; ok && i < 4; i++) {
+        w->tl[i] = w->be.alloc(n);
+        w->ad[i] = w->be.alloc(n);
+        ok = w->tl[i] && w->ad[i];
+    }
+    if (!ok) {
+        c->lin = w;
+        atm_lin_free(c);
+        return atm_fail(c, ATM_ERR_HIP, "hipMalloc failed for the linearised step's arrays");
+    }
+    ATM_HIP(c, hipGetLastError());
+    c->lin = w;
+    return ATM_OK;
+}
+
+// Derivative arrays hold tangents in a tangent sweep and adjoints in an
+// adjoint sweep; switching clears them (constant levels and the adjoint
+// bookkeeping rely on zeros).
+static void lin_prepare(atm_ctx *c, int kind) {
+    LinWork *w = c->lin;
+    LinCore<GpuBackend> &k = w->core;
+    w->be.s = c->stream;
+    k.c = c->coef;
+    k.ak0 = c->ak0;
+    k.pktop = c->pktop;
+    k.ak = c->ak;
+    k.bk = c->bk;
+    if (w->last != kind) {
+        for (int i = 0; i < A_COUNT; i++)
+            if (k.drv[i]) w->be.fill(k.drv[i], k.count(i), 0.);
+        w->last = kind;
+    }
+    // constants of this step: surface gz and its corner values, the net flux
+    k.set_constants(c->gz, c->gzb, c->net_flux);
+    // the start state
+    const size_t n = c->g.plane() * c->g.L;
+    const double *st[4] = {c->dp, c->pt, c->ud, c->vd};
+    for (int i = 0; i < 4; i++) w->be.copy(k.val[A_XDP + i], st[i], n);
+}
+
+int atm_lin_tl_step(atm_ctx *c, double dt, int remap) {
+    LinWork *w = c->lin;
+    LinCore<GpuBackend> &k = w->core;
+    lin_prepare(c, LIN_TAN);
+    const size_t n = c->g.plane() * c->g.L;
+    for (int i = 0; i < 4; i++) w->be.copy(k.drv[A_XDP + i], w->tl[i], n);
+    k.sweep(LIN_TAN, dt, remap != 0);
+    int end[4];
+    lin_end_arrays(remap != 0, end);
+    double *st[4] = {c->dp, c->pt, c->ud, c->vd};
+    for (int i = 0; i < 4; i++) {
+        w->be.copy(st[i], k.val[end[i]], n);
+        w->be.copy(w->tl[i], k.drv[end[i]], n);
+    }
+    const int nk[] = {c->g.L, c->g.L, c->g.L, c->g.L};
+    int rc = atm_launch_halo_multi(c, st, nk, 4);
+    if (rc) return rc;
+    ATM_HIP(c, hipGetLastError());
+    return ATM_OK;
+}
+
+int atm_lin_ad_step(atm_ctx *c, double dt, int remap) {
+    LinWork *w = c->lin;
+    LinCore<GpuBackend> &k = w->core;
+    lin_prepare(c, LIN_ADJ);
+    int end[4];
+    lin_end_arrays(remap != 0, end);
+    for (int i = 0; i < 4; i++) w->be.copy_interior(c->g, c->g.L, w->ad[i], k.drv[end[i]]);
+    k.adjoint(dt, remap != 0);
+    const size_t n = c->g.plane() * c->g.L;
+    for (int i = 0; i < 4; i++) {
+        w->be.copy(w->ad[i], k.drv[A_XDP + i], n);
+        w->be.fill(k.drv[A_XDP + i], n, 0.);
+    }
+    ATM_HIP(c, hipGetLastError());
+    return ATM_OK;
+}

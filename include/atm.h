@@ -56,7 +56,11 @@ typedef enum atm_field {
     /* mod_sw_dyn module-private C-grid state (mod_sw_dyn.f90:23-27); the
        reference host never sees these, they are exposed for testing. */
     ATM_DPC = 10, ATM_PTC = 11, ATM_UC = 12, ATM_VC = 13,
-    ATM_NFIELDS = 14
+    /* tangent and adjoint of dp, pt, ud, vd for the linearised step
+       (atm_lin_init); only their interior is read and meaningful */
+    ATM_TL_DP = 14, ATM_TL_PT = 15, ATM_TL_UD = 16, ATM_TL_VD = 17,
+    ATM_AD_DP = 18, ATM_AD_PT = 19, ATM_AD_UD = 20, ATM_AD_VD = 21,
+    ATM_NFIELDS = 22
 } atm_field;
 
 typedef struct atm_grid {
@@ -220,6 +224,30 @@ int atm_graph_begin(atm_ctx *ctx);
 int atm_graph_end(atm_ctx *ctx);
 int atm_graph_launch(atm_ctx *ctx, int times);
 int atm_graph_free(atm_ctx *ctx);
+
+/* Tangent-linear and adjoint of one model step (no reference counterpart;
+   for gradient computations).  The step is the reference's sequence
+   cgrid_dynamics_step(dt/2), cgrid_halo_exchange, dgrid_dynamics_step(dt),
+   perform_vertical_remapping when `remap` is nonzero, dgrid_halo_exchange
+   (src/mod_model.f90:144-173) with radiation and physics off (the net flux
+   is a fixed forcing), linearised about the current state with every PPM
+   limiter branch and upwind switch frozen as the state takes them.  The
+   state is the interior of dp, pt, ud, vd at a step boundary; the
+   linearised kernels form the forward values themselves (the FP64 path).
+     atm_lin_init  allocates the linearised step's arrays (needs
+                   atm_init_sw_dyn).
+     atm_tl_step   advances the state by one step and the tangent fields
+                   (ATM_TL_*) by its tangent-linear model about the state
+                   at the step's start.
+     atm_ad_step   replaces the adjoint fields (ATM_AD_*: the adjoint of the
+                   step's end state) with the adjoint of the start state,
+                   linearised about the current state, which it leaves
+                   unchanged.
+   Both refuse while recording a graph or between atm_dgrid_step and
+   atm_dgrid_halo. */
+int atm_lin_init(atm_ctx *ctx);
+int atm_tl_step(atm_ctx *ctx, double dt, int remap);
+int atm_ad_step(atm_ctx *ctx, double dt, int remap);
 
 /* Library build information (kernel path, compile flags), for logs. */
 const char *atm_build_info(void);
