@@ -65,6 +65,26 @@ __global__ void k_lin_scan(int mode, Geo g, Rng r, const double *sv, double *sd,
         scan_column_adj<GZ>(g, c0, sd, tv, td, pv, pd, kv, kd, gd);
 }
 
+// a column operation of the remap, one thread per column of the box
+template <int KIND>
+__global__ void __launch_bounds__(256) k_lin_column(int mode, LinArgs x, Rng r, double *ws) {
+    const size_t W = (size_t)(r.I1 - r.I0 + 1), ncol = W * (size_t)(r.J1 - r.J0 + 1);
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncol) return;
+    const int I = r.I0 + (int)(t % W), J = r.J0 + (int)(t / W);
+    const RmWS w{ws, ncol, t, x.g.L};
+    const size_t c0 = x.g.ix2(I, J);
+    if (KIND == RM_COL_REMAP) {
+        if (mode == LIN_FWD) rm_column<double>(x, c0, w);
+        else if (mode == LIN_TAN) rm_column<Jac<1>>(x, c0, w);
+        else rm_column_adj(x, c0, w);
+    } else {
+        if (mode == LIN_FWD) rm_pt_column<double>(x, c0, w);
+        else if (mode == LIN_TAN) rm_pt_column<Jac<1>>(x, c0, w);
+        else rm_pt_column_adj(x, c0, w);
+    }
+}
+
 // dst = periodic fill of src's interior over the whole halo'd plane
 __global__ void k_lin_halo(Geo g, int nk, const double *__restrict__ sv, const double *__restrict__ st,
                            double *__restrict__ dv, double *__restrict__ dt) {
@@ -152,6 +172,14 @@ struct GpuBackend {
         else
             hipLaunchKernelGGL(k_lin_scan<false>, grid, LBLK, 0, s, mode, g, r, sv, sd, tv, td, pv, pd, kv, kd, gv,
                                gd);
+    }
+    void column(int mode, int kind, const LinArgs &x, Rng r, double *ws) {
+        const size_t ncol = (size_t)(r.I1 - r.I0 + 1) * (size_t)(r.J1 - r.J0 + 1);
+        const dim3 grid((unsigned)((ncol + 255) / 256));
+        if (kind == RM_COL_REMAP)
+            hipLaunchKernelGGL(k_lin_column<RM_COL_REMAP>, grid, dim3(256), 0, s, mode, x, r, ws);
+        else
+            hipLaunchKernelGGL(k_lin_column<RM_COL_PT>, grid, dim3(256), 0, s, mode, x, r, ws);
     }
     void halo_copy(int mode, const Geo &g, int nk, const double *sv, double *sd, double *dv, double *dd) {
         const dim3 grid((unsigned)((g.NI + LBLK.x - 1) / LBLK.x), (unsigned)((g.NJ + LBLK.y - 1) / LBLK.y),

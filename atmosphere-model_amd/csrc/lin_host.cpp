@@ -74,6 +74,24 @@ struct HostBackend {
                 }
             }
     }
+    void column(int mode, int kind, const LinArgs &x, Rng r, double *ws) {
+        const Geo &g = x.g;
+        const size_t W = (size_t)(r.I1 - r.I0 + 1), ncol = W * (size_t)(r.J1 - r.J0 + 1);
+        for (int J = r.J0; J <= r.J1; J++)
+            for (int I = r.I0; I <= r.I1; I++) {
+                const RmWS w{ws, ncol, (size_t)(J - r.J0) * W + (size_t)(I - r.I0), g.L};
+                const size_t c0 = g.ix2(I, J);
+                if (kind == RM_COL_REMAP) {
+                    if (mode == LIN_FWD) rm_column<double>(x, c0, w);
+                    else if (mode == LIN_TAN) rm_column<Jac<1>>(x, c0, w);
+                    else rm_column_adj(x, c0, w);
+                } else {
+                    if (mode == LIN_FWD) rm_pt_column<double>(x, c0, w);
+                    else if (mode == LIN_TAN) rm_pt_column<Jac<1>>(x, c0, w);
+                    else rm_pt_column_adj(x, c0, w);
+                }
+            }
+    }
     void halo_copy(int mode, const Geo &g, int nk, const double *sv, double *sd, double *dv, double *dd) {
         for (int K = 0; K < nk; K++)
             for (int J = 0; J < g.NJ; J++)

@@ -30,6 +30,8 @@ struct LinArgs {
     Geo g;
     DynCoef c;
     int e0, e1;
+    const double *ak, *bk;  // hybrid coefficients (the remap's target grid)
+    double top;             // Ak(1), the top pressure
     const double *iv[LIN_MAXIN];
     double *it[LIN_MAXIN];
     double *ov[LIN_MAXOUT];

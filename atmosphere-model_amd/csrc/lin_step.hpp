@@ -22,7 +22,6 @@
 #include <vector>
 
 #include "lin_ops.hpp"
-#include "lin_remap.hpp"
 
 struct Rng {
     int I0, I1, J0, J1;  // inclusive
@@ -48,8 +47,8 @@ enum LinArr {
     A_PT1, A_PLEV_D2, A_PKAP_D2, A_GZ_D2, A_PKB, A_GZB, A_KE_D, A_VORT_D, A_PGFX_D, A_PGFY_D, A_DIV, A_TMP11,
     A_DIV4, A_TMP12, A_LAP6, A_UD1, A_TMP13, A_VD1,
     // remap (mod_remap.f90:27-163)
-    A_RPLEV, A_RPKAP, A_RPLK, A_RGZ, A_TPLEV, A_TPKAP, A_TPLK, A_UFX, A_UFD, A_VFX, A_VFD, A_UR, A_VR,
-    A_ENER, A_KER, A_EREM, A_DP2, A_PT2, A_UD2, A_VD2,
+    A_RPLEV, A_RPKAP, A_RPLK, A_RGZ, A_TPLEV, A_TPKAP, A_TPLK, A_UFX, A_UFD, A_VFX, A_VFD, A_UDM, A_VDM,
+    A_UR, A_VR, A_ENER, A_KER, A_EREM, A_DP2, A_PT2,
     A_COUNT
 };
 
@@ -70,6 +69,8 @@ inline int lin_levels(int id, int L) {
 
 inline Rng lin_box(const Geo &g, int a, int b) { return Rng{a, g.NI - 1 - b, a, g.NJ - 1 - b}; }
 inline Rng lin_box4(const Geo &g, int ai, int bi, int aj, int bj) { return Rng{ai, g.NI - 1 - bi, aj, g.NJ - 1 - bj}; }
+
+#include "lin_remap.hpp"
 
 // The step program.  E: the runner (LinCore below) with op<F>, scan,
 // halo_copy and remap.
@@ -158,7 +159,7 @@ void lin_program(E &e, bool remap) {
 // the arrays holding the end state's interior
 inline void lin_end_arrays(bool remap, int out[4]) {
     if (remap) {
-        out[0] = A_DP2; out[1] = A_PT2; out[2] = A_UD2; out[3] = A_VD2;
+        out[0] = A_DP2; out[1] = A_PT2; out[2] = A_UR; out[3] = A_VR;
     } else {
         out[0] = A_DP1; out[1] = A_PT1; out[2] = A_UD1; out[3] = A_VD1;
     }
@@ -177,10 +178,13 @@ struct LinCore {
     double step_dt = 0.;  // the model step; the C-grid half step uses step_dt / 2
     std::vector<std::function<void()>> rev;
     const double *ak = nullptr, *bk = nullptr;  // coordinates (device or host, as the backend)
+    double *ws = nullptr;                       // column workspace of the remap
 
     LinCore(B &be, const Geo &geo) : b(be), g(geo) {}
 
     size_t count(int id) const { return g.plane() * (size_t)lin_levels(id, g.L); }
+    // the remap's widest column box is isd-1..ied+1
+    size_t ws_count() const { return (size_t)RM_SLOTS * (g.L + 2) * (size_t)(g.nx + 2) * (g.ny + 2); }
 
     bool allocate() {
         for (int i = 0; i < A_COUNT; i++) {
@@ -190,9 +194,12 @@ struct LinCore {
             drv[i] = b.alloc(count(i));
             if (!drv[i]) return false;
         }
-        return true;
+        ws = b.alloc(ws_count());
+        return ws != nullptr;
     }
     void release() {
+        b.release(ws);
+        ws = nullptr;
         for (int i = 0; i < A_COUNT; i++) {
             b.release(val[i]);
             b.release(drv[i]);
@@ -220,6 +227,9 @@ struct LinCore {
         x.c = c;
         x.e0 = e0;
         x.e1 = e1;
+        x.ak = ak;
+        x.bk = bk;
+        x.top = ak0;
         int k = 0;
         for (int id : in) {
             x.iv[k] = val[id];
@@ -266,6 +276,16 @@ struct LinCore {
             rev.push_back([=]() { b.halo_copy(LIN_ADJ, g, nk, sv, sd, dv, dd); });
         else
             b.halo_copy(mode, g, nk, sv, sd, dv, dd);
+    }
+
+    // a column operation of the remap (lin_remap.hpp) over a box of columns
+    void column(int kind, Rng r, std::initializer_list<int> in, std::initializer_list<int> out) {
+        const LinArgs x = args(in, out, 0, 0);
+        double *w = ws;
+        if (mode == LIN_ADJ)
+            rev.push_back([=]() { b.column(LIN_ADJ, kind, x, r, w); });
+        else
+            b.column(mode, kind, x, r, w);
     }
 
     void set_dt(double dt) {
