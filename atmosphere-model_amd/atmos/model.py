@@ -18,7 +18,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libatmos_hip.so")
 
 FIELDS = {"dp": 0, "pt": 1, "ud": 2, "vd": 3, "ts": 4, "gz": 5, "net_flux": 6,
-          "plev": 7, "pkap": 8, "radius": 9, "dpc": 10, "ptc": 11, "uc": 12, "vc": 13}
+          "plev": 7, "pkap": 8, "radius": 9, "dpc": 10, "ptc": 11, "uc": 12, "vc": 13,
+          "tl_dp": 14, "tl_pt": 15, "tl_ud": 16, "tl_vd": 17,
+          "ad_dp": 18, "ad_pt": 19, "ad_ud": 20, "ad_vd": 21}
 STATUS = {0: "ok", 1: "argument", 2: "hip", 3: "table range", 4: "non-finite", 5: "state"}
 EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm_set_coordinate",
            "atm_put_field", "atm_get_field", "atm_put_field_device", "atm_get_field_device",
@@ -30,7 +32,7 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_physics_init", "atm_physics_tendencies", "atm_apply_physics", "atm_physics_halo",
            "atm_output_init", "atm_accumulate_output", "atm_output_mean", "atm_stream_create",
            "atm_stream_destroy", "atm_graph_begin", "atm_graph_end", "atm_graph_launch", "atm_graph_free",
-           "atm_autotune", "atm_get_tuning", "atm_set_tuning"]
+           "atm_autotune", "atm_get_tuning", "atm_set_tuning", "atm_lin_init", "atm_tl_step", "atm_ad_step"]
 PATHS = {"fused": 0, "reference": 1}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 TIMERS = {"c1": 0, "c2": 1, "d1": 2, "d2": 3, "scan": 4, "halo": 5, "remap": 6, "radiation": 7,
@@ -95,6 +97,9 @@ def load_library(path: str = LIB_PATH):
     L.atm_get_path.argtypes = [vp, C.POINTER(C.c_int)]
     L.atm_timer_enable.argtypes = [vp, ip]
     L.atm_timer_read.argtypes = [vp, ip, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    L.atm_lin_init.argtypes = [vp]
+    for n in ["atm_tl_step", "atm_ad_step"]:
+        getattr(L, n).argtypes = [vp, C.c_double, ip]
     L.atm_autotune.argtypes = [vp, ip]
     L.atm_get_tuning.argtypes = [vp, C.POINTER(C.c_int)]
     L.atm_set_tuning.argtypes = [vp, ip]
@@ -312,3 +317,25 @@ class Model:
 
     def set_stream(self, stream_handle):
         self._ck(self._L.atm_set_stream(self._h, C.c_void_p(stream_handle)))
+
+    # ---- device-resident transfers (halo'd layout, stream-ordered) ----
+    def put_device(self, name, ptr):
+        """Copy a device array (nk, ny+14, nx+14) at address ptr into a field."""
+        self._ck(self._L.atm_put_field_device(self._h, FIELDS[name], C.c_void_p(ptr)))
+
+    def get_device(self, name, ptr):
+        """Copy a field into the device array at address ptr."""
+        self._ck(self._L.atm_get_field_device(self._h, FIELDS[name], C.c_void_p(ptr)))
+
+    # ---- tangent-linear and adjoint of one step (no reference counterpart) ----
+    def lin_init(self):
+        self._ck(self._L.atm_lin_init(self._h))
+
+    def tl_step(self, dt, remap=False):
+        """Advance the state one step and the tangent fields (tl_*) alongside."""
+        self._ck(self._L.atm_tl_step(self._h, dt, int(bool(remap))))
+
+    def ad_step(self, dt, remap=False):
+        """Replace the adjoint fields (ad_*) of the step's end state by those
+        of its start state, linearised about the current state."""
+        self._ck(self._L.atm_ad_step(self._h, dt, int(bool(remap))))
