@@ -17,8 +17,10 @@ The forward trajectory runs on the model's own path and precision (the
 mixed form when the model was created so); the linearised step forms its
 forward values in FP64 from each stored state.
 
-The model's stream is set to torch's current stream, so the library's work
-and torch's are ordered on one stream.
+The library runs on the model's own stream; every hand-over of a tensor
+between torch and the library waits for the side that wrote it (torch's
+device work before the library reads a tensor, the library's stream before
+torch reads one or frees one the library copied from).
 """
 from __future__ import annotations
 
@@ -80,7 +82,7 @@ class Window:
     """A schedule of steps with binomial checkpointing on a Model (which must
     have run init_model; remap steps need its remap arrays)."""
 
-    def __init__(self, model, schedule, snapshots=10, stream=None):
+    def __init__(self, model, schedule, snapshots=10):
         import torch
 
         self.torch = torch
@@ -90,7 +92,6 @@ class Window:
         if self.snaps < 1:
             raise ValueError("at least one snapshot")
         self.shape_full = (model.nlay, model.ny + 2 * HALO, model.nx + 2 * HALO)
-        self.m.set_stream(torch.cuda.current_stream().cuda_stream if stream is None else stream)
         self.m.lin_init()
         self.store = {}  # step index -> halo'd state tensors
         self.counts = {"forward_steps": 0, "adjoint_steps": 0, "max_snapshots": 0}
@@ -98,32 +99,39 @@ class Window:
     # ---- state transfer ----
     def _take(self):
         t = self.torch
-        snap = []
-        for n in STATE:
-            a = t.empty(self.shape_full, dtype=t.float64, device="cuda")
+        snap = [t.empty(self.shape_full, dtype=t.float64, device="cuda") for _ in STATE]
+        t.cuda.synchronize()
+        for n, a in zip(STATE, snap):
             self.m.get_device(n, a.data_ptr())
-            snap.append(a)
+        self.m.sync()
         return snap
 
     def _give(self, snap):
+        self.torch.cuda.synchronize()
         for n, a in zip(STATE, snap):
             self.m.put_device(n, a.data_ptr())
+        self.m.sync()
 
     def _interior(self, names):
         t = self.torch
-        out = []
-        for n in names:
-            a = t.empty(self.shape_full, dtype=t.float64, device="cuda")
+        full = [t.empty(self.shape_full, dtype=t.float64, device="cuda") for _ in names]
+        t.cuda.synchronize()
+        for n, a in zip(names, full):
             self.m.get_device(n, a.data_ptr())
-            out.append(a[:, HALO:-HALO, HALO:-HALO].contiguous())
-        return out
+        self.m.sync()
+        return [a[:, HALO:-HALO, HALO:-HALO].contiguous() for a in full]
 
     def _put_interior(self, names, tensors):
         t = self.torch
-        for n, x in zip(names, tensors):
+        full = []
+        for x in tensors:
             a = t.zeros(self.shape_full, dtype=t.float64, device="cuda")
             a[:, HALO:-HALO, HALO:-HALO] = x
+            full.append(a)
+        t.cuda.synchronize()
+        for n, a in zip(names, full):
             self.m.put_device(n, a.data_ptr())
+        self.m.sync()
 
     def set_state(self, dp, pt, ud, vd):
         """Initial interior fields; the halos are filled periodically."""

@@ -75,6 +75,12 @@ def parse():
                          "mixed precision, radiation every 10 steps")
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--ensemble-streams", type=int, default=4)
+    ap.add_argument("--adjoint", choices=["on", "off"], default="on",
+                    help="gradient through a checkpointed window (tangent-linear/adjoint workload)")
+    ap.add_argument("--adj-nx", type=int, default=256)
+    ap.add_argument("--adj-nlay", type=int, default=48)
+    ap.add_argument("--adj-steps", type=int, default=90)
+    ap.add_argument("--adj-snapshots", type=int, default=10)
     ap.add_argument("--autotune", choices=["on", "off"], default="off",
                     help="time the fused path's launch-shape candidates on each grid first (results unchanged; "
                          "off by default: isolated timings have picked shapes slower inside a step)")
@@ -331,6 +337,97 @@ def radiation_timing(m, args, time_s):
             "flop_per_call": flops}
 
 
+def adjoint_timing(args):
+    """Gradient of a surface-pressure cost through a window of steps: the
+    tangent-linear/adjoint workload (256 x 256 x 48, dx = 50 km, dt = 40 s,
+    a 90-step window with remaps every 5 steps, 10 snapshots of binomial
+    checkpointing; J = mean-square deviation of the surface pressure over a
+    32 x 32 box at the end).  The forward trajectory runs in the mixed form
+    (or FP64), the adjoint kernels in FP64.  Ratio = wall time of the
+    gradient (checkpoint recomputation + adjoint steps) / wall time of the
+    forward window.  A central finite difference of J (eps 1e-7 along a
+    random direction of pt, ud, vd) checks the gradient on an FP64 run."""
+    import numpy as np
+    import torch
+
+    from atmos import driver
+    from atmos.adjoint import (Window, binomial_reps, recompute_steps, schedule_of, surface_pressure_cost,
+                               window_apply)
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+
+    nx, L, dx, dt, n, snaps = args.adj_nx, args.adj_nlay, 50e3, 40.0, args.adj_steps, args.adj_snapshots
+    ic = make_initial_state(nx, nx, L, dx, seed=20261203 + 400)
+    cfg = Config(dt_max=dt, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12, dt_remap=5 * dt,
+                 radiation_on=False, physics_on=False)
+    sched = schedule_of(n, dt, 5)
+    c0 = nx // 2 - 16
+    box = (c0, c0 + 32, c0, c0 + 32)
+    p_top = float(ic["ak"][0])
+    res = {"grid": f"{nx}x{nx}x{L}", "dx_km": dx / 1e3, "dt": dt, "window_steps": n, "snapshots": snaps,
+           "remap_every": 5, "repetitions": binomial_reps(n, snaps - 1),
+           "recomputed_steps": recompute_steps(n, snaps)}
+
+    def make(precision):
+        m = Model(nx, nx, L, dx, precision=precision)
+        driver.init_model(m, ic, cfg)
+        return m
+
+    def grad(m, timing):
+        w = Window(m, sched, snapshots=snaps)
+        x0 = [torch.tensor(m.get(name), dtype=torch.float64, device="cuda", requires_grad=True)
+              for name in ("dp", "pt", "ud", "vd")]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = window_apply(w, *x0)
+        J = surface_pressure_cost(out[0], p_top, box)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        J.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if timing is not None:
+            timing.update(forward_s=t1 - t0, gradient_s=t2 - t1, counts=dict(w.counts))
+        return J.item(), [x.grad for x in x0], [x.detach() for x in x0], w
+
+    for precision in ("mixed", "fp64"):
+        m = make(precision)
+        grad(m, None)  # warm-up (allocations, kernel loading)
+        t = {}
+        J, g, x0, w = grad(m, t)
+        res[precision] = {"forward_ms": t["forward_s"] * 1e3, "gradient_ms": t["gradient_s"] * 1e3,
+                          "ratio": t["gradient_s"] / t["forward_s"], "J": J, **t["counts"]}
+        if precision == "fp64":
+            # finite differences of J along pt, ud, vd (T and the winds)
+            rng = np.random.default_rng(7)
+            d = [torch.zeros_like(x0[0]), 1e-3 * torch.tensor(rng.standard_normal(x0[1].shape), device="cuda"),
+                 torch.tensor(rng.standard_normal(x0[2].shape), device="cuda"),
+                 torch.tensor(rng.standard_normal(x0[3].shape), device="cuda")]
+            gd = float(sum((gi * di).sum() for gi, di in zip(g, d)))
+            eps = 1e-7
+
+            def J_at(e):
+                ww = Window(m, sched, snapshots=snaps)
+                out = window_apply(ww, *[xi + e * di for xi, di in zip(x0, d)])
+                return float(surface_pressure_cost(out[0], p_top, box))
+
+            fd = (J_at(eps) - J_at(-eps)) / (2 * eps)
+            res["fd_check"] = {"eps": eps, "grad_dot_d": gd, "fd": fd, "rel_err": abs(fd - gd) / abs(gd)}
+        # one adjoint step alone and one linearised forward step alone
+        m.timer_enable(False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            m.ad_step(dt, False)
+        torch.cuda.synchronize()
+        res[precision]["ad_step_ms"] = (time.perf_counter() - t0) / 5 * 1e3
+        del m, w
+        torch.cuda.empty_cache()
+    res["ratio"] = res["mixed"]["ratio"]
+    return res
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -400,8 +497,6 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": alg, "avg_launch_ms": avg_s * 1e3},
         "kernels_ms_per_step": {k: v[1] / args.steps for k, v in kernels.items()},
-        # the metric's third part: no tangent-linear or adjoint kernels exist
-        # in this library, so there is no ratio to report
         "adjoint_forward_cost_ratio": None,
         "tuning": {"corner_levels_per_workgroup": tuning, "autotune": args.autotune}
     }
@@ -417,6 +512,9 @@ def main():
             mix = out["mixed"]["ms_per_step"] + out["mixed"]["radiation_ms_per_call"] / 10
             out["mixed"]["with_radiation_every_10"] = {"fp64_ms_per_step": f64, "mixed_ms_per_step": mix,
                                                        "speedup": f64 / mix, "derived": True}
+    if args.adjoint == "on":
+        out["adjoint"] = adjoint_timing(args)
+        out["adjoint_forward_cost_ratio"] = out["adjoint"]["ratio"]
     if args.ensemble == "on" and args.path == "fused":
         out["ensemble"] = ensemble_timing(args)
     if args.graph == "on" and args.path == "fused":
