@@ -17,6 +17,10 @@ The forward trajectory runs on the model's own path and precision (the
 mixed form when the model was created so); the linearised step forms its
 forward values in FP64 from each stored state.
 
+torch carries its own HIP runtime beside the one the library links; in a
+process that uses both, torch must bring up the device (torch.cuda.init())
+before the first Model is created.
+
 The library runs on the model's own stream; every hand-over of a tensor
 between torch and the library waits for the side that wrote it (torch's
 device work before the library reads a tensor, the library's stream before

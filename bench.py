@@ -439,6 +439,11 @@ def main():
         sys.exit(2)
     import numpy as np  # noqa: F401
 
+    if args.adjoint == "on":
+        # torch carries its own HIP runtime beside the library's; it has to
+        # bring up the device first in a process that uses both
+        import torch
+        torch.cuda.init()
     from atmos import driver
     from atmos.config import Config
     from atmos.ic import make_initial_state
