@@ -81,6 +81,11 @@ def lib():
         L.orc_sw_column.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, dp, dp]
         L.orc_lw_column.restype = C.c_int
         L.orc_lw_column.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, C.c_double, dp, dp]
+        L.orc_ts_set_jitter.argtypes = [C.c_int]
+        L.orc_ts_column_g.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dp] + \
+            [dp] * 8 + [C.c_double] * 6 + [dp, dp]
+        L.orc_twostream.argtypes = [C.c_long, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dp] + \
+            [dp] * 8 + [dp] * 4 + [C.c_double, C.c_double, dp]
         _LIB = L
     return _LIB
 
@@ -347,3 +352,53 @@ def reference_gas_optics(queries):
         subprocess.check_call([exe, fin, fout])
         res = np.fromfile(fout, dtype="<f8")
     return res[:len(q)], res[len(q):]
+
+
+# ---- general two-stream columns (arrays [column][level], level 0 = surface) ----
+_TS_LAYER = ("play", "tlay", "h2o", "cloud_tau", "cloud_ssa", "cloud_asy")
+
+
+def _ts_arrays(cols):
+    a = {n: np.ascontiguousarray(cols[n], dtype=np.float64) for n in
+         _TS_LAYER + ("plev", "tlev", "mu0", "sfc_alb", "tsfc", "irradiance", "optics")}
+    return a
+
+
+def twostream(cols, guard, jitter=False):
+    """Net flux [column][level]: orc_twostream over the columns of a dict as
+    built by atmos.twostream.synthetic_columns.  jitter: every exponential
+    moved by one ulp (orc_ts_set_jitter), to measure the formulas' own
+    sensitivity to the rounding of exp."""
+    a = _ts_arrays(cols)
+    ncol, L = int(cols["ncol"]), int(cols["nlay"])
+    net = np.zeros((ncol, L + 1))
+    lib().orc_ts_set_jitter(1 if jitter else 0)
+    try:
+        lib().orc_twostream(ncol, L, int(cols["n_sw"]), int(cols["n_lw"]), float(cols["p_ref"]),
+                            float(cols["t_ref"]), _p(a["optics"]), _p(a["play"]), _p(a["plev"]), _p(a["tlay"]),
+                            _p(a["tlev"]), _p(a["h2o"]), _p(a["cloud_tau"]), _p(a["cloud_ssa"]),
+                            _p(a["cloud_asy"]), _p(a["mu0"]), _p(a["sfc_alb"]), _p(a["tsfc"]),
+                            _p(a["irradiance"]), float(cols["gravity"]), float(guard), _p(net))
+    finally:
+        lib().orc_ts_set_jitter(0)
+    return net
+
+
+def twostream_column_g(cols, c, kind, g, guard, jitter=False):
+    """(flux up, flux down) of column c for one g-point (kind 0 shortwave,
+    1 longwave; the shortwave down flux includes the direct beam)."""
+    a = _ts_arrays(cols)
+    L = int(cols["nlay"])
+    row = {n: np.ascontiguousarray(a[n][c]) for n in _TS_LAYER + ("plev", "tlev")}
+    fu, fd = np.zeros(L + 1), np.zeros(L + 1)
+    lib().orc_ts_set_jitter(1 if jitter else 0)
+    try:
+        lib().orc_ts_column_g(L, kind, g, int(cols["n_sw"]), int(cols["n_lw"]), float(cols["p_ref"]),
+                              float(cols["t_ref"]), _p(a["optics"]), _p(row["play"]), _p(row["plev"]),
+                              _p(row["tlay"]), _p(row["tlev"]), _p(row["h2o"]), _p(row["cloud_tau"]),
+                              _p(row["cloud_ssa"]), _p(row["cloud_asy"]), float(a["mu0"][c]),
+                              float(a["sfc_alb"][c]), float(a["tsfc"][c]), float(a["irradiance"][c]),
+                              float(cols["gravity"]), float(guard), _p(fu), _p(fd))
+    finally:
+        lib().orc_ts_set_jitter(0)
+    return fu, fd

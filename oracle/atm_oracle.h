@@ -89,6 +89,19 @@ int orc_lw_column(int nlay, int g, const double *play, const double *dm,
                   const double *T, const double *Tlev, double ts,
                   double *flux_up, double *flux_dn);
 
+/* general two-stream columns (synthetic correlated-k workload) */
+void orc_ts_set_jitter(int on);
+void orc_ts_column_g(int L, int kind, int g, int nsw, int nlw, double pref, double tref, const double *optics,
+                     const double *play, const double *plev, const double *tlay, const double *tlev,
+                     const double *q, const double *ctau, const double *cssa, const double *casy, double mu0,
+                     double alb, double tsfc, double irr, double gravity, double guard, double *flux_up,
+                     double *flux_dn);
+void orc_twostream(long ncol, int L, int nsw, int nlw, double pref, double tref, const double *optics,
+                   const double *play, const double *plev, const double *tlay, const double *tlev, const double *q,
+                   const double *ctau, const double *cssa, const double *casy, const double *mu0,
+                   const double *alb, const double *tsfc, const double *irr, double gravity, double guard,
+                   double *net_flux);
+
 #ifdef __cplusplus
 }
 #endif
