@@ -20,6 +20,8 @@ U[0.05, 0.3], absorption coefficients log-uniform in [1e-6, 1e2] m^2/kg at
 from the layer temperatures.
 
 `twostream` runs the library on device arrays; there is no CPU path.
+torch brings its own HIP runtime: in a process that uses both, torch must
+bring up the device (torch.cuda.init()) before the library's first call.
 """
 from __future__ import annotations
 
@@ -34,7 +36,7 @@ GUARD_FP64 = 1e-4
 LAYER_FIELDS = ("play", "tlay", "h2o", "cloud_tau", "cloud_ssa", "cloud_asy")
 LEVEL_FIELDS = ("plev", "tlev")
 COLUMN_FIELDS = ("mu0", "sfc_alb", "tsfc", "irradiance")
-STATS = ("guarded_layers", "bad_inputs")
+STATS = ("guarded_layers", "bad_columns")
 
 
 def optics_size(n_sw: int, n_lw: int) -> int:
@@ -86,9 +88,13 @@ def synthetic_columns(ncol: int, nlay: int = 64, n_sw: int = 32, n_lw: int = 32,
     zl = height(play)
     tlay = temperature(zl) + rng.normal(0.0, 0.5, (ncol, nlay))
     tlev = np.empty((ncol, nlay + 1))
-    tlev[:, 1:-1] = 0.5 * (tlay[:, :-1] + tlay[:, 1:])
-    tlev[:, 0] = 1.5 * tlay[:, 0] - 0.5 * tlay[:, 1]
-    tlev[:, -1] = 1.5 * tlay[:, -1] - 0.5 * tlay[:, -2]
+    if nlay > 1:
+        tlev[:, 1:-1] = 0.5 * (tlay[:, :-1] + tlay[:, 1:])
+        tlev[:, 0] = 1.5 * tlay[:, 0] - 0.5 * tlay[:, 1]
+        tlev[:, -1] = 1.5 * tlay[:, -1] - 0.5 * tlay[:, -2]
+    else:
+        tlev[:, 0] = tlay[:, 0] + 3.0
+        tlev[:, 1] = tlay[:, 0] - 3.0
     h2o = np.maximum(1e-2 * np.exp(-zl / 2e3), h2o_floor)
     cloud = rng.uniform(size=(ncol, nlay)) < cloud_fraction
     cloud_tau = np.where(cloud, np.exp(rng.uniform(np.log(0.1), np.log(50.0), (ncol, nlay))), 0.0)
@@ -140,8 +146,39 @@ def _lib():
                 f.argtypes = [C.POINTER(Columns), C.c_int, C.c_void_p, C.c_void_p, C.POINTER(Stats), C.c_void_p]
                 f.restype = C.c_int
         L.atm_twostream_last_error.restype = C.c_char_p
+        L.atm_stream_synchronize.argtypes = [C.c_void_p]
+        L.atm_stream_timer_begin.argtypes = [C.c_void_p]
+        L.atm_stream_timer_end.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L._ts_declared = True
     return L
+
+
+_STREAM = None
+
+
+def stream():
+    """The library-runtime stream the columns run on (created once)."""
+    global _STREAM
+    if _STREAM is None:
+        from .model import stream_create
+        _STREAM = stream_create()
+    return _STREAM
+
+
+def synchronize():
+    _lib().atm_stream_synchronize(C.c_void_p(stream()))
+
+
+def timer_begin():
+    if _lib().atm_stream_timer_begin(C.c_void_p(stream())) != 0:
+        raise RuntimeError("atm_stream_timer_begin failed")
+
+
+def timer_end() -> float:
+    ms = C.c_double(0.0)
+    if _lib().atm_stream_timer_end(C.c_void_p(stream()), C.byref(ms)) != 0:
+        raise RuntimeError("atm_stream_timer_end failed")
+    return ms.value
 
 
 def to_device(cols: dict, device="cuda"):
@@ -170,11 +207,13 @@ def _struct(cols: dict, guard: float) -> Columns:
 
 
 def twostream(cols: dict, precision: str = "fp64", guard: float = GUARD_FP64, fluxes: bool = False,
-              noalloc: bool = False, out=None):
+              noalloc: bool = False, stats: bool = True, out=None, wait: bool = True):
     """Net flux [column][level] (and, with fluxes, the (4, ncol, nlay + 1)
     shortwave up, shortwave down, longwave up, longwave down) of device
-    columns (to_device); returns (net, fluxes or None, stats dict).  Runs on
-    torch's current stream and waits for it."""
+    columns (to_device); returns (net, fluxes or None, stats dict or None).
+    The library runs on its own stream (stream()); torch's work is waited
+    for first, and the library's after the call unless wait is False (then
+    call synchronize() before torch reads the outputs)."""
     import torch
     L = _lib()
     s = _struct(cols, guard)
@@ -183,10 +222,12 @@ def twostream(cols: dict, precision: str = "fp64", guard: float = GUARD_FP64, fl
     fl = torch.empty((4,) + shape, dtype=torch.float64, device="cuda") if fluxes else None
     st = Stats()
     fn = L.atm_twostream_noalloc if noalloc else L.atm_twostream
-    stream = torch.cuda.current_stream().cuda_stream
+    torch.cuda.synchronize()
     rc = fn(C.byref(s), {"fp64": 0, "mixed": 1}[precision], net.data_ptr(), fl.data_ptr() if fluxes else None,
-            C.byref(st), stream)
+            C.byref(st) if stats else None, C.c_void_p(stream()))
     if rc != 0:
         from .model import AtmError
         raise AtmError(rc, L.atm_twostream_last_error().decode())
-    return net, fl, {n: int(getattr(st, n)) for n in STATS}
+    if wait:
+        synchronize()
+    return net, fl, ({n: int(getattr(st, n)) for n in STATS} if stats else None)

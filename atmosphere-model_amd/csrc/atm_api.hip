@@ -219,6 +219,42 @@ int atm_stream_destroy(void *s) {
     return hipStreamDestroy((hipStream_t)s) == hipSuccess ? ATM_OK : ATM_ERR_HIP;
 }
 
+int atm_stream_synchronize(void *s) {
+    return hipStreamSynchronize((hipStream_t)s) == hipSuccess ? ATM_OK : ATM_ERR_HIP;
+}
+
+namespace {
+struct StreamTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    bool begun = false;
+    ~StreamTimer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+thread_local StreamTimer g_stream_timer;
+}  // namespace
+
+int atm_stream_timer_begin(void *s) {
+    StreamTimer &t = g_stream_timer;
+    if (!t.a && (hipEventCreate(&t.a) != hipSuccess || hipEventCreate(&t.b) != hipSuccess)) return ATM_ERR_HIP;
+    if (hipEventRecord(t.a, (hipStream_t)s) != hipSuccess) return ATM_ERR_HIP;
+    t.begun = true;
+    return ATM_OK;
+}
+
+int atm_stream_timer_end(void *s, double *ms) {
+    StreamTimer &t = g_stream_timer;
+    if (!ms || !t.begun) return ATM_ERR_ARGUMENT;
+    t.begun = false;
+    float f = 0.f;
+    if (hipEventRecord(t.b, (hipStream_t)s) != hipSuccess || hipEventSynchronize(t.b) != hipSuccess ||
+        hipEventElapsedTime(&f, t.a, t.b) != hipSuccess)
+        return ATM_ERR_HIP;
+    *ms = f;
+    return ATM_OK;
+}
+
 int atm_set_stream(atm_ctx *c, void *s) {
     CHECK_CTX(c);
     CHECK_NOT_CAPTURING(c);

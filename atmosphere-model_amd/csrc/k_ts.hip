@@ -1,0 +1,611 @@
+// k_ts.hip -- general two-stream correlated-k columns (gfx950).
+//
+// src/mod_radiation.f90: solve_shortwave_column (158-268),
+// solve_longwave_column (270-347), calculate_fluxes (349-381) and
+// delta_eddington_rescaling (383-398), over per-layer cloud optics, a
+// per-column zenith angle, surface albedo and surface temperature, and an
+// analytic k-distribution (the contract is in include/atm.h).
+//
+// One wavefront per column, as in k_rad.hip: per pass over RG g-points,
+//   optics  every lane takes its layers (lane, lane + 64, ...): gas, Rayleigh
+//           and cloud optical depths, delta-Eddington, the two-stream
+//           coefficients, R, T and the shortwave beam factors (or the
+//           longwave sources) into LDS;
+//   chains  16 lanes per g-point run the beam and the adding method over
+//           LDS (rad_chain.hpp);
+//   sums    every lane adds the pass's fluxes of its levels into its
+//           accumulators in g order.
+// Inputs are read with the level index contiguous, so a wavefront's load of
+// one array is one contiguous row of the column.
+//
+// FP64 (R = double): the reference's expressions with its operation order.
+// MIXED (R = float): the same optics in float with the hardware exp,
+// reciprocal and square root, except the longwave sources, whose
+// (1 + R - T) / tau cancels for thin layers (tau ~ 1e-13 in the upper
+// layers of the synthetic workload: R = 0 and T = 1 in float there).  With
+// u = 1 - e^(-lambda tau) from expm1 and D = lambda (1 + e^2) +
+// gamma1 u (2 - u) (the reference's denominator, rearranged):
+//   R = gamma2 u (2 - u) / D,          T = 2 lambda e / D,
+//   1 - T = (lambda u^2 + gamma1 u (2 - u)) / D,
+//   q = (1 + R - T) / (tau (gamma1 + gamma2))
+//     = (lambda u^2 + (gamma1 + gamma2) u (2 - u)) / (D tau (gamma1 + gamma2)),
+//   Su / pi = (B1 - B0) (1 - q) + (1 - T) B0 - R B1,
+//   Sd / pi = (B0 - B1) (1 - q) + (1 - T) B1 - R B0,
+// algebraically the reference's sources, each term free of cancellation.
+// Layers that scatter nearly conservatively (Rayleigh above the water
+// vapour: 1 - omega ~ 1e-9) would round omega to 1 in float, where
+// gamma1 = gamma2, lambda = 0 and the reference's denominator
+// lambda + gamma1 + (lambda - gamma1) e^2 vanishes.  The mixed form carries
+// the co-albedo 1 - omega = tau_abs / tau instead: after the scaling it is
+// (1 - omega) / (1 - omega f); gamma1 - gamma2 = (1 - omega) (2 - g^2) /
+// (1 - g^2 (1 - mu0)) for the direct beam and 2 (1 - omega) for the diffuse
+// and thermal streams, lambda^2 = (gamma1 - gamma2)(gamma1 + gamma2), and
+// the denominator is lambda (1 + e^2) + gamma1 (1 - e^2) with
+// 1 - e^2 = u (2 - u).
+// The beam, the adding method and the flux sums stay in double.
+#include <cstdint>
+#include <string>
+#include <type_traits>
+
+#include "atm_common.hpp"
+#include "rad_chain.hpp"
+
+namespace {
+
+using namespace radchain;
+
+constexpr double TS_SIGMA = 5.670374419e-8;  // Stefan-Boltzmann, W m-2 K-4
+constexpr double TS_EMISS = (double)0.99f;   // surface emissivity (binary32 literal)
+constexpr int TS_MAXL = 64 * RMAXL;
+
+struct TsArgs {
+    long long ncol;
+    int L, nsw, nlw;
+    double pref, tref, gravity, guard;
+    const double *optics;
+    const double *play, *tlay, *h2o, *ctau, *cssa, *casy, *plev, *tlev;
+    const double *mu0, *alb, *tsfc, *irr;
+    double *net, *fluxes;
+    unsigned *status;  // per column: guarded layers (bits 0-30), bad input (bit 31); null: not kept
+};
+
+__device__ __forceinline__ float texp(float x) { return __expf(x); }
+__device__ __forceinline__ double texp(double x) { return exp(x); }
+__device__ __forceinline__ float tsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
+
+// Planck source of one g-point at a level: f_g sigma T^4 / pi, with T^4 as
+// (T T)(T T) and the quotient correctly rounded (div_by with 1 / pi).
+__device__ __forceinline__ double planck(double pf, double t4, double zpi) {
+    return div_by((pf * TS_SIGMA) * t4, ATM_PI, zpi);
+}
+
+// delta_eddington_rescaling (383-398)
+template <class R>
+__device__ __forceinline__ void delta_eddington(R &tau, R &ssa, R &asy) {
+    const R f = asy * asy;
+    const R coef = R(1) - ssa * f;
+    tau = tau * coef;
+    ssa = odiv(ssa * (R(1) - f), coef);
+    asy = odiv(asy, R(1) + asy);
+}
+
+// one shortwave layer of one g-point (158-260); returns true where the
+// guard moved lambda.  Writes Tdir, Rdif, Tdif (direct beam) and the
+// diffuse R, T.
+template <class R>
+__device__ __forceinline__ bool sw_layer(R tau, R ssa, R asy, R mu0, R zmu, R guard, R &Tdir, R &Rdif, R &Tdif,
+                                         R &Rd, R &Td) {
+    delta_eddington(tau, ssa, asy);
+    const R asy2 = asy * asy;
+    R back = odiv(R(2) - R(3) * asy * mu0, R(4));
+    R tmp1 = R(4) * (R(1) - asy2 * (R(1) - mu0));
+    R gamma3 = ssa * asy2;
+    R gamma4 = R(4) * back + R(3) * asy;
+    R tmp2 = R(3) * asy;
+    R gamma1 = odiv(R(7) - R(3) * asy2 - ssa * (R(4) + tmp2) + gamma3 * gamma4, tmp1);
+    R gamma2 = odiv(R(-1) + asy2 + ssa * (R(4) - tmp2) + gamma3 * (gamma4 - R(4)), tmp1);
+    gamma3 = back;
+    gamma4 = R(1) - gamma3;
+    const R alpha1 = gamma1 * gamma4 + gamma2 * gamma3;
+    const R alpha2 = gamma2 * gamma3 + gamma2 * gamma4;  // the reference's alpha2 (214)
+    R lambda = tsqrt(gamma1 * gamma1 - gamma2 * gamma2);
+    R lam_mu = lambda * mu0;
+    bool guarded = false;
+    if (fabs(R(1) - lam_mu * lam_mu) < guard) {
+        lambda = lambda * (R(1) - R(2) * guard);
+        lam_mu = lambda * mu0;
+        guarded = true;
+    }
+    const R e_minus = texp(-lambda * tau);
+    R e_2minus = e_minus * e_minus;
+    // -tau / mu0: the correctly rounded quotient in FP64 (div_by), the
+    // reciprocal's product in float
+    R q;
+    if constexpr (std::is_same<R, double>::value)
+        q = div_by(-tau, mu0, zmu);
+    else
+        q = -tau * zmu;
+    Tdir = texp(q);
+    tmp1 = odiv(ssa, (R(1) - lam_mu * lam_mu) * (lambda + gamma1 + (lambda - gamma1) * e_2minus));
+    tmp2 = lambda * gamma3;
+    Rdif = tmp1 * ((R(1) - lam_mu) * (alpha2 + tmp2) - (R(1) + lam_mu) * (alpha2 - tmp2) * e_2minus -
+                   R(2) * lambda * (gamma3 - alpha2 * mu0) * e_minus * Tdir);
+    tmp2 = lambda * gamma4;
+    Tdif = tmp1 * (R(2) * lambda * (gamma4 + alpha1 * mu0) * e_minus -
+                   Tdir * ((R(1) + lam_mu) * (alpha1 + tmp2) - (R(1) - lam_mu) * (alpha1 - tmp2) * e_2minus));
+    // diffuse R/T: backscatter of Ritter & Geleyn; e^(-lambda tau) of the
+    // direct beam's lambda (254-260)
+    back = odiv(R(4) + asy, R(8) * (R(1) + asy));
+    const R g1 = R(2) * (R(1) - ssa * (R(1) - back));
+    const R g2 = R(2) * ssa * back;
+    lambda = tsqrt(g1 * g1 - g2 * g2);
+    e_2minus = e_minus * e_minus;
+    tmp1 = odiv(R(1), lambda + g1 + (lambda - g1) * e_2minus);
+    Rd = tmp1 * g2 * (R(1) - e_2minus);
+    Td = R(2) * tmp1 * lambda * e_minus;
+    return guarded;
+}
+
+// the mixed form's shortwave layer: scaling, coefficients and the diffuse
+// R/T in float with the co-albedo carried; the direct-beam R/T from those
+// float values in double.  Their closed form cancels O(1) terms down to
+// O(tau) and divides by lambda (near zero for nearly conservative layers)
+// and by 1 - (lambda mu0)^2, so it is evaluated in double from
+// exponentials accurate both relatively and near 1: e = 1 - u with
+// u = 1 - e^(-z) from expm1 for z < 1/2, e^(-z) itself above; within 1e-2
+// of lambda mu0 = 1, where the numerator's own cancellation needs e and
+// T_dir consistent to double precision, both exponentials in double.
+__device__ __forceinline__ bool sw_layer_mixed(float tau, float ssa, float coalb, float asy, double mu0d, float zmu,
+                                               float guard, double &Tdir, double &Rdif, double &Tdif, float &Rd,
+                                               float &Td) {
+    const float mu0 = (float)mu0d;
+    const float f = asy * asy;
+    const float den = 1.f - ssa * f;
+    const float rden = __builtin_amdgcn_rcpf(den);
+    tau = tau * den;
+    ssa = ssa * (1.f - f) * rden;
+    coalb = coalb * rden;
+    asy = asy * __builtin_amdgcn_rcpf(1.f + asy);
+    const float asy2 = asy * asy;
+    float back = (2.f - 3.f * asy * mu0) * 0.25f;
+    const float rd4 = __builtin_amdgcn_rcpf(1.f - asy2 * (1.f - mu0));
+    float gamma3 = ssa * asy2;
+    float gamma4 = 4.f * back + 3.f * asy;
+    const float tmp2a = 3.f * asy;
+    const float gamma1 = (7.f - 3.f * asy2 - ssa * (4.f + tmp2a) + gamma3 * gamma4) * 0.25f * rd4;
+    const float gamma2 = (-1.f + asy2 + ssa * (4.f - tmp2a) + gamma3 * (gamma4 - 4.f)) * 0.25f * rd4;
+    const float dgam = coalb * (2.f - asy2) * rd4;  // gamma1 - gamma2
+    gamma3 = back;
+    gamma4 = 1.f - gamma3;
+    const float alpha1 = gamma1 * gamma4 + gamma2 * gamma3;
+    const float alpha2 = gamma2 * gamma3 + gamma2 * gamma4;
+    float lambda = __builtin_sqrtf(dgam * (gamma1 + gamma2));
+    float lam_mu = lambda * mu0;
+    bool guarded = false;
+    if (fabsf(1.f - lam_mu * lam_mu) < guard) {
+        lambda = lambda * (1.f - 2.f * guard);
+        lam_mu = lambda * mu0;
+        guarded = true;
+    }
+    const float z = lambda * tau, zt = tau * zmu;
+    const float u = -expm1f(-z);
+    const double w = (double)u * (2. - (double)u);  // 1 - e^2
+    const double lam = lambda, lm = lam * mu0d;
+    double e, td;
+    if (fabsf(1.f - lam_mu * lam_mu) < 1e-2f) {
+        e = exp(-lam * (double)tau);
+        td = exp(-(double)tau / mu0d);
+    } else {
+        e = z < 0.5f ? 1. - (double)u : (double)expf(-z);
+        td = zt < 0.5f ? 1. + (double)expm1f(-zt) : (double)expf(-zt);
+    }
+    const double e2 = e * e;
+    const double g3 = gamma3, g4 = gamma4, a1 = alpha1, a2 = alpha2;
+    const double t1 = (double)ssa / ((1. - lm * lm) * (lam * (1. + e2) + (double)gamma1 * w));
+    Rdif = t1 * ((1. - lm) * (a2 + lam * g3) - (1. + lm) * (a2 - lam * g3) * e2 - 2. * lam * (g3 - a2 * mu0d) * e * td);
+    Tdif = t1 * (2. * lam * (g4 + a1 * mu0d) * e - td * ((1. + lm) * (a1 + lam * g4) - (1. - lm) * (a1 - lam * g4) * e2));
+    Tdir = td;
+    // diffuse R/T (float): e^(-lambda tau) of the direct beam's lambda
+    back = (4.f + asy) * __builtin_amdgcn_rcpf(8.f * (1.f + asy));
+    const float g1 = 2.f * (1.f - ssa * (1.f - back));
+    const float g2 = 2.f * ssa * back;
+    const float lam_d = __builtin_sqrtf(2.f * coalb * (g1 + g2));
+    const float ef = (float)e, wf = (float)w;
+    const float rD = __builtin_amdgcn_rcpf(lam_d * (1.f + ef * ef) + g1 * wf);
+    Rd = g2 * wf * rD;
+    Td = 2.f * lam_d * ef * rD;
+    return guarded;
+}
+
+// one longwave layer of one g-point (270-339): R, T and the sources for
+// the Planck values B0 (lower level) and B1 (upper level)
+__device__ __forceinline__ void lw_layer(double tau, double ssa, double asy, double B0, double B1, double &Rl,
+                                         double &Tl, double &Su, double &Sd) {
+    delta_eddington(tau, ssa, asy);
+    const double back = (4. - 3. * asy) / 8.;
+    const double g1 = 2. * (1. - ssa * (1. - back));
+    const double g2 = 2. * ssa * back;
+    const double lambda = sqrt(g1 * g1 - g2 * g2);
+    const double e_minus = exp(-lambda * tau);
+    const double e_2minus = e_minus * e_minus;
+    double tmp1 = 1. / (lambda + g1 + (lambda - g1) * e_2minus);
+    Rl = tmp1 * g2 * (1. - e_2minus);
+    Tl = 2. * tmp1 * lambda * e_minus;
+    const double pdiff = (1. + Rl - Tl) * (B0 - B1) / (tau * (g1 + g2));
+    tmp1 = 1. - Rl;
+    Su = ATM_PI * (tmp1 * B1 - Tl * B0 + pdiff);
+    Sd = ATM_PI * (tmp1 * B0 - Tl * B1 - pdiff);
+}
+
+// the mixed form's longwave layer (the rearrangement above)
+__device__ __forceinline__ void lw_layer_mixed(float tau, float ssa, float coalb, float asy, double B0d, double B1d,
+                                               double &Rl, double &Tl, double &Su, double &Sd) {
+    const float f = asy * asy;
+    const float den = 1.f - ssa * f;
+    const float rden = __builtin_amdgcn_rcpf(den);
+    tau = tau * den;
+    ssa = ssa * (1.f - f) * rden;
+    coalb = coalb * rden;
+    asy = asy * __builtin_amdgcn_rcpf(1.f + asy);
+    const float back = (4.f - 3.f * asy) * 0.125f;
+    const float g1 = 2.f * (1.f - ssa * (1.f - back));
+    const float g2 = 2.f * ssa * back;
+    const float gs = g1 + g2;
+    const float lambda = __builtin_sqrtf(2.f * coalb * gs);  // (g1 - g2) = 2 (1 - omega)
+    const float z = lambda * tau;
+    const float u = -expm1f(-z);
+    const float e = 1.f - u;
+    const float w = u * (2.f - u);  // 1 - e^2
+    const float inv = __builtin_amdgcn_rcpf(lambda * (1.f + e * e) + g1 * w);
+    const float r = g2 * w * inv;
+    const float t = 2.f * lambda * e * inv;
+    const float omt = (lambda * u * u + g1 * w) * inv;
+    const float q = (lambda * u * u + gs * w) * inv * __builtin_amdgcn_rcpf(tau * gs);
+    const float B0 = (float)B0d, B1 = (float)B1d, dB = (float)(B1d - B0d);
+    const float oq = 1.f - q;
+    Rl = r;
+    Tl = t;
+    Su = ATM_PI * (double)(dB * oq + omt * B0 - r * B1);
+    Sd = ATM_PI * (double)(-dB * oq + omt * B1 - r * B0);
+}
+
+__device__ __forceinline__ bool finite_pos(double x) { return x > 0. && x < __builtin_inf(); }
+
+// NL layers per lane (L <= 64 NL)
+template <int NL, class R>
+__global__ void __launch_bounds__(64, NL == 1 ? 3 : 1) k_ts_columns(TsArgs a) {
+    extern __shared__ double lds[];
+    const long long col = blockIdx.x;
+    const int lane = threadIdx.x, L = a.L, N = L + 1;
+    auto arr = [&](int q, int gi) { return lds + (size_t)(q * RG + gi) * N; };
+    // arrays: 0 Tdir -> beam, 1 Rdif -> Su, 2 Tdif -> Sd, 3 R -> up, 4 T -> down
+    double *acc = lds + (size_t)RARR * RG * N;  // sw up, sw down, lw up, lw down
+    const double *play = a.play + col * L, *tlay = a.tlay + col * L, *h2o = a.h2o + col * L;
+    const double *ctau = a.ctau + col * L, *cssa = a.cssa + col * L, *casy = a.casy + col * L;
+    const double *plev = a.plev + col * N, *tlev = a.tlev + col * N;
+    const double mu0 = a.mu0[col], alb = a.alb[col], tsfc = a.tsfc[col], irr = a.irr[col];
+    const double zmu = 1. / mu0, zpi = 1. / ATM_PI;
+    const double *csw = a.optics, *clw = a.optics + 5 * a.nsw, *ray = a.optics + 5 * (a.nsw + a.nlw);
+    const double *sol = ray + a.nsw, *pfr = sol + a.nsw;
+    bool bad = !(mu0 > 0. && mu0 <= 1.);
+    unsigned guarded = 0;
+    constexpr int NV = NL + 1;
+    // per-lane inputs of its layers K = lane + 64 v
+    R xv[NL], yv[NL];
+    double dmv[NL], qv[NL], ctv[NL], csv[NL], cav[NL], t4lo[NL], t4hi[NL];
+#pragma unroll
+    for (int v = 0; v < NL; v++) {
+        const int K = lane + 64 * v;
+        xv[v] = yv[v] = R(0);
+        dmv[v] = qv[v] = ctv[v] = csv[v] = cav[v] = t4lo[v] = t4hi[v] = 0.;
+        if (K < L) {
+            xv[v] = R(log(play[K] / a.pref));
+            yv[v] = R(tlay[K] - a.tref);
+            dmv[v] = (plev[K] - plev[K + 1]) / a.gravity;
+            qv[v] = h2o[K];
+            ctv[v] = ctau[K];
+            csv[v] = cssa[K];
+            cav[v] = casy[K];
+            const double tl = tlev[K], th = tlev[K + 1];
+            t4lo[v] = (tl * tl) * (tl * tl);
+            t4hi[v] = (th * th) * (th * th);
+            if (!finite_pos(dmv[v])) bad = true;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const int k = lane + 64 * v;
+        if (k <= L) acc[k] = acc[N + k] = acc[2 * N + k] = acc[3 * N + k] = 0.;
+    }
+    const R guard = R(a.guard);
+    for (int g0 = 0; g0 < a.nsw; g0 += RG) {
+        const int ng = min(RG, a.nsw - g0);
+        // ---- shortwave optics (158-260) ----
+#pragma unroll
+        for (int v = 0; v < NL; v++) {
+            const int K = lane + 64 * v;
+            if (K >= L) continue;
+            const R x = xv[v], y = yv[v];
+            const double sca_c = csv[v] * ctv[v], abs_c = ctv[v] * (1. - csv[v]);
+#pragma unroll 1
+            for (int gi = 0; gi < ng; gi++) {
+                const int gp = g0 + gi;
+                const double *c = csw + 5 * gp;
+                const R lnk = R(c[0]) + R(c[1]) * x + R(c[2]) * x * x + R(c[3]) * y + R(c[4]) * y * y;
+                const double tau_g = (double)texp(lnk) * qv[v] * dmv[v];
+                const double tau_r = ray[gp] * dmv[v];
+                const double sca = tau_r + sca_c;
+                const double tau = tau_g + tau_r + ctv[v];
+                const double tau_abs = tau_g + abs_c;
+                if (!finite_pos(tau) || !(tau_abs > 0.)) bad = true;
+                double Tdir, Rdif, Tdif;
+                R Rd, Td;
+                bool gd;
+                if constexpr (std::is_same<R, double>::value) {
+                    const double ssa = sca / tau;
+                    const double asy = sca > 0. ? sca_c * cav[v] / sca : 0.;
+                    gd = sw_layer<double>(tau, ssa, asy, mu0, zmu, guard, Tdir, Rdif, Tdif, Rd, Td);
+                } else {
+                    const float rt = __builtin_amdgcn_rcpf((float)tau);
+                    const float asy = sca > 0. ? (float)(sca_c * cav[v]) * __builtin_amdgcn_rcpf((float)sca) : 0.f;
+                    gd = sw_layer_mixed((float)tau, (float)sca * rt, (float)tau_abs * rt, asy, mu0, (float)zmu, guard,
+                                        Tdir, Rdif, Tdif, Rd, Td);
+                }
+                if (gd) guarded++;
+                arr(0, gi)[K] = Tdir;
+                arr(1, gi)[K] = Rdif;
+                arr(2, gi)[K] = Tdif;
+                arr(3, gi)[K] = Rd;
+                arr(4, gi)[K] = Td;
+            }
+        }
+        __syncthreads();
+        {  // beam down the column (221-249), then the adding method (349-381)
+            const int gi = lane / CG, j = lane % CG, gp = g0 + gi;
+            if (gi < ng) {
+                double *B = arr(0, gi), *Su = arr(1, gi), *Sd = arr(2, gi);
+                const double toa = beam_group(L, j, mu0 * (irr * sol[gp]), B, Su, Sd);
+                adding_group<true, 4 * NL>(L, j, arr(3, gi), arr(4, gi), Su, Sd, B, toa, 0., alb);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const int k = lane + 64 * v;
+            if (k > L) continue;
+            double su = acc[k], sd = acc[N + k];
+            for (int gi = 0; gi < ng; gi++) {
+                su = su + arr(3, gi)[k];
+                sd = sd + arr(4, gi)[k];
+            }
+            acc[k] = su;
+            acc[N + k] = sd;
+        }
+        __syncthreads();
+    }
+    const double ts4 = (tsfc * tsfc) * (tsfc * tsfc);
+    for (int g0 = 0; g0 < a.nlw; g0 += RG) {
+        const int ng = min(RG, a.nlw - g0);
+        // ---- longwave optics (270-339) ----
+#pragma unroll
+        for (int v = 0; v < NL; v++) {
+            const int K = lane + 64 * v;
+            if (K >= L) continue;
+            const R x = xv[v], y = yv[v];
+            const double sca = csv[v] * ctv[v], abs_c = ctv[v] * (1. - csv[v]);
+#pragma unroll 1
+            for (int gi = 0; gi < ng; gi++) {
+                const int gp = g0 + gi;
+                const double *c = clw + 5 * gp;
+                const R lnk = R(c[0]) + R(c[1]) * x + R(c[2]) * x * x + R(c[3]) * y + R(c[4]) * y * y;
+                const double tau_g = (double)texp(lnk) * qv[v] * dmv[v];
+                const double tau = tau_g + ctv[v];
+                const double tau_abs = tau_g + abs_c;
+                if (!finite_pos(tau) || !(tau_abs > 0.)) bad = true;
+                const double B0 = planck(pfr[gp], t4lo[v], zpi), B1 = planck(pfr[gp], t4hi[v], zpi);
+                double Rl, Tl, Su, Sd;
+                if constexpr (std::is_same<R, double>::value) {
+                    lw_layer(tau, sca / tau, sca > 0. ? cav[v] : 0., B0, B1, Rl, Tl, Su, Sd);
+                } else {
+                    const float rt = __builtin_amdgcn_rcpf((float)tau);
+                    lw_layer_mixed((float)tau, (float)sca * rt, (float)tau_abs * rt, sca > 0. ? (float)cav[v] : 0.f, B0,
+                                   B1, Rl, Tl, Su, Sd);
+                }
+                arr(3, gi)[K] = Rl;
+                arr(4, gi)[K] = Tl;
+                arr(1, gi)[K] = Su;
+                arr(2, gi)[K] = Sd;
+            }
+        }
+        __syncthreads();
+        {
+            const int gi = lane / CG, j = lane % CG, gp = g0 + gi;
+            if (gi < ng) {
+                const double Bs = planck(pfr[gp], ts4, zpi);
+                adding_group<true, 4 * NL>(L, j, arr(3, gi), arr(4, gi), arr(1, gi), arr(2, gi), nullptr, 0.,
+                                           TS_EMISS * ATM_PI * Bs, 1. - TS_EMISS);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const int k = lane + 64 * v;
+            if (k > L) continue;
+            double lu = acc[2 * N + k], ld = acc[3 * N + k];
+            for (int gi = 0; gi < ng; gi++) {
+                lu = lu + arr(3, gi)[k];
+                ld = ld + arr(4, gi)[k];
+            }
+            acc[2 * N + k] = lu;
+            acc[3 * N + k] = ld;
+        }
+        __syncthreads();
+    }
+    const size_t o = (size_t)col * N;
+    const size_t plane = (size_t)a.ncol * N;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const int k = lane + 64 * v;
+        if (k > L) continue;
+        const double su = acc[k], sd = acc[N + k], lu = acc[2 * N + k], ld = acc[3 * N + k];
+        a.net[o + k] = (sd + ld) - (su + lu);
+        if (a.fluxes) {
+            a.fluxes[o + k] = su;
+            a.fluxes[plane + o + k] = sd;
+            a.fluxes[2 * plane + o + k] = lu;
+            a.fluxes[3 * plane + o + k] = ld;
+        }
+    }
+    if (a.status) {
+        // the column's guard count and bad-input flag: a wave sum by
+        // butterfly shuffles, written by lane 0
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) guarded += __shfl_xor(guarded, m, 64);
+        const bool any_bad = __ballot(bad) != 0;
+        if (lane == 0) a.status[col] = (guarded & 0x7fffffffu) | (any_bad ? 0x80000000u : 0u);
+    }
+}
+
+// totals of the per-column status words: [0] guarded layers, [1] bad
+// columns; one workgroup, integer sums (order-independent)
+__global__ void __launch_bounds__(1024) k_ts_totals(const unsigned *status, long long ncol,
+                                                    unsigned long long *tot) {
+    __shared__ unsigned long long sg[1024], sb[1024];
+    unsigned long long g = 0, b = 0;
+    for (long long c = threadIdx.x; c < ncol; c += 1024) {
+        const unsigned s = status[c];
+        g += s & 0x7fffffffu;
+        b += s >> 31;
+    }
+    sg[threadIdx.x] = g;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 512; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sg[threadIdx.x] += sg[threadIdx.x + w];
+            sb[threadIdx.x] += sb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        tot[0] = sg[0];
+        tot[1] = sb[0];
+    }
+}
+
+thread_local std::string g_ts_error;
+
+int ts_fail(int code, const std::string &msg) {
+    g_ts_error = msg;
+    return code;
+}
+
+// false where this runtime knows p as host memory; memory it does not know
+// (another runtime's allocation) is taken as a device address
+bool not_host_ptr(const void *p) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return true;
+    }
+    return at.type != hipMemoryTypeHost;
+}
+
+// Checks the column description and the outputs; "" when usable.
+std::string ts_check(const atm_ts_columns *c, int precision, const double *net, const double *fluxes) {
+    if (!c) return "cols is NULL";
+    if (precision != ATM_PRECISION_FP64 && precision != ATM_PRECISION_MIXED) return "precision must be FP64 or MIXED";
+    if (c->ncol < 0 || c->ncol > 0x7fffffffLL) return "ncol must be in [0, 2^31 - 1]";
+    if (c->nlay < 1 || c->nlay > TS_MAXL) return "nlay must be in [1, " + std::to_string(TS_MAXL) + "]";
+    if (c->n_sw < 0 || c->n_lw < 0) return "n_sw and n_lw must be >= 0";
+    if (!(c->p_ref > 0.) || !(c->gravity > 0.)) return "p_ref and gravity must be > 0";
+    if (!(c->guard >= 0. && c->guard < 0.5)) return "guard must be in [0, 0.5)";
+    if (c->ncol == 0) return "";
+    const struct {
+        const void *p;
+        const char *name;
+    } ptrs[] = {{c->optics, "optics"}, {c->play, "play"}, {c->tlay, "tlay"}, {c->h2o, "h2o"},
+                {c->cloud_tau, "cloud_tau"}, {c->cloud_ssa, "cloud_ssa"}, {c->cloud_asy, "cloud_asy"},
+                {c->plev, "plev"}, {c->tlev, "tlev"}, {c->mu0, "mu0"}, {c->sfc_alb, "sfc_alb"},
+                {c->tsfc, "tsfc"}, {c->irradiance, "irradiance"}, {net, "net_flux"}};
+    for (const auto &q : ptrs)
+        if (!q.p || !not_host_ptr(q.p) || ((uintptr_t)q.p & 7)) return std::string(q.name) + " must be an 8-byte aligned device pointer";
+    if (c->n_sw + c->n_lw == 0 && !c->optics) return "optics is NULL";
+    if (fluxes && (!not_host_ptr(fluxes) || ((uintptr_t)fluxes & 7))) return "fluxes must be an 8-byte aligned device pointer or NULL";
+    return "";
+}
+
+TsArgs ts_args(const atm_ts_columns *c, double *net, double *fluxes, unsigned *status) {
+    return TsArgs{c->ncol, c->nlay, c->n_sw, c->n_lw, c->p_ref, c->t_ref, c->gravity,
+                  c->guard, c->optics, c->play, c->tlay, c->h2o, c->cloud_tau, c->cloud_ssa, c->cloud_asy,
+                  c->plev, c->tlev, c->mu0, c->sfc_alb, c->tsfc, c->irradiance, net, fluxes, status};
+}
+
+// launches the column kernel for the layer count and precision
+hipError_t ts_launch(const TsArgs &a, int precision, hipStream_t s) {
+    const size_t lds = sizeof(double) * (RARR * RG + 4) * (size_t)(a.L + 1);
+    const dim3 grid((unsigned)a.ncol), blk(64);
+    const bool mixed = precision == ATM_PRECISION_MIXED;
+    if (a.L <= 64) {
+        if (mixed)
+            hipLaunchKernelGGL((k_ts_columns<1, float>), grid, blk, lds, s, a);
+        else
+            hipLaunchKernelGGL((k_ts_columns<1, double>), grid, blk, lds, s, a);
+    } else if (a.L <= 128) {
+        if (mixed)
+            hipLaunchKernelGGL((k_ts_columns<2, float>), grid, blk, lds, s, a);
+        else
+            hipLaunchKernelGGL((k_ts_columns<2, double>), grid, blk, lds, s, a);
+    } else {
+        if (mixed)
+            hipLaunchKernelGGL((k_ts_columns<RMAXL, float>), grid, blk, lds, s, a);
+        else
+            hipLaunchKernelGGL((k_ts_columns<RMAXL, double>), grid, blk, lds, s, a);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *atm_twostream_last_error(void) { return g_ts_error.c_str(); }
+
+int atm_twostream(const atm_ts_columns *cols, int precision, double *net_flux, double *fluxes, atm_ts_stats *stats,
+                  void *hip_stream) {
+    const std::string why = ts_check(cols, precision, net_flux, fluxes);
+    if (!why.empty()) return ts_fail(ATM_ERR_ARGUMENT, "atm_twostream: " + why);
+    if (stats) *stats = atm_ts_stats{0, 0};
+    if (cols->ncol == 0) return ATM_OK;
+    hipStream_t s = (hipStream_t)hip_stream;
+    void *ws = nullptr;
+    unsigned *status = nullptr;
+    if (stats) {
+        // status words, then the two totals at a 64-byte aligned offset
+        const size_t off = ((size_t)cols->ncol * sizeof(unsigned) + 63) & ~(size_t)63;
+        hipError_t e = hipMalloc(&ws, off + 2 * sizeof(unsigned long long));
+        if (e != hipSuccess) return ts_fail(ATM_ERR_HIP, std::string("atm_twostream: hipMalloc: ") + hipGetErrorString(e));
+        status = (unsigned *)ws;
+        hipError_t e1 = ts_launch(ts_args(cols, net_flux, fluxes, status), precision, s);
+        unsigned long long tot[2] = {0, 0};
+        if (e1 == hipSuccess) {
+            hipLaunchKernelGGL(k_ts_totals, dim3(1), dim3(1024), 0, s, status, (long long)cols->ncol,
+                               (unsigned long long *)((char *)ws + off));
+            e1 = hipGetLastError();
+        }
+        if (e1 == hipSuccess) e1 = hipMemcpyAsync(tot, (char *)ws + off, sizeof tot, hipMemcpyDeviceToHost, s);
+        if (e1 == hipSuccess) e1 = hipStreamSynchronize(s);
+        hipError_t e2 = hipFree(ws);
+        if (e1 != hipSuccess) return ts_fail(ATM_ERR_HIP, std::string("atm_twostream: ") + hipGetErrorString(e1));
+        if (e2 != hipSuccess) return ts_fail(ATM_ERR_HIP, std::string("atm_twostream: hipFree: ") + hipGetErrorString(e2));
+        stats->guarded_layers = (int64_t)tot[0];
+        stats->bad_columns = (int64_t)tot[1];
+        return ATM_OK;
+    }
+    hipError_t e = ts_launch(ts_args(cols, net_flux, fluxes, nullptr), precision, s);
+    if (e != hipSuccess) return ts_fail(ATM_ERR_HIP, std::string("atm_twostream: ") + hipGetErrorString(e));
+    return ATM_OK;
+}
+
+}  // extern "C"

@@ -101,6 +101,15 @@ int atm_set_stream(atm_ctx *ctx, void *hip_stream);
    release.  No reference counterpart (plumbing of this library). */
 int atm_stream_create(void **hip_stream);
 int atm_stream_destroy(void *hip_stream);
+/* Wait for the work enqueued on a stream of this library's runtime; time
+   the work enqueued between timer_begin and timer_end with HIP events on
+   that stream (ms; timer_end waits for the end event).  One timer per
+   calling thread.  For hosts that drive the library from another runtime
+   (torch bundles its own HIP runtime, whose streams and events this one
+   does not see).  No reference counterpart. */
+int atm_stream_synchronize(void *hip_stream);
+int atm_stream_timer_begin(void *hip_stream);
+int atm_stream_timer_end(void *hip_stream, double *ms);
 
 /* Hybrid coordinate coefficients Ak, Bk (nlay+1 each) -- the reference's
    coord_Ak/coord_Bk read by init_prognostic_fields (mod_input.f90:161-168). */
@@ -248,6 +257,71 @@ int atm_graph_free(atm_ctx *ctx);
 int atm_lin_init(atm_ctx *ctx);
 int atm_tl_step(atm_ctx *ctx, double dt, int remap);
 int atm_ad_step(atm_ctx *ctx, double dt, int remap);
+
+/* General two-stream columns.
+   Replaces the column loop of calculate_radiative_heating
+   (mod_radiation.f90:117-154) with solve_shortwave_column (158-268),
+   solve_longwave_column (270-347), calculate_fluxes (349-381) and
+   delta_eddington_rescaling (383-398) over inputs the reference fixes:
+   per-layer cloud optical depth, single-scattering albedo and asymmetry
+   mixed with the gas and Rayleigh optical depths, per-column cosine of the
+   solar zenith angle, surface albedo, surface temperature and irradiance,
+   and gas optics from an analytic k-distribution, per g-point g:
+     ln k = c0 + c1 x + c2 x^2 + c3 y + c4 y^2,  x = ln(p / p_ref),
+     y = T - t_ref,  tau_gas = k q dm  (k in m^2 per kg of water vapour,
+     q the layer's mixing ratio, dm = (p_lower - p_upper) / gravity);
+   shortwave Rayleigh tau = k_ray dm; Planck source f_g sigma T^4 / pi
+   (sigma 5.670374419e-8, pi the binary32 value); surface emissivity 0.99.
+   The reference's column formulas are kept as written, with its own
+   properties: alpha2 = gamma2 gamma3 + gamma2 gamma4 (214), the diffuse R/T
+   with the direct beam's e^(-lambda tau) (254-260), the top's diffuse
+   downward flux equal to the incident beam (264), no surface reflection of
+   the direct beam.  guard: where |1 - (lambda mu0)^2| < guard, lambda is
+   scaled by (1 - 2 guard) for that layer (every term), moving lambda mu0
+   off the removable singularity of the direct-beam R/T.
+   Layout: every array is FP64 device memory, [column][level] with the level
+   index contiguous; level 0 is the surface, layer k lies between levels k
+   and k + 1.  optics: n_sw x 5 shortwave coefficients (c0..c4), n_lw x 5
+   longwave, n_sw Rayleigh coefficients (m^2 per kg of air), n_sw solar
+   fractions, n_lw Planck fractions (6 n_sw + 6 n_lw + n_sw values). */
+typedef struct atm_ts_columns {
+    int64_t ncol;
+    int32_t nlay, n_sw, n_lw;          /* 1 <= nlay <= 256; n_sw, n_lw >= 0 */
+    double p_ref, t_ref, gravity, guard;
+    const double *optics;
+    const double *play, *tlay, *h2o;                 /* [ncol][nlay] */
+    const double *cloud_tau, *cloud_ssa, *cloud_asy; /* [ncol][nlay] */
+    const double *plev, *tlev;                       /* [ncol][nlay + 1] */
+    const double *mu0, *sfc_alb, *tsfc, *irradiance; /* [ncol] */
+} atm_ts_columns;
+
+/* Column diagnostics: shortwave layers (x g-points) where the guard moved
+   lambda; columns with an input outside the solver's domain (non-positive
+   or non-finite optical depth, no absorption in a layer (omega = 1, where
+   the reference's R/T divide 0 by 0), mu0 outside (0, 1], non-positive
+   pressure thickness), whose fluxes are then not meaningful. */
+typedef struct atm_ts_stats {
+    int64_t guarded_layers, bad_columns;
+} atm_ts_stats;
+
+/* net_flux [ncol][nlay + 1] = (sw down + lw down) - (sw up + lw up), each
+   summed over its g-points in order; fluxes (optional, NULL for none):
+   4 x [ncol][nlay + 1] shortwave up, shortwave down (direct beam
+   included), longwave up, longwave down.  precision: ATM_PRECISION_FP64
+   (the reference's arithmetic) or ATM_PRECISION_MIXED (g-point optics in
+   float, the longwave sources in a rearranged form that avoids the
+   cancellation of 1 + R - T for thin layers; beam, adding method and sums
+   in double).  Runs on hip_stream (NULL: the default stream; a stream of
+   this library's runtime, see atm_stream_create).  stats NULL:
+   asynchronous, allocates nothing.  stats non-NULL: allocates a per-column
+   status buffer (4 ncol + 64 bytes), waits for the stream and fills stats.
+   Arrays must be device memory: pointers this runtime identifies as host
+   memory are refused, others are taken as device addresses (memory of
+   another runtime in the process, such as torch's, is not known to this
+   one).  Errors are reported by atm_twostream_last_error (per thread). */
+int atm_twostream(const atm_ts_columns *cols, int precision, double *net_flux, double *fluxes,
+                  atm_ts_stats *stats, void *hip_stream);
+const char *atm_twostream_last_error(void);
 
 /* Library build information (kernel path, compile flags), for logs. */
 const char *atm_build_info(void);

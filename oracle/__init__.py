@@ -82,10 +82,12 @@ def lib():
         L.orc_lw_column.restype = C.c_int
         L.orc_lw_column.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, C.c_double, dp, dp]
         L.orc_ts_set_jitter.argtypes = [C.c_int]
-        L.orc_ts_column_g.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dp] + \
-            [dp] * 8 + [C.c_double] * 6 + [dp, dp]
-        L.orc_twostream.argtypes = [C.c_long, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dp] + \
-            [dp] * 8 + [dp] * 4 + [C.c_double, C.c_double, dp]
+        for name in ("orc_ts_column_g", "orc_ts_column_g_ld", "orc_ts_column_g_q"):
+            getattr(L, name).argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                         dp] + [dp] * 8 + [C.c_double] * 6 + [dp, dp]
+        for name in ("orc_twostream", "orc_twostream_ld", "orc_twostream_q"):
+            getattr(L, name).argtypes = [C.c_long, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dp] + \
+                [dp] * 8 + [dp] * 4 + [C.c_double, C.c_double, dp, dp]
         _LIB = L
     return _LIB
 
@@ -364,41 +366,46 @@ def _ts_arrays(cols):
     return a
 
 
-def twostream(cols, guard, jitter=False):
-    """Net flux [column][level]: orc_twostream over the columns of a dict as
-    built by atmos.twostream.synthetic_columns.  jitter: every exponential
-    moved by one ulp (orc_ts_set_jitter), to measure the formulas' own
-    sensitivity to the rounding of exp."""
+def twostream(cols, guard, jitter=False, extended=None, fluxes=False):
+    """Net flux [column][level] (and, with fluxes, the (4, ncol, nlay + 1)
+    sw up, sw down, lw up, lw down): orc_twostream over the columns of a
+    dict as built by atmos.twostream.synthetic_columns.  jitter: every
+    exponential moved by one ulp (orc_ts_set_jitter), to measure the
+    formulas' own sensitivity to the rounding of exp.  extended: the same
+    statements in "ld" (long double, orc_twostream_ld) or "q" (binary128,
+    orc_twostream_q), accuracy references."""
     a = _ts_arrays(cols)
     ncol, L = int(cols["ncol"]), int(cols["nlay"])
     net = np.zeros((ncol, L + 1))
+    fl = np.zeros((4, ncol, L + 1)) if fluxes else None
+    fn = {None: lib().orc_twostream, "ld": lib().orc_twostream_ld, "q": lib().orc_twostream_q}[extended]
     lib().orc_ts_set_jitter(1 if jitter else 0)
     try:
-        lib().orc_twostream(ncol, L, int(cols["n_sw"]), int(cols["n_lw"]), float(cols["p_ref"]),
-                            float(cols["t_ref"]), _p(a["optics"]), _p(a["play"]), _p(a["plev"]), _p(a["tlay"]),
-                            _p(a["tlev"]), _p(a["h2o"]), _p(a["cloud_tau"]), _p(a["cloud_ssa"]),
-                            _p(a["cloud_asy"]), _p(a["mu0"]), _p(a["sfc_alb"]), _p(a["tsfc"]),
-                            _p(a["irradiance"]), float(cols["gravity"]), float(guard), _p(net))
+        fn(ncol, L, int(cols["n_sw"]), int(cols["n_lw"]), float(cols["p_ref"]), float(cols["t_ref"]),
+           _p(a["optics"]), _p(a["play"]), _p(a["plev"]), _p(a["tlay"]), _p(a["tlev"]), _p(a["h2o"]),
+           _p(a["cloud_tau"]), _p(a["cloud_ssa"]), _p(a["cloud_asy"]), _p(a["mu0"]), _p(a["sfc_alb"]),
+           _p(a["tsfc"]), _p(a["irradiance"]), float(cols["gravity"]), float(guard), _p(net),
+           _p(fl) if fluxes else None)
     finally:
         lib().orc_ts_set_jitter(0)
-    return net
+    return (net, fl) if fluxes else net
 
 
-def twostream_column_g(cols, c, kind, g, guard, jitter=False):
+def twostream_column_g(cols, c, kind, g, guard, jitter=False, extended=None):
     """(flux up, flux down) of column c for one g-point (kind 0 shortwave,
     1 longwave; the shortwave down flux includes the direct beam)."""
     a = _ts_arrays(cols)
     L = int(cols["nlay"])
     row = {n: np.ascontiguousarray(a[n][c]) for n in _TS_LAYER + ("plev", "tlev")}
     fu, fd = np.zeros(L + 1), np.zeros(L + 1)
+    fn = {None: lib().orc_ts_column_g, "ld": lib().orc_ts_column_g_ld, "q": lib().orc_ts_column_g_q}[extended]
     lib().orc_ts_set_jitter(1 if jitter else 0)
     try:
-        lib().orc_ts_column_g(L, kind, g, int(cols["n_sw"]), int(cols["n_lw"]), float(cols["p_ref"]),
-                              float(cols["t_ref"]), _p(a["optics"]), _p(row["play"]), _p(row["plev"]),
-                              _p(row["tlay"]), _p(row["tlev"]), _p(row["h2o"]), _p(row["cloud_tau"]),
-                              _p(row["cloud_ssa"]), _p(row["cloud_asy"]), float(a["mu0"][c]),
-                              float(a["sfc_alb"][c]), float(a["tsfc"][c]), float(a["irradiance"][c]),
-                              float(cols["gravity"]), float(guard), _p(fu), _p(fd))
+        fn(L, kind, g, int(cols["n_sw"]), int(cols["n_lw"]), float(cols["p_ref"]), float(cols["t_ref"]),
+           _p(a["optics"]), _p(row["play"]), _p(row["plev"]), _p(row["tlay"]), _p(row["tlev"]), _p(row["h2o"]),
+           _p(row["cloud_tau"]), _p(row["cloud_ssa"]), _p(row["cloud_asy"]), float(a["mu0"][c]),
+           float(a["sfc_alb"][c]), float(a["tsfc"][c]), float(a["irradiance"][c]), float(cols["gravity"]),
+           float(guard), _p(fu), _p(fd))
     finally:
         lib().orc_ts_set_jitter(0)
     return fu, fd

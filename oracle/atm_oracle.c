@@ -16,6 +16,7 @@
  */
 #include "atm_oracle.h"
 
+#include <quadmath.h>
 #include <stdint.h>
 #include <math.h>
 #include <stdlib.h>
@@ -1588,7 +1589,8 @@ void orc_output_mean(orc_t *o, double *out) {
  *     tau_gas = k q dm; Rayleigh tau = k_ray dm (shortwave only);
  *   - a cloud per layer (optical depth, single-scattering albedo,
  *     asymmetry), mixed with the gas and Rayleigh optical depths;
- *   - Planck source B_g(T) = f_g sigma T^4 / pi (pi the binary32 value);
+ *   - Planck source B_g(T) = f_g sigma T^4 / pi (pi the binary32 value,
+ *     T^4 formed as (T T)(T T));
  *   - the guard on |1 - (lambda mu0)^2| < guard: lambda *= (1 - 2 guard),
  *     used by every term of that layer, so (lambda mu0) moves off the
  *     removable singularity of the direct-beam R/T.
@@ -1600,10 +1602,10 @@ void orc_output_mean(orc_t *o, double *out) {
 #define TS_SIGMA 5.670374419e-8
 
 /* Rounding-sensitivity probe (test infrastructure): when set, every
-   exponential of the two-stream columns is moved by one ulp up or down,
-   chosen by a hash of its argument, as an exp() that rounds differently
-   would.  The spread of the results between the two settings measures how
-   far the reference's formulas amplify one rounding of exp. */
+   exponential of the FP64 two-stream columns is moved by one ulp up or
+   down, chosen by a hash of its argument, as an exp() that rounds
+   differently would.  The spread of the results between the two settings
+   measures how far the reference's formulas amplify one rounding of exp. */
 static int ts_jitter = 0;
 void orc_ts_set_jitter(int on) { ts_jitter = on; }
 
@@ -1618,157 +1620,37 @@ static double ts_exp(double x) {
     return (b & 1) ? nextafter(e, INFINITY) : nextafter(e, 0.);
 }
 
-static double ts_k(const double *c, double x, double y) {
-    return ts_exp(c[0] + c[1] * x + c[2] * x * x + c[3] * y + c[4] * y * y);
-}
-
-/* one shortwave g-point: 158-268 */
-static void ts_sw(int L, const double *c, double k_ray, double irr, const double *play, const double *dm,
-                  const double *tlay, const double *q, const double *ctau, const double *cssa,
-                  const double *casy, double mu0, double sfc_alb, double pref, double tref, double guard,
-                  double *flux_up, double *flux_dn) {
-    double *R = zalloc(L), *T = zalloc(L), *Su = zalloc(L + 1), *Sd = zalloc(L + 1), *beam = zalloc(L + 1);
-    beam[L] = mu0 * irr;
-    for (int k = L; k >= 1; k--) {
-        const int K = k - 1;
-        double tau_g = ts_k(c, log(play[K] / pref), tlay[K] - tref) * q[K] * dm[K];
-        double tau_r = k_ray * dm[K];
-        double sca = tau_r + cssa[K] * ctau[K];
-        double tau = tau_g + tau_r + ctau[K];
-        double ssa = sca / tau;
-        double asy = sca > 0. ? cssa[K] * ctau[K] * casy[K] / sca : 0.;
-        orc_delta_eddington(&tau, &ssa, &asy);
-        double asy2 = asy * asy;
-        double back = (2. - 3. * asy * mu0) / 4.;
-        double tmp1 = 4. * (1. - asy2 * (1. - mu0));
-        double gamma3 = ssa * asy2;
-        double gamma4 = 4. * back + 3. * asy;
-        double tmp2 = 3. * asy;
-        double gamma1 = (7. - 3. * asy2 - ssa * (4. + tmp2) + gamma3 * gamma4) / tmp1;
-        double gamma2 = (-1. + asy2 + ssa * (4. - tmp2) + gamma3 * (gamma4 - 4.)) / tmp1;
-        gamma3 = back;
-        gamma4 = 1. - gamma3;
-        double alpha1 = gamma1 * gamma4 + gamma2 * gamma3;
-        double alpha2 = gamma2 * gamma3 + gamma2 * gamma4;
-        double lambda = sqrt(gamma1 * gamma1 - gamma2 * gamma2);
-        double lam_mu = lambda * mu0;
-        if (fabs(1. - lam_mu * lam_mu) < guard) {
-            lambda = lambda * (1. - 2. * guard);
-            lam_mu = lambda * mu0;
-        }
-        double e_minus = ts_exp(-lambda * tau);
-        double e_2minus = e_minus * e_minus;
-        double Tdir = ts_exp(-tau / mu0);
-        tmp1 = ssa / ((1. - lam_mu * lam_mu) * (lambda + gamma1 + (lambda - gamma1) * e_2minus));
-        tmp2 = lambda * gamma3;
-        double Rdif = tmp1 * ((1. - lam_mu) * (alpha2 + tmp2) - (1. + lam_mu) * (alpha2 - tmp2) * e_2minus -
-                              2. * lambda * (gamma3 - alpha2 * mu0) * e_minus * Tdir);
-        tmp2 = lambda * gamma4;
-        double Tdif = tmp1 * (2. * lambda * (gamma4 + alpha1 * mu0) * e_minus -
-                              Tdir * ((1. + lam_mu) * (alpha1 + tmp2) - (1. - lam_mu) * (alpha1 - tmp2) * e_2minus));
-        beam[K] = Tdir * beam[k];
-        Su[K] = Rdif * beam[k];
-        Sd[K] = Tdif * beam[k];
-        back = (4. + asy) / (8. * (1. + asy));
-        gamma1 = 2. * (1. - ssa * (1. - back));
-        gamma2 = 2. * ssa * back;
-        lambda = sqrt(gamma1 * gamma1 - gamma2 * gamma2);
-        e_2minus = e_minus * e_minus; /* Q6 */
-        tmp1 = 1. / (lambda + gamma1 + (lambda - gamma1) * e_2minus);
-        R[K] = tmp1 * gamma2 * (1. - e_2minus);
-        T[K] = 2. * tmp1 * lambda * e_minus;
-    }
-    orc_calculate_fluxes(L, R, T, Su, Sd, beam[L], 0., sfc_alb, flux_up, flux_dn);
-    for (int k = 0; k <= L; k++) flux_dn[k] = flux_dn[k] + beam[k];
-    free(R); free(T); free(Su); free(Sd); free(beam);
-}
-
-/* one longwave g-point: 270-347 */
-static void ts_lw(int L, const double *c, double pf, const double *play, const double *dm, const double *tlay,
-                  const double *tlev, const double *q, const double *ctau, const double *cssa,
-                  const double *casy, double tsfc, double pref, double tref, double *flux_up,
-                  double *flux_dn) {
-    double *R = zalloc(L), *T = zalloc(L), *Su = zalloc(L + 1), *Sd = zalloc(L + 1), *B = zalloc(L + 1);
-    const double emiss = (double)0.99f;
-    const double Bs = pf * TS_SIGMA * pow(tsfc, 4) / PI_F;
-    B[0] = pf * TS_SIGMA * pow(tlev[0], 4) / PI_F;
-    for (int k = 1; k <= L; k++) {
-        const int K = k - 1;
-        double tau_g = ts_k(c, log(play[K] / pref), tlay[K] - tref) * q[K] * dm[K];
-        double sca = cssa[K] * ctau[K];
-        double tau = tau_g + ctau[K];
-        double ssa = sca / tau;
-        double asy = sca > 0. ? casy[K] : 0.;
-        orc_delta_eddington(&tau, &ssa, &asy);
-        double back = (4. - 3. * asy) / 8.;
-        double gamma1 = 2. * (1. - ssa * (1. - back));
-        double gamma2 = 2. * ssa * back;
-        double lambda = sqrt(gamma1 * gamma1 - gamma2 * gamma2);
-        double e_minus = ts_exp(-lambda * tau);
-        double e_2minus = e_minus * e_minus;
-        double tmp1 = 1. / (lambda + gamma1 + (lambda - gamma1) * e_2minus);
-        R[K] = tmp1 * gamma2 * (1. - e_2minus);
-        T[K] = 2. * tmp1 * lambda * e_minus;
-        B[k] = pf * TS_SIGMA * pow(tlev[k], 4) / PI_F;
-        double pdiff = (1. + R[K] - T[K]) * (B[k - 1] - B[k]) / (tau * (gamma1 + gamma2));
-        tmp1 = 1. - R[K];
-        Su[K] = PI_F * (tmp1 * B[k] - T[K] * B[k - 1] + pdiff);
-        Sd[K] = PI_F * (tmp1 * B[k - 1] - T[K] * B[k] - pdiff);
-    }
-    orc_calculate_fluxes(L, R, T, Su, Sd, 0., emiss * PI_F * Bs, 1. - emiss, flux_up, flux_dn);
-    free(R); free(T); free(Su); free(Sd); free(B);
-}
-
-/* One g-point of one column (kind 0 shortwave, 1 longwave), fluxes on the
-   L + 1 levels; inputs of that column as below. */
-void orc_ts_column_g(int L, int kind, int g, int nsw, int nlw, double pref, double tref, const double *optics,
-                     const double *play, const double *plev, const double *tlay, const double *tlev,
-                     const double *q, const double *ctau, const double *cssa, const double *casy, double mu0,
-                     double alb, double tsfc, double irr, double gravity, double guard, double *flux_up,
-                     double *flux_dn) {
-    double *dm = zalloc(L);
-    for (int k = 0; k < L; k++) dm[k] = (plev[k] - plev[k + 1]) / gravity;
-    const double *csw = optics, *clw = optics + 5 * nsw, *ray = optics + 5 * (nsw + nlw),
-                 *sol = ray + nsw, *pf = sol + nsw;
-    if (kind == 0)
-        ts_sw(L, csw + 5 * g, ray[g], irr * sol[g], play, dm, tlay, q, ctau, cssa, casy, mu0, alb, pref, tref,
-              guard, flux_up, flux_dn);
-    else
-        ts_lw(L, clw + 5 * g, pf[g], play, dm, tlay, tlev, q, ctau, cssa, casy, tsfc, pref, tref, flux_up,
-              flux_dn);
-    free(dm);
-}
-
-/* Net flux (nlay + 1 levels) of ncol columns laid out [column][level]:
-   (sw down + lw down) - (sw up + lw up), each summed over its g-points in
-   order (117-154). */
-void orc_twostream(long ncol, int L, int nsw, int nlw, double pref, double tref, const double *optics,
-                   const double *play, const double *plev, const double *tlay, const double *tlev, const double *q,
-                   const double *ctau, const double *cssa, const double *casy, const double *mu0,
-                   const double *alb, const double *tsfc, const double *irr, double gravity, double guard,
-                   double *net_flux) {
-#pragma omp parallel for schedule(dynamic, 16)
-    for (long c = 0; c < ncol; c++) {
-        const size_t a = (size_t)c * L, b = (size_t)c * (L + 1);
-        double *su = zalloc(L + 1), *sd = zalloc(L + 1), *lu = zalloc(L + 1), *ld = zalloc(L + 1);
-        double *fu = zalloc(L + 1), *fd = zalloc(L + 1);
-        for (int g = 0; g < nsw; g++) {
-            orc_ts_column_g(L, 0, g, nsw, nlw, pref, tref, optics, play + a, plev + b, tlay + a, tlev + b, q + a,
-                            ctau + a, cssa + a, casy + a, mu0[c], alb[c], tsfc[c], irr[c], gravity, guard, fu, fd);
-            for (int k = 0; k <= L; k++) {
-                su[k] = su[k] + fu[k];
-                sd[k] = sd[k] + fd[k];
-            }
-        }
-        for (int g = 0; g < nlw; g++) {
-            orc_ts_column_g(L, 1, g, nsw, nlw, pref, tref, optics, play + a, plev + b, tlay + a, tlev + b, q + a,
-                            ctau + a, cssa + a, casy + a, mu0[c], alb[c], tsfc[c], irr[c], gravity, guard, fu, fd);
-            for (int k = 0; k <= L; k++) {
-                lu[k] = lu[k] + fu[k];
-                ld[k] = ld[k] + fd[k];
-            }
-        }
-        for (int k = 0; k <= L; k++) net_flux[b + k] = (sd[k] + ld[k]) - (su[k] + lu[k]);
-        free(su); free(sd); free(lu); free(ld); free(fu); free(fd);
-    }
-}
+/* The column code, once in FP64 (the oracle proper: orc_ts_column_g,
+   orc_twostream), and as accuracy references for formulas whose FP64
+   rounding is amplified (the longwave 1 + R - T of thin layers) the same
+   statements in long double (x87 extended, 64-bit significand: the _ld
+   functions) and in binary128 (113-bit significand, libquadmath: the _q
+   functions). */
+#define TS_R double
+#define TS_N(name) name
+#define TS_EXP ts_exp
+#define TS_LOG log
+#define TS_SQRT sqrt
+#include "ts_columns.inc"
+#undef TS_R
+#undef TS_N
+#undef TS_EXP
+#undef TS_LOG
+#undef TS_SQRT
+#define TS_R long double
+#define TS_N(name) name##_ld
+#define TS_EXP expl
+#define TS_LOG logl
+#define TS_SQRT sqrtl
+#include "ts_columns.inc"
+#undef TS_R
+#undef TS_N
+#undef TS_EXP
+#undef TS_LOG
+#undef TS_SQRT
+#define TS_R __float128
+#define TS_N(name) name##_q
+#define TS_EXP expq
+#define TS_LOG logq
+#define TS_SQRT sqrtq
+#include "ts_columns.inc"

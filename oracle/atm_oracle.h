@@ -89,18 +89,26 @@ int orc_lw_column(int nlay, int g, const double *play, const double *dm,
                   const double *T, const double *Tlev, double ts,
                   double *flux_up, double *flux_dn);
 
-/* general two-stream columns (synthetic correlated-k workload) */
+/* general two-stream columns (synthetic correlated-k workload); the _ld
+   and _q variants run the same statements in long double and binary128 */
 void orc_ts_set_jitter(int on);
-void orc_ts_column_g(int L, int kind, int g, int nsw, int nlw, double pref, double tref, const double *optics,
-                     const double *play, const double *plev, const double *tlay, const double *tlev,
-                     const double *q, const double *ctau, const double *cssa, const double *casy, double mu0,
-                     double alb, double tsfc, double irr, double gravity, double guard, double *flux_up,
-                     double *flux_dn);
-void orc_twostream(long ncol, int L, int nsw, int nlw, double pref, double tref, const double *optics,
-                   const double *play, const double *plev, const double *tlay, const double *tlev, const double *q,
-                   const double *ctau, const double *cssa, const double *casy, const double *mu0,
-                   const double *alb, const double *tsfc, const double *irr, double gravity, double guard,
-                   double *net_flux);
+#define ORC_TS_DECL(name)                                                                                          \
+    void name(int L, int kind, int g, int nsw, int nlw, double pref, double tref, const double *optics,            \
+              const double *play, const double *plev, const double *tlay, const double *tlev, const double *q,     \
+              const double *ctau, const double *cssa, const double *casy, double mu0, double alb, double tsfc,    \
+              double irr, double gravity, double guard, double *flux_up, double *flux_dn);
+#define ORC_TS_DECL_N(name)                                                                                        \
+    void name(long ncol, int L, int nsw, int nlw, double pref, double tref, const double *optics,                  \
+              const double *play, const double *plev, const double *tlay, const double *tlev, const double *q,     \
+              const double *ctau, const double *cssa, const double *casy, const double *mu0, const double *alb,    \
+              const double *tsfc, const double *irr, double gravity, double guard, double *net_flux,             \
+              double *fluxes);
+ORC_TS_DECL(orc_ts_column_g)
+ORC_TS_DECL(orc_ts_column_g_ld)
+ORC_TS_DECL_N(orc_twostream)
+ORC_TS_DECL_N(orc_twostream_ld)
+ORC_TS_DECL(orc_ts_column_g_q)
+ORC_TS_DECL_N(orc_twostream_q)
 
 #ifdef __cplusplus
 }
