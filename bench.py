@@ -81,6 +81,10 @@ def parse():
     ap.add_argument("--adj-nlay", type=int, default=48)
     ap.add_argument("--adj-steps", type=int, default=90)
     ap.add_argument("--adj-snapshots", type=int, default=10)
+    ap.add_argument("--twostream", choices=["on", "off"], default="on",
+                    help="general two-stream columns: 262144 x 64, 32 + 32 g-points, FP64 and mixed")
+    ap.add_argument("--ts-ncol", type=int, default=262144)
+    ap.add_argument("--ts-calls", type=int, default=5)
     ap.add_argument("--autotune", choices=["on", "off"], default="off",
                     help="time the fused path's launch-shape candidates on each grid first (results unchanged; "
                          "off by default: isolated timings have picked shapes slower inside a step)")
@@ -337,6 +341,92 @@ def radiation_timing(m, args, time_s):
             "flop_per_call": flops}
 
 
+# Algorithmic flops of the two-stream columns, counted from the
+# reference's statements (src/mod_radiation.f90:158-398) with every +, -, *,
+# /, sqrt and exp one flop: per layer and shortwave g-point 149 (gas,
+# Rayleigh and cloud optical depths 21 with the k evaluation, delta-Eddington
+# 9, coefficients, guard and direct-beam R/T 89, the beam and its sources 3,
+# diffuse R/T 27), 19 for calculate_fluxes' two sweeps, 1 for adding the
+# beam, 2 for the g-point sums; per layer and longwave g-point 72 (optics
+# 40, R/T and the Planck sources 32) + 19 + 2; per layer and column 9 shared
+# by the g-points (ln p, T - T_ref, dm, cloud scattering, T^4).
+TS_FLOPS_SW = 149 + 19 + 1 + 2
+TS_FLOPS_LW = 72 + 19 + 2
+TS_FLOPS_LAYER = 9
+FP64_VALU_PEAK_TFLOPS = 78.6
+
+
+def twostream_timing(args):
+    """The general two-stream columns at the workload's size: FP64 (the
+    reference's formulas) and the mixed form, timed with HIP events on the
+    library's stream over back-to-back calls; errors against each other on
+    every column, against the binary128 evaluation on a sample, and per
+    g-point; the oracle on a bounded sample of columns on the host."""
+    import numpy as np
+    import torch
+
+    import oracle as O
+    from atmos import twostream as TS
+
+    ncol, L, nsw, nlw = args.ts_ncol, 64, 32, 32
+    cols = TS.synthetic_columns(ncol, nlay=L, n_sw=nsw, n_lw=nlw, seed=20261203 + 300)
+    dev = TS.to_device(cols)
+    flops = ncol * L * (nsw * TS_FLOPS_SW + nlw * TS_FLOPS_LW + TS_FLOPS_LAYER)
+    res = {"workload": f"{ncol} columns x {L} layers, {nsw} SW + {nlw} LW g-points, synthetic k-distribution, "
+                       "clouds in 20% of layers (g 0.85), mu0 U[0.1,1], albedo U[0.05,0.3]",
+           "flops_per_call": flops}
+    nets = {}
+    for prec in ("fp64", "mixed"):
+        out = torch.empty((ncol, L + 1), dtype=torch.float64, device="cuda")
+        _, _, st = TS.twostream(dev, prec, out=out)  # warm-up, with diagnostics
+        TS.timer_begin()
+        for _ in range(args.ts_calls):
+            TS.twostream(dev, prec, out=out, stats=False, wait=False)
+        ms = TS.timer_end() / args.ts_calls
+        nets[prec] = out.cpu().numpy()
+        tf = flops / (ms * 1e-3) / 1e12
+        res[prec] = {"ms_per_call": ms, "columns_per_s": ncol / (ms * 1e-3), "tflops": tf,
+                     "frac_fp64_valu_peak": tf / FP64_VALU_PEAK_TFLOPS, **st}
+    res["speedup_mixed"] = res["fp64"]["ms_per_call"] / res["mixed"]["ms_per_call"]
+
+    def rel(a, b):
+        sc = np.maximum(np.max(np.abs(b), axis=-1), 1e-300)
+        e = np.max(np.abs(a - b), axis=-1) / sc
+        return {"max": float(e.max()), "rms": float(np.sqrt(np.mean(e ** 2)))}
+
+    res["mixed_vs_fp64"] = rel(nets["mixed"], nets["fp64"])
+    idx = np.arange(0, ncol, max(1, ncol // 256))
+    sub = {**cols, **{n: cols[n][idx] for n in TS.LAYER_FIELDS + TS.LEVEL_FIELDS + TS.COLUMN_FIELDS},
+           "ncol": len(idx)}
+    q = O.twostream(sub, TS.GUARD_FP64, extended="q")
+    res["vs_binary128"] = {"sample_columns": len(idx), "fp64": rel(nets["fp64"][idx], q),
+                           "mixed": rel(nets["mixed"][idx], q)}
+    # per g-point: the same columns with one g-point's optics
+    per_g = []
+    for kind, n in (("sw", nsw), ("lw", nlw)):
+        for g in range(n):
+            one = TS.select_gpoints(cols, sw=[g] if kind == "sw" else [], lw=[g] if kind == "lw" else [])
+            d1 = {**dev, "optics": torch.tensor(one["optics"], dtype=torch.float64, device="cuda"),
+                  "n_sw": one["n_sw"], "n_lw": one["n_lw"]}
+            a, _, _ = TS.twostream(d1, "fp64", stats=False)
+            b, _, _ = TS.twostream(d1, "mixed", stats=False)
+            per_g.append((kind, g, rel(b.cpu().numpy(), a.cpu().numpy())))
+    worst = max(per_g, key=lambda t: t[2]["max"])
+    res["mixed_vs_fp64_per_g"] = {"max_over_g": worst[2]["max"], "worst_g": f"{worst[0]}{worst[1]}",
+                                  "rms_over_g": float(np.sqrt(np.mean([t[2]["rms"] ** 2 for t in per_g])))}
+    # the oracle (C restatement, FP64) on a bounded sample of columns
+    nthreads = O.set_threads(0)
+    ncpu = min(ncol, 8192)
+    sub = {**cols, **{n: cols[n][:ncpu] for n in TS.LAYER_FIELDS + TS.LEVEL_FIELDS + TS.COLUMN_FIELDS},
+           "ncol": ncpu}
+    t0 = time.perf_counter()
+    O.twostream(sub, TS.GUARD_FP64)
+    sec = time.perf_counter() - t0
+    res["cpu_oracle"] = {"columns_per_s": ncpu / sec, "cores": nthreads,
+                         "sample": f"{ncpu} columns, {sec:.2f} s on {nthreads} host threads (oracle/ts_columns.inc)"}
+    return res
+
+
 def adjoint_timing(args):
     """Gradient of a surface-pressure cost through a window of steps: the
     tangent-linear/adjoint workload (256 x 256 x 48, dx = 50 km, dt = 40 s,
@@ -452,7 +542,7 @@ def main():
         sys.exit(2)
     import numpy as np  # noqa: F401
 
-    if args.adjoint == "on":
+    if args.adjoint == "on" or args.twostream == "on":
         # torch carries its own HIP runtime beside the library's; it has to
         # bring up the device first in a process that uses both
         import torch
@@ -533,6 +623,8 @@ def main():
     if args.adjoint == "on":
         out["adjoint"] = adjoint_timing(args)
         out["adjoint_forward_cost_ratio"] = out["adjoint"]["ratio"]
+    if args.twostream == "on":
+        out["twostream"] = twostream_timing(args)
     if args.ensemble == "on" and args.path == "fused":
         out["ensemble"] = ensemble_timing(args)
     if args.graph == "on" and args.path == "fused":
