@@ -33,7 +33,7 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_output_init", "atm_accumulate_output", "atm_output_mean", "atm_stream_create",
            "atm_stream_destroy", "atm_graph_begin", "atm_graph_end", "atm_graph_launch", "atm_graph_free",
            "atm_autotune", "atm_get_tuning", "atm_set_tuning", "atm_lin_init", "atm_tl_step", "atm_ad_step",
-           "atm_twostream", "atm_twostream_last_error", "atm_stream_synchronize", "atm_stream_timer_begin",
+           "atm_twostream", "atm_twostream_noalloc", "atm_twostream_last_error", "atm_stream_synchronize", "atm_stream_timer_begin",
            "atm_stream_timer_end"]
 PATHS = {"fused": 0, "reference": 1}
 PRECISIONS = {"fp64": 0, "mixed": 1}

@@ -44,6 +44,7 @@
 // 1 - e^2 = u (2 - u).
 // The beam, the adding method and the flux sums stay in double.
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -568,9 +569,79 @@ hipError_t ts_launch(const TsArgs &a, int precision, hipStream_t s) {
 
 }  // namespace
 
+// Scratch of atm_twostream_noalloc: the per-column status words of up to
+// TS_NOALLOC_MAX_COLS columns, then the two 64-bit totals (4 MiB + 256 B).
+// Each GPU holds its own copy until the process ends, and every call on a
+// GPU shares it: calls that overlap on one GPU race.  Every word a call
+// reads is written by that call first (status by the column kernel, totals
+// by k_ts_totals), so nothing is cleared between calls.
+constexpr long long TS_NOALLOC_MAX_COLS = 1LL << 20;
+static __device__ __align__(256) unsigned scratch[TS_NOALLOC_MAX_COLS + 64];
+
+namespace {
+
+constexpr int TS_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned *g_scratch_ptr[TS_MAX_DEVICES] = {};
+
+// The current device's address of scratch, found once per device with that
+// device current.  In host code the name scratch is a host-side stand-in,
+// never a device address: every use below goes through this pointer.
+int scratch_address(unsigned **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return ts_fail(ATM_ERR_HIP, std::string("atm_twostream_noalloc: hipGetDevice: ") +
+                                                         hipGetErrorString(e));
+    if (dev < 0 || dev >= TS_MAX_DEVICES) return ts_fail(ATM_ERR_HIP, "atm_twostream_noalloc: device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return ts_fail(ATM_ERR_HIP, std::string("atm_twostream_noalloc: hipGetSymbolAddress(scratch): ") +
+                                            hipGetErrorString(e));
+        g_scratch_ptr[dev] = (unsigned *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return ATM_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *atm_twostream_last_error(void) { return g_ts_error.c_str(); }
+
+int atm_twostream_noalloc(const atm_ts_columns *cols, int precision, double *net_flux, double *fluxes,
+                          atm_ts_stats *stats, void *hip_stream) {
+    const std::string why = ts_check(cols, precision, net_flux, fluxes);
+    if (!why.empty()) return ts_fail(ATM_ERR_ARGUMENT, "atm_twostream_noalloc: " + why);
+    if (cols->ncol > TS_NOALLOC_MAX_COLS)
+        return ts_fail(ATM_ERR_ARGUMENT, "atm_twostream_noalloc: ncol " + std::to_string(cols->ncol) +
+                                             " exceeds the scratch's " + std::to_string(TS_NOALLOC_MAX_COLS) +
+                                             " columns (use atm_twostream)");
+    if (stats) *stats = atm_ts_stats{0, 0};
+    if (cols->ncol == 0) return ATM_OK;
+    unsigned *scr = nullptr;
+    const int rc = scratch_address(&scr);
+    if (rc != ATM_OK) return rc;
+    unsigned long long *tot = (unsigned long long *)(scr + TS_NOALLOC_MAX_COLS);
+    hipStream_t s = (hipStream_t)hip_stream;
+    hipError_t e = ts_launch(ts_args(cols, net_flux, fluxes, scr), precision, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_ts_totals, dim3(1), dim3(1024), 0, s, scr, (long long)cols->ncol, tot);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && stats) {
+        unsigned long long t[2] = {0, 0};
+        e = hipMemcpyAsync(t, tot, sizeof t, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        stats->guarded_layers = (int64_t)t[0];
+        stats->bad_columns = (int64_t)t[1];
+    }
+    if (e != hipSuccess) return ts_fail(ATM_ERR_HIP, std::string("atm_twostream_noalloc: ") + hipGetErrorString(e));
+    return ATM_OK;
+}
 
 int atm_twostream(const atm_ts_columns *cols, int precision, double *net_flux, double *fluxes, atm_ts_stats *stats,
                   void *hip_stream) {

@@ -321,6 +321,17 @@ typedef struct atm_ts_stats {
    one).  Errors are reported by atm_twostream_last_error (per thread). */
 int atm_twostream(const atm_ts_columns *cols, int precision, double *net_flux, double *fluxes,
                   atm_ts_stats *stats, void *hip_stream);
+/* atm_twostream without any allocation, for callers that cannot allocate
+   (code under HIP graph capture, serving loops, hosts that forbid
+   hipMalloc).  No counterpart in the reference.  Same kernels and results;
+   the per-column status words and the totals live in a static device array
+   of the library (4 MiB + 256 B per GPU, held until the process ends), so
+   ncol is limited to 1048576 (larger: ATM_ERR_ARGUMENT).  Every call on a
+   GPU shares that array: calls that overlap on one GPU race.  stats NULL:
+   asynchronous (no copy, no wait); non-NULL: waits for the stream and
+   copies the totals. */
+int atm_twostream_noalloc(const atm_ts_columns *cols, int precision, double *net_flux, double *fluxes,
+                          atm_ts_stats *stats, void *hip_stream);
 const char *atm_twostream_last_error(void);
 
 /* Library build information (kernel path, compile flags), for logs. */
