@@ -8,6 +8,41 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
+
+// Frozen-branch mode: a tape of the program's branch decisions (LIN_DECIDE
+// in lin_types.hpp).  Recording appends each decision of a run; replaying
+// returns the recorded decisions in the same order, so a run from a
+// perturbed state takes the branches of the recorded one, and counts the
+// decisions the perturbed state alone would have taken otherwise.  The host
+// program runs its loops in a fixed order, so the k-th decision of a replay
+// is the k-th of the recording.
+namespace lintape {
+struct Tape {
+    int mode = 0;  // 0 off, 1 record, 2 replay
+    std::vector<bool> bits;
+    size_t pos = 0;
+    long long flips = 0, overrun = 0;
+};
+inline Tape g;
+inline bool decide(bool b) {
+    if (g.mode == 1) {
+        g.bits.push_back(b);
+        return b;
+    }
+    if (g.mode == 2) {
+        if (g.pos >= g.bits.size()) {
+            g.overrun++;
+            return b;
+        }
+        const bool r = g.bits[g.pos++];
+        if (r != b) g.flips++;
+        return r;
+    }
+    return b;
+}
+}  // namespace lintape
+#define LIN_DECIDE(b) (lintape::decide(b))
 
 #include "lin_step.hpp"
 
@@ -194,6 +229,28 @@ void *lin_host_create(int nx, int ny, int nlay, double dx, double dy, const doub
     k.ak = h->ak;
     k.bk = h->bk;
     return h;
+}
+
+// tape mode: 0 off, 1 record (clears the tape), 2 replay (from its start)
+void lin_host_tape(int mode) {
+    lintape::Tape &t = lintape::g;
+    t.mode = mode;
+    if (mode == 1) t.bits.clear();
+    t.pos = 0;
+    t.flips = t.overrun = 0;
+}
+
+// 0: decisions recorded, 1: decisions replayed, 2: replayed decisions the
+// live comparison would have taken the other way, 3: decisions past the
+// tape's end
+long long lin_host_tape_stat(int which) {
+    const lintape::Tape &t = lintape::g;
+    switch (which) {
+    case 0: return (long long)t.bits.size();
+    case 1: return (long long)t.pos;
+    case 2: return t.flips;
+    default: return t.overrun;
+    }
 }
 
 void lin_host_destroy(void *p) {

@@ -61,10 +61,10 @@ __host__ __device__ inline S rm_clamp(int j, int L, const S &bs, const S &vlo, c
         l = dmax2(l, dmin2(vhi, vlo));
     } else if (j >= 3 && j <= L - 1) {
         const S lo = dmin2(vlo, vhi), hi = dmax2(vlo, vhi);
-        if (ge * gc > 0) {
+        if (LIN_DECIDE(ge * gc > 0)) {
             l = dmin2(l, hi);
             l = dmax2(l, lo);
-        } else if (ge > 0) {
+        } else if (LIN_DECIDE(ge > 0)) {
             l = dmax2(l, lo);
         } else {
             l = dmin2(l, hi);
@@ -222,7 +222,7 @@ __host__ __device__ inline void rm_column(const LinArgs &x, size_t c0, const RmW
         S acc(0.);
         for (int k = 1; k <= L; k++) {
             const S xk = X(k), xk1 = X(k + 1);
-            if (val(xk1) >= val(dm0) || val(xk) <= val(dm1)) continue;
+            if (LIN_DECIDE(val(xk1) >= val(dm0)) || LIN_DECIDE(val(xk) <= val(dm1))) continue;
             acc = acc + rm_overlap<S>(rm_get<S>(w, RM_B, k), rm_get<S>(w, RM_T, k), rm_get<S>(w, RM_CV, k), V(k), xk,
                                       xk1, dm0, dm1);
         }

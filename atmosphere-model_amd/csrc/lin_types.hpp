@@ -16,6 +16,17 @@
 
 #include "atm_common.hpp"
 
+// Every branch decision of the linearised program passes through
+// LIN_DECIDE: the value comparisons below, the Fortran MAX, MIN, SIGN and
+// ABS selections, and the remap's overlap and clamp tests.  It is the
+// decision itself unless a build defines it first: the host build of the
+// program records the decisions of one run and replays them in another
+// (its frozen-branch mode), so that finite differences differentiate the
+// branches the trajectory took.
+#ifndef LIN_DECIDE
+#define LIN_DECIDE(b) (b)
+#endif
+
 template <int K>
 struct Jac {
     double v;
@@ -138,16 +149,17 @@ __host__ __device__ inline Jac<K> operator/(double a, const Jac<K> &b) {
 }
 
 // comparisons on the value (frozen switches)
-template <int K> __host__ __device__ inline bool operator>(const Jac<K> &a, const Jac<K> &b) { return a.v > b.v; }
-template <int K> __host__ __device__ inline bool operator<(const Jac<K> &a, const Jac<K> &b) { return a.v < b.v; }
-template <int K> __host__ __device__ inline bool operator>=(const Jac<K> &a, const Jac<K> &b) { return a.v >= b.v; }
-template <int K> __host__ __device__ inline bool operator<=(const Jac<K> &a, const Jac<K> &b) { return a.v <= b.v; }
-template <int K> __host__ __device__ inline bool operator>(const Jac<K> &a, double b) { return a.v > b; }
-template <int K> __host__ __device__ inline bool operator<(const Jac<K> &a, double b) { return a.v < b; }
+template <int K> __host__ __device__ inline bool operator>(const Jac<K> &a, const Jac<K> &b) { return LIN_DECIDE(a.v > b.v); }
+template <int K> __host__ __device__ inline bool operator<(const Jac<K> &a, const Jac<K> &b) { return LIN_DECIDE(a.v < b.v); }
+template <int K> __host__ __device__ inline bool operator>=(const Jac<K> &a, const Jac<K> &b) { return LIN_DECIDE(a.v >= b.v); }
+template <int K> __host__ __device__ inline bool operator<=(const Jac<K> &a, const Jac<K> &b) { return LIN_DECIDE(a.v <= b.v); }
+template <int K> __host__ __device__ inline bool operator>(const Jac<K> &a, double b) { return LIN_DECIDE(a.v > b); }
+template <int K> __host__ __device__ inline bool operator<(const Jac<K> &a, double b) { return LIN_DECIDE(a.v < b); }
 
 template <int K>
 __host__ __device__ inline Jac<K> fabs(const Jac<K> &a) {
-    JAC_UNARY_MAP(a.v < 0. ? -a.v : a.v, a.v < 0. ? -1. : 1.)
+    const bool neg = LIN_DECIDE(a.v < 0.);
+    JAC_UNARY_MAP(neg ? -a.v : a.v, neg ? -1. : 1.)
 }
 template <int K>
 __host__ __device__ inline Jac<K> log(const Jac<K> &a) {
@@ -155,14 +167,16 @@ __host__ __device__ inline Jac<K> log(const Jac<K> &a) {
 }
 // Fortran MAX / MIN of two values: the derivative of the operand selected
 template <int K>
-__host__ __device__ inline Jac<K> dmax2(const Jac<K> &a, const Jac<K> &b) { return a.v >= b.v ? a : b; }
+__host__ __device__ inline Jac<K> dmax2(const Jac<K> &a, const Jac<K> &b) { return LIN_DECIDE(a.v >= b.v) ? a : b; }
 template <int K>
-__host__ __device__ inline Jac<K> dmin2(const Jac<K> &a, const Jac<K> &b) { return a.v <= b.v ? a : b; }
+__host__ __device__ inline Jac<K> dmin2(const Jac<K> &a, const Jac<K> &b) { return LIN_DECIDE(a.v <= b.v) ? a : b; }
 // Fortran SIGN(a, b) = |a| with the sign of b; piecewise constant in b
 template <int K>
 __host__ __device__ inline Jac<K> fsign(const Jac<K> &a, const Jac<K> &b) {
-    const double s = (a.v < 0. ? -1. : 1.) * (__builtin_signbit(b.v) ? -1. : 1.);
-    JAC_UNARY_MAP(__builtin_copysign(a.v < 0. ? -a.v : a.v, b.v), s)
+    const bool an = LIN_DECIDE(a.v < 0.), bn = LIN_DECIDE(__builtin_signbit(b.v) != 0);
+    const double s = (an ? -1. : 1.) * (bn ? -1. : 1.);
+    const double m = an ? -a.v : a.v, sv = bn ? -m : m;
+    JAC_UNARY_MAP(sv == 0. ? __builtin_copysign(0., bn ? -1. : 1.) : sv, s)
 }
 // x / 3 (the forward value is div3's correctly rounded quotient)
 template <int K>

@@ -22,6 +22,9 @@ def _lib():
     L.lin_host_get.argtypes = [C.c_void_p, C.c_int, dp]
     L.lin_host_tl_step.argtypes = [C.c_void_p, C.c_double, C.c_int]
     L.lin_host_ad_step.argtypes = [C.c_void_p, C.c_double, C.c_int]
+    L.lin_host_tape.argtypes = [C.c_int]
+    L.lin_host_tape_stat.argtypes = [C.c_int]
+    L.lin_host_tape_stat.restype = C.c_longlong
     return L
 
 
@@ -92,3 +95,14 @@ class HostLin:
 
     def ad_step(self, dt, remap=False):
         assert self.L.lin_host_ad_step(self.h, dt, int(remap)) == 0
+
+
+def tape(mode):
+    """Branch-decision tape of the host program: "off", "record" (clears),
+    "replay" (from the start)."""
+    _lib().lin_host_tape({"off": 0, "record": 1, "replay": 2}[mode])
+
+
+def tape_stats():
+    L = _lib()
+    return {k: int(L.lin_host_tape_stat(i)) for i, k in enumerate(("recorded", "replayed", "flips", "overrun"))}

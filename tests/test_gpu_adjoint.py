@@ -198,3 +198,37 @@ def test_window_gradient_matches_finite_differences(gpu_available):
     fd = (J_at(eps) - J_at(-eps)) / (2 * eps)
     print(f"J {J0:.6e} <grad, d> {gdot:.12e} fd {fd:.12e}", flush=True)
     assert abs(fd - gdot) <= 1e-6 * abs(gdot), (fd, gdot)
+
+
+def test_window_gradient_is_the_host_adjoint_and_the_frozen_branch_derivative(gpu_available):
+    """The GPU window's gradient (binomial checkpointing, 3 snapshots) is the
+    host build's adjoint through stored states, and matches finite
+    differences of the model with its branches frozen (the host build's
+    decision tape, eps 1e-7) to 1e-6."""
+    from lin_host import HostLin
+    from test_lin_frozen_fd import frozen_J, window_forward, window_gradient
+
+    from atmos.adjoint import schedule_of
+    print("window gradient vs host adjoint and frozen-branch FD", flush=True)
+    m, ic = _model(nx=64, ny=32, nlay=10, seed=12)
+    x0 = _state(m)
+    p_top = float(ic["ak"][0])
+    sched = schedule_of(12, DT, 5)
+    box = (8, 24, 16, 48)
+    J0, g, _ = _window_grad(m, x0, sched, 3, box, p_top)
+    h = HostLin(ic)
+    remaps = [r for _, r in sched]
+    states, final = window_forward(h, x0, remaps, record=True)
+    gh = window_gradient(h, states, final, remaps, p_top, box)
+    for a, n in zip(g, STATE):
+        assert np.max(np.abs(a - gh[n])) <= 1e-10 * np.max(np.abs(gh[n])), n
+    rng = np.random.default_rng(8)
+    d = {"dp": np.zeros_like(x0["dp"]), "pt": 1e-3 * rng.standard_normal(x0["pt"].shape),
+         "ud": rng.standard_normal(x0["ud"].shape), "vd": rng.standard_normal(x0["vd"].shape)}
+    gdot = sum(float(np.sum(gi * d[n])) for gi, n in zip(g, STATE))
+    eps = 1e-7
+    jp, _ = frozen_J(h, {n: x0[n] + eps * d[n] for n in STATE}, remaps, p_top, box)
+    jm, _ = frozen_J(h, {n: x0[n] - eps * d[n] for n in STATE}, remaps, p_top, box)
+    fd = (jp - jm) / (2 * eps)
+    print(f"<grad, d> {gdot:.12e} frozen FD {fd:.12e}", flush=True)
+    assert abs(fd - gdot) <= 1e-6 * abs(gdot)
