@@ -3,11 +3,12 @@
 // lin_step.hpp / lin_ops.hpp; this file runs them with HIP kernels:
 //   forward   one thread per point of an operation's box, the body in double;
 //   tangent   the same body on Jac<1> (value and tangent read, both written);
-//   adjoint   the same body on Jac<NR>: the output's adjoint times each
-//             partial is added to the adjoint of the value read, with
-//             global FP64 atomic adds (neighbouring points read the same
-//             values); outputs whose adjoint is zero return at once, so an
-//             adjoint confined to a small region costs little of the grid.
+//   adjoint   the same body on Jac<NR>: each output's adjoint times each
+//             partial, summed over the body's outputs, is added to the
+//             adjoint of the value read with one global FP64 atomic add per
+//             read (neighbouring points read the same values); outputs whose
+//             adjoint is zero add nothing, so an adjoint confined to a small
+//             region costs little of the grid.
 // Column scans run one thread per column, the periodic halo fills one
 // thread per point (the adjoint folds each halo point's adjoint onto its
 // interior source with an atomic add).
@@ -47,6 +48,7 @@ __global__ void __launch_bounds__(256) k_lin_adj(LinArgs x, Rng r, int k0) {
     LIN_POINT
     AccAdj<F::NR, DevAdd> a(x);
     F::run(a, x, I, J, K);
+    a.flush();
 }
 #undef LIN_POINT
 

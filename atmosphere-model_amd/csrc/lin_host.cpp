@@ -84,6 +84,7 @@ struct HostBackend {
                 for (int I = r.I0; I <= r.I1; I++) {
                     AccAdj<F::NR, HostAdd> a(x);
                     F::run(a, x, I, J, K);
+                    a.flush();
                 }
     }
     void scan(int mode, bool gz, const Geo &g, Rng r, const double *sv, double *sd, const double *tv, double *td,

@@ -80,7 +80,10 @@ struct AccAdj {
     const LinArgs &x;
     int sarr[NR];
     size_t six[NR];
-    __host__ __device__ explicit AccAdj(const LinArgs &a) : x(a) {}
+    double acc[NR];  // adjoint owed to each read, summed over the outputs
+    __host__ __device__ explicit AccAdj(const LinArgs &a) : x(a) {
+        for (int s = 0; s < NR; s++) acc[s] = 0.;
+    }
     template <int s>
     __host__ __device__ S rd(int arr, int I, int J, int K) {
         static_assert(s < NR, "read slot");
@@ -95,9 +98,15 @@ struct AccAdj {
         if (ab == 0.) return;
         x.ot[o][i] = 0.;
 #pragma unroll
+        for (int s = 0; s < NR; s++) acc[s] += ab * v.d[s];
+    }
+    // after the body: one addition per read, none where the adjoint owed
+    // is zero (a partial that a frozen branch zeroes)
+    __host__ __device__ void flush() {
+#pragma unroll
         for (int s = 0; s < NR; s++) {
             double *t = x.it[sarr[s]];
-            if (t) ADD::add(t + six[s], ab * v.d[s]);
+            if (t && acc[s] != 0.) ADD::add(t + six[s], acc[s]);
         }
     }
 };
