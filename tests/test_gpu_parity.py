@@ -470,3 +470,62 @@ def test_mixed_precision_radiation(gpu_available):
     e = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
     _print("mixed radiation", net_flux=e)
     assert e <= 1e-4
+
+
+def test_cpu_reference_configuration_100_steps(gpu_available):
+    """The CPU reference configuration's exact schedule: 64x64x20, dx = dy =
+    100 km, dt = 60 s, seeded recipe, radiation off, dry convective mixing
+    on (physics and remap every 5 minutes), 100 steps; GPU fused path
+    against the oracle at 1e-12 on every prognostic field and ts."""
+    import oracle as O
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+    nx, ny, nlay = 64, 64, 20
+    print("cpu reference configuration 64x64x20 x100 physics on, radiation off", flush=True)
+    ic = make_initial_state(nx, ny, nlay, 100e3)
+    cfg = dict(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=300.0,
+               dt_remap=300.0, radiation_on=False, physics_on=True)
+    gpu, orc = Model(nx, ny, nlay, 100e3), O.Oracle(nx, ny, nlay, 100e3, 100e3)
+    for b in (gpu, orc):
+        st = driver.init_model(b, ic, Config(**cfg))
+        t, n, stable = driver.run_model(b, Config(**cfg), st, max_steps=100)
+        assert stable and n == 100
+    errs = {n: compare(gpu, orc, n) for n in ("dp", "pt", "ud", "vd", "ts")}
+    _print("cpu configuration x100", **errs)
+    for n, e in errs.items():
+        assert e <= PARITY, (n, e)
+
+
+def test_full_size_parity_100_steps(gpu_available):
+    """The benchmark workload's whole run: 512x512x64, dx = 25 km, dt = 20 s,
+    remap every 5 steps, 100 steps (20 remaps) from the seeded state, GPU
+    fused path against the threaded oracle at 1e-12 on the prognostic
+    fields; the total mass of both conserved to FP64 rounding."""
+    import oracle as O
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+    nx, ny, nlay, dx = 512, 512, 64, 25e3
+    print(f"full size {nx}x{ny}x{nlay} x100", flush=True)
+    ic = make_initial_state(nx, ny, nlay, dx)
+    kw = dict(dt_max=20.0, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12, dt_remap=100.0,
+              radiation_on=False, physics_on=False)
+    gpu, orc = Model(nx, ny, nlay, dx), O.Oracle(nx, ny, nlay, dx, dx)
+    for b in (gpu, orc):
+        cfg = Config(**kw)
+        st = driver.init_model(b, ic, cfg)
+        t = 0.0
+        for k in range(100):
+            t += driver.step(b, cfg, st, t)
+            if b is orc and (k + 1) % 20 == 0:
+                print(f"  oracle step {k + 1}", flush=True)
+    errs = {n: compare(gpu, orc, n) for n in ("dp", "pt", "ud", "vd")}
+    _print("full size x100", **errs)
+    for n, e in errs.items():
+        assert e <= PARITY, (n, e)
+    m0 = ic["dp"].sum()
+    for b in (gpu, orc):
+        assert abs(b.get("dp").sum() - m0) / m0 <= 1e-13
