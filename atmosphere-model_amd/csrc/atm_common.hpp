@@ -386,6 +386,16 @@ __device__ __forceinline__ void xcd_tile(int gx, int gy, int gz, int &bx, int &b
 }
 
 // launch helpers: 2-D tiles over I x J, z = level
+// Workgroup index -> work item so that each XCD gets a contiguous range:
+// the dispatcher hands workgroups to the 8 XCDs in turn, so the m-th
+// workgroup an XCD x receives is workgroup 8 m + x; with n = 8 q + r work
+// items, XCDs 0..r-1 take q + 1 consecutive items and the others q.  A
+// bijection of [0, n); neighbouring items then share their XCD's L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned n) {
+    const unsigned x = b % 8u, m = b / 8u, q = n / 8u, r = n % 8u;
+    return (x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q) + m;
+}
+
 static inline dim3 grid3(int nI, int nJ, int nK, dim3 b) {
     return dim3((nI + b.x - 1) / b.x, (nJ + b.y - 1) / b.y, nK);
 }

@@ -212,7 +212,9 @@ __global__ void __launch_bounds__(64, NL == 1 ? 3 : 1) k_rad_columns(Geo g, Tabl
                                                     const double *__restrict__ radius, double *__restrict__ net_flux,
                                                     int *flag) {
     extern __shared__ double lds[];
-    const int col = blockIdx.x;
+    // consecutive columns (in i) on one XCD: they share the 128-byte lines
+    // of every level they read
+    const int col = (int)xcd_block(blockIdx.x, gridDim.x);
     const int I = ATM_HALO + col % g.nx, J = ATM_HALO + col / g.nx;
     const int lane = threadIdx.x;
     const int L = g.L, N = L + 1;
