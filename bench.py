@@ -39,6 +39,11 @@ RAD_FLOP_PER_LAYER_G = 280
 RAD_G_POINTS = 16
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 
+# Algorithmic bytes of a whole step per cell (FP64 fields): the C stage's
+# compulsory reads and writes (9 x 8 B), the D stage's (13 x 8 B), two halo
+# fills (~7 B) and a remap every 5 steps (8 x 8 B / 5): ~196 B.
+WHOLE_STEP_BYTES = 72 + 104 + 7 + 12.8
+
 ALG_BYTES_PER_CELL = {
     "c1": 8 * (6 + 4),   # read dp, pt, ud, vd, playkap, net_flux; write dpc, ptc, uc, vc
     "c2": 8 * (6 + 2),   # read uc, vc, ud, vd, pkap, gz; write uc, vc
@@ -111,6 +116,14 @@ def calibrate_copy(mib):
 ENSEMBLE_SPECS = [(24, 192, 192, 48), (6, 256, 256, 48), (2, 384, 384, 64)]
 
 
+def whole_step_roofline(cell_steps_per_s):
+    """The whole step's algorithmic bytes per cell-step times the rate,
+    against the HBM peak (radiation and physics not counted)."""
+    gbs = WHOLE_STEP_BYTES * cell_steps_per_s / 1e9
+    return {"bytes_per_cell_step": WHOLE_STEP_BYTES, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS}
+
+
 def ensemble_timing(args):
     """The uneven ensemble: 32 members of three grid sizes, each from its own
     seeded state (k^-3 temperature noise of 0.5 K), mixed precision, dx and
@@ -136,6 +149,7 @@ def ensemble_timing(args):
            "value": cells * args.ensemble_steps / sec, "unit": "cell-steps/s",
            "ms_per_step": sec / args.ensemble_steps * 1e3,
            "launch": f"each member's own launches, members dealt to {args.ensemble_streams} HIP streams",
+           "whole_step_roofline": whole_step_roofline(cells * args.ensemble_steps / sec),
            "corner_levels_per_workgroup": {f"{nx}x{ny}x{L}": sorted({mb.model.tuning for mb in ens.members
                                                                      if mb.cells == nx * ny * L})
                                            for _, nx, ny, L in ENSEMBLE_SPECS}}
@@ -626,7 +640,10 @@ def main():
                    "nx": nx, "ny": ny, "nlay": L, "path": args.path, "precision": "fp64"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "profiles/traffic.json: FETCH_SIZE + WRITE_SIZE of this kernel "
+                                       "from a committed rocprofv3 pass, scaled by the copy calibration",
                      "alg_bytes_per_launch": alg, "avg_launch_ms": avg_s * 1e3},
+        "whole_step_roofline": whole_step_roofline(value),
         "kernels_ms_per_step": {k: v[1] / args.steps for k, v in kernels.items()},
         "adjoint_forward_cost_ratio": None,
         "tuning": {"corner_levels_per_workgroup": tuning, "autotune": args.autotune}
