@@ -529,3 +529,31 @@ def test_full_size_parity_100_steps(gpu_available):
     m0 = ic["dp"].sum()
     for b in (gpu, orc):
         assert abs(b.get("dp").sum() - m0) / m0 <= 1e-13
+
+
+def test_mixed_form_against_the_oracle_100_steps(gpu_available):
+    """The mixed form's error measured against the oracle itself (not the
+    FP64 GPU path): 64x64x20, dt = 60 s, remap every 5 steps, radiation every
+    10 steps, 100 steps.  The north star's bound is 1e-5 relative RMS in
+    surface pressure and temperature."""
+    import oracle as O
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+    nx, ny, nlay = 64, 64, 20
+    print("mixed form vs oracle 64x64x20 x100, radiation every 10", flush=True)
+    ic = make_initial_state(nx, ny, nlay, 100e3)
+    kw = dict(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=600.0, dt_physics=1e12, dt_remap=300.0,
+              radiation_on=True, physics_on=False)
+    gpu, orc = Model(nx, ny, nlay, 100e3, precision="mixed"), O.Oracle(nx, ny, nlay, 100e3, 100e3)
+    for b in (gpu, orc):
+        st = driver.init_model(b, ic, Config(**kw))
+        t, n, stable = driver.run_model(b, Config(**kw), st, max_steps=100)
+        assert stable and n == 100
+    ps_g, T_g = _ps_T(gpu, ic["ak"][0])
+    ps_o, T_o = _ps_T(orc, ic["ak"][0])
+    rr = {"ps": float(np.sqrt(np.mean((ps_g - ps_o) ** 2)) / np.sqrt(np.mean(ps_o ** 2))),
+          "T": float(np.sqrt(np.mean((T_g - T_o) ** 2)) / np.sqrt(np.mean(T_o ** 2)))}
+    _print("mixed vs oracle x100 rel rms", **rr)
+    assert rr["ps"] <= 1e-5 and rr["T"] <= 1e-5, rr
