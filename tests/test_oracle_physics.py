@@ -8,7 +8,9 @@
 * surface heating: ts advances by dt * net_flux(1) / (41680 * 5);
 * apply_physics_tendencies interpolates with the binary32 weights 7/12 and
   1/12 (Q1) along j for u and along i for v: an x<->y transpose of the
-  state transposes every result.
+  state transposes every result;
+* the Richardson-number mixing law for 0 < Ri < 1/4 from its shear
+  dependence at fixed stratification (the binary32 shear floor included).
 """
 import numpy as np
 
@@ -133,3 +135,46 @@ def test_apply_weights_binary32_and_transpose_symmetry():
     expect = u0[:, 3, 3] + 10.0 * (w7 * (ut + ut) - w1 * (ut + ut))
     np.testing.assert_array_equal(o.get("ud")[:, 3, 3], expect)
     assert abs((w7 * 2 - w1 * 2) - 1.0) > 1e-8
+
+
+def _fraction(du, dpt=1e-8):
+    """Mixing fraction f of a stable two-layer column with shear du (u of the
+    lower layer du, upper 0): the lower layer's wind tendency is
+    -f dp2/(dp1 + dp2) du / 600, the upper's +f dp1/(dp1 + dp2) du / 600."""
+    dp = np.array([40000.0, 25000.0])
+    o = _column_state(dp, [300.0, 300.0 + dpt], [du, 0.0], [0.0, 0.0])
+    o.calculate_physics_tendencies()
+    ut, _, _ = o.physics_column_tendencies(3, 4)
+    f0 = -ut[0] * 600.0 * (dp[0] + dp[1]) / (dp[1] * du)
+    f1 = ut[1] * 600.0 * (dp[0] + dp[1]) / (dp[0] * du)
+    return f0, f1
+
+
+def test_richardson_mixing_law():
+    """For 0 < Ri < 1/4 the exchange is (1 - Ri/Ric)^2 dp1 dp2/(dp1 + dp2) times
+    the difference (mod_physics.f90:99-111) with Ri proportional to
+    1/(du^2 + dv^2 + min_v2): at fixed stratification 1 - sqrt(f) =
+    4 A/(du^2 + min_v2) with one A for every shear.  The floor min_v2 is the
+    binary32 1e-4 (Q1); a linear law, no floor, or the double 1e-4 give a
+    different A at each shear.  Just past the threshold there is no mixing."""
+    floor = float(np.float32(1e-4))
+    shears = [0.025, 0.03, 0.04, 0.06, 0.1, 0.3]
+    s = []
+    for du in shears:
+        f0, f1 = _fraction(du)
+        assert 0.0 < f0 < 1.0 and abs(f0 - f1) <= 1e-12 * f0
+        s.append(1.0 - np.sqrt(f0))
+    s = np.array(s)
+    d2 = np.array(shears) ** 2
+    spread = lambda a: float(np.max(np.abs(a / a[0] - 1.0)))  # noqa: E731
+    A = s * (d2 + floor) / 4.0
+    assert spread(A) < 1e-11, A
+    # the alternatives the data rule out
+    assert spread(s * (d2 + 1e-4)) > 1e-9                      # the double 1e-4
+    assert spread(s * d2) > 1e-3                               # no floor
+    lin = np.array([1.0 - _fraction(du)[0] for du in shears]) * (d2 + floor)
+    assert spread(lin) > 1e-3                                  # (1 - Ri/Ric) unsquared
+    # Ri = 1/4 at du_c^2 = 4 A - min_v2: no exchange just below du_c, some just above
+    du_c = np.sqrt(4.0 * A[0] - floor)
+    assert _fraction(0.999 * du_c)[0] == 0.0
+    assert _fraction(1.001 * du_c)[0] > 0.0
