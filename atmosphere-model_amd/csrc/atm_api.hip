@@ -49,9 +49,11 @@ static double *field_ptr(atm_ctx *c, int id, int *nk) {
     }
 }
 
+// n doubles plus the tail that overhanging tiles read (zero-filled)
 static int dalloc(atm_ctx *c, double **p, size_t n) {
-    ATM_HIP(c, hipMalloc(p, sizeof(double) * (n ? n : 1)));
-    ATM_HIP(c, hipMemsetAsync(*p, 0, sizeof(double) * (n ? n : 1), c->stream));
+    const size_t m = n + atm_tile_slack(c->g);
+    ATM_HIP(c, hipMalloc(p, sizeof(double) * m));
+    ATM_HIP(c, hipMemsetAsync(*p, 0, sizeof(double) * m, c->stream));
     return ATM_OK;
 }
 
@@ -155,7 +157,7 @@ int atm_create(const atm_grid *grid, const atm_params *params, int precision, at
     c->own_stream = true;
     const size_t p = c->g.plane(), L = c->g.L;
     const char *path = getenv("ATM_PATH");
-    c->fused = atm_fused_supported(c->g) && !(path && std::string(path) == "reference");
+    c->fused = !(path && std::string(path) == "reference");
     c->mixed = precision == ATM_PRECISION_MIXED;
     if (c->mixed && !c->fused) return fail(ATM_ERR_ARGUMENT);  // the mixed form is on the fused path only
     struct { double **ptr; size_t n; } bufs[] = {
@@ -599,9 +601,9 @@ int atm_set_path(atm_ctx *c, int path) {
     c->plk_valid = false;
     c->nfw_tracked = false;
     if (path == ATM_PATH_FUSED) {
-        if (!atm_fused_supported(c->g) || !c->uc2)
-            return atm_fail(c, ATM_ERR_ARGUMENT, "fused path needs nx % " + std::to_string(ATM_TILE_X) + " == 0 and ny % " +
-                                                      std::to_string(ATM_TILE_Y) + " == 0");
+        if (!c->uc2)
+            return atm_fail(c, ATM_ERR_ARGUMENT,
+                            "the context was created on the reference path (ATM_PATH=reference): no fused buffers");
         c->fused = true;
     } else {
         if (c->mixed) return atm_fail(c, ATM_ERR_ARGUMENT, "the mixed-precision form runs on the fused path only");

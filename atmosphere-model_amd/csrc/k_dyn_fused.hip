@@ -22,6 +22,14 @@
 //
 // Tile coordinates: x = I - I0, y = J - J0 with (I0, J0) the tile origin in
 // the halo'd index space; LDS arrays cover [x0, x1) x [y0, y1) regions.
+//
+// Grids of any nx, ny: the last tile of a row or column may overhang the
+// domain.  Its threads past the domain run the same stages on whatever the
+// memory holds there (the halo, the row padding, the next row, or the tail
+// every buffer carries, atm_tile_slack), which only ever feeds cells past
+// the domain, and store nothing: every store, and every report to the
+// stability word, is for a cell inside the domain.  The stencils of cells
+// inside reach at most 4 cells out, all inside the halo of 7.
 #include <type_traits>
 
 #include "atm_common.hpp"
@@ -98,6 +106,15 @@ __device__ __forceinline__ unsigned own_off(const Geo &g, int tx_n) {
     return (unsigned)((ty + 4) * g.SI + tx + 4);
 }
 
+// Whether this thread's cell lies inside the domain (a tile may overhang
+// it), formed afresh like own_off so that no register holds it between the
+// stores.
+__device__ __forceinline__ bool own_inside(const Geo &g, int I0, int J0, int tx_n) {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return I0 + t % tx_n < ATM_HALO + g.nx && J0 + t / tx_n < ATM_HALO + g.ny;
+}
+
 // ---------------------------------------------------------------------------
 // C1 / D1: the dp and pt updates of one level (cgrid 94-268, dgrid 439-598).
 // CGRID: the C winds come from the D winds (109-126) and are also written
@@ -165,7 +182,7 @@ __global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo 
     __syncthreads();
 
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    double dpv = 0., dpcv = 0.;
+    double dpv = 0., dpcv = 0., pt_new = 0.;
     R fx0 = 0., fx1 = 0., fy0 = 0., fy1 = 0.;  // dp mass fluxes on the cell's faces
     for (int pass = 0; pass < 2; pass++) {
         // inner y step (tmp of the outer x step), with deny; inner x step with denx; Laplacian
@@ -220,8 +237,10 @@ __global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo 
             double d = dpv - (D(fx1) - D(fx0) + c.dtdx * c.vx * (D(lxp) - 2. * D(l0) + D(lxm)));
             d = d - (D(fy1) - D(fy0) + c.dtdy * c.vy * (D(lyp) - 2. * D(l0) + D(lym)));
             dpcv = d;
-            (dpo + kb0)[CGRID ? own_off(g, TX) : TOFF(tx, ty)] = d;
-            if (!CGRID) atm_report_nonfinite(nfw, d);
+            if (own_inside(g, I0, J0, TX)) {
+                (dpo + kb0)[CGRID ? own_off(g, TX) : TOFF(tx, ty)] = d;
+                if (!CGRID) atm_report_nonfinite(nfw, d);
+            }
             __syncthreads();
             FOR_REG(RQ) Q(x, y) = G(pt, x, y); END_FOR
             __syncthreads();
@@ -246,11 +265,17 @@ __global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo 
                       c.vy * (D((lyp - l0) * fy1) - D((l0 - lym) * fy0))) +
                  c.dt * dpv * hr) /
                 dpcv;
-            (pto + kb0)[CGRID ? own_off(g, TX) : TOFF(tx, ty)] = p;
-            if (!CGRID) atm_report_nonfinite(nfw, p);
+            // the float C step stores pt with its winds below: one masked
+            // branch instead of two keeps its registers from spilling
+            if (CGRID && !FP64) {
+                pt_new = p;
+            } else if (own_inside(g, I0, J0, TX)) {
+                (pto + kb0)[CGRID ? own_off(g, TX) : TOFF(tx, ty)] = p;
+                if (!CGRID) atm_report_nonfinite(nfw, p);
+            }
         }
     }
-    if (CGRID) {
+    if (CGRID && own_inside(g, I0, J0, TX)) {
         double u, v;
         if constexpr (FP64) {
             u = UC(tx, ty);
@@ -262,6 +287,7 @@ __global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo 
             v = .5 * (vb + .5 * (G(vw, tx + 1, ty) + G(vw, tx + 1, ty - 1)));
         }
         const unsigned o = own_off(g, TX);
+        if (!FP64) (pto + kb0)[o] = pt_new;
         (uco + kb0)[o] = u;
         (vco + kb0)[o] = v;
         if (K == g.L - 1) {  // A winds of the last level (119-125), read by the KE pass
@@ -347,6 +373,7 @@ __global__ __launch_bounds__(NT_WIND) void k_c2(Geo g, DynCoef c, const double *
     END_FOR
     __syncthreads();
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    if (!(I0 + tx < ATM_HALO + g.nx && J0 + ty < ATM_HALO + g.ny)) return;  // no barrier follows
     // pressure gradient force (325-339)
     double pgfx, pgfy;
     {
@@ -490,6 +517,7 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
     END_FOR
     __syncthreads();
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    if (!(I0 + tx < ATM_HALO + g.nx && J0 + ty < ATM_HALO + g.ny)) return;  // no barrier follows
     // B-grid pressure gradient (665-679) from the corner values of pkap and
     // gz at levels k and k+1 (k_bcorners; pkapb(nlay+1) and gzb(1) from
     // init_sw_dyn 991, 995)
@@ -931,12 +959,13 @@ int atm_fused_autotune(atm_ctx *c, int reps) {
     return rc;
 }
 
-bool atm_fused_supported(const Geo &g) { return g.nx % TX == 0 && g.ny % TY_PAIR == 0 && g.ny % TY_WIND == 0; }
-
 // ---------------------------------------------------------------------------
 // Step orchestration of the fused path.  Buffers: prognostic fields have two
 // copies (cur, nxt); the C step writes its own arrays; the D step writes nxt.
 // ---------------------------------------------------------------------------
+// tiles of t cells over n cells (the last may overhang)
+static unsigned tiles(int n, int t) { return (unsigned)((n + t - 1) / t); }
+
 int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
     const Geo &g = c->g;
     c->coef.dt = dt;
@@ -950,8 +979,8 @@ int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
         atm_timer_end(c, ATM_T_SCAN);
         c->plk_valid = true;
     }
-    dim3 grid(g.nx / TX, g.ny / TY_PAIR, g.L), blk(NT_PAIR);
-    dim3 gridw(g.nx / TX, g.ny / TY_WIND, g.L), blkw(NT_WIND);
+    dim3 grid(tiles(g.nx, TX), tiles(g.ny, TY_PAIR), g.L), blk(NT_PAIR);
+    dim3 gridw(tiles(g.nx, TX), tiles(g.ny, TY_WIND), g.L), blkw(NT_WIND);
     atm_timer_begin(c, ATM_T_C1);
     if (c->mixed)
         hipLaunchKernelGGL((k_pair<V_PPM_U, true, float>), grid, blk, 0, c->stream, g, k, b.dp, b.pt, b.ud, b.vd,
@@ -1003,8 +1032,8 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
     c->coef.dtdx = dt * c->coef.rdx;
     c->coef.dtdy = dt * c->coef.rdy;
     const DynCoef &k = c->coef;
-    dim3 grid(g.nx / TX, g.ny / TY_PAIR, g.L), blk(NT_PAIR);
-    dim3 gridw(g.nx / TX, g.ny / TY_WIND, g.L), blkw(NT_WIND);
+    dim3 grid(tiles(g.nx, TX), tiles(g.ny, TY_PAIR), g.L), blk(NT_PAIR);
+    dim3 gridw(tiles(g.nx, TX), tiles(g.ny, TY_WIND), g.L), blkw(NT_WIND);
     atm_timer_begin(c, ATM_T_D1);
     if (c->mixed)
         hipLaunchKernelGGL((k_pair<V_PPM_NN, false, float>), grid, blk, 0, c->stream, g, k, cur.dp, cur.pt, c->uc2,

@@ -82,8 +82,8 @@ typedef struct atm_params {
    float and runs the adding method in double; the column scans, pressure
    gradient, heating rate, remap and physics stay FP64.
    Each face flux is one value shared by the two cells it separates, so mass
-   is conserved to FP64 rounding.  MIXED needs the fused path (nx % 32 == 0,
-   ny % 16 == 0); it has no counterpart in the reference. */
+   is conserved to FP64 rounding.  MIXED runs on the fused path (any grid);
+   it has no counterpart in the reference. */
 enum { ATM_PRECISION_FP64 = 0, ATM_PRECISION_MIXED = 1 };
 
 /* Context lifetime.  Allocates every device buffer for one model instance.
@@ -177,10 +177,13 @@ int atm_halo_exchange(atm_ctx *ctx, int field);
 int atm_check_stability(atm_ctx *ctx, int *stable);
 
 /* Execution path of the dynamics (no counterpart in the reference; both
-   give identical results).  ATM_PATH_FUSED (default where nx % 32 == 0 and
-   ny % 16 == 0): LDS-tiled kernels, four per step.  ATM_PATH_REFERENCE: one
-   kernel per reference loop nest, in place.  Settable between steps; the
-   environment variable ATM_PATH=reference selects it at creation. */
+   give identical results).  ATM_PATH_FUSED (the default, any nx and ny):
+   LDS-tiled kernels, four per step; tiles of 32 x 8 and 32 x 16 cells, the
+   last of a row or column overhanging the domain when nx or ny is not a
+   multiple.  ATM_PATH_REFERENCE: one kernel per reference loop nest, in
+   place.  Settable between steps; the environment variable
+   ATM_PATH=reference selects it at creation (the context then holds no
+   fused buffers and cannot switch to the fused path). */
 enum { ATM_PATH_FUSED = 0, ATM_PATH_REFERENCE = 1 };
 int atm_set_path(atm_ctx *ctx, int path);
 int atm_get_path(atm_ctx *ctx, int *path);

@@ -97,10 +97,13 @@ def test_driver_run(gpu_available, shape, steps, rad):
         assert e <= PARITY, (n, e)
 
 
-@pytest.mark.parametrize("shape,steps", [((32, 32, 20), 11), ((64, 32, 16), 6)])
+@pytest.mark.parametrize("shape,steps", [((32, 32, 20), 11), ((64, 32, 16), 6), ((37, 23, 10), 11),
+                                         ((200, 200, 20), 6), ((8, 8, 4), 11), ((33, 17, 5), 11)])
 def test_fused_path_bitwise_equals_reference_path(gpu_available, shape, steps):
     """The fused LDS-tiled kernels and the reference-structured kernels
-    produce identical bits, including remap and radiation steps."""
+    produce identical bits, including remap and radiation steps; on grids
+    that are not multiples of the tiles (37x23, 200x200, 8x8, 33x17) the
+    last tiles overhang the domain and store only the cells inside."""
     from atmos import driver
     from atmos.config import Config
     from atmos.ic import make_initial_state
@@ -376,18 +379,19 @@ def _ps_T(m, ptop):
     return plev[0], pt * ((pk[1:] - pk[:-1]) / np.log(pk[1:] / pk[:-1]))
 
 
-@pytest.mark.parametrize("dy", [100e3, 75e3])
-def test_mixed_precision_error_and_mass(gpu_available, dy):
+@pytest.mark.parametrize("shape,dy", [((64, 64, 20), 100e3), ((64, 64, 20), 75e3), ((40, 24, 8), 100e3)])
+def test_mixed_precision_error_and_mass(gpu_available, shape, dy):
     """ATM_PRECISION_MIXED against the FP64 path: 30 steps with remaps every
-    5 at 64x64x20 from the same state, at dx = dy and dx != dy.  Surface
-    pressure and temperature agree to float-level error, and the total mass
-    (sum of dp) changes only by FP64 rounding in both forms (the float
-    fluxes are shared by the two cells of each face)."""
+    5 at 64x64x20 from the same state, at dx = dy and dx != dy, and on a
+    grid the tiles overhang (40x24x8).  Surface pressure and temperature
+    agree to float-level error, and the total mass (sum of dp) changes only
+    by FP64 rounding in both forms (the float fluxes are shared by the two
+    cells of each face)."""
     from atmos import driver
     from atmos.config import Config
     from atmos.ic import make_initial_state
     from atmos.model import AtmError, Model
-    nx, ny, nlay = 64, 64, 20
+    nx, ny, nlay = shape
     ic = make_initial_state(nx, ny, nlay, 100e3, dy)
     kw = dict(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12, dt_remap=300.0,
               radiation_on=False, physics_on=False)
@@ -408,12 +412,10 @@ def test_mixed_precision_error_and_mass(gpu_available, dy):
                 m.set_path("reference")
     a, b = out["mixed"], out["fp64"]
     rr = {k: float(np.sqrt(np.mean((a[k] - b[k]) ** 2)) / np.sqrt(np.mean(b[k] ** 2))) for k in ("ps", "T", "ud", "vd")}
-    _print(f"mixed vs fp64 x30 dy={dy:.0f} rel rms", **rr, drift_mixed=abs(a["drift"]), drift_fp64=abs(b["drift"]))
+    _print(f"mixed vs fp64 x30 {shape} dy={dy:.0f} rel rms", **rr, drift_mixed=abs(a["drift"]), drift_fp64=abs(b["drift"]))
     assert rr["ps"] <= 1e-6 and rr["T"] <= 1e-6, rr
     assert rr["ud"] <= 1e-4 and rr["vd"] <= 1e-4, rr
     assert abs(a["drift"]) <= 1e-13 and abs(b["drift"]) <= 1e-13
-    with pytest.raises(AtmError):  # the mixed form needs the fused path's tiling
-        Model(40, 24, 8, 100e3, precision="mixed")
 
 
 def test_full_size_parity(gpu_available):
