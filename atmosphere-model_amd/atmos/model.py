@@ -34,7 +34,7 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_stream_destroy", "atm_graph_begin", "atm_graph_end", "atm_graph_launch", "atm_graph_free",
            "atm_autotune", "atm_get_tuning", "atm_set_tuning", "atm_lin_init", "atm_tl_step", "atm_ad_step",
            "atm_twostream", "atm_twostream_noalloc", "atm_twostream_last_error", "atm_stream_synchronize", "atm_stream_timer_begin",
-           "atm_stream_timer_end"]
+           "atm_stream_timer_end", "atm_poison"]
 PATHS = {"fused": 0, "reference": 1}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 TIMERS = {"c1": 0, "c2": 1, "d1": 2, "d2": 3, "scan": 4, "halo": 5, "remap": 6, "radiation": 7,
@@ -98,6 +98,7 @@ def load_library(path: str = LIB_PATH):
     L.atm_set_path.argtypes = [vp, ip]
     L.atm_get_path.argtypes = [vp, C.POINTER(C.c_int)]
     L.atm_timer_enable.argtypes = [vp, ip]
+    L.atm_poison.argtypes = [vp, ip, C.c_double]
     L.atm_timer_read.argtypes = [vp, ip, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     L.atm_lin_init.argtypes = [vp]
     for n in ["atm_tl_step", "atm_ad_step"]:
@@ -297,6 +298,12 @@ class Model:
 
     def timer_enable(self, on=True):
         self._ck(self._L.atm_timer_enable(self._h, int(bool(on))))
+
+    def poison(self, padding=True, outer_halo=True, value=float("nan")):
+        """Write `value` into memory no kernel may read for a result inside
+        the domain: the row padding and buffer tails, and/or the halo cells
+        farther than 4 from the interior (atm_poison)."""
+        self._ck(self._L.atm_poison(self._h, (1 if padding else 0) | (2 if outer_halo else 0), value))
 
     def timer_read(self, name):
         """(launches, total milliseconds) of one kernel group since enabling."""

@@ -53,6 +53,7 @@ static double *field_ptr(atm_ctx *c, int id, int *nk) {
 static int dalloc(atm_ctx *c, double **p, size_t n) {
     const size_t m = n + atm_tile_slack(c->g);
     ATM_HIP(c, hipMalloc(p, sizeof(double) * m));
+    c->field_bufs.push_back({*p, n});
     ATM_HIP(c, hipMemsetAsync(*p, 0, sizeof(double) * m, c->stream));
     return ATM_OK;
 }
@@ -71,6 +72,7 @@ static void release(atm_ctx *c) {
     for (int f = 0; f < 4; f++)
         for (int b = 0; b < 2; b++)
             if (c->pbuf[f][b]) { (void)hipFree(c->pbuf[f][b]); c->pbuf[f][b] = nullptr; }
+    c->field_bufs.clear();
     double **fz[] = {&c->uc2, &c->vc2, &c->heatd, &c->plk};
     for (auto p : fz)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
@@ -640,6 +642,17 @@ int atm_set_tuning(atm_ctx *c, int corner_levels) {
     if (corner_levels != 1 && corner_levels != 2 && corner_levels != 4 && corner_levels != 8)
         return atm_fail(c, ATM_ERR_ARGUMENT, "corner levels per workgroup: 1, 2, 4 or 8");
     c->bc_levels = corner_levels;
+    return ATM_OK;
+}
+
+int atm_poison(atm_ctx *c, int what, double value) {
+    CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
+    if (what & ~(ATM_POISON_PADDING | ATM_POISON_OUTER_HALO)) return atm_fail(c, ATM_ERR_ARGUMENT, "poison: bad class");
+    for (const auto &b : c->field_bufs) {
+        const int rc = atm_launch_poison(c, b.first, b.second, what, value);
+        if (rc) return rc;
+    }
     return ATM_OK;
 }
 

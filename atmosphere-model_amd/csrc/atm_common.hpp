@@ -289,8 +289,8 @@ struct atm_ctx {
     bool have_coord = false, sw_init = false, remap_init = false, rad_init = false, phys_init = false;
     double ak0 = 0;    // Ak(1) = top pressure
     double pktop = 0;  // Ak(1)**kappa, evaluated on the host as the reference does
-    // execution path: fused LDS-tiled kernels (default where the grid tiles
-    // evenly) or the reference-structured kernels of k_dyn_ref.hip
+    // execution path: fused LDS-tiled kernels (the default) or the
+    // reference-structured kernels of k_dyn_ref.hip
     bool fused = false;
     // mixed precision (fused path only): the tile kernels reconstruct and
     // form fluxes in float, update the fields in double
@@ -322,6 +322,9 @@ struct atm_ctx {
            *deny = nullptr, *dpfx = nullptr, *dpfy = nullptr, *fxq = nullptr, *fyq = nullptr,
            *vort = nullptr, *ke = nullptr, *pgfx = nullptr, *pgfy = nullptr, *divg = nullptr;
     double *ua = nullptr, *va = nullptr, *ub = nullptr, *vb = nullptr;  // 2-D (last level)
+    // every field buffer of the context with its size in doubles (without
+    // the tail), for atm_poison
+    std::vector<std::pair<double *, size_t>> field_bufs;
     int *flag = nullptr;        // device status word
     int *flag_host = nullptr;   // pinned mirror
     // Stability tracking.  Every kernel that writes interior dp, pt, ud or
@@ -405,6 +408,9 @@ int atm_launch_halo(atm_ctx *c, double *a, int nk);
 #define ATM_HALO_MAXF 6
 // periodic halo fill of n <= ATM_HALO_MAXF fields (nk[i] levels each) in one launch
 int atm_launch_halo_multi(atm_ctx *c, double *const *a, const int *nk, int n);
+// fills the poisoned classes (ATM_POISON_*) of a field buffer of n doubles
+// (whole planes, or fewer than one plane: then only its tail)
+int atm_launch_poison(atm_ctx *c, double *buf, size_t n, int what, double value);
 int atm_ref_cgrid(atm_ctx *c, double dt);
 int atm_ref_dgrid(atm_ctx *c, double dt);
 int atm_ref_init_sw_dyn(atm_ctx *c);

@@ -8,6 +8,7 @@
 // bottom/top strips (nx x 7 each).  Reads touch the interior only and
 // writes the halo only, so the fill is race-free in one launch.
 #include "atm_common.hpp"
+#include "atm_fused.hpp"
 
 namespace {
 
@@ -60,7 +61,45 @@ __global__ void k_finite(Geo g, const double *__restrict__ a, const double *__re
     if (bad) atomicOr(flag, 1);
 }
 
+// atm_poison over the planes of one buffer: the row padding (i >= NI)
+// and/or the halo cells farther than 4 from the interior
+__global__ void k_poison_planes(Geo g, double *__restrict__ a, int what, double value) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y;
+    const size_t K = blockIdx.z;
+    if (i >= g.SI) return;
+    const int far = ATM_HALO - 4;  // halo cells [0, far) and past the interior + 4
+    bool hit;
+    if (i >= g.NI)
+        hit = (what & ATM_POISON_PADDING) != 0;
+    else
+        hit = (what & ATM_POISON_OUTER_HALO) != 0 &&
+              (i < far || i >= ATM_HALO + g.nx + 4 || J < far || J >= ATM_HALO + g.ny + 4);
+    if (hit) a[K * g.plane() + (size_t)J * g.SI + i] = value;
+}
+
+__global__ void k_poison_span(double *__restrict__ a, size_t n, double value) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = value;
+}
+
 }  // namespace
+
+int atm_launch_poison(atm_ctx *c, double *buf, size_t n, int what, double value) {
+    const Geo &g = c->g;
+    const size_t pl = g.plane();
+    if (n >= pl && n % pl == 0) {
+        const dim3 b(256), gr((g.SI + 255) / 256, g.NJ, (unsigned)(n / pl));
+        hipLaunchKernelGGL(k_poison_planes, gr, b, 0, c->stream, g, buf, what, value);
+    }
+    if (what & ATM_POISON_PADDING) {  // the tail past the last plane
+        const size_t m = atm_tile_slack(g);
+        hipLaunchKernelGGL(k_poison_span, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->stream, buf + n, m,
+                           value);
+    }
+    ATM_HIP(c, hipGetLastError());
+    return ATM_OK;
+}
 
 int atm_launch_halo_multi(atm_ctx *c, double *const *a, const int *nk, int n) {
     const Geo &g = c->g;

@@ -197,6 +197,21 @@ int atm_get_path(atm_ctx *ctx, int *path);
 int atm_timer_enable(atm_ctx *ctx, int on);
 int atm_timer_read(atm_ctx *ctx, int kernel, int64_t *launches, double *total_ms);
 
+/* Read-extent diagnostics (no counterpart in the reference; for tests).
+   atm_poison writes `value` (a NaN, say) into device memory that no kernel
+   may read for a result inside the domain, in every field buffer of the
+   context: ATM_POISON_PADDING, the row padding past ie (the device's row
+   stride is nx+14 rounded up to 16) and the tail past the last plane (what
+   the overhanging tiles of the fused path read); ATM_POISON_OUTER_HALO, the
+   halo cells more than 4 from the interior (no kernel of the fused path
+   reads farther for a cell inside; the halo fills rewrite them; the
+   reference-structured path evaluates the first phase of a step on the
+   ring, as the reference does, from cells farther out).  Stream-ordered; refused
+   while recording a graph.  Fields read back through atm_get_field show the
+   value in the poisoned outer halo until the next halo fill. */
+enum { ATM_POISON_PADDING = 1, ATM_POISON_OUTER_HALO = 2 };
+int atm_poison(atm_ctx *ctx, int what, double value);
+
 /* Launch-shape tuning (no counterpart in the reference): atm_autotune times
    each candidate launch shape of the fused path's tunable kernels on this
    context's grid and stream (one warm-up launch, then `reps` timed ones)
