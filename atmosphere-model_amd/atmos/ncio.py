@@ -31,7 +31,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libatmos_io.so")
+# ATM_IO_LIB names another build of the library (the sanitizer build of the
+# host tests)
+LIB_PATH = os.environ.get("ATM_IO_LIB") or os.path.join(os.path.dirname(_HERE), "lib", "libatmos_io.so")
 
 NOWRITE, WRITE, CLOBBER, NOCLOBBER = 0, 1, 0, 4
 CDF1, CDF2, CDF5 = 0, 0x0200, 0x0020

@@ -158,7 +158,10 @@ __host__ __device__ inline void pow_kappa_n(double (&x)[N]) {
         const double h = 1. + rh[i];
         const double l = (1. - h) + rh[i];
         const double t = rh[i] * rh[i] * q[i];  // e^rh = h + l + t
-        v[i] = ldexp(h + (t + fma(rl, h + t, l)), (int)n[i]);
+        // n is clamped before its conversion to int, which would be undefined
+        // for the NaN of x < 0 or x = NaN (whose result is replaced below);
+        // |n| < 310 for every finite positive x, so no value changes
+        v[i] = ldexp(h + (t + fma(rl, h + t, l)), (int)dmin2(dmax2(n[i], -2000.), 2000.));
         const double xi = x[i];
         x[i] = (xi > 0. && xi < __builtin_inf()) ? v[i] : xi == 0. ? 0. : xi == __builtin_inf() ? xi : __builtin_nan("");
     }

@@ -7,7 +7,8 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB = os.path.join(ROOT, "atmosphere-model_amd", "lib", "libatmos_lin_host.so")
+# ATM_LIN_HOST_LIB names another build (the sanitizer build)
+LIB = os.environ.get("ATM_LIN_HOST_LIB") or os.path.join(ROOT, "atmosphere-model_amd", "lib", "libatmos_lin_host.so")
 H = 7
 STATE = ("dp", "pt", "ud", "vd")
 

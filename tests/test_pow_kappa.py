@@ -14,7 +14,8 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB = os.path.join(ROOT, "atmosphere-model_amd", "lib", "libatmos_column_host.so")
+# ATM_COLUMN_HOST_LIB names another build (the sanitizer build)
+LIB = os.environ.get("ATM_COLUMN_HOST_LIB") or os.path.join(ROOT, "atmosphere-model_amd", "lib", "libatmos_column_host.so")
 KAPPA = float(np.float32(287.052874)) / 1005.0  # R_dry (binary32 literal) / cp_dry
 
 

@@ -19,7 +19,8 @@ import pytest
 import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB = os.path.join(ROOT, "atmosphere-model_amd", "lib", "libatmos_column_host.so")
+# ATM_COLUMN_HOST_LIB names another build (the sanitizer build)
+LIB = os.environ.get("ATM_COLUMN_HOST_LIB") or os.path.join(ROOT, "atmosphere-model_amd", "lib", "libatmos_column_host.so")
 
 
 @pytest.fixture(scope="module")
