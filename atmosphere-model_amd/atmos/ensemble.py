@@ -3,12 +3,26 @@
 Each member is a `Model` with its own device state and its own seeded
 initial state (temperature noise with a k^-3 spectrum on top of the common
 recipe).  The members share one schedule: the time step, the remap
-interval and the radiation interval.  A step issues every member's
-launches in the reference's per-step call order (`driver.step`) on that
-member's HIP stream, then checks every member's stability; members are
-dealt round-robin to a few streams of default priority, so the launches of
-different members overlap on the GPU while each member's own launches stay
-in order.
+interval and the radiation interval.  Members are dealt round-robin to a
+few HIP streams of default priority, so the launches of different members
+overlap on the GPU while each member's own launches stay in order.
+
+A step issues every member's calls in the reference's per-step order
+(`driver.step`, mod_model.f90:117-183) and checks every member's stability
+(check_stability, mod_fields.f90:69-106).  With deferred checks (the
+default) a member's check is queued behind its step and read after the
+next step has been issued, so the streams never drain between steps: the
+result `step()` returns is that of the previous step, and `finish()` reads
+the last one.  `checks="sync"` waits for every member's check each step.
+
+Graph mode (`graph_period=P`): each member records P steps of dynamics and
+remaps as one HIP graph (atm_graph_begin/end) during the first period and
+replays it for every later one, with radiation issued eagerly at each
+period's start and one deferred check per period.  The period must hold
+whole remap intervals and an even number of buffer switches (each D step
+and each remap switches), and radiation may fall due only at period
+starts.  The recorded time steps are those of the first period; the remap
+pattern of every later period is checked against it.
 """
 from __future__ import annotations
 
@@ -27,6 +41,8 @@ class Member:
     config: Config
     time: float
     cells: int
+    pending: bool = False  # a deferred check is queued
+    pattern: list | None = None  # graph mode: remap flags of the recorded period
 
 
 class Ensemble:
@@ -36,9 +52,17 @@ class Ensemble:
 
     def __init__(self, specs, dx: float, dt: float, remap_every: int = 5, radiation_every: int = 10,
                  precision: str = "fp64", nstreams: int = 4, seed: int = 20261205, t_noise: float = 0.5,
-                 autotune: bool = False):
+                 autotune: bool = False, checks: str = "deferred", graph_period: int = 0):
         if nstreams < 1:
             raise ValueError("nstreams must be >= 1")
+        if checks not in ("deferred", "sync"):
+            raise ValueError("checks: 'deferred' or 'sync'")
+        self.remap_every, self.radiation_every = remap_every, radiation_every
+        self.checks = checks
+        self.graph_period = 0
+        if graph_period:
+            self._validate_period(graph_period)
+            self.graph_period = graph_period
         self.streams: list[int] = []
         self.members: list[Member] = []
         try:
@@ -68,13 +92,97 @@ class Ensemble:
 
     def step(self) -> bool:
         """One step of every member; False when a member is no longer stable
-        (check_stability, mod_fields.f90:69-106)."""
+        (with deferred checks: was not stable after the previous step)."""
+        if self.graph_period:
+            raise RuntimeError("graph mode steps whole periods: period()")
         dts = [driver.step(mb.model, mb.config, mb.state, mb.time) for mb in self.members]
-        ok = True
         for mb, dt in zip(self.members, dts):
-            ok = mb.model.check_stability() and ok
             mb.time = mb.time + dt
+        return self._check()
+
+    def _validate_period(self, P: int):
+        if self.remap_every < 1 or P % self.remap_every or (P + P // self.remap_every) % 2:
+            raise ValueError("graph_period must hold whole remap intervals and an even number of "
+                             "buffer switches (D steps + remaps)")
+        if self.radiation_every > 0 and self.radiation_every % P:
+            raise ValueError("radiation must fall due at period starts (radiation_every % graph_period == 0)")
+
+    def use_graphs(self, period: int):
+        """Switch to graph mode from here on (at a period start of the
+        schedule: radiation due now or not within the period)."""
+        self._validate_period(period)
+        self.graph_period = period
+
+    def period(self) -> bool:
+        """Graph mode: graph_period steps of every member (the first call
+        records each member's graph); False when a member was not stable
+        after the previous period."""
+        if not self.graph_period:
+            raise RuntimeError("period() needs graph_period")
+        for mb in self.members:
+            self._member_period(mb)
+        return self._check()
+
+    def finish(self) -> bool:
+        """Reads the deferred checks still queued; False when a member was
+        not stable after its last step."""
+        ok = True
+        for mb in self.members:
+            if mb.pending:
+                mb.pending = False
+                ok = mb.model.check_stability_end() and ok
         return ok
+
+    def _check(self) -> bool:
+        ok = True
+        for mb in self.members:
+            if self.checks == "sync":
+                ok = mb.model.check_stability() and ok
+                continue
+            if mb.pending:
+                mb.pending = False
+                ok = mb.model.check_stability_end() and ok
+            mb.model.check_stability_begin()
+            mb.pending = True
+        return ok
+
+    def _member_period(self, mb: Member):
+        m, st, cfg = mb.model, mb.state, mb.config
+        if st.calculate_radiation_this_step:  # due at the period start
+            if cfg.radiation_on:
+                m.calculate_radiative_heating(mb.time)
+                m.radiation_halo_exchange()
+            st.calculate_radiation_this_step = False
+        record = mb.pattern is None
+        if record:
+            if mb.pending:  # a graph recording starts with no check queued
+                mb.pending = False
+                if not m.check_stability_end():
+                    raise FloatingPointError("ensemble member unstable before its graph was recorded")
+            m.graph_begin()
+        t, pattern = mb.time, []
+        for k in range(self.graph_period):
+            dt = driver.calculate_dt(t, cfg, st)
+            remap = st.remap_this_step
+            st.remap_this_step = False
+            if record:
+                m.cgrid_dynamics_step(dt / 2.0)
+                m.cgrid_halo_exchange()
+                m.dgrid_dynamics_step(dt)
+                if remap:
+                    m.perform_vertical_remapping()
+                m.dgrid_halo_exchange()
+            pattern.append(remap)
+            t = t + dt
+            if st.calculate_radiation_this_step and k + 1 < self.graph_period:
+                raise RuntimeError("radiation fell due inside a graph period")
+        if record:
+            m.graph_end()
+            mb.pattern = pattern
+        elif pattern != mb.pattern:
+            raise RuntimeError("the remap pattern of this period differs from the recorded one")
+        m.graph_launch(1)
+        mb.time = t
 
     def sync(self):
         for mb in self.members:

@@ -34,7 +34,7 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_stream_destroy", "atm_graph_begin", "atm_graph_end", "atm_graph_launch", "atm_graph_free",
            "atm_autotune", "atm_get_tuning", "atm_set_tuning", "atm_lin_init", "atm_tl_step", "atm_ad_step",
            "atm_twostream", "atm_twostream_noalloc", "atm_twostream_last_error", "atm_stream_synchronize", "atm_stream_timer_begin",
-           "atm_stream_timer_end", "atm_poison"]
+           "atm_stream_timer_end", "atm_poison", "atm_check_stability_begin", "atm_check_stability_end"]
 PATHS = {"fused": 0, "reference": 1}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 TIMERS = {"c1": 0, "c2": 1, "d1": 2, "d2": 3, "scan": 4, "halo": 5, "remap": 6, "radiation": 7,
@@ -95,6 +95,8 @@ def load_library(path: str = LIB_PATH):
     L.atm_halo_exchange.argtypes = [vp, ip]
     L.atm_output_mean.argtypes = [vp, dp]
     L.atm_check_stability.argtypes = [vp, C.POINTER(C.c_int)]
+    L.atm_check_stability_begin.argtypes = [vp]
+    L.atm_check_stability_end.argtypes = [vp, C.POINTER(C.c_int)]
     L.atm_set_path.argtypes = [vp, ip]
     L.atm_get_path.argtypes = [vp, C.POINTER(C.c_int)]
     L.atm_timer_enable.argtypes = [vp, ip]
@@ -266,6 +268,16 @@ class Model:
     def check_stability(self):
         s = C.c_int(0)
         self._ck(self._L.atm_check_stability(self._h, C.byref(s)))
+        return bool(s.value)
+
+    def check_stability_begin(self):
+        """Queue a stability check behind the calls issued so far (returns at once)."""
+        self._ck(self._L.atm_check_stability_begin(self._h))
+
+    def check_stability_end(self):
+        """The result of the check queued by check_stability_begin (waits for it only)."""
+        s = C.c_int(0)
+        self._ck(self._L.atm_check_stability_end(self._h, C.byref(s)))
         return bool(s.value)
 
     def sync(self):

@@ -84,6 +84,11 @@ static void release(atm_ctx *c) {
     }
     if (c->flag) (void)hipFree(c->flag);
     if (c->flag_host) (void)hipHostFree(c->flag_host);
+    if (c->chk_host) (void)hipHostFree(c->chk_host);
+    c->chk_host = nullptr;
+    if (c->chk_ev) (void)hipEventDestroy(c->chk_ev);
+    c->chk_ev = nullptr;
+    c->chk_pending = false;
     if (c->nfw) (void)hipFree(c->nfw);
     atm_remap_free(c);
     atm_rad_free(c);
@@ -323,7 +328,7 @@ static int xfer(atm_ctx *c, int id, void *dst, const void *src, bool to_dev, hip
         return hipMemcpy2DAsync(to, hp, dev, dpch, w, rows, kind, c->stream);
     };
     if (to_dev && id == ATM_DP) c->plk_valid = false;
-    if (to_dev && (id == ATM_DP || id == ATM_PT || id == ATM_UD || id == ATM_VD)) c->nfw_tracked = false;
+    if (to_dev && (id == ATM_DP || id == ATM_PT || id == ATM_UD || id == ATM_VD)) atm_untrack(c);
     if (split_winds(c, id)) {
         double *ring = c->pbuf[id == ATM_UD ? 2 : 3][1 - c->cur];
         if (to_dev) {
@@ -434,7 +439,7 @@ int atm_dgrid_step(atm_ctx *c, double dt) {
         return ATM_OK;
     }
     c->plk_valid = false;
-    c->nfw_tracked = false;  // the reference-structured kernels do not report
+    atm_untrack(c);  // the reference-structured kernels do not report
     return atm_ref_dgrid(c, dt);
 }
 
@@ -577,6 +582,7 @@ int atm_check_stability(atm_ctx *c, int *stable) {
     CHECK_CTX(c);
     CHECK_NOT_CAPTURING(c);
     if (!stable) return atm_fail(c, ATM_ERR_ARGUMENT, "null result pointer");
+    if (c->chk_pending) return atm_fail(c, ATM_ERR_STATE, "a deferred check is pending: atm_check_stability_end first");
     if (c->nfw_tracked) {
         ATM_HIP(c, hipMemcpyAsync(c->flag_host, c->nfw, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         ATM_HIP(c, hipStreamSynchronize(c->stream));
@@ -595,13 +601,58 @@ int atm_check_stability(atm_ctx *c, int *stable) {
     return ATM_OK;
 }
 
+// The deferred form of the check: _begin queues it on the stream behind
+// the work issued so far and returns; _end waits for it alone (not for
+// work issued after _begin) and reports.  A non-finite value written into
+// a prognostic field by a reporting kernel reads as unstable here, without
+// the confirming scan atm_check_stability makes.
+int atm_check_stability_begin(atm_ctx *c) {
+    CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
+    if (c->chk_pending) return atm_fail(c, ATM_ERR_STATE, "a deferred check is pending: atm_check_stability_end first");
+    if (!c->chk_ev) ATM_HIP(c, hipEventCreateWithFlags(&c->chk_ev, hipEventDisableTiming));
+    if (!c->chk_host) ATM_HIP(c, hipHostMalloc((void **)&c->chk_host, sizeof(int), hipHostMallocDefault));
+    c->chk_scan = !c->nfw_tracked;
+    if (c->chk_scan) {
+        int rc = atm_launch_stability(c);
+        if (rc) return rc;
+        ATM_HIP(c, hipMemcpyAsync(c->chk_host, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        ATM_HIP(c, hipMemsetAsync(c->nfw, 0, sizeof(int), c->stream));
+    } else {
+        ATM_HIP(c, hipMemcpyAsync(c->chk_host, c->nfw, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    }
+    ATM_HIP(c, hipEventRecord(c->chk_ev, c->stream));
+    c->chk_epoch = c->untrack_epoch;
+    c->chk_pending = true;
+    return ATM_OK;
+}
+
+int atm_check_stability_end(atm_ctx *c, int *stable) {
+    CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
+    if (!stable) return atm_fail(c, ATM_ERR_ARGUMENT, "null result pointer");
+    if (!c->chk_pending) return atm_fail(c, ATM_ERR_STATE, "no deferred check: atm_check_stability_begin first");
+    c->chk_pending = false;
+    ATM_HIP(c, hipEventSynchronize(c->chk_ev));
+    const bool bad = (*c->chk_host & 1) != 0;
+    *stable = bad ? 0 : 1;
+    if (bad) {
+        atm_untrack(c);
+    } else if (c->chk_scan && c->untrack_epoch == c->chk_epoch) {
+        // the scan found the fields finite and every write since has
+        // reported into the word it cleared
+        c->nfw_tracked = true;
+    }
+    return ATM_OK;
+}
+
 int atm_set_path(atm_ctx *c, int path) {
     CHECK_CTX(c);
     CHECK_NOT_CAPTURING(c);
     if (path != ATM_PATH_FUSED && path != ATM_PATH_REFERENCE) return atm_fail(c, ATM_ERR_ARGUMENT, "bad path");
     if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "finish the step (atm_dgrid_halo) first");
     c->plk_valid = false;
-    c->nfw_tracked = false;
+    atm_untrack(c);
     if (path == ATM_PATH_FUSED) {
         if (!c->uc2)
             return atm_fail(c, ATM_ERR_ARGUMENT,
@@ -703,6 +754,7 @@ int atm_graph_begin(atm_ctx *c) {
     CHECK_NOT_CAPTURING(c);
     if (!c->sw_init) return atm_fail(c, ATM_ERR_STATE, "atm_init_sw_dyn first");
     if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "a graph starts at a step boundary");
+    if (c->chk_pending) return atm_fail(c, ATM_ERR_STATE, "a deferred check is pending: atm_check_stability_end first");
     ATM_HIP(c, hipStreamSynchronize(c->stream));
     if (c->gexec) {
         ATM_HIP(c, hipGraphExecDestroy(c->gexec));
@@ -784,7 +836,7 @@ int atm_graph_launch(atm_ctx *c, int times) {
         // the stability tracking (lost when a recorded write does not report)
         for (double dt : c->graph_dts) c->acc_time = c->acc_time + dt;
         c->plk_valid = c->graph_plk_end;
-        if (!c->graph_tracks) c->nfw_tracked = false;
+        if (!c->graph_tracks) atm_untrack(c);
     }
     return ATM_OK;
 }
@@ -823,7 +875,7 @@ int atm_tl_step(atm_ctx *c, double dt, int remap) {
     rc = atm_lin_tl_step(c, dt, remap);
     // the state changed outside the tracked kernels
     c->plk_valid = false;
-    c->nfw_tracked = false;
+    atm_untrack(c);
     return rc;
 }
 

@@ -338,6 +338,14 @@ struct atm_ctx {
     // is finite; otherwise the check scans the four fields.
     int *nfw = nullptr;
     bool nfw_tracked = false;
+    // bumped whenever nfw_tracked is cleared (atm_untrack), so that a
+    // deferred check started before knows tracking was lost since
+    unsigned long long untrack_epoch = 0;
+    // the deferred check (atm_check_stability_begin / _end)
+    int *chk_host = nullptr;  // pinned: the word the check copied
+    hipEvent_t chk_ev = nullptr;
+    bool chk_pending = false, chk_scan = false;
+    unsigned long long chk_epoch = 0;
     RemapWork *remap = nullptr;
     RadWork *rad = nullptr;
     PhysWork *phys = nullptr;
@@ -410,6 +418,12 @@ static inline dim3 grid3(int nI, int nJ, int nK, dim3 b) {
 int atm_launch_halo(atm_ctx *c, double *a, int nk);
 #define ATM_HALO_MAXF 6
 // periodic halo fill of n <= ATM_HALO_MAXF fields (nk[i] levels each) in one launch
+// the state changed outside the kernels that report into nfw: the next
+// stability check scans the fields
+inline void atm_untrack(atm_ctx *c) {
+    c->nfw_tracked = false;
+    c->untrack_epoch++;
+}
 int atm_launch_halo_multi(atm_ctx *c, double *const *a, const int *nk, int n);
 // fills the poisoned classes (ATM_POISON_*) of a field buffer of n doubles
 // (whole planes, or fewer than one plane: then only its tail)

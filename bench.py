@@ -129,27 +129,54 @@ def ensemble_timing(args):
     seeded state (k^-3 temperature noise of 0.5 K), mixed precision, dx and
     dt of the main workload, remap every 5 steps and radiation every 10.
     Whole steps of all members (dynamics, remap, radiation, the stability
-    check of every member), timed after 5 warm-up steps."""
+    check of every member after every step), timed after 10 warm-up steps:
+    first with each check waited for before the next step is issued (the
+    streams drain every step), then with deferred checks (the value: each
+    member's check is queued behind its step and read after the next step is
+    issued), then with each member's 10-step period replayed from a HIP graph
+    (radiation eager at the period start, one deferred check per period)."""
     from atmos.ensemble import Ensemble
     ens = Ensemble(ENSEMBLE_SPECS, args.dx, args.dt, remap_every=5, radiation_every=10, precision="mixed",
-                   nstreams=args.ensemble_streams, autotune=args.autotune == "on")
-    for _ in range(5):
+                   nstreams=args.ensemble_streams, autotune=args.autotune == "on", checks="sync")
+    n_sync = max(10, args.ensemble_steps // 20 * 10)
+    n_def = max(10, args.ensemble_steps // 10 * 10)
+    for _ in range(10):
         if not ens.step():
             raise RuntimeError("ensemble member unstable in warm-up")
     ens.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.ensemble_steps):
-        if not ens.step():
-            raise RuntimeError("ensemble member unstable")
-    ens.sync()
-    sec = time.perf_counter() - t0
     cells = ens.cells
+
+    def timed(run, n):
+        t0 = time.perf_counter()
+        for _ in range(n):
+            if not run():
+                raise RuntimeError("ensemble member unstable")
+        if not ens.finish():
+            raise RuntimeError("ensemble member unstable")
+        ens.sync()
+        return time.perf_counter() - t0
+
+    sec_sync = timed(ens.step, n_sync)
+    ens.checks = "deferred"
+    sec = timed(ens.step, n_def)
+    ens.use_graphs(10)
+    if not ens.period() or not ens.finish():  # records each member's graph, runs the first period
+        raise RuntimeError("ensemble member unstable")
+    ens.sync()
+    n_per = max(1, n_def // 10)
+    sec_graph = timed(ens.period, n_per)
     out = {"members": " + ".join(f"{c} x {nx}x{ny}x{L}" for c, nx, ny, L in ENSEMBLE_SPECS), "cells": cells,
-           "precision": "mixed", "steps": args.ensemble_steps, "remap_every": 5, "radiation_every": 10,
-           "value": cells * args.ensemble_steps / sec, "unit": "cell-steps/s",
-           "ms_per_step": sec / args.ensemble_steps * 1e3,
-           "launch": f"each member's own launches, members dealt to {args.ensemble_streams} HIP streams",
-           "whole_step_roofline": whole_step_roofline(cells * args.ensemble_steps / sec),
+           "precision": "mixed", "steps": n_def, "remap_every": 5, "radiation_every": 10,
+           "value": cells * n_def / sec, "unit": "cell-steps/s",
+           "ms_per_step": sec / n_def * 1e3,
+           "launch": f"each member's own launches, members dealt to {args.ensemble_streams} HIP streams; "
+                     "each member's stability check deferred by one step (no drain between steps)",
+           "whole_step_roofline": whole_step_roofline(cells * n_def / sec),
+           "sync_checks": {"steps": n_sync, "value": cells * n_sync / sec_sync,
+                           "ms_per_step": sec_sync / n_sync * 1e3},
+           "graph": {"period_steps": 10, "periods": n_per, "value": cells * 10 * n_per / sec_graph,
+                     "ms_per_step": sec_graph / (10 * n_per) * 1e3,
+                     "checks": "one deferred check per member per period"},
            "corner_levels_per_workgroup": {f"{nx}x{ny}x{L}": sorted({mb.model.tuning for mb in ens.members
                                                                      if mb.cells == nx * ny * L})
                                            for _, nx, ny, L in ENSEMBLE_SPECS}}

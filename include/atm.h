@@ -176,6 +176,18 @@ int atm_halo_exchange(atm_ctx *ctx, int field);
    are finite over the interior.  Blocks. */
 int atm_check_stability(atm_ctx *ctx, int *stable);
 
+/* The same check, deferred (no counterpart in the reference; for drivers
+   of many contexts that keep the GPU fed): _begin queues the check behind
+   the calls issued so far and returns at once; _end waits for that check
+   only, not for calls issued after _begin, and sets *stable for the state
+   at _begin.  One deferred check per context at a time; atm_check_stability
+   and atm_graph_begin are refused while one is pending.  Where the fields
+   are tracked (every write since the last scan came from a kernel that
+   reports non-finite values), a reported non-finite write reads as
+   unstable without atm_check_stability's confirming scan. */
+int atm_check_stability_begin(atm_ctx *ctx);
+int atm_check_stability_end(atm_ctx *ctx, int *stable);
+
 /* Execution path of the dynamics (no counterpart in the reference; both
    give identical results).  ATM_PATH_FUSED (the default, any nx and ny):
    LDS-tiled kernels, four per step; tiles of 32 x 8 and 32 x 16 cells, the

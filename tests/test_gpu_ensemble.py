@@ -78,3 +78,87 @@ def test_ensemble_at_the_workload_sizes(gpu_available):
             assert np.array_equal(ens.members[i].model.get(name), m.get(name)), (i, name)
         print(f"  member {i} {nx}x{ny}x{nlay} equals its solo run", flush=True)
     ens.close()
+
+
+@pytest.mark.parametrize("mode", ["deferred", "graph"])
+def test_deferred_checks_and_graph_periods_match_sync_steps(gpu_available, mode):
+    """Deferred stability checks (read one step late, no drain) and 10-step
+    periods replayed from each member's HIP graph give every member the
+    same bits as steps with a waited check each (radiation every 10 steps,
+    remap every 5; one member's grid is overhung by the tiles)."""
+    from atmos.ensemble import Ensemble
+    specs = [(1, 64, 32, 12), (2, 40, 24, 10), (1, 32, 16, 6)]
+    dx, dt, seed = 100e3, 60.0, 91
+    print(f"ensemble {mode} against sync checks {specs} x20", flush=True)
+    kw = dict(remap_every=5, radiation_every=10, precision="mixed", nstreams=2, seed=seed, t_noise=0.5)
+    ref = Ensemble(specs, dx, dt, checks="sync", **kw)
+    for _ in range(20):
+        assert ref.step()
+    ens = Ensemble(specs, dx, dt, checks="deferred", graph_period=10 if mode == "graph" else 0, **kw)
+    if mode == "graph":
+        for _ in range(2):
+            assert ens.period()
+    else:
+        for _ in range(20):
+            assert ens.step()
+    assert ens.finish()
+    ref.sync()
+    ens.sync()
+    for i, (a, b) in enumerate(zip(ref.members, ens.members)):
+        assert a.time == b.time
+        for name in ("dp", "pt", "ud", "vd", "net_flux"):
+            assert np.array_equal(a.model.get(name), b.model.get(name)), (i, name)
+    ref.close()
+    ens.close()
+
+
+def test_deferred_check_reports_what_the_waited_check_reports(gpu_available):
+    """The deferred check finds a NaN put into dp (a scan) and a NaN that a
+    step carries into the state from net_flux (the D-step kernels' tracking
+    word); it is refused when none is queued, when one is already queued,
+    and the waited check and a graph recording are refused while one is."""
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import AtmError, Model
+    nx, ny, nlay = 40, 24, 8
+    print("deferred check detection and refusals", flush=True)
+    ic = make_initial_state(nx, ny, nlay, 100e3)
+    cfg = Config(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12, dt_remap=300.0,
+                 radiation_on=False, physics_on=False)
+    m = Model(nx, ny, nlay, 100e3)
+    st = driver.init_model(m, ic, cfg)
+    with pytest.raises(AtmError):
+        m.check_stability_end()
+    t = 0.0
+    for _ in range(2):  # the first deferred check scans, the second reads the tracking word
+        t += driver.step(m, cfg, st, t)
+        m.check_stability_begin()
+        with pytest.raises(AtmError):
+            m.check_stability_begin()
+        with pytest.raises(AtmError):
+            m.check_stability()
+        with pytest.raises(AtmError):
+            m.graph_begin()
+        assert m.check_stability_end()
+    # a NaN through net_flux reaches pt in the next step's D kernels (tracked)
+    a = m.get("net_flux")
+    a[3, 10, 20] = np.nan
+    m.put("net_flux", a)
+    t += driver.step(m, cfg, st, t)
+    m.check_stability_begin()
+    assert not m.check_stability_end()
+    # a NaN put into dp is found by the scan the put makes necessary
+    m2 = Model(nx, ny, nlay, 100e3)
+    st2 = driver.init_model(m2, ic, cfg)
+    driver.step(m2, cfg, st2, 0.0)
+    m2.check_stability_begin()
+    assert m2.check_stability_end()
+    dp = m2.get_full("dp")
+    dp[2, 12, 15] = np.nan
+    m2.put_full("dp", dp)
+    m2.check_stability_begin()
+    assert not m2.check_stability_end()
+    assert not m2.check_stability()
+    m.close()
+    m2.close()
