@@ -4,24 +4,22 @@
 //   forward   one thread per point of an operation's box, the body in double;
 //   tangent   the same body on Jac<1> (value and tangent read, both written);
 //   adjoint   the same body on Jac<NR>: each output's adjoint times each
-//             partial, summed over the body's outputs, is added to the
-//             adjoint of the value read with one global FP64 atomic add per
-//             read (neighbouring points read the same values); outputs whose
-//             adjoint is zero add nothing, so an adjoint confined to a small
-//             region costs little of the grid.
+//             partial, summed over the body's outputs, is what the point
+//             owes each value it read; a workgroup sums these in LDS planes
+//             and adds each touched value to global memory once per plane
+//             with an FP64 atomic (neighbouring workgroups touch the same
+//             values); outputs whose adjoint is zero add nothing, so an
+//             adjoint confined to a small region costs little of the grid.
 // Column scans run one thread per column, the periodic halo fills one
 // thread per point (the adjoint folds each halo point's adjoint onto its
 // interior source with an atomic add).
+#include <algorithm>
 #include <cstring>
 
 #include "atm_common.hpp"
 #include "lin_step.hpp"
 
 namespace {
-
-struct DevAdd {
-    __device__ static void add(double *p, double v) { unsafeAtomicAdd(p, v); }
-};
 
 #define LIN_POINT                                                     \
     const int I = r.I0 + blockIdx.x * blockDim.x + threadIdx.x;       \
@@ -43,14 +41,117 @@ __global__ void __launch_bounds__(256) k_lin_tan(LinArgs x, Rng r, int k0) {
     F::run(a, x, I, J, K);
 }
 
-template <class F>
-__global__ void __launch_bounds__(256) k_lin_adj(LinArgs x, Rng r, int k0) {
-    LIN_POINT
-    AccAdj<F::NR, DevAdd> a(x);
-    F::run(a, x, I, J, K);
-    a.flush();
-}
 #undef LIN_POINT
+
+// ---------------------------------------------------------------------------
+// Adjoint bodies accumulated in LDS.  A workgroup takes ADJ_X x ADJ_Y points
+// of one level.  Each point evaluates its body on Jac<NR>; what it owes a
+// value it read goes into an LDS plane kept for that operand and level
+// offset, which covers the workgroup's points plus ADJ_H on every side.
+// Neighbouring points read the same values (a stencil reads one array at up
+// to 16 offsets), so their contributions sum in LDS, and every value that
+// received any gets one global atomic add per plane, instead of one per
+// read per point.  A read beyond the planes' reach, or of an operand past
+// the first ADJ_PMAX planes, adds to global memory directly.
+//
+// Every read of a body names a compile-time slot, operand and offset from
+// the point (lin_ops.hpp), so each thread derives the same plane table from
+// its own slots.  Threads past the box run the body on the box's last point
+// (valid memory) with their writes and additions switched off, so they keep
+// the table and every barrier.
+// ---------------------------------------------------------------------------
+constexpr int ADJ_X = 32, ADJ_Y = 8, ADJ_H = 3, ADJ_PMAX = 6;
+constexpr int ADJ_W = ADJ_X + 2 * ADJ_H, ADJ_PL = ADJ_W * (ADJ_Y + 2 * ADJ_H);
+
+template <int NR>
+struct AccAdjTile {
+    using S = Jac<NR>;
+    const LinArgs &x;
+    const int I, J, K;
+    const bool active;
+    int sarr[NR], sdi[NR], sdj[NR], sdk[NR];
+    double acc[NR];
+    __device__ AccAdjTile(const LinArgs &a, int I_, int J_, int K_, bool act)
+        : x(a), I(I_), J(J_), K(K_), active(act) {
+        for (int s = 0; s < NR; s++) acc[s] = 0.;
+    }
+    template <int s>
+    __device__ S rd(int arr, int Ir, int Jr, int Kr) {
+        static_assert(s < NR, "read slot");
+        sarr[s] = arr;
+        sdi[s] = Ir - I;
+        sdj[s] = Jr - J;
+        sdk[s] = Kr - K;
+        return S::seed(x.iv[arr][x.g.ix(Ir, Jr, Kr)], s);
+    }
+    __device__ void wr(int o, int Iw, int Jw, int Kw, const S &v) {
+        if (!active) return;
+        const size_t i = x.g.ix(Iw, Jw, Kw);
+        const double ab = x.ot[o][i];
+        if (ab == 0.) return;
+        x.ot[o][i] = 0.;
+#pragma unroll
+        for (int s = 0; s < NR; s++) acc[s] += ab * v.d[s];
+    }
+};
+
+template <class F>
+__global__ void __launch_bounds__(ADJ_X * ADJ_Y) k_lin_adj(LinArgs x, Rng r, int k0) {
+    constexpr int NR = F::NR, NT = ADJ_X * ADJ_Y;
+    __shared__ double pl[ADJ_PMAX * ADJ_PL];
+    const int tid = threadIdx.y * ADJ_X + threadIdx.x;
+    const int Ib = r.I0 + blockIdx.x * ADJ_X, Jb = r.J0 + blockIdx.y * ADJ_Y, K = k0 + blockIdx.z;
+    const int Iq = Ib + threadIdx.x, Jq = Jb + threadIdx.y;
+    const bool active = Iq <= r.I1 && Jq <= r.J1;
+    AccAdjTile<NR> a(x, min(Iq, r.I1), min(Jq, r.J1), K, active);
+    for (int c = tid; c < ADJ_PMAX * ADJ_PL; c += NT) pl[c] = 0.;
+    F::run(a, x, a.I, a.J, K);
+    // plane of each slot: one per (operand, level offset) in slot order
+    // (register arrays indexed only by unrolled loop counters)
+    int np = 0, parr[ADJ_PMAX], pdk[ADJ_PMAX], sp[NR];
+#pragma unroll
+    for (int q = 0; q < ADJ_PMAX; q++) parr[q] = pdk[q] = 0;
+#pragma unroll
+    for (int s = 0; s < NR; s++) {
+        int p = -1;
+#pragma unroll
+        for (int q = 0; q < ADJ_PMAX; q++)
+            if (p < 0 && q < np && parr[q] == a.sarr[s] && pdk[q] == a.sdk[s]) p = q;
+        if (p < 0 && np < ADJ_PMAX && x.it[a.sarr[s]]) {
+#pragma unroll
+            for (int q = 0; q < ADJ_PMAX; q++)
+                if (q == np) {
+                    parr[q] = a.sarr[s];
+                    pdk[q] = a.sdk[s];
+                }
+            p = np++;
+        }
+        sp[s] = p;
+    }
+    __syncthreads();  // planes cleared
+#pragma unroll
+    for (int s = 0; s < NR; s++) {
+        const double v = a.acc[s];
+        double *t = x.it[a.sarr[s]];
+        if (!t || v == 0.) continue;
+        const int px = (int)threadIdx.x + ADJ_H + a.sdi[s], py = (int)threadIdx.y + ADJ_H + a.sdj[s];
+        if (sp[s] >= 0 && px >= 0 && px < ADJ_W && py >= 0 && py < ADJ_Y + 2 * ADJ_H)
+            atomicAdd(&pl[sp[s] * ADJ_PL + py * ADJ_W + px], v);
+        else
+            unsafeAtomicAdd(t + x.g.ix(a.I + a.sdi[s], a.J + a.sdj[s], K + a.sdk[s]), v);
+    }
+    __syncthreads();  // contributions summed
+#pragma unroll
+    for (int q = 0; q < ADJ_PMAX; q++) {
+        if (q >= np) break;
+        double *t = x.it[parr[q]];
+        for (int c = tid; c < ADJ_PL; c += NT) {
+            const double v = pl[q * ADJ_PL + c];
+            if (v != 0.)
+                unsafeAtomicAdd(t + x.g.ix(Ib - ADJ_H + c % ADJ_W, Jb - ADJ_H + c / ADJ_W, K + pdk[q]), v);
+        }
+    }
+}
 
 template <bool GZ>
 __global__ void k_lin_scan(int mode, Geo g, Rng r, const double *sv, double *sd, const double *tv, double *td,
@@ -67,14 +168,22 @@ __global__ void k_lin_scan(int mode, Geo g, Rng r, const double *sv, double *sd,
         scan_column_adj<GZ>(g, c0, sd, tv, td, pv, pd, kv, kd, gd);
 }
 
-// a column operation of the remap, one thread per column of the box
+// column operations of the remap, one thread per column of each box;
+// blockIdx.y picks the operation (independent operations run together)
+struct LinColOps {
+    LinColOp op[LIN_MAXCOL];
+};
+
 template <int KIND>
-__global__ void __launch_bounds__(256) k_lin_column(int mode, LinArgs x, Rng r, double *ws) {
+__global__ void __launch_bounds__(256) k_lin_column(int mode, LinColOps ops) {
+    const LinColOp &o = ops.op[blockIdx.y];
+    const Rng r = o.r;
+    const LinArgs &x = o.x;
     const size_t W = (size_t)(r.I1 - r.I0 + 1), ncol = W * (size_t)(r.J1 - r.J0 + 1);
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ncol) return;
     const int I = r.I0 + (int)(t % W), J = r.J0 + (int)(t / W);
-    const RmWS w{ws, ncol, t, x.g.L};
+    const RmWS w{o.w, ncol, t, x.g.L};
     const size_t c0 = x.g.ix2(I, J);
     if (KIND == RM_COL_REMAP) {
         if (mode == LIN_FWD) rm_column<double>(x, c0, w);
@@ -164,7 +273,9 @@ struct GpuBackend {
     }
     template <class F>
     void adj(const LinArgs &x, Rng r, int k0, int nk) {
-        hipLaunchKernelGGL(k_lin_adj<F>, lin_grid(r, nk), LBLK, 0, s, x, r, k0);
+        const dim3 grid((unsigned)((r.I1 - r.I0 + ADJ_X) / ADJ_X), (unsigned)((r.J1 - r.J0 + ADJ_Y) / ADJ_Y),
+                        (unsigned)nk);
+        hipLaunchKernelGGL(k_lin_adj<F>, grid, dim3(ADJ_X, ADJ_Y), 0, s, x, r, k0);
     }
     void scan(int mode, bool gz, const Geo &g, Rng r, const double *sv, double *sd, const double *tv, double *td,
               double *pv, double *pd, double *kv, double *kd, double *gv, double *gd) {
@@ -175,13 +286,19 @@ struct GpuBackend {
             hipLaunchKernelGGL(k_lin_scan<false>, grid, LBLK, 0, s, mode, g, r, sv, sd, tv, td, pv, pd, kv, kd, gv,
                                gd);
     }
-    void column(int mode, int kind, const LinArgs &x, Rng r, double *ws) {
-        const size_t ncol = (size_t)(r.I1 - r.I0 + 1) * (size_t)(r.J1 - r.J0 + 1);
-        const dim3 grid((unsigned)((ncol + 255) / 256));
+    void columns(int mode, int kind, const LinColOp *ops, int n) {
+        LinColOps a{};
+        size_t ncol = 0;
+        for (int i = 0; i < n; i++) {
+            a.op[i] = ops[i];
+            const Rng &r = ops[i].r;
+            ncol = std::max(ncol, (size_t)(r.I1 - r.I0 + 1) * (size_t)(r.J1 - r.J0 + 1));
+        }
+        const dim3 grid((unsigned)((ncol + 255) / 256), (unsigned)n);
         if (kind == RM_COL_REMAP)
-            hipLaunchKernelGGL(k_lin_column<RM_COL_REMAP>, grid, dim3(256), 0, s, mode, x, r, ws);
+            hipLaunchKernelGGL(k_lin_column<RM_COL_REMAP>, grid, dim3(256), 0, s, mode, a);
         else
-            hipLaunchKernelGGL(k_lin_column<RM_COL_PT>, grid, dim3(256), 0, s, mode, x, r, ws);
+            hipLaunchKernelGGL(k_lin_column<RM_COL_PT>, grid, dim3(256), 0, s, mode, a);
     }
     void halo_copy(int mode, const Geo &g, int nk, const double *sv, double *sd, double *dv, double *dd) {
         const dim3 grid((unsigned)((g.NI + LBLK.x - 1) / LBLK.x), (unsigned)((g.NJ + LBLK.y - 1) / LBLK.y),

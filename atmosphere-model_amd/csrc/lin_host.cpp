@@ -110,6 +110,9 @@ struct HostBackend {
                 }
             }
     }
+    void columns(int mode, int kind, const LinColOp *ops, int n) {
+        for (int i = 0; i < n; i++) column(mode, kind, ops[i].x, ops[i].r, ops[i].w);
+    }
     void column(int mode, int kind, const LinArgs &x, Rng r, double *ws) {
         const Geo &g = x.g;
         const size_t W = (size_t)(r.I1 - r.I0 + 1), ncol = W * (size_t)(r.J1 - r.J0 + 1);

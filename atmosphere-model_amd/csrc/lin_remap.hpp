@@ -4,11 +4,11 @@
 // remap_column and the potential-temperature reconstruction.
 //
 // A column operation runs in one thread per column.  Forward and tangent
-// evaluate the column on their scalar type; the adjoint first evaluates it
-// in double, keeping every intermediate the reverse pass needs (the
+// evaluate the column on their scalar type, keeping every intermediate (the
 // eliminated edge values, gam, the back-substituted and limited level
-// values, the edges and curvature) in a workspace laid out [slot][level]
-// [column], then walks the column backwards.  Each backward step applies
+// values, the edges and curvature) in the operation's own workspace, laid
+// out [slot][level][column]; the adjoint sweep, which runs after the
+// forward sweep of the same step, walks the column backwards over them.  Each backward step applies
 // the partials of one small local function (an elimination step, a clamp,
 // a Huynh limit, one source/destination overlap), obtained by evaluating
 // the same template the forward pass used on Jac<K>, so the reverse pass
@@ -117,6 +117,20 @@ __host__ __device__ inline S rm_overlap(const S &b, const S &t, const S &cv, con
     return acc;
 }
 
+// The source layers that overlap destination layer m (193-281 let a pair
+// through when X(k+1) < D(m) and X(k) > D(m+1)).  Faces decrease upward in
+// both grids, so for each m these are a contiguous run of k, and the run's
+// start only moves up as m grows: the first layer of the run is the first k
+// from the previous run's start with X(k+1) < D(m), and the run ends before
+// the first k with X(k) <= D(m+1).  Walking the runs visits exactly the
+// pairs the reference's loop over all (k, m) accumulates, in the same
+// ascending order, with O(L) face reads per column instead of O(L^2).
+template <class XF>
+__host__ __device__ inline int rm_first_overlap(const XF &X, int k, int L, double dm0) {
+    while (k <= L && LIN_DECIDE(val(X(k + 1)) >= dm0)) k++;
+    return k;
+}
+
 // Column workspace: slot s, 1-based level k (0..L+1), column t.
 struct RmWS {
     double *w;
@@ -128,6 +142,11 @@ struct RmWS {
 // the adjoint keeps its accumulators at slot + RM_NV
 enum { RM_LV = 0, RM_GM, RM_BS, RM_LF, RM_T, RM_B, RM_CV, RM_NV };
 constexpr int RM_SLOTS = 2 * RM_NV;
+// Workspaces of the step's column operations: the u, v and energy columns
+// of remap_column and the pt reconstruction each keep their own, so the
+// forward sweep's intermediates are still there when the adjoint sweep
+// reaches the operation, and the three remap columns can run together.
+enum { RM_WS_U = 0, RM_WS_V, RM_WS_E, RM_WS_PT, RM_NWS };
 
 template <class S>
 __host__ __device__ inline S rm_get(const RmWS &w, int s, int k);
@@ -217,12 +236,14 @@ __host__ __device__ inline void rm_column(const LinArgs &x, size_t c0, const RmW
         rm_put(w, RM_CV, k, 6. * v - 3. * (t + b));
     }
     // overlaps, for each destination layer in ascending source layer
+    int k0 = 1;
     for (int m = 1; m <= L; m++) {
         const S dm0 = D(m), dm1 = D(m + 1);
         S acc(0.);
-        for (int k = 1; k <= L; k++) {
+        k0 = rm_first_overlap(X, k0, L, val(dm0));
+        for (int k = k0; k <= L; k++) {
             const S xk = X(k), xk1 = X(k + 1);
-            if (LIN_DECIDE(val(xk1) >= val(dm0)) || LIN_DECIDE(val(xk) <= val(dm1))) continue;
+            if (LIN_DECIDE(val(xk) <= val(dm1))) break;
             acc = acc + rm_overlap<S>(rm_get<S>(w, RM_B, k), rm_get<S>(w, RM_T, k), rm_get<S>(w, RM_CV, k), V(k), xk,
                                       xk1, dm0, dm1);
         }
@@ -230,12 +251,12 @@ __host__ __device__ inline void rm_column(const LinArgs &x, size_t c0, const RmW
     }
 }
 
-// Adjoint of rm_column: the forward pass in double (its intermediates in
-// the workspace), then the reverse pass.  The output's adjoint (consumed)
-// is added into the adjoints of V, X and D; each column is one thread's.
+// Adjoint of rm_column: the reverse pass over the intermediates that the
+// forward sweep of the same step left in this operation's workspace.  The
+// output's adjoint (consumed) is added into the adjoints of V, X and D;
+// each column is one thread's.
 __host__ __device__ inline void rm_column_adj(const LinArgs &x, size_t c0, const RmWS &w) {
     const int L = x.g.L;
-    rm_column<double>(x, c0, w);
     auto V = [&](int k) { return col_rd<double>(x, 0, c0, k); };
     auto X = [&](int k) { return col_rd<double>(x, 1, c0, k); };
     auto D = [&](int k) { return col_rd<double>(x, 2, c0, k); };
@@ -247,16 +268,18 @@ __host__ __device__ inline void rm_column_adj(const LinArgs &x, size_t c0, const
     auto A = [&](int s, int k) -> double & { return w.at(s + RM_NV, k); };
     for (int s = 0; s < RM_NV; s++)
         for (int k = 0; k <= L + 1; k++) A(s, k) = 0.;
-    // overlaps
+    // overlaps (the pairs the forward pass visited)
+    int k0 = 1;
     for (int m = 1; m <= L; m++) {
         double &ao = x.ot[0][c0 + (size_t)(m - 1) * pl];
         const double am = ao;
         ao = 0.;
         if (am == 0.) continue;
         const double dm0 = D(m), dm1 = D(m + 1);
-        for (int k = 1; k <= L; k++) {
+        k0 = rm_first_overlap(X, k0, L, dm0);
+        for (int k = k0; k <= L; k++) {
             const double xk = X(k), xk1 = X(k + 1);
-            if (xk1 >= dm0 || xk <= dm1) continue;
+            if (LIN_DECIDE(xk <= dm1)) break;
             using J8 = Jac<8>;
             const J8 r = rm_overlap<J8>(J8::seed(W(RM_B, k), 0), J8::seed(W(RM_T, k), 1), J8::seed(W(RM_CV, k), 2),
                                         J8::seed(V(k), 3), J8::seed(xk, 4), J8::seed(xk1, 5), J8::seed(dm0, 6),
@@ -383,7 +406,7 @@ __host__ __device__ inline void rm_pt_column(const LinArgs &x, size_t c0, const 
 
 __host__ __device__ inline void rm_pt_column_adj(const LinArgs &x, size_t c0, const RmWS &w) {
     const int L = x.g.L;
-    rm_pt_column<double>(x, c0, w);  // the geopotential chain into the workspace
+    // the geopotential chain: in the workspace from the forward sweep
     const size_t pl = x.g.plane();
     double agz = 0.;  // adjoint of gz_remap(m + 1)
     for (int m = L; m >= 1; m--) {
@@ -542,13 +565,16 @@ void lin_remap_program(E &e) {
     e.template op<OpFaceAvg<0>>(R(6, 6), 0, L + 1, {A_TPLEV}, {A_VFD});
     e.template op<OpWindSource>(R(6, 6), 0, L, {A_UD1, A_UD0}, {A_UDM});
     e.template op<OpWindSource>(R(6, 6), 0, L, {A_VD1, A_VD0}, {A_VDM});
-    e.column(RM_COL_REMAP, R(6, 6), {A_UDM, A_UFX, A_UFD}, {A_UR});
-    e.column(RM_COL_REMAP, R(6, 6), {A_VDM, A_VFX, A_VFD}, {A_VR});
-    // total energy (107-155)
+    // total energy of each layer (111-128): from the winds before the remap
     e.template op<OpEnergyIn>(R(7, 7), 0, L, {A_PT1, A_RPLK, A_RGZ, A_UDM, A_VDM}, {A_ENER});
+    // remap_column of the u and v winds and of the energy: independent
+    // columns, one launch
+    e.columns(RM_COL_REMAP, {e.col(R(6, 6), {A_UDM, A_UFX, A_UFD}, {A_UR}, RM_WS_U),
+                             e.col(R(6, 6), {A_VDM, A_VFX, A_VFD}, {A_VR}, RM_WS_V),
+                             e.col(R(7, 7), {A_ENER, A_RPLEV, A_TPLEV}, {A_EREM}, RM_WS_E)});
+    // pt from the remapped energy and the remapped winds' kinetic energy (129-155)
     e.template op<OpKeRemap>(R(7, 7), 0, L, {A_UR, A_VR}, {A_KER});
-    e.column(RM_COL_REMAP, R(7, 7), {A_ENER, A_RPLEV, A_TPLEV}, {A_EREM});
-    e.column(RM_COL_PT, R(7, 7), {A_EREM, A_KER, A_TPLK, A_TPKAP, A_RGZ}, {A_PT2});
+    e.columns(RM_COL_PT, {e.col(R(7, 7), {A_EREM, A_KER, A_TPLK, A_TPKAP, A_RGZ}, {A_PT2}, RM_WS_PT)});
     // write-back (157-161): dp from the target grid; the winds are A_UR, A_VR
     e.template op<OpDpOut>(R(7, 7), 0, L, {A_TPLEV}, {A_DP2});
 }

@@ -13,9 +13,11 @@
 //   template <class F> fwd / tan / adj (const LinArgs &, Rng, k0, nk)
 //   scan(mode, gz, Rng, ...)            the column scans (lin_ops.hpp)
 //   halo_copy(mode, nk, sv, st, dv, dt) dst = periodic fill of src's interior
+//   columns(mode, kind, ops, n)         column operations of the remap
 //   copy, fill, alloc, release          plain memory operations
 #pragma once
 
+#include <array>
 #include <functional>
 #include <initializer_list>
 #include <type_traits>
@@ -26,6 +28,16 @@
 struct Rng {
     int I0, I1, J0, J1;  // inclusive
 };
+
+// One column operation of the remap: its box of columns, its operands and
+// its workspace.  A backend's columns(mode, kind, ops, n) runs n of them,
+// independent of each other, together.
+struct LinColOp {
+    Rng r;
+    LinArgs x;
+    double *w;
+};
+constexpr int LIN_MAXCOL = 3;
 
 enum LinMode { LIN_FWD = 0, LIN_TAN = 1, LIN_ADJ = 2 };
 
@@ -178,7 +190,7 @@ struct LinCore {
     double step_dt = 0.;  // the model step; the C-grid half step uses step_dt / 2
     std::vector<std::function<void()>> rev;
     const double *ak = nullptr, *bk = nullptr;  // coordinates (device or host, as the backend)
-    double *ws = nullptr;                       // column workspace of the remap
+    double *ws[RM_NWS] = {};                    // column workspaces of the remap
 
     LinCore(B &be, const Geo &geo) : b(be), g(geo) {}
 
@@ -194,12 +206,17 @@ struct LinCore {
             drv[i] = b.alloc(count(i));
             if (!drv[i]) return false;
         }
-        ws = b.alloc(ws_count());
-        return ws != nullptr;
+        for (int i = 0; i < RM_NWS; i++) {
+            ws[i] = b.alloc(ws_count());
+            if (!ws[i]) return false;
+        }
+        return true;
     }
     void release() {
-        b.release(ws);
-        ws = nullptr;
+        for (int i = 0; i < RM_NWS; i++) {
+            b.release(ws[i]);
+            ws[i] = nullptr;
+        }
         for (int i = 0; i < A_COUNT; i++) {
             b.release(val[i]);
             b.release(drv[i]);
@@ -278,14 +295,19 @@ struct LinCore {
             b.halo_copy(mode, g, nk, sv, sd, dv, dd);
     }
 
-    // a column operation of the remap (lin_remap.hpp) over a box of columns
-    void column(int kind, Rng r, std::initializer_list<int> in, std::initializer_list<int> out) {
-        const LinArgs x = args(in, out, 0, 0);
-        double *w = ws;
+    // column operations of the remap (lin_remap.hpp), each over a box of
+    // columns with its own workspace, independent of each other
+    LinColOp col(Rng r, std::initializer_list<int> in, std::initializer_list<int> out, int wsi) const {
+        return LinColOp{r, args(in, out, 0, 0), ws[wsi]};
+    }
+    void columns(int kind, std::initializer_list<LinColOp> ops) {
+        std::array<LinColOp, LIN_MAXCOL> a{};
+        int n = 0;
+        for (const LinColOp &o : ops) a[n++] = o;
         if (mode == LIN_ADJ)
-            rev.push_back([=]() { b.column(LIN_ADJ, kind, x, r, w); });
+            rev.push_back([=]() { b.columns(LIN_ADJ, kind, a.data(), n); });
         else
-            b.column(mode, kind, x, r, w);
+            b.columns(mode, kind, a.data(), n);
     }
 
     void set_dt(double dt) {
