@@ -191,12 +191,12 @@ __host__ __device__ __forceinline__ T dq_mono(T qm, T q0, T qp) {
     return fsign(dmin3(T(fabs(d)), dmin, dmax), d);
 }
 
-// Edge anomalies of the middle cell from q(m-2..m+2): ppm 877-911.
+// Edge anomalies of the middle cell from q(m-1..m+1) and the dqi_mono of
+// cells m-1, m, m+1 (ppm 877-911).  Each cell's dqi_mono depends only on
+// its own three values, so a tile can form it once per cell and share it
+// between the faces that need it; ppm_edges below forms it in place.
 template <int V, class T>
-__host__ __device__ __forceinline__ void ppm_edges(T qm2, T qm1, T q0, T qp1, T qp2, T &eL, T &eR) {
-    T dmm = dq_mono(qm2, qm1, q0);
-    T dm0 = dq_mono(qm1, q0, qp1);
-    T dmp = dq_mono(q0, qp1, qp2);
+__host__ __device__ __forceinline__ void ppm_edges_dq(T qm1, T q0, T qp1, T dmm, T dm0, T dmp, T &eL, T &eR) {
     eL = (qm1 - q0) / T(2) + div3(dmm - dm0);
     eR = (qp1 - q0) / T(2) + div3(dm0 - dmp);
     if (V == V_PPM_C || V == V_PPM_NN) {
@@ -231,17 +231,25 @@ __host__ __device__ __forceinline__ void ppm_edges(T qm2, T qm1, T q0, T qp1, T 
     }
 }
 
-// Upwind PPM value at the face between cells i-1 and i, from
-// q(i-3..i+2) and the Courant number c(i): flux 802-829.
+// Edge anomalies of the middle cell from q(m-2..m+2): ppm 877-911.
 template <int V, class T>
-__host__ __device__ __forceinline__ T ppm_face(T qm3, T qm2, T qm1, T q0, T qp1, T qp2, T c) {
+__host__ __device__ __forceinline__ void ppm_edges(T qm2, T qm1, T q0, T qp1, T qp2, T &eL, T &eR) {
+    ppm_edges_dq<V>(qm1, q0, qp1, dq_mono(qm2, qm1, q0), dq_mono(qm1, q0, qp1), dq_mono(q0, qp1, qp2), eL, eR);
+}
+
+// Upwind PPM value at the face between cells i-1 and i (flux 802-829) from
+// q(i-2..i+1), the dqi_mono of cells i-2..i+1 (dm2, dm1, d0, dp1) and the
+// Courant number c(i).  q(i-3) and q(i+2) enter the reference's flux only
+// through dqi_mono(i-2) and dqi_mono(i+1).
+template <int V, class T>
+__host__ __device__ __forceinline__ T ppm_face_dq(T qm2, T qm1, T q0, T qp1, T dm2, T dm1, T d0, T dp1, T c) {
     T s = T(fabs(c));
     T k1 = (T(1) - T(2) * s) * (T(1) - s);
     T k2 = s * (s - T(1));
     if (V == V_PPM_C) {
         T eLm, eRm, eL0, eR0;
-        ppm_edges<V>(qm3, qm2, qm1, q0, qp1, eLm, eRm);
-        ppm_edges<V>(qm2, qm1, q0, qp1, qp2, eL0, eR0);
+        ppm_edges_dq<V>(qm2, qm1, q0, dm2, dm1, d0, eLm, eRm);
+        ppm_edges_dq<V>(qm1, q0, qp1, dm1, d0, dp1, eL0, eR0);
         if ((T(3) * T(fabs(eL0 + eR0)) > T(fabs(eL0 - eR0))) && (T(3) * T(fabs(eLm + eRm)) > T(fabs(eLm - eRm))))
             return c > T(0) ? qm1 : q0;
         if (c > T(0)) return qm1 + k1 * eRm + k2 * eLm;
@@ -249,13 +257,32 @@ __host__ __device__ __forceinline__ T ppm_face(T qm3, T qm2, T qm1, T q0, T qp1,
     } else {
         // only the upwind cell's edges are needed
         bool up = c > T(0);
-        T a2 = up ? qm3 : qm2, a1 = up ? qm2 : qm1, a0 = up ? qm1 : q0;
-        T b1 = up ? q0 : qp1, b2 = up ? qp1 : qp2;
+        T a1 = up ? qm2 : qm1, a0 = up ? qm1 : q0, b1 = up ? q0 : qp1;
+        T da = up ? dm2 : dm1, db = up ? dm1 : d0, dc = up ? d0 : dp1;
         T eL, eR;
-        ppm_edges<V>(a2, a1, a0, b1, b2, eL, eR);
+        ppm_edges_dq<V>(a1, a0, b1, da, db, dc, eL, eR);
         if (up) return a0 + k1 * eR + k2 * eL;
         return a0 + k1 * eL + k2 * eR;
     }
+}
+
+// The same from q(i-3..i+2), each dqi_mono formed here.
+template <int V, class T>
+__host__ __device__ __forceinline__ T ppm_face(T qm3, T qm2, T qm1, T q0, T qp1, T qp2, T c) {
+    if (V == V_PPM_C)
+        return ppm_face_dq<V>(qm2, qm1, q0, qp1, dq_mono(qm3, qm2, qm1), dq_mono(qm2, qm1, q0),
+                              dq_mono(qm1, q0, qp1), dq_mono(q0, qp1, qp2), c);
+    // only the upwind cell's edges are needed
+    T s = T(fabs(c));
+    T k1 = (T(1) - T(2) * s) * (T(1) - s);
+    T k2 = s * (s - T(1));
+    bool up = c > T(0);
+    T a2 = up ? qm3 : qm2, a1 = up ? qm2 : qm1, a0 = up ? qm1 : q0;
+    T b1 = up ? q0 : qp1, b2 = up ? qp1 : qp2;
+    T eL, eR;
+    ppm_edges<V>(a2, a1, a0, b1, b2, eL, eR);
+    if (up) return a0 + k1 * eR + k2 * eL;
+    return a0 + k1 * eL + k2 * eR;
 }
 
 // Piecewise-constant upwind value at the face between cells i-1 and i: flux 832-838.

@@ -308,7 +308,7 @@ __global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo 
 // one level (cgrid 294-414).
 // ---------------------------------------------------------------------------
 template <class R>
-__global__ __launch_bounds__(NT_WIND) void k_c2(Geo g, DynCoef c, const double *__restrict__ uc1,
+__global__ __launch_bounds__(NT_WIND, 8) void k_c2(Geo g, DynCoef c, const double *__restrict__ uc1,
                                            const double *__restrict__ vc1, const double *__restrict__ ud,
                                            const double *__restrict__ vd, const double *__restrict__ pkap,
                                            const double *__restrict__ gz, const double *__restrict__ ua,
@@ -336,6 +336,24 @@ __global__ __launch_bounds__(NT_WIND) void k_c2(Geo g, DynCoef c, const double *
     U(x, y) = G(uc1, x, y);
     V(x, y) = G(vc1, x, y);
     END_FOR
+    // pressure gradient force (325-339): global values only, formed before
+    // the LDS stages so its loads' latency overlaps them (a thread past the
+    // domain reads the halo or the tile slack and stores nothing)
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    double pgfx, pgfy;
+    {
+        const double *pk0 = pkap + kb0, *pk1 = pkap + TBASE(K + 1), *gz0 = gz + kb0, *gz1 = gz + TBASE(K + 1);
+        double a = pk0[TOFF(tx, ty)], d = pk1[TOFF(tx, ty)];
+        double b = pk1[TOFF(tx - 1, ty)], cc = pk0[TOFF(tx - 1, ty)];
+        pgfx = ((a - b) * (gz0[TOFF(tx - 1, ty)] - gz1[TOFF(tx, ty)]) +
+                (cc - d) * (gz1[TOFF(tx - 1, ty)] - gz0[TOFF(tx, ty)])) /
+               (a - b + cc - d);
+        b = pk1[TOFF(tx, ty - 1)];
+        cc = pk0[TOFF(tx, ty - 1)];
+        pgfy = ((a - b) * (gz0[TOFF(tx, ty - 1)] - gz1[TOFF(tx, ty)]) +
+                (cc - d) * (gz1[TOFF(tx, ty - 1)] - gz0[TOFF(tx, ty)])) /
+               (a - b + cc - d);
+    }
     __syncthreads();
     FOR_REG(RW)  // vorticity (319-323)
     W(x, y) = r.coriolis - (U(x, y) - U(x, y - 1)) * r.rdy + (V(x, y) - V(x - 1, y)) * r.rdx;
@@ -372,23 +390,7 @@ __global__ __launch_bounds__(NT_WIND) void k_c2(Geo g, DynCoef c, const double *
                         (W(x, y + 1) + W(x, y - 1)) * r.rdy2;
     END_FOR
     __syncthreads();
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     if (!(I0 + tx < ATM_HALO + g.nx && J0 + ty < ATM_HALO + g.ny)) return;  // no barrier follows
-    // pressure gradient force (325-339)
-    double pgfx, pgfy;
-    {
-        const double *pk0 = pkap + kb0, *pk1 = pkap + TBASE(K + 1), *gz0 = gz + kb0, *gz1 = gz + TBASE(K + 1);
-        double a = pk0[TOFF(tx, ty)], d = pk1[TOFF(tx, ty)];
-        double b = pk1[TOFF(tx - 1, ty)], cc = pk0[TOFF(tx - 1, ty)];
-        pgfx = ((a - b) * (gz0[TOFF(tx - 1, ty)] - gz1[TOFF(tx, ty)]) +
-                (cc - d) * (gz1[TOFF(tx - 1, ty)] - gz0[TOFF(tx, ty)])) /
-               (a - b + cc - d);
-        b = pk1[TOFF(tx, ty - 1)];
-        cc = pk0[TOFF(tx, ty - 1)];
-        pgfy = ((a - b) * (gz0[TOFF(tx, ty - 1)] - gz1[TOFF(tx, ty)]) +
-                (cc - d) * (gz1[TOFF(tx, ty - 1)] - gz0[TOFF(tx, ty)])) /
-               (a - b + cc - d);
-    }
     const double udv = G(ud, tx, ty), vdv = G(vd, tx, ty);
     double ucv, vcv;  // this cell's C winds in double
     if constexpr (FP64) {
@@ -423,6 +425,11 @@ __global__ __launch_bounds__(NT_WIND) void k_c2(Geo g, DynCoef c, const double *
 // ---------------------------------------------------------------------------
 // D2: kinetic energy, vorticity, divergence damping, B-grid pressure
 // gradient and the ud/vd update of one level (dgrid 606-771).
+//
+// Its PPM_CONSTRAINED faces (the kinetic energy's and the vorticity
+// fluxes') each need the dqi_mono of four cells; every cell's dqi_mono is
+// formed once per tile, into LDS, before the faces that read it (storage
+// of arrays that are dead or not yet written at that stage).
 // ---------------------------------------------------------------------------
 template <class R>
 __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *__restrict__ ud,
@@ -431,7 +438,6 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
                                            const double *__restrict__ gzb, double *__restrict__ udo,
                                            double *__restrict__ vdo, int *nfw) {
     constexpr int TY = TY_WIND, NT = NT_WIND;
-    constexpr bool FP64 = std::is_same<R, double>::value;
     const CoefR<R> r(c);
     using RU = Reg<-3, TX + 3, -3, TY + 4>;
     using RV = Reg<-3, TX + 4, -3, TY + 3>;
@@ -442,11 +448,22 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
     using RTU = Reg<0, TX, -3, TY + 3>;
     using RTW = Reg<-3, TX + 3, 0, TY>;
     using RLP = Reg<-1, TX, -1, TY>;
+    // dqi_mono of the kinetic energy's faces: U along x, V along y
+    using RQU = Reg<-2, TX + 2, 0, TY + 1>;
+    using RQV = Reg<0, TX + 1, -2, TY + 2>;
+    // ... and of the vorticity fluxes': TU along y, TW along x
+    using RQTU = Reg<0, TX, -2, TY + 1>;
+    using RQTW = Reg<-2, TX + 1, 0, TY>;
     __shared__ R sU[RU::n], sV[RV::n], sKE[RKE::n], sW[RW::n], sDV[RDV::n], sD2[RD2::n];
+    __shared__ R sT[RTU::n + RTW::n], sL[RLP::n];
+    R *const sTU = sT, *const sTW = sT + RTU::n;
     // del^4 divergence reuses the divergence's storage (dead once del^2 is formed)
     static_assert(RKE::n <= RDV::n, "sD4 fits in sDV");
     R *const sD4 = sDV;
-    __shared__ R sTU[RTU::n], sTW[RTW::n], sL[RLP::n];
+    // the kinetic energy's dqi_mono live where TU and TW are formed later;
+    // the vorticity fluxes' where U and V were (dead after the kinetic energy)
+    static_assert(RQU::n + RQV::n <= RTU::n + RTW::n && RQTU::n <= RU::n && RQTW::n <= RV::n, "dq storage");
+    R *const sQU = sT, *const sQV = sT + RQU::n, *const sQTU = sU, *const sQTW = sV;
     int bx, by, K;
     xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
     const int I0 = ATM_HALO + bx * TX, J0 = ATM_HALO + by * TY;
@@ -459,19 +476,33 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
 #define D2(x, y) sD2[RD2::at(x, y)]
     FOR_REG(RU) U(x, y) = G(ud, x, y); END_FOR
     FOR_REG(RV) V(x, y) = G(vd, x, y); END_FOR
-    __syncthreads();
-    FOR_REG(RKE)  // kinetic energy (639-657), PPM_CONSTRAINED, last-level B winds
+    // The B-grid pressure gradient (665-679) needs global values only: it is
+    // formed before the LDS stages, so its loads' latency overlaps them.  The
+    // corner values of pkap and gz at levels k and k+1 come from k_bcorners
+    // (pkapb(nlay+1) and gzb(1) from init_sw_dyn 991, 995).  A thread past
+    // the domain reads the halo or the tile slack and stores nothing.
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     const int Lm = g.L - 1;
-    const double *ucl = uc + TBASE(Lm), *vcl = vc + TBASE(Lm);
-    const R ubij = R(.5 * (ucl[TOFF(x, y)] + ucl[TOFF(x, y - 1)]));
-    const R vbij = R(.5 * (vcl[TOFF(x, y)] + vcl[TOFF(x - 1, y)]));
-    const R fy = ppm_face<V_PPM_C>(V(x, y - 3), V(x, y - 2), V(x, y - 1), V(x, y), V(x, y + 1), V(x, y + 2),
-                                   r.dtdy * vbij);
-    const R fx = ppm_face<V_PPM_C>(U(x - 3, y), U(x - 2, y), U(x - 1, y), U(x, y), U(x + 1, y), U(x + 2, y),
-                                   r.dtdx * ubij);
-    const R k = vbij * fy;
-    sKE[RKE::at(x, y)] = R(.5) * (k + ubij * fx);
-    END_FOR
+    const size_t kbL = TBASE(Lm);
+#define GL(a, x, y) (a + kbL)[TOFF(x, y)]
+    double pgfx, pgfy;
+    {
+        const size_t kb1 = TBASE(K + 1);
+#define CB(a, kq, x, y) (a + ((kq) ? kb1 : kb0))[TOFF(x, y)]
+        double a = CB(pkapb, 0, tx + 1, ty), b = CB(pkapb, 1, tx, ty), cc = CB(pkapb, 0, tx, ty),
+               d = CB(pkapb, 1, tx + 1, ty);
+        const double g00 = CB(gzb, 0, tx, ty), g01 = CB(gzb, 1, tx, ty);
+        pgfx = ((a - b) * (g00 - CB(gzb, 1, tx + 1, ty)) + (cc - d) * (g01 - CB(gzb, 0, tx + 1, ty))) /
+               (a - b + cc - d);
+        a = CB(pkapb, 0, tx, ty + 1);
+        d = CB(pkapb, 1, tx, ty + 1);
+        pgfy = ((a - b) * (g00 - CB(gzb, 1, tx, ty + 1)) + (cc - d) * (g01 - CB(gzb, 0, tx, ty + 1))) /
+               (a - b + cc - d);
+#undef CB
+    }
+    __syncthreads();
+    FOR_REG(RQU) sQU[RQU::at(x, y)] = dq_mono(U(x - 1, y), U(x, y), U(x + 1, y)); END_FOR
+    FOR_REG(RQV) sQV[RQV::at(x, y)] = dq_mono(V(x, y - 1), V(x, y), V(x, y + 1)); END_FOR
     FOR_REG(RW)  // vorticity (659-663)
     W(x, y) = r.coriolis - (U(x, y + 1) - U(x, y)) * r.rdy + (V(x + 1, y) - V(x, y)) * r.rdx;
     END_FOR
@@ -479,17 +510,33 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
     DV(x, y) = r.rdx * (U(x, y) - U(x - 1, y)) + r.rdy * (V(x, y) - V(x, y - 1));
     END_FOR
     __syncthreads();
+    FOR_REG(RKE)  // kinetic energy (639-657), PPM_CONSTRAINED, last-level B winds
+    const R ubij = R(.5 * (GL(uc, x, y) + GL(uc, x, y - 1)));
+    const R vbij = R(.5 * (GL(vc, x, y) + GL(vc, x - 1, y)));
+#define QV(m) sQV[RQV::at(x, y + (m))]
+    const R fy = ppm_face_dq<V_PPM_C>(V(x, y - 2), V(x, y - 1), V(x, y), V(x, y + 1), QV(-2), QV(-1), QV(0), QV(1),
+                                      r.dtdy * vbij);
+#undef QV
+#define QU(m) sQU[RQU::at(x + (m), y)]
+    const R fx = ppm_face_dq<V_PPM_C>(U(x - 2, y), U(x - 1, y), U(x, y), U(x + 1, y), QU(-2), QU(-1), QU(0), QU(1),
+                                      r.dtdx * ubij);
+#undef QU
+    const R k = vbij * fy;
+    sKE[RKE::at(x, y)] = R(.5) * (k + ubij * fx);
+    END_FOR
     FOR_REG(RD2)  // del^2 divergence (688-692)
     D2(x, y) = R(-2) * DV(x, y) * r.rdx2rdy2 + (DV(x + 1, y) + DV(x - 1, y)) * r.rdx2 +
                (DV(x, y + 1) + DV(x, y - 1)) * r.rdy2;
     END_FOR
+    FOR_REG(RLP)  // Laplacian of vorticity (717-720)
+    sL[RLP::at(x, y)] = R(-2) * W(x, y) * r.rdx2rdy2 + (W(x + 1, y) + W(x - 1, y)) * r.rdx2 +
+                        (W(x, y + 1) + W(x, y - 1)) * r.rdy2;
+    END_FOR
+    __syncthreads();
     // The reference's second level loop (634-771) reuses the 2-D arrays cx,
     // cy, denom_x, denom_y (and ub, vb) that its first loop (441-454) left
     // at the last level, k = nlay: the vorticity fluxes of every level use
     // the C winds of level nlay for their Courant numbers and denominators.
-    const int Lm = g.L - 1;
-    const size_t kbL = TBASE(Lm);
-#define GL(a, x, y) (a + kbL)[TOFF(x, y)]
     FOR_REG(RTU)  // inner x step of vorticity for ud (708-716)
     const R w0 = W(x, y);
     const R u0 = GL(uc, x, y), u1 = GL(uc, x + 1, y);
@@ -506,51 +553,25 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
     const R f1 = pcm_face(w0, W(x, y + 1), r.dtdx * v1);
     sTW[RTW::at(x, y)] = R(.5) * (w0 + dny * (w0 - r.rA * (f1 - f0)));
     END_FOR
-    FOR_REG(RLP)  // Laplacian of vorticity (717-720)
-    sL[RLP::at(x, y)] = R(-2) * W(x, y) * r.rdx2rdy2 + (W(x + 1, y) + W(x - 1, y)) * r.rdx2 +
-                        (W(x, y + 1) + W(x, y - 1)) * r.rdy2;
-    END_FOR
-    __syncthreads();
     FOR_REG(RKE)  // del^4 divergence times the damping coefficient (695-702)
     sD4[RKE::at(x, y)] = r.divdamp * (R(-2) * D2(x, y) * r.rdx2rdy2 + (D2(x + 1, y) + D2(x - 1, y)) * r.rdx2 +
                                       (D2(x, y + 1) + D2(x, y - 1)) * r.rdy2);
     END_FOR
     __syncthreads();
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    FOR_REG(RQTU) sQTU[RQTU::at(x, y)] = dq_mono(sTU[RTU::at(x, y - 1)], sTU[RTU::at(x, y)], sTU[RTU::at(x, y + 1)]); END_FOR
+    FOR_REG(RQTW) sQTW[RQTW::at(x, y)] = dq_mono(sTW[RTW::at(x - 1, y)], sTW[RTW::at(x, y)], sTW[RTW::at(x + 1, y)]); END_FOR
+    __syncthreads();
     if (!(I0 + tx < ATM_HALO + g.nx && J0 + ty < ATM_HALO + g.ny)) return;  // no barrier follows
-    // B-grid pressure gradient (665-679) from the corner values of pkap and
-    // gz at levels k and k+1 (k_bcorners; pkapb(nlay+1) and gzb(1) from
-    // init_sw_dyn 991, 995)
-    const size_t kb1 = TBASE(K + 1);
-#define CB(a, kq, x, y) (a + ((kq) ? kb1 : kb0))[TOFF(x, y)]
-    double pgfx, pgfy;
-    {
-        double a = CB(pkapb, 0, tx + 1, ty), b = CB(pkapb, 1, tx, ty), cc = CB(pkapb, 0, tx, ty),
-               d = CB(pkapb, 1, tx + 1, ty);
-        const double g00 = CB(gzb, 0, tx, ty), g01 = CB(gzb, 1, tx, ty);
-        pgfx = ((a - b) * (g00 - CB(gzb, 1, tx + 1, ty)) + (cc - d) * (g01 - CB(gzb, 0, tx + 1, ty))) /
-               (a - b + cc - d);
-        a = CB(pkapb, 0, tx, ty + 1);
-        d = CB(pkapb, 1, tx, ty + 1);
-        pgfy = ((a - b) * (g00 - CB(gzb, 1, tx, ty + 1)) + (cc - d) * (g01 - CB(gzb, 0, tx, ty + 1))) /
-               (a - b + cc - d);
-    }
-#undef CB
     const double ucv = G(uc, tx, ty), vcv = G(vc, tx, ty);
     const double ucl = GL(uc, tx, ty), vcl = GL(vc, tx, ty);
-#undef GL
-    double udv, vdv;  // this cell's D winds in double
-    if constexpr (FP64) {
-        udv = U(tx, ty);
-        vdv = V(tx, ty);
-    } else {
-        udv = G(ud, tx, ty);
-        vdv = G(vd, tx, ty);
-    }
+    // this cell's D winds in double (their LDS tiles now hold dqi_mono)
+    const double udv = G(ud, tx, ty), vdv = G(vd, tx, ty);
     {  // ud (724-738)
 #define T(m) sTU[RTU::at(tx, ty + (m))]
-        const R fy = ppm_face<V_PPM_C>(T(-3), T(-2), T(-1), T(0), T(1), T(2), r.dtdx * R(vcl));
+#define Q(m) sQTU[RQTU::at(tx, ty + (m))]
+        const R fy = ppm_face_dq<V_PPM_C>(T(-2), T(-1), T(0), T(1), Q(-2), Q(-1), Q(0), Q(1), r.dtdx * R(vcl));
 #undef T
+#undef Q
         const double un = udv +
                           c.dt * (vcv * D(fy) + c.vy * (D(sL[RLP::at(tx, ty)]) - D(sL[RLP::at(tx, ty - 1)]))) -
                           c.dtdx * (D(sKE[RKE::at(tx + 1, ty)]) - D(sKE[RKE::at(tx, ty)]) -
@@ -560,8 +581,10 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
     }
     {  // vd (755-769)
 #define T(m) sTW[RTW::at(tx + (m), ty)]
-        const R fx = ppm_face<V_PPM_C>(T(-3), T(-2), T(-1), T(0), T(1), T(2), r.dtdx * R(ucl));
+#define Q(m) sQTW[RQTW::at(tx + (m), ty)]
+        const R fx = ppm_face_dq<V_PPM_C>(T(-2), T(-1), T(0), T(1), Q(-2), Q(-1), Q(0), Q(1), r.dtdx * R(ucl));
 #undef T
+#undef Q
         const double vn = vdv -
                           c.dt * (ucv * D(fx) + c.vx * (D(sL[RLP::at(tx, ty)]) - D(sL[RLP::at(tx - 1, ty)]))) -
                           c.dtdy * (D(sKE[RKE::at(tx, ty + 1)]) - D(sKE[RKE::at(tx, ty)]) -
@@ -570,6 +593,7 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
         atm_report_nonfinite(nfw, vn);
     }
 #undef G
+#undef GL
 #undef U
 #undef V
 #undef W

@@ -15,6 +15,7 @@
 // interior source with an atomic add).
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "atm_common.hpp"
 #include "lin_step.hpp"
@@ -41,6 +42,18 @@ __global__ void __launch_bounds__(256) k_lin_tan(LinArgs x, Rng r, int k0) {
     F::run(a, x, I, J, K);
 }
 
+// Adjoint body with one global atomic add per read and point.
+struct DevAdd {
+    __device__ static void add(double *p, double v) { unsafeAtomicAdd(p, v); }
+};
+
+template <class F>
+__global__ void __launch_bounds__(256) k_lin_adj_direct(LinArgs x, Rng r, int k0) {
+    LIN_POINT
+    AccAdj<F::NR, DevAdd> a(x);
+    F::run(a, x, I, J, K);
+    a.flush();
+}
 #undef LIN_POINT
 
 // ---------------------------------------------------------------------------
@@ -61,6 +74,16 @@ __global__ void __launch_bounds__(256) k_lin_tan(LinArgs x, Rng r, int k0) {
 // the table and every barrier.
 // ---------------------------------------------------------------------------
 constexpr int ADJ_X = 32, ADJ_Y = 8, ADJ_H = 3, ADJ_PMAX = 6;
+
+// An operation whose reads share few values between neighbouring points
+// gains less from the planes than the planes cost; it declares
+// ADJ_DIRECT = true and takes the direct kernel above.
+template <class F, class = void>
+struct AdjDirect : std::false_type {};
+template <class F>
+struct AdjDirect<F, std::void_t<decltype(F::ADJ_DIRECT)>> : std::integral_constant<bool, F::ADJ_DIRECT> {};
+template <class F>
+constexpr bool adj_in_lds() { return !AdjDirect<F>::value; }
 constexpr int ADJ_W = ADJ_X + 2 * ADJ_H, ADJ_PL = ADJ_W * (ADJ_Y + 2 * ADJ_H);
 
 template <int NR>
@@ -104,7 +127,6 @@ __global__ void __launch_bounds__(ADJ_X * ADJ_Y) k_lin_adj(LinArgs x, Rng r, int
     const int Iq = Ib + threadIdx.x, Jq = Jb + threadIdx.y;
     const bool active = Iq <= r.I1 && Jq <= r.J1;
     AccAdjTile<NR> a(x, min(Iq, r.I1), min(Jq, r.J1), K, active);
-    for (int c = tid; c < ADJ_PMAX * ADJ_PL; c += NT) pl[c] = 0.;
     F::run(a, x, a.I, a.J, K);
     // plane of each slot: one per (operand, level offset) in slot order
     // (register arrays indexed only by unrolled loop counters)
@@ -128,6 +150,7 @@ __global__ void __launch_bounds__(ADJ_X * ADJ_Y) k_lin_adj(LinArgs x, Rng r, int
         }
         sp[s] = p;
     }
+    for (int c = tid; c < np * ADJ_PL; c += NT) pl[c] = 0.;
     __syncthreads();  // planes cleared
 #pragma unroll
     for (int s = 0; s < NR; s++) {
@@ -273,6 +296,10 @@ struct GpuBackend {
     }
     template <class F>
     void adj(const LinArgs &x, Rng r, int k0, int nk) {
+        if (!adj_in_lds<F>()) {
+            hipLaunchKernelGGL(k_lin_adj_direct<F>, lin_grid(r, nk), LBLK, 0, s, x, r, k0);
+            return;
+        }
         const dim3 grid((unsigned)((r.I1 - r.I0 + ADJ_X) / ADJ_X), (unsigned)((r.J1 - r.J0 + ADJ_Y) / ADJ_Y),
                         (unsigned)nk);
         hipLaunchKernelGGL(k_lin_adj<F>, grid, dim3(ADJ_X, ADJ_Y), 0, s, x, r, k0);

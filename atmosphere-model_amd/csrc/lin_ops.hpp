@@ -118,6 +118,7 @@ struct AccAdj {
 // flux, dp; out: heating rate
 struct OpHeat {
     static constexpr int NR = 5;
+    static constexpr bool ADJ_DIRECT = true;  // neighbouring points share few reads: no LDS planes
     template <class A>
     __host__ __device__ static void run(A &a, const LinArgs &x, int I, int J, int K) {
         using S = typename A::S;
@@ -345,6 +346,7 @@ struct OpCPgf {
 // in: ud, vd; out: denx, deny, cx, cy
 struct OpCDenoms2 {
     static constexpr int NR = 4;
+    static constexpr bool ADJ_DIRECT = true;  // neighbouring points share few reads: no LDS planes
     template <class A>
     __host__ __device__ static void run(A &a, const LinArgs &x, int I, int J, int K) {
         using S = typename A::S;
@@ -437,6 +439,7 @@ struct OpCVc {
 // Courant numbers from the C winds (446-449).  in: uc, vc; out: cx, cy
 struct OpDCourant {
     static constexpr int NR = 2;
+    static constexpr bool ADJ_DIRECT = true;  // neighbouring points share few reads: no LDS planes
     template <class A>
     __host__ __device__ static void run(A &a, const LinArgs &x, int I, int J, int K) {
         using S = typename A::S;
@@ -450,6 +453,7 @@ struct OpDCourant {
 // last level (Q21).  in: uc, vc (level nlay-1); out: ub, vb (planes)
 struct OpDUbVb {
     static constexpr int NR = 4;
+    static constexpr bool ADJ_DIRECT = true;  // neighbouring points share few reads: no LDS planes
     template <class A>
     __host__ __device__ static void run(A &a, const LinArgs &x, int I, int J, int K) {
         using S = typename A::S;

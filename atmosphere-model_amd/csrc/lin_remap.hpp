@@ -438,6 +438,7 @@ __host__ __device__ inline void rm_pt_column_adj(const LinArgs &x, size_t c0, co
 // playkap = (pkap(k+1) - pkap(k)) / log(pkap(k+1) / pkap(k)) (48-53, 64-70)
 struct OpRemapPlk {
     static constexpr int NR = 2;
+    static constexpr bool ADJ_DIRECT = true;  // neighbouring points share few reads: no LDS planes
     template <class A>
     __host__ __device__ static void run(A &a, const LinArgs &x, int I, int J, int K) {
         using S = typename A::S;
@@ -449,6 +450,7 @@ struct OpRemapPlk {
 // target_plev = Ak + Bk (plev(1) - top) (58-59); ak, bk, ak0 of LinArgs
 struct OpTargetPlev {
     static constexpr int NR = 1;
+    static constexpr bool ADJ_DIRECT = true;  // neighbouring points share few reads: no LDS planes
     template <class A>
     __host__ __device__ static void run(A &a, const LinArgs &x, int I, int J, int K) {
         using S = typename A::S;
@@ -460,6 +462,7 @@ struct OpTargetPlev {
 // x**kappa (61-62)
 struct OpPowKappa {
     static constexpr int NR = 1;
+    static constexpr bool ADJ_DIRECT = true;  // neighbouring points share few reads: no LDS planes
     template <class A>
     __host__ __device__ static void run(A &a, const LinArgs &x, int I, int J, int K) {
         using S = typename A::S;
@@ -487,6 +490,7 @@ struct OpFaceAvg {
 // halo of the start state (Q5).  in: the D step's winds, the start winds
 struct OpWindSource {
     static constexpr int NR = 2;
+    static constexpr bool ADJ_DIRECT = true;  // neighbouring points share few reads: no LDS planes
     template <class A>
     __host__ __device__ static void run(A &a, const LinArgs &x, int I, int J, int K) {
         using S = typename A::S;
@@ -532,6 +536,7 @@ struct OpKeRemap {
 // dp = target_plev(k) - target_plev(k+1) (157-158)
 struct OpDpOut {
     static constexpr int NR = 2;
+    static constexpr bool ADJ_DIRECT = true;  // neighbouring points share few reads: no LDS planes
     template <class A>
     __host__ __device__ static void run(A &a, const LinArgs &x, int I, int J, int K) {
         using S = typename A::S;
