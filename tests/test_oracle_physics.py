@@ -178,3 +178,48 @@ def test_richardson_mixing_law():
     du_c = np.sqrt(4.0 * A[0] - floor)
     assert _fraction(0.999 * du_c)[0] == 0.0
     assert _fraction(1.001 * du_c)[0] > 0.0
+
+
+def test_richardson_number_is_the_bulk_richardson_number():
+    """The stability factor of the local Richardson number (mod_physics.f90:
+    99-104) pinned by its physical meaning.  For two layers in hydrostatic
+    balance the bulk Richardson number across their interface is
+    (dtheta / theta_m) * g dz / S^2: g dz is the mass-weighted mean of the
+    layers' geopotential thicknesses, each cp theta (p_bot^kappa - p_top^kappa)
+    for an isentropic layer, theta_m the mass-weighted mean potential
+    temperature and S^2 the squared shear plus the binary32 floor.  The
+    oracle's mixing fractions at several shears, read through the law
+    (1 - Ri/Ric)^2, give A = Ri S^2; it must equal that product from the
+    column's pressures alone.  One layer's thickness in place of the mean,
+    the lower layer's theta in place of the mean, or the sum of the two
+    thicknesses, each miss by > 1e-3."""
+    dp = np.array([40000.0, 25000.0])
+    ptop, dpt = 1000.0, 3.0
+    th = np.array([300.0, 300.0 + dpt])
+    kappa = float(np.float32(287.052874)) / 1005.0
+    floor = float(np.float32(1e-4))
+    p = np.array([ptop + dp[0] + dp[1], ptop + dp[1], ptop])  # surface, interface, top
+    dgz = 1005.0 * th * (p[:-1] ** kappa - p[1:] ** kappa)
+    A_phys = dpt / ((dp @ th) / dp.sum()) * (dp @ dgz) / dp.sum()
+
+    def frac(du):
+        o = _column_state(dp, th, [du, 0.0], [0.0, 0.0], ptop=ptop)
+        o.calculate_physics_tendencies()
+        ut, _, _ = o.physics_column_tendencies(3, 4)
+        return -ut[0] * 600.0 * (dp[0] + dp[1]) / (dp[1] * du)
+
+    du_c = np.sqrt(4.0 * A_phys - floor)  # Ri = 1/4
+    A = []
+    for du in (1.2 * du_c, 1.5 * du_c, 2.5 * du_c):
+        f = frac(du)
+        assert 0.0 < f < 1.0
+        A.append((1.0 - np.sqrt(f)) * (du * du + floor) / 4.0)
+    A = np.array(A)
+    assert float(np.max(np.abs(A / A_phys - 1.0))) < 1e-9, (A, A_phys)
+    # the threshold sits where the physical bulk Richardson number is 1/4
+    assert frac(0.999 * du_c) == 0.0 and frac(1.001 * du_c) > 0.0
+    # plausible misreadings of the stability factor
+    for wrong in (dpt / ((dp @ th) / dp.sum()) * dgz[0],
+                  dpt / th[0] * (dp @ dgz) / dp.sum(),
+                  dpt / ((dp @ th) / dp.sum()) * dgz.sum()):
+        assert abs(wrong / A_phys - 1.0) > 1e-3
