@@ -194,21 +194,32 @@ __host__ __device__ inline void rm_column(const LinArgs &x, size_t c0, const RmW
     auto V = [&](int k) { return col_rd<S>(x, 0, c0, k); };
     auto X = [&](int k) { return col_rd<S>(x, 1, c0, k); };
     auto D = [&](int k) { return col_rd<S>(x, 2, c0, k); };
-    // elimination sweep
+    // elimination sweep; the recurrence (level_var(k+1), gam(k)) and the
+    // faces and values of the current layers are carried in registers
     {
+        S xp = X(L + 1), x0 = X(L), xm = X(L - 1), v0 = V(L), vm = V(L - 1);
         S lv, gm;
-        rm_elim_top<S>(X(L + 1), X(L), X(L - 1), V(L), V(L - 1), lv, gm);
+        rm_elim_top<S>(xp, x0, xm, v0, vm, lv, gm);
         rm_put(w, RM_LV, L + 1, lv);
         rm_put(w, RM_GM, L, gm);
         for (int k = L; k >= 2; k--) {
+            // xp = X(k+1), x0 = X(k), xm = X(k-1), v0 = V(k), vm = V(k-1)
             S l2, g2;
-            rm_elim_step<S>(X(k + 1), X(k), X(k - 1), V(k), V(k - 1), rm_get<S>(w, RM_LV, k + 1),
-                            rm_get<S>(w, RM_GM, k), l2, g2);
+            rm_elim_step<S>(xp, x0, xm, v0, vm, lv, gm, l2, g2);
             rm_put(w, RM_LV, k, l2);
             rm_put(w, RM_GM, k - 1, g2);
+            lv = l2;
+            gm = g2;
+            if (k > 2) {
+                xp = x0;
+                x0 = xm;
+                xm = X(k - 2);
+                v0 = vm;
+                vm = V(k - 2);
+            }
         }
-        rm_put(w, RM_LV, 1,
-               rm_elim_bottom<S>(X(3), X(2), X(1), V(1), V(2), rm_get<S>(w, RM_LV, 2), rm_get<S>(w, RM_GM, 1)));
+        // xp = X(3), x0 = X(2), xm = X(1), v0 = V(2), vm = V(1)
+        rm_put(w, RM_LV, 1, rm_elim_bottom<S>(xp, x0, xm, vm, v0, lv, gm));
     }
     // back substitution (313-315) and limits of the level values
     auto G = [&](int i) { return (i >= 1 && i <= L) ? val(rm_get<S>(w, RM_GM, i)) : 0.; };
@@ -254,7 +265,11 @@ __host__ __device__ inline void rm_column(const LinArgs &x, size_t c0, const RmW
 // Adjoint of rm_column: the reverse pass over the intermediates that the
 // forward sweep of the same step left in this operation's workspace.  The
 // output's adjoint (consumed) is added into the adjoints of V, X and D;
-// each column is one thread's.
+// each column is one thread's.  Every sweep keeps the adjoints that
+// consecutive steps of its loop add to (a layer's edges while the overlap
+// walk stays on it, the back substitution's chain, a window of faces) in
+// registers and writes each to memory once, when no later step of the
+// sweep adds to it.
 __host__ __device__ inline void rm_column_adj(const LinArgs &x, size_t c0, const RmWS &w) {
     const int L = x.g.L;
     auto V = [&](int k) { return col_rd<double>(x, 0, c0, k); };
@@ -266,115 +281,168 @@ __host__ __device__ inline void rm_column_adj(const LinArgs &x, size_t c0, const
     auto aD = [&](int k) -> double & { return x.it[2][c0 + (size_t)(k - 1) * pl]; };
     auto W = [&](int s, int k) { return w.at(s, k); };
     auto A = [&](int s, int k) -> double & { return w.at(s + RM_NV, k); };
-    for (int s = 0; s < RM_NV; s++)
-        for (int k = 0; k <= L + 1; k++) A(s, k) = 0.;
-    // overlaps (the pairs the forward pass visited)
-    int k0 = 1;
-    for (int m = 1; m <= L; m++) {
-        double &ao = x.ot[0][c0 + (size_t)(m - 1) * pl];
-        const double am = ao;
-        ao = 0.;
-        if (am == 0.) continue;
-        const double dm0 = D(m), dm1 = D(m + 1);
-        k0 = rm_first_overlap(X, k0, L, dm0);
-        for (int k = k0; k <= L; k++) {
-            const double xk = X(k), xk1 = X(k + 1);
-            if (LIN_DECIDE(xk <= dm1)) break;
-            using J8 = Jac<8>;
-            const J8 r = rm_overlap<J8>(J8::seed(W(RM_B, k), 0), J8::seed(W(RM_T, k), 1), J8::seed(W(RM_CV, k), 2),
-                                        J8::seed(V(k), 3), J8::seed(xk, 4), J8::seed(xk1, 5), J8::seed(dm0, 6),
-                                        J8::seed(dm1, 7));
-            A(RM_B, k) += am * r.d[0];
-            A(RM_T, k) += am * r.d[1];
-            A(RM_CV, k) += am * r.d[2];
-            aV(k) += am * r.d[3];
-            aX(k) += am * r.d[4];
-            aX(k + 1) += am * r.d[5];
-            aD(m) += am * r.d[6];
-            aD(m + 1) += am * r.d[7];
+
+    // overlaps (the pairs the forward pass visited).  The walk never returns
+    // to a layer it left, so the sums of the current layer kc are written
+    // when it moves on; a layer it never reaches keeps zero adjoints.
+    for (int k = 1; k <= L; k++) A(RM_B, k) = A(RM_T, k) = A(RM_CV, k) = 0.;
+    {
+        int kc = 0;
+        double pB = 0., pT = 0., pCV = 0., pV = 0., pX0 = 0., pX1 = 0.;  // of layer kc: B, T, CV, V, X(kc), X(kc+1)
+        double pD0 = 0., pD1 = 0.;                                      // D(m), D(m+1)
+        auto leave = [&](int knew) {
+            if (kc > 0) {
+                A(RM_B, kc) = pB;
+                A(RM_T, kc) = pT;
+                A(RM_CV, kc) = pCV;
+                aV(kc) += pV;
+                aX(kc) += pX0;
+                if (knew == kc + 1) {
+                    pX0 = pX1;
+                } else {
+                    aX(kc + 1) += pX1;
+                    pX0 = 0.;
+                }
+            }
+            pX1 = pB = pT = pCV = pV = 0.;
+            kc = knew;
+        };
+        int k0 = 1;
+        for (int m = 1; m <= L; m++) {
+            if (m > 1) {
+                aD(m - 1) += pD0;
+                pD0 = pD1;
+                pD1 = 0.;
+            }
+            double &ao = x.ot[0][c0 + (size_t)(m - 1) * pl];
+            const double am = ao;
+            ao = 0.;
+            if (am == 0.) continue;
+            const double dm0 = D(m), dm1 = D(m + 1);
+            k0 = rm_first_overlap(X, k0, L, dm0);
+            for (int k = k0; k <= L; k++) {
+                const double xk = X(k), xk1 = X(k + 1);
+                if (LIN_DECIDE(xk <= dm1)) break;
+                using J8 = Jac<8>;
+                const J8 r = rm_overlap<J8>(J8::seed(W(RM_B, k), 0), J8::seed(W(RM_T, k), 1),
+                                            J8::seed(W(RM_CV, k), 2), J8::seed(V(k), 3), J8::seed(xk, 4),
+                                            J8::seed(xk1, 5), J8::seed(dm0, 6), J8::seed(dm1, 7));
+                if (k != kc) leave(k);
+                pB += am * r.d[0];
+                pT += am * r.d[1];
+                pCV += am * r.d[2];
+                pV += am * r.d[3];
+                pX0 += am * r.d[4];
+                pX1 += am * r.d[5];
+                pD0 += am * r.d[6];
+                pD1 += am * r.d[7];
+            }
         }
+        aD(L) += pD0;
+        aD(L + 1) += pD1;
+        if (kc > 0) leave(L + 2);
     }
-    // edges and curvature
-    for (int k = L; k >= 1; k--) {
-        const double acv = A(RM_CV, k);
-        aV(k) += 6. * acv;
-        double at = A(RM_T, k) - 3. * acv, ab = A(RM_B, k) - 3. * acv;
-        if (k >= 3) {
-            using J4 = Jac<4>;
-            const J4 t = rm_huynh_top<J4>(J4::seed(W(RM_LF, k + 1), 0), J4::seed(V(k), 1),
-                                          J4::seed(W(RM_GM, k - 1), 2), J4::seed(W(RM_GM, k - 2), 3));
-            A(RM_LF, k + 1) += at * t.d[0];
-            aV(k) += at * t.d[1];
-            A(RM_GM, k - 1) += at * t.d[2];
-            A(RM_GM, k - 2) += at * t.d[3];
-            const J4 b = rm_huynh_btm<J4>(J4::seed(W(RM_LF, k), 0), J4::seed(V(k), 1), J4::seed(W(RM_GM, k), 2),
-                                          J4::seed(W(RM_GM, k - 1), 3));
-            A(RM_LF, k) += ab * b.d[0];
-            aV(k) += ab * b.d[1];
-            A(RM_GM, k) += ab * b.d[2];
-            A(RM_GM, k - 1) += ab * b.d[3];
-        } else {
-            A(RM_LF, k + 1) += at;
-            A(RM_LF, k) += ab;
+    // edges and curvature, from the top layer down: level_var(k+1) and
+    // gam(k) receive their last adjoint at layer k
+    {
+        double pLF = 0.;                // level_var(k+1), from layer k+1's bottom edge
+        double g0 = 0., g1 = 0., g2 = 0.;  // gam(k), gam(k-1), gam(k-2)
+        for (int k = L; k >= 1; k--) {
+            const double acv = A(RM_CV, k);
+            double av = 6. * acv;
+            const double at = A(RM_T, k) - 3. * acv, ab = A(RM_B, k) - 3. * acv;
+            double lt = at, lb = ab;
+            if (k >= 3) {
+                using J4 = Jac<4>;
+                const J4 t = rm_huynh_top<J4>(J4::seed(W(RM_LF, k + 1), 0), J4::seed(V(k), 1),
+                                              J4::seed(W(RM_GM, k - 1), 2), J4::seed(W(RM_GM, k - 2), 3));
+                lt = at * t.d[0];
+                av += at * t.d[1];
+                g1 += at * t.d[2];
+                g2 += at * t.d[3];
+                const J4 b = rm_huynh_btm<J4>(J4::seed(W(RM_LF, k), 0), J4::seed(V(k), 1), J4::seed(W(RM_GM, k), 2),
+                                              J4::seed(W(RM_GM, k - 1), 3));
+                lb = ab * b.d[0];
+                av += ab * b.d[1];
+                g0 += ab * b.d[2];
+                g1 += ab * b.d[3];
+            }
+            aV(k) += av;
+            A(RM_LF, k + 1) = pLF + lt;
+            pLF = lb;
+            A(RM_GM, k) = g0;
+            g0 = g1;
+            g1 = g2;
+            g2 = 0.;
         }
+        A(RM_LF, 1) = pLF;
     }
-    // limits and back substitution, from the top level down
+    // limits and back substitution, from the top level down:
+    // bs(j) = lv(j) - gam(j-1) bs(j-1)
     auto G = [&](int i) { return (i >= 1 && i <= L) ? W(RM_GM, i) : 0.; };
-    for (int j = L + 1; j >= 2; j--) {
-        double abs_ = A(RM_LF, j);
-        if (j <= L) {
-            using J3 = Jac<3>;
-            const J3 l = rm_clamp<J3>(j, L, J3::seed(W(RM_BS, j), 0), J3::seed(V(j - 1), 1), J3::seed(V(j), 2),
-                                      G(j + 1), G(j - 1));
-            abs_ = A(RM_LF, j) * l.d[0];
-            aV(j - 1) += A(RM_LF, j) * l.d[1];
-            aV(j) += A(RM_LF, j) * l.d[2];
+    {
+        double pBS = 0., pAV = 0.;  // adjoints of bs(j) and of V(j)
+        for (int j = L + 1; j >= 2; j--) {
+            const double alf = A(RM_LF, j);
+            double abs_ = alf, avj = pAV, avm = 0.;
+            if (j <= L) {
+                using J3 = Jac<3>;
+                const J3 l = rm_clamp<J3>(j, L, J3::seed(W(RM_BS, j), 0), J3::seed(V(j - 1), 1), J3::seed(V(j), 2),
+                                          G(j + 1), G(j - 1));
+                abs_ = alf * l.d[0];
+                avm = alf * l.d[1];
+                avj += alf * l.d[2];
+                aV(j) += avj;
+            }
+            pAV = avm;
+            abs_ += pBS;
+            A(RM_LV, j) = abs_;
+            A(RM_GM, j - 1) -= abs_ * W(RM_BS, j - 1);
+            pBS = -abs_ * W(RM_GM, j - 1);
         }
-        abs_ += A(RM_BS, j);
-        // bs(j) = lv(j) - gam(j-1) bs(j-1)
-        A(RM_LV, j) += abs_;
-        A(RM_GM, j - 1) -= abs_ * W(RM_BS, j - 1);
-        A(RM_BS, j - 1) += -abs_ * W(RM_GM, j - 1);
+        aV(1) += pAV;
+        A(RM_LV, 1) = A(RM_LF, 1) + pBS;
     }
-    A(RM_LV, 1) += A(RM_LF, 1) + A(RM_BS, 1);
-    // elimination sweep backwards: bottom, then k = 2..L, then the top
+    // elimination sweep backwards: bottom, then k = 2..L, then the top.
+    // At step k the adjoints of X(k-1) and V(k-1) are complete.
     {
         using J7 = Jac<7>;
         const J7 l1 = rm_elim_bottom<J7>(J7::seed(X(3), 0), J7::seed(X(2), 1), J7::seed(X(1), 2), J7::seed(V(1), 3),
                                          J7::seed(V(2), 4), J7::seed(W(RM_LV, 2), 5), J7::seed(W(RM_GM, 1), 6));
         const double a = A(RM_LV, 1);
-        aX(3) += a * l1.d[0];
-        aX(2) += a * l1.d[1];
-        aX(1) += a * l1.d[2];
-        aV(1) += a * l1.d[3];
-        aV(2) += a * l1.d[4];
-        A(RM_LV, 2) += a * l1.d[5];
-        A(RM_GM, 1) += a * l1.d[6];
+        double xm = a * l1.d[2], x0 = a * l1.d[1], xp = a * l1.d[0];  // X(k-1), X(k), X(k+1) at k = 2
+        double vm = a * l1.d[3], v0 = a * l1.d[4];                    // V(k-1), V(k)
+        double cLV = a * l1.d[5], cGM = a * l1.d[6];                  // level_var(k), gam(k-1)
         for (int k = 2; k <= L; k++) {
             J7 lv, gm;
             rm_elim_step<J7>(J7::seed(X(k + 1), 0), J7::seed(X(k), 1), J7::seed(X(k - 1), 2), J7::seed(V(k), 3),
                              J7::seed(V(k - 1), 4), J7::seed(W(RM_LV, k + 1), 5), J7::seed(W(RM_GM, k), 6), lv, gm);
-            const double al = A(RM_LV, k), ag = A(RM_GM, k - 1);
-            const double p[7] = {al * lv.d[0] + ag * gm.d[0], al * lv.d[1] + ag * gm.d[1], al * lv.d[2] + ag * gm.d[2],
-                                 al * lv.d[3] + ag * gm.d[3], al * lv.d[4] + ag * gm.d[4], al * lv.d[5] + ag * gm.d[5],
-                                 al * lv.d[6] + ag * gm.d[6]};
-            aX(k + 1) += p[0];
-            aX(k) += p[1];
-            aX(k - 1) += p[2];
-            aV(k) += p[3];
-            aV(k - 1) += p[4];
-            A(RM_LV, k + 1) += p[5];
-            A(RM_GM, k) += p[6];
+            const double al = A(RM_LV, k) + cLV, ag = A(RM_GM, k - 1) + cGM;
+            xp += al * lv.d[0] + ag * gm.d[0];
+            x0 += al * lv.d[1] + ag * gm.d[1];
+            xm += al * lv.d[2] + ag * gm.d[2];
+            v0 += al * lv.d[3] + ag * gm.d[3];
+            vm += al * lv.d[4] + ag * gm.d[4];
+            cLV = al * lv.d[5] + ag * gm.d[5];
+            cGM = al * lv.d[6] + ag * gm.d[6];
+            aX(k - 1) += xm;
+            aV(k - 1) += vm;
+            xm = x0;
+            x0 = xp;
+            xp = 0.;
+            vm = v0;
+            v0 = 0.;
         }
+        // xm = X(L), x0 = X(L+1), vm = V(L) pending
         using J5 = Jac<5>;
         J5 lv, gm;
         rm_elim_top<J5>(J5::seed(X(L + 1), 0), J5::seed(X(L), 1), J5::seed(X(L - 1), 2), J5::seed(V(L), 3),
                         J5::seed(V(L - 1), 4), lv, gm);
-        const double al = A(RM_LV, L + 1), ag = A(RM_GM, L);
-        aX(L + 1) += al * lv.d[0] + ag * gm.d[0];
-        aX(L) += al * lv.d[1] + ag * gm.d[1];
+        const double al = A(RM_LV, L + 1) + cLV, ag = A(RM_GM, L) + cGM;
+        aX(L + 1) += x0 + (al * lv.d[0] + ag * gm.d[0]);
+        aX(L) += xm + (al * lv.d[1] + ag * gm.d[1]);
         aX(L - 1) += al * lv.d[2] + ag * gm.d[2];
-        aV(L) += al * lv.d[3] + ag * gm.d[3];
+        aV(L) += vm + (al * lv.d[3] + ag * gm.d[3]);
         aV(L - 1) += al * lv.d[4] + ag * gm.d[4];
     }
 }
