@@ -80,8 +80,10 @@ def parse():
                          "mixed precision, radiation every 10 steps")
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--ensemble-streams", type=int, default=4)
-    ap.add_argument("--adjoint", choices=["on", "off"], default="on",
-                    help="gradient through a checkpointed window (tangent-linear/adjoint workload)")
+    ap.add_argument("--adjoint", choices=["on", "off", "steps"], default="on",
+                    help="gradient through a checkpointed window (tangent-linear/adjoint workload); "
+                         "steps: only single tangent-linear and adjoint steps at the same size, "
+                         "through the C ABI without torch (for profiling the step kernels)")
     ap.add_argument("--adj-nx", type=int, default=256)
     ap.add_argument("--adj-nlay", type=int, default=48)
     ap.add_argument("--adj-steps", type=int, default=90)
@@ -595,6 +597,57 @@ def adjoint_timing(args):
     return res
 
 
+def adjoint_step_timing(args):
+    """Single linearised steps of the tangent-linear/adjoint workload's grid
+    (256 x 256 x 48, dx = 50 km, dt = 40 s), with and without a remap:
+    wall time per adjoint step and per tangent-linear step, against a
+    forward step of the fused FP64 path, with every adjoint and tangent
+    field nonzero (seeded noise).  No torch: only the C ABI."""
+    import numpy as np
+
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
+    from atmos.model import Model
+
+    nx, L, dx, dt = args.adj_nx, args.adj_nlay, 50e3, 40.0
+    ic = make_initial_state(nx, nx, L, dx, seed=20261203 + 400)
+    cfg = Config(dt_max=dt, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12, dt_remap=5 * dt,
+                 radiation_on=False, physics_on=False)
+    m = Model(nx, nx, L, dx)
+    st = driver.init_model(m, ic, cfg)
+    m.lin_init()
+    rng = np.random.default_rng(20261203 + 402)
+    res = {"grid": f"{nx}x{nx}x{L}", "dt": dt}
+    for kind in ("ad", "tl"):
+        for remap in (False, True):
+            for name in ("dp", "pt", "ud", "vd"):
+                m.put(f"{kind}_{name}", rng.standard_normal((L, nx, nx)))
+            step = m.ad_step if kind == "ad" else m.tl_step
+            step(dt, remap)
+            m.sync()
+            n = 5
+            t0 = time.perf_counter()
+            for _ in range(n):
+                step(dt, remap)
+            m.sync()
+            res[f"{kind}_step_ms{'_remap' if remap else ''}"] = (time.perf_counter() - t0) / n * 1e3
+    t = 0.0
+    for _ in range(3):
+        t += driver.step(m, cfg, st, t)
+    m.sync()
+    t0 = time.perf_counter()
+    n = 10
+    for _ in range(n):
+        t += driver.step(m, cfg, st, t)
+        if not m.check_stability():
+            raise SystemExit("adjoint steps: the forward run is unstable")
+    m.sync()
+    res["forward_step_ms"] = (time.perf_counter() - t0) / n * 1e3
+    m.close()
+    return res
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -690,6 +743,8 @@ def main():
     if args.adjoint == "on":
         out["adjoint"] = adjoint_timing(args)
         out["adjoint_forward_cost_ratio"] = out["adjoint"]["ratio"]
+    elif args.adjoint == "steps":
+        out["adjoint_steps"] = adjoint_step_timing(args)
     if args.twostream == "on":
         out["twostream"] = twostream_timing(args)
     if args.ensemble == "on" and args.path == "fused":
