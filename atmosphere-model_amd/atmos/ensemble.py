@@ -23,6 +23,14 @@ whole remap intervals and an even number of buffer switches (each D step
 and each remap switches), and radiation may fall due only at period
 starts.  The recorded time steps are those of the first period; the remap
 pattern of every later period is checked against it.
+
+Batched mode (`launch="batched"`): the dynamics stages of a step (C step,
+its halo fill, D step, the D halo fill) run as one launch per stage for all
+members together (atm_ensemble_*: a member table indexed by workgroup), on
+the first member's stream, joined to the other members' streams by events;
+radiation and the remap stay per member, on the members' own streams, where
+they overlap.  The members share the time step.  Every member's values are
+those of its own launches, bit for bit.
 """
 from __future__ import annotations
 
@@ -31,7 +39,7 @@ import dataclasses
 from . import driver
 from .config import Config
 from .ic import make_initial_state
-from .model import Model, stream_create, stream_destroy
+from .model import BatchedSteps, Model, stream_create, stream_destroy
 
 
 @dataclasses.dataclass
@@ -48,15 +56,25 @@ class Member:
 class Ensemble:
     """specs: (count, nx, ny, nlay) groups.  Member i is seeded with seed + i.
     autotune: each member times the fused path's launch shapes on its own
-    grid at construction (results do not depend on the choice)."""
+    grid at construction (results do not depend on the choice).
+    launch: "streams" (each member's own launches, members dealt to nstreams
+    streams) or "batched" (one launch per dynamics stage for all members;
+    remap and radiation on the members' streams)."""
 
     def __init__(self, specs, dx: float, dt: float, remap_every: int = 5, radiation_every: int = 10,
                  precision: str = "fp64", nstreams: int = 4, seed: int = 20261205, t_noise: float = 0.5,
-                 autotune: bool = False, checks: str = "deferred", graph_period: int = 0):
+                 autotune: bool = False, checks: str = "deferred", graph_period: int = 0,
+                 launch: str = "streams"):
         if nstreams < 1:
             raise ValueError("nstreams must be >= 1")
         if checks not in ("deferred", "sync"):
             raise ValueError("checks: 'deferred' or 'sync'")
+        if launch not in ("streams", "batched"):
+            raise ValueError("launch: 'streams' or 'batched'")
+        if launch == "batched" and graph_period:
+            raise ValueError("batched launches are not recorded into graphs")
+        self.launch = launch
+        self.batch = None
         self.remap_every, self.radiation_every = remap_every, radiation_every
         self.checks = checks
         self.graph_period = 0
@@ -82,6 +100,8 @@ class Ensemble:
                         m.autotune()
                     self.members[-1] = Member(m, st, cfg, cfg.t_initial, nx * ny * nlay)
                     i += 1
+            if launch == "batched":
+                self.batch = BatchedSteps([mb.model for mb in self.members])
         except BaseException:
             self.close()
             raise
@@ -95,10 +115,47 @@ class Ensemble:
         (with deferred checks: was not stable after the previous step)."""
         if self.graph_period:
             raise RuntimeError("graph mode steps whole periods: period()")
+        if self.batch is not None:
+            self._batched_step()
+            return self._check()
         dts = [driver.step(mb.model, mb.config, mb.state, mb.time) for mb in self.members]
         for mb, dt in zip(self.members, dts):
             mb.time = mb.time + dt
         return self._check()
+
+    def _batched_step(self):
+        """driver.step (mod_model.f90:117-183) for every member, its dynamics
+        stages batched over the members."""
+        dts = []
+        for mb in self.members:
+            m, cfg, st = mb.model, mb.config, mb.state
+            if st.calculate_radiation_this_step:
+                if cfg.radiation_on:
+                    m.calculate_radiative_heating(mb.time)
+                    m.radiation_halo_exchange()
+                st.calculate_radiation_this_step = False
+            dts.append(driver.calculate_dt(mb.time, cfg, st))
+            if cfg.physics_on and st.calculate_physics_this_step:
+                m.calculate_physics_tendencies()
+            st.calculate_physics_this_step = False
+        dt = dts[0]
+        if any(d != dt for d in dts):
+            raise RuntimeError("batched members must share the time step")
+        b = self.batch
+        b.cgrid_dynamics_step(dt / 2.0)
+        b.cgrid_halo_exchange()
+        b.dgrid_dynamics_step(dt)
+        for mb in self.members:
+            if mb.config.physics_on:
+                mb.model.apply_physics_tendencies(dt)
+            if mb.state.remap_this_step:
+                mb.model.perform_vertical_remapping()
+                mb.state.remap_this_step = False
+        b.dgrid_halo_exchange()
+        for mb in self.members:
+            if mb.config.physics_on:
+                mb.model.physics_halo_exchange()
+            mb.time = mb.time + dt
 
     def _validate_period(self, P: int):
         if self.remap_every < 1 or P % self.remap_every or (P + P // self.remap_every) % 2:
@@ -189,8 +246,11 @@ class Ensemble:
             mb.model.sync()
 
     def close(self):
-        """Release the members' contexts, then the streams they ran on
-        (closing twice is harmless)."""
+        """Release the batched steps, the members' contexts, then the
+        streams they ran on (closing twice is harmless)."""
+        if self.batch is not None:
+            self.batch.close()
+            self.batch = None
         for mb in self.members:
             mb.model.close()  # waits for the member's stream
         self.members.clear()

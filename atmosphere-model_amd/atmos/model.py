@@ -34,7 +34,9 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_stream_destroy", "atm_graph_begin", "atm_graph_end", "atm_graph_launch", "atm_graph_free",
            "atm_autotune", "atm_get_tuning", "atm_set_tuning", "atm_lin_init", "atm_tl_step", "atm_ad_step",
            "atm_twostream", "atm_twostream_noalloc", "atm_twostream_last_error", "atm_stream_synchronize", "atm_stream_timer_begin",
-           "atm_stream_timer_end", "atm_poison", "atm_check_stability_begin", "atm_check_stability_end"]
+           "atm_stream_timer_end", "atm_poison", "atm_check_stability_begin", "atm_check_stability_end",
+           "atm_ensemble_create", "atm_ensemble_destroy", "atm_ensemble_cgrid_step", "atm_ensemble_cgrid_halo",
+           "atm_ensemble_dgrid_step", "atm_ensemble_dgrid_halo"]
 PATHS = {"fused": 0, "reference": 1}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 TIMERS = {"c1": 0, "c2": 1, "d1": 2, "d2": 3, "scan": 4, "halo": 5, "remap": 6, "radiation": 7,
@@ -108,6 +110,11 @@ def load_library(path: str = LIB_PATH):
     L.atm_autotune.argtypes = [vp, ip]
     L.atm_get_tuning.argtypes = [vp, C.POINTER(C.c_int)]
     L.atm_set_tuning.argtypes = [vp, ip]
+    L.atm_ensemble_create.argtypes = [C.POINTER(vp), ip, C.POINTER(vp)]
+    for n in ["atm_ensemble_destroy", "atm_ensemble_cgrid_halo", "atm_ensemble_dgrid_halo"]:
+        getattr(L, n).argtypes = [vp]
+    for n in ["atm_ensemble_cgrid_step", "atm_ensemble_dgrid_step"]:
+        getattr(L, n).argtypes = [vp, C.c_double]
     _LIB = L
     return L
 
@@ -126,6 +133,49 @@ def stream_destroy(handle: int) -> None:
     rc = load_library().atm_stream_destroy(C.c_void_p(handle))
     if rc:
         raise AtmError(rc, "atm_stream_destroy")
+
+
+class BatchedSteps:
+    """The batched ensemble steps (atm_ensemble_*) over Models that run the
+    fused path in one precision: each call does for every member what the
+    Model's own call does, with one launch per stage for all of them on the
+    first member's stream (joined to the others' streams by events).  Close
+    it before the members."""
+
+    def __init__(self, models):
+        self._L = load_library()
+        self.models = list(models)
+        arr = (C.c_void_p * len(self.models))(*[m._h for m in self.models])
+        h = C.c_void_p()
+        rc = self._L.atm_ensemble_create(arr, len(self.models), C.byref(h))
+        if rc:
+            self._raise(rc)
+        self._h = h
+
+    def _raise(self, rc):
+        msgs = [m._L.atm_last_error(m._h).decode() for m in self.models if m._h]
+        raise AtmError(rc, "; ".join(x for x in msgs if x) or "atm_ensemble")
+
+    def _ck(self, rc):
+        if rc:
+            self._raise(rc)
+
+    def cgrid_dynamics_step(self, dt):
+        self._ck(self._L.atm_ensemble_cgrid_step(self._h, dt))
+
+    def cgrid_halo_exchange(self):
+        self._ck(self._L.atm_ensemble_cgrid_halo(self._h))
+
+    def dgrid_dynamics_step(self, dt):
+        self._ck(self._L.atm_ensemble_dgrid_step(self._h, dt))
+
+    def dgrid_halo_exchange(self):
+        self._ck(self._L.atm_ensemble_dgrid_halo(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.atm_ensemble_destroy(self._h)
+            self._h = None
 
 
 def _dptr(a):

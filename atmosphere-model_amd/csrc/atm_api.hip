@@ -7,6 +7,8 @@
 #include <cstring>
 
 #include <cstdlib>
+#include <algorithm>
+#include <vector>
 
 #include "atm_common.hpp"
 #include "atm_fused.hpp"
@@ -850,6 +852,266 @@ int atm_graph_free(atm_ctx *c) {
         c->gexec = nullptr;
     }
     return ATM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched ensemble steps: the dynamics stages of every member in one launch
+// each (atm_fused.hpp).  The calls mirror atm_cgrid_step, atm_cgrid_halo,
+// atm_dgrid_step and atm_dgrid_halo member by member, with the same checks
+// and the same host-side state changes; errors are reported on the member
+// concerned, or on the first.
+// ---------------------------------------------------------------------------
+struct atm_ensemble {
+    std::vector<atm_ctx *> m;
+    EnsMember *tab = nullptr;  // device
+    // per launch kind: the members' workgroup spans and the launch size (a
+    // multiple of 8)
+    EnsIndex ix_pair{}, ix_wind{}, ix_corner{}, ix_scan{}, ix_halo[4]{};
+    unsigned n_pair = 0, n_wind = 0, n_corner = 0, n_scan = 0, n_halo[4] = {0, 0, 0, 0};
+    // The batched launches go to the first member's stream.  Members may run
+    // on other streams (their remap and radiation overlap there): a batched
+    // call first makes the launch stream wait for every other member stream
+    // and afterwards makes those wait for it (events, one per stream).
+    std::vector<hipEvent_t> ev;          // n + 1
+    std::vector<hipStream_t> others;     // the other member streams of the current call
+};
+
+// the launch stream waits for the work already issued on every other member stream
+static int ens_join(atm_ensemble *e) {
+    atm_ctx *c0 = e->m[0];
+    const hipStream_t s0 = c0->stream;
+    e->others.clear();
+    for (atm_ctx *c : e->m) {
+        if (c->stream == s0 || std::find(e->others.begin(), e->others.end(), c->stream) != e->others.end()) continue;
+        hipEvent_t ev = e->ev[e->others.size()];
+        e->others.push_back(c->stream);
+        ATM_HIP(c0, hipEventRecord(ev, c->stream));
+        ATM_HIP(c0, hipStreamWaitEvent(s0, ev, 0));
+    }
+    return ATM_OK;
+}
+
+// every other member stream waits for the batched launches
+static int ens_fork(atm_ensemble *e) {
+    atm_ctx *c0 = e->m[0];
+    if (e->others.empty()) return ATM_OK;
+    hipEvent_t ev = e->ev.back();
+    ATM_HIP(c0, hipEventRecord(ev, c0->stream));
+    for (hipStream_t s : e->others) ATM_HIP(c0, hipStreamWaitEvent(s, ev, 0));
+    return ATM_OK;
+}
+
+static unsigned round8(unsigned long long n) { return (unsigned)((n + 7ull) / 8ull * 8ull); }
+
+// every member on the fused path, in one precision, on one device, with
+// its dynamics initialised, not recording a graph
+static int ens_usable(atm_ensemble *e) {
+    atm_ctx *c0 = e->m[0];
+    for (atm_ctx *c : e->m) {
+        if (!c->sw_init) return atm_fail(c, ATM_ERR_STATE, "ensemble: atm_init_sw_dyn first");
+        if (!c->fused) return atm_fail(c, ATM_ERR_STATE, "ensemble: every member on the fused path");
+        if (c->mixed != c0->mixed) return atm_fail(c, ATM_ERR_STATE, "ensemble: every member in one precision");
+        if (c->device != c0->device) return atm_fail(c, ATM_ERR_STATE, "ensemble: every member on one device");
+        if (c->capturing) return atm_fail(c, ATM_ERR_STATE, "ensemble: not while recording a graph");
+    }
+    return ATM_OK;
+}
+
+// bit m: member m's current copy of dp, pt, ud, vd
+static unsigned long long ens_parity(atm_ensemble *e) {
+    unsigned long long par = 0;
+    for (size_t i = 0; i < e->m.size(); i++)
+        if (e->m[i]->cur) par |= 1ull << i;
+    return par;
+}
+
+int atm_ensemble_create(atm_ctx *const *members, int n, atm_ensemble **out) {
+    if (!out || !members || n < 1) return ATM_ERR_ARGUMENT;
+    *out = nullptr;
+    for (int i = 0; i < n; i++)
+        if (!members[i]) return ATM_ERR_ARGUMENT;
+    if (n > ATM_ENS_MAX) return atm_fail(members[0], ATM_ERR_ARGUMENT, "ensemble: at most 64 members");
+    atm_ensemble *e = new atm_ensemble();
+    e->m.assign(members, members + n);
+    int rc = ens_usable(e);
+    std::vector<EnsMember> h(n);
+    unsigned long long np = 0, nw = 0, nc = 0, ns = 0, nh[4] = {0, 0, 0, 0};
+    for (int i = 0; rc == ATM_OK && i < n; i++) {
+        e->ix_pair.first[i] = (unsigned)np;
+        e->ix_wind.first[i] = (unsigned)nw;
+        e->ix_corner.first[i] = (unsigned)nc;
+        e->ix_scan.first[i] = (unsigned)ns;
+        for (int k = 0; k < 4; k++) e->ix_halo[k].first[i] = (unsigned)nh[k];
+        atm_ctx *c = members[i];
+        if (!atm_ens_scan_fits(c->g)) {
+            rc = atm_fail(c, ATM_ERR_ARGUMENT, "ensemble: nlay <= 64 for the batched column scans");
+            break;
+        }
+        EnsMember &m = h[i];
+        m.g = c->g;
+        m.coef = c->coef;
+        m.ak0 = c->ak0;
+        m.pktop = c->pktop;
+        for (int f = 0; f < 4; f++)
+            for (int w = 0; w < 2; w++) m.p[f][w] = c->pbuf[f][w];
+        m.dpc = c->dpc; m.ptc = c->ptc; m.uc = c->uc; m.vc = c->vc; m.ua = c->ua; m.va = c->va;
+        m.uc2 = c->uc2; m.vc2 = c->vc2; m.plk = c->plk; m.net_flux = c->net_flux; m.heatd = c->heatd;
+        m.plev = c->plev; m.pkap = c->pkap; m.gz = c->gz; m.pkapb = c->pkapb; m.gzb = c->gzb; m.nfw = c->nfw;
+        np += atm_ens_blocks_pair(c->g);
+        nw += atm_ens_blocks_wind(c->g);
+        nc += atm_ens_blocks_corner(c->g);
+        ns += atm_ens_blocks_scan(c->g);
+        for (int k = 0; k < 4; k++) nh[k] += atm_ens_blocks_halo(c->g, k);
+    }
+    if (rc == ATM_OK && std::max(std::max(np, nw), std::max(nc, ns)) > 0x7fffffffull)
+        rc = atm_fail(members[0], ATM_ERR_ARGUMENT, "ensemble: too many workgroups for one launch");
+    if (rc == ATM_OK) {
+        e->ix_pair.first[n] = (unsigned)np;
+        e->ix_wind.first[n] = (unsigned)nw;
+        e->ix_corner.first[n] = (unsigned)nc;
+        e->ix_scan.first[n] = (unsigned)ns;
+        for (int k = 0; k < 4; k++) e->ix_halo[k].first[n] = (unsigned)nh[k];
+    }
+    if (rc == ATM_OK) {
+        hipError_t err = hipMalloc(&e->tab, sizeof(EnsMember) * n);
+        if (err == hipSuccess) err = hipMemcpy(e->tab, h.data(), sizeof(EnsMember) * n, hipMemcpyHostToDevice);
+        for (int i = 0; err == hipSuccess && i <= n; i++) {
+            hipEvent_t ev = nullptr;
+            err = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+            if (err == hipSuccess) e->ev.push_back(ev);
+        }
+        if (err != hipSuccess) rc = atm_fail(members[0], ATM_ERR_HIP, std::string("ensemble: ") + hipGetErrorString(err));
+    }
+    if (rc != ATM_OK) {
+        for (hipEvent_t ev : e->ev) (void)hipEventDestroy(ev);
+        if (e->tab) (void)hipFree(e->tab);
+        delete e;
+        return rc;
+    }
+    e->n_pair = round8(np);
+    e->n_wind = round8(nw);
+    e->n_corner = round8(nc);
+    e->n_scan = round8(ns);
+    for (int k = 0; k < 4; k++) e->n_halo[k] = round8(nh[k]);
+    *out = e;
+    return ATM_OK;
+}
+
+int atm_ensemble_destroy(atm_ensemble *e) {
+    if (!e) return ATM_OK;
+    for (atm_ctx *c : e->m) (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t ev : e->ev) (void)hipEventDestroy(ev);
+    if (e->tab) (void)hipFree(e->tab);
+    delete e;
+    return ATM_OK;
+}
+
+int atm_ensemble_cgrid_step(atm_ensemble *e, double dt) {
+    if (!e) return ATM_ERR_ARGUMENT;
+    int rc = ens_usable(e);
+    if (rc) return rc;
+    for (atm_ctx *c : e->m)
+        if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "atm_dgrid_halo must follow atm_dgrid_step");
+    atm_ctx *c0 = e->m[0];
+    const hipStream_t s = c0->stream;
+    const int nm = (int)e->m.size();
+    for (atm_ctx *c : e->m) {
+        c->coef.dt = dt;
+        c->coef.dtdx = dt * c->coef.rdx;
+        c->coef.dtdy = dt * c->coef.rdy;
+        if (!c->plk_valid) {  // dp changed since the last D step's scan (a remap, a put)
+            rc = atm_fused_plk(c);
+            if (rc) return rc;
+        }
+    }
+    const unsigned long long par = ens_parity(e);
+    rc = ens_join(e);
+    if (rc) return rc;
+    atm_timer_begin(c0, ATM_T_C1);
+    atm_ens_launch_pair(s, e->tab, e->ix_pair, nm, par, dt, true, c0->mixed, e->n_pair);
+    atm_timer_end(c0, ATM_T_C1);
+    atm_timer_begin(c0, ATM_T_HALO);
+    atm_ens_launch_halo(s, e->tab, e->ix_halo[ENS_HALO_C1], nm, par, ENS_HALO_C1, e->n_halo[ENS_HALO_C1]);
+    atm_timer_end(c0, ATM_T_HALO);
+    atm_timer_begin(c0, ATM_T_SCAN);
+    atm_ens_launch_scan(s, e->tab, e->ix_scan, nm, par, false, e->n_scan);
+    atm_timer_end(c0, ATM_T_SCAN);
+    atm_timer_begin(c0, ATM_T_C2);
+    atm_ens_launch_c2(s, e->tab, e->ix_wind, nm, par, dt, c0->mixed, e->n_wind);
+    atm_timer_end(c0, ATM_T_C2);
+    ATM_HIP(c0, hipGetLastError());
+    return ens_fork(e);
+}
+
+int atm_ensemble_cgrid_halo(atm_ensemble *e) {
+    if (!e) return ATM_ERR_ARGUMENT;
+    int rc = ens_usable(e);
+    if (rc) return rc;
+    atm_ctx *c0 = e->m[0];
+    rc = ens_join(e);
+    if (rc) return rc;
+    atm_timer_begin(c0, ATM_T_HALO);
+    atm_ens_launch_halo(c0->stream, e->tab, e->ix_halo[ENS_HALO_CH], (int)e->m.size(), ens_parity(e), ENS_HALO_CH, e->n_halo[ENS_HALO_CH]);
+    atm_timer_end(c0, ATM_T_HALO);
+    ATM_HIP(c0, hipGetLastError());
+    return ens_fork(e);
+}
+
+int atm_ensemble_dgrid_step(atm_ensemble *e, double dt) {
+    if (!e) return ATM_ERR_ARGUMENT;
+    int rc = ens_usable(e);
+    if (rc) return rc;
+    for (atm_ctx *c : e->m)
+        if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "atm_dgrid_halo must follow atm_dgrid_step");
+    atm_ctx *c0 = e->m[0];
+    const hipStream_t s = c0->stream;
+    const int nm = (int)e->m.size();
+    for (atm_ctx *c : e->m) {
+        c->coef.dt = dt;
+        c->coef.dtdx = dt * c->coef.rdx;
+        c->coef.dtdy = dt * c->coef.rdy;
+    }
+    const unsigned long long par = ens_parity(e);
+    rc = ens_join(e);
+    if (rc) return rc;
+    atm_timer_begin(c0, ATM_T_D1);
+    atm_ens_launch_pair(s, e->tab, e->ix_pair, nm, par, dt, false, c0->mixed, e->n_pair);
+    atm_timer_end(c0, ATM_T_D1);
+    atm_timer_begin(c0, ATM_T_HALO);
+    atm_ens_launch_halo(s, e->tab, e->ix_halo[ENS_HALO_D1], nm, par, ENS_HALO_D1, e->n_halo[ENS_HALO_D1]);
+    atm_timer_end(c0, ATM_T_HALO);
+    atm_timer_begin(c0, ATM_T_SCAN);
+    atm_ens_launch_scan(s, e->tab, e->ix_scan, nm, par, true, e->n_scan);
+    atm_ens_launch_corners(s, e->tab, e->ix_corner, nm, e->n_corner);
+    atm_timer_end(c0, ATM_T_SCAN);
+    atm_timer_begin(c0, ATM_T_D2);
+    atm_ens_launch_d2(s, e->tab, e->ix_wind, nm, par, dt, c0->mixed, e->n_wind);
+    atm_timer_end(c0, ATM_T_D2);
+    ATM_HIP(c0, hipGetLastError());
+    rc = ens_fork(e);
+    if (rc) return rc;
+    for (atm_ctx *c : e->m) {  // as atm_dgrid_step on the fused path
+        c->plk_valid = true;
+        c->cur = 1 - c->cur;
+        atm_point_current(c);
+        c->after_d = true;
+    }
+    return ATM_OK;
+}
+
+int atm_ensemble_dgrid_halo(atm_ensemble *e) {
+    if (!e) return ATM_ERR_ARGUMENT;
+    int rc = ens_usable(e);
+    if (rc) return rc;
+    atm_ctx *c0 = e->m[0];
+    rc = ens_join(e);
+    if (rc) return rc;
+    atm_timer_begin(c0, ATM_T_HALO);
+    atm_ens_launch_halo(c0->stream, e->tab, e->ix_halo[ENS_HALO_DH], (int)e->m.size(), ens_parity(e), ENS_HALO_DH, e->n_halo[ENS_HALO_DH]);
+    atm_timer_end(c0, ATM_T_HALO);
+    ATM_HIP(c0, hipGetLastError());
+    for (atm_ctx *c : e->m) c->after_d = false;
+    return ens_fork(e);
 }
 
 int atm_lin_init(atm_ctx *c) {

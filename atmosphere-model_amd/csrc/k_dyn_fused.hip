@@ -127,7 +127,7 @@ template <class R, bool CGRID>
 constexpr int pair_waves() { return std::is_same<R, double>::value ? 6 : CGRID ? 7 : 8; }
 
 template <int VAR, bool CGRID, class R>
-__global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo g, DynCoef c, const double *__restrict__ dp,
+__device__ __forceinline__ void pair_body(int bx, int by, int K, Geo g, DynCoef c, const double *__restrict__ dp,
                                              const double *__restrict__ pt, const double *__restrict__ uw,
                                              const double *__restrict__ vw, const double *__restrict__ heat,
                                              const double *__restrict__ plk, const double *__restrict__ nf,
@@ -151,8 +151,6 @@ __global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo 
     // four around its cell in registers), the pt face values in the pt pass
     __shared__ R sFX[RFX::n], sFY[RFY::n];
 
-    int bx, by, K;
-    xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
     const int I0 = ATM_HALO + bx * TX, J0 = ATM_HALO + by * TY;
     const size_t kb0 = TBASE(K);
 #define G(a, x, y) (a + kb0)[TOFF(x, y)]
@@ -303,12 +301,25 @@ __global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo 
 #undef LP
 }
 
+template <int VAR, bool CGRID, class R>
+__global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo g, DynCoef c, const double *__restrict__ dp,
+                                             const double *__restrict__ pt, const double *__restrict__ uw,
+                                             const double *__restrict__ vw, const double *__restrict__ heat,
+                                             const double *__restrict__ plk, const double *__restrict__ nf,
+                                             double *__restrict__ dpo, double *__restrict__ pto,
+                                             double *__restrict__ uco, double *__restrict__ vco,
+                                             double *__restrict__ ua, double *__restrict__ va, int *nfw) {
+    int bx, by, K;
+    xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
+    pair_body<VAR, CGRID, R>(bx, by, K, g, c, dp, pt, uw, vw, heat, plk, nf, dpo, pto, uco, vco, ua, va, nfw);
+}
+
 // ---------------------------------------------------------------------------
 // C2: kinetic energy, vorticity, pressure gradient and the uc/vc update of
 // one level (cgrid 294-414).
 // ---------------------------------------------------------------------------
 template <class R>
-__global__ __launch_bounds__(NT_WIND, 8) void k_c2(Geo g, DynCoef c, const double *__restrict__ uc1,
+__device__ __forceinline__ void c2_body(int bx, int by, int K, Geo g, DynCoef c, const double *__restrict__ uc1,
                                            const double *__restrict__ vc1, const double *__restrict__ ud,
                                            const double *__restrict__ vd, const double *__restrict__ pkap,
                                            const double *__restrict__ gz, const double *__restrict__ ua,
@@ -324,8 +335,6 @@ __global__ __launch_bounds__(NT_WIND, 8) void k_c2(Geo g, DynCoef c, const doubl
     using RTW = Reg<-2, TX + 3, 0, TY>;
     using RLV = Reg<0, TX + 1, 0, TY + 1>;
     __shared__ R sU[RB::n], sV[RB::n], sW[RW::n], sK[RK::n], sTV[RTV::n], sTW[RTW::n], sL[RLV::n];
-    int bx, by, K;
-    xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
     const int I0 = ATM_HALO + bx * TX, J0 = ATM_HALO + by * TY;
     const size_t kb0 = TBASE(K);
 #define G(a, x, y) (a + kb0)[TOFF(x, y)]
@@ -422,6 +431,18 @@ __global__ __launch_bounds__(NT_WIND, 8) void k_c2(Geo g, DynCoef c, const doubl
 #undef W
 }
 
+template <class R>
+__global__ __launch_bounds__(NT_WIND, 8) void k_c2(Geo g, DynCoef c, const double *__restrict__ uc1,
+                                           const double *__restrict__ vc1, const double *__restrict__ ud,
+                                           const double *__restrict__ vd, const double *__restrict__ pkap,
+                                           const double *__restrict__ gz, const double *__restrict__ ua,
+                                           const double *__restrict__ va, double *__restrict__ uco,
+                                           double *__restrict__ vco) {
+    int bx, by, K;
+    xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
+    c2_body<R>(bx, by, K, g, c, uc1, vc1, ud, vd, pkap, gz, ua, va, uco, vco);
+}
+
 // ---------------------------------------------------------------------------
 // D2: kinetic energy, vorticity, divergence damping, B-grid pressure
 // gradient and the ud/vd update of one level (dgrid 606-771).
@@ -432,7 +453,7 @@ __global__ __launch_bounds__(NT_WIND, 8) void k_c2(Geo g, DynCoef c, const doubl
 // of arrays that are dead or not yet written at that stage).
 // ---------------------------------------------------------------------------
 template <class R>
-__global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *__restrict__ ud,
+__device__ __forceinline__ void d2_body(int bx, int by, int K, Geo g, DynCoef c, const double *__restrict__ ud,
                                            const double *__restrict__ vd, const double *__restrict__ uc,
                                            const double *__restrict__ vc, const double *__restrict__ pkapb,
                                            const double *__restrict__ gzb, double *__restrict__ udo,
@@ -464,8 +485,6 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
     // the vorticity fluxes' where U and V were (dead after the kinetic energy)
     static_assert(RQU::n + RQV::n <= RTU::n + RTW::n && RQTU::n <= RU::n && RQTW::n <= RV::n, "dq storage");
     R *const sQU = sT, *const sQV = sT + RQU::n, *const sQTU = sU, *const sQTW = sV;
-    int bx, by, K;
-    xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
     const int I0 = ATM_HALO + bx * TX, J0 = ATM_HALO + by * TY;
     const size_t kb0 = TBASE(K);
 #define G(a, x, y) (a + kb0)[TOFF(x, y)]
@@ -601,6 +620,17 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
 #undef D2
 }
 
+template <class R>
+__global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *__restrict__ ud,
+                                           const double *__restrict__ vd, const double *__restrict__ uc,
+                                           const double *__restrict__ vc, const double *__restrict__ pkapb,
+                                           const double *__restrict__ gzb, double *__restrict__ udo,
+                                           double *__restrict__ vdo, int *nfw) {
+    int bx, by, K;
+    xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
+    d2_body<R>(bx, by, K, g, c, ud, vd, uc, vc, pkapb, gzb, udo, vdo, nfw);
+}
+
 // B-grid corner values of pkap and gz for the D step (dgrid 606-632): for
 // every level k, pkapb(k) = corners(pkap(k)) and gzb(k+1) = corners(gz(k+1))
 // on the interior plus one row and column (what the pressure gradient of
@@ -614,7 +644,7 @@ __global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *
 // LDS stages; fewer levels per block give small grids more blocks.  NLEV
 // is the context's tuning (atm_autotune); every choice gives the same values.
 template <int NLEV>
-__global__ void __launch_bounds__(NT_WIND) k_bcorners(Geo g, const double *__restrict__ pkap,
+__device__ __forceinline__ void bcorners_body(int bxi, int byi, int bzi, Geo g, const double *__restrict__ pkap,
                                                  const double *__restrict__ gz, double *__restrict__ pkapb,
                                                  double *__restrict__ gzb) {
     constexpr int TY = TY_WIND, NT = NT_WIND;
@@ -623,8 +653,8 @@ __global__ void __launch_bounds__(NT_WIND) k_bcorners(Geo g, const double *__res
     using RAY = Reg<-2, TX + 1, 0, TY>;      // y averages
     constexpr int NR = (RQ::n + NT - 1) / NT;  // staged points per thread and field
     __shared__ double sQ[2][RQ::n], sAX[2][RAX::n], sAY[2][RAY::n];
-    const int I0 = ATM_HALO + blockIdx.x * TX, J0 = ATM_HALO + blockIdx.y * TY;
-    const int K0 = blockIdx.z * NLEV, K1 = min(K0 + NLEV, g.L);
+    const int I0 = ATM_HALO + bxi * TX, J0 = ATM_HALO + byi * TY;
+    const int K0 = bzi * NLEV, K1 = min(K0 + NLEV, g.L);
     // this thread's staged points: plane offsets, or -1 past the halo'd
     // plane (the last tile row / column holds one valid output row / column
     // and does not use them)
@@ -683,6 +713,13 @@ __global__ void __launch_bounds__(NT_WIND) k_bcorners(Geo g, const double *__res
             gzb[g.ix(I, J, K + 1)] = out[1];
         }
     }
+}
+
+template <int NLEV>
+__global__ void __launch_bounds__(NT_WIND) k_bcorners(Geo g, const double *__restrict__ pkap,
+                                                 const double *__restrict__ gz, double *__restrict__ pkapb,
+                                                 double *__restrict__ gzb) {
+    bcorners_body<NLEV>((int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, g, pkap, gz, pkapb, gzb);
 }
 
 // k_bcorners with n levels per block (1, 2, 4 or 8)
@@ -816,7 +853,7 @@ __global__ void __launch_bounds__(64) k_scan_pg(Geo g, double ptop, double pktop
 constexpr int SC_COLS = 16, SC_PARTS = 4;
 
 template <int CMAX>
-__global__ void __launch_bounds__(64, 4) k_scan_cols(Geo g, int i0, int j0, int ni, double ptop, double pktop,
+__device__ __forceinline__ void scan_cols_body(int bxs, int bys, Geo g, int i0, int j0, int ni, double ptop, double pktop,
                                                   double gravity, const double *__restrict__ src,
                                                   const double *__restrict__ th, const double *__restrict__ nf,
                                                   double *__restrict__ plev, double *__restrict__ pkap,
@@ -824,7 +861,7 @@ __global__ void __launch_bounds__(64, 4) k_scan_cols(Geo g, int i0, int j0, int 
                                                   double *__restrict__ plk_out) {
     static_assert(SC_COLS * SC_PARTS == 64, "one wavefront");
     const int lane = threadIdx.x, q = lane / SC_COLS;
-    const int I = i0 + blockIdx.x * SC_COLS + lane % SC_COLS, J = j0 + blockIdx.y;
+    const int I = i0 + bxs * SC_COLS + lane % SC_COLS, J = j0 + bys;
     const bool col = I < i0 + ni;  // lanes past the last column still pass the sums on
     const int C = (g.L + SC_PARTS - 1) / SC_PARTS;
     const int k0 = min(q * C, g.L), n = min(C, g.L - k0);
@@ -920,6 +957,16 @@ __global__ void __launch_bounds__(64, 4) k_scan_cols(Geo g, int i0, int j0, int 
         if (u < n && col) gz[xb + (size_t)(u + 1) * pl] = tz[u];
 }
 
+template <int CMAX>
+__global__ void __launch_bounds__(64, 4) k_scan_cols(Geo g, int i0, int j0, int ni, double ptop, double pktop,
+                                                  double gravity, const double *__restrict__ src,
+                                                  const double *__restrict__ th, const double *__restrict__ nf,
+                                                  double *__restrict__ plev, double *__restrict__ pkap,
+                                                  double *__restrict__ gz, double *__restrict__ heat,
+                                                  double *__restrict__ plk_out) {
+    scan_cols_body<CMAX>((int)blockIdx.x, (int)blockIdx.y, g, i0, j0, ni, ptop, pktop, gravity, src, th, nf, plev, pkap, gz, heat, plk_out);
+}
+
 constexpr int SC_CMAX = 16;  // k_scan_cols for nlay <= 64
 
 // the scans of the columns (i0.., j0..) x ni x nj: k_scan_cols when nlay
@@ -939,9 +986,158 @@ void launch_scans(atm_ctx *c, int i0, int j0, int ni, int nj, double gravity, co
     }
 }
 
+// ---------------------------------------------------------------------------
+// Batched ensemble stages: each kernel runs one stage for every member of an
+// ensemble in one launch (the member table, ens_find in atm_fused.hpp); a
+// workgroup does what the member's own launch does for the same tile, with
+// the same body, so every member's values are those of its own launches.
+// par holds each member's current copy of the double-buffered fields (bit m).
+// ---------------------------------------------------------------------------
+constexpr int ENS_CORNER_LEVELS = 4;  // corner kernel levels per workgroup (every choice gives the same values)
+
+__device__ __forceinline__ unsigned tiles_d(int n, int t) { return (unsigned)((n + t - 1) / t); }
+
+// the member's coefficients for a step of dt (as atm_fused_cgrid/dgrid set them)
+__device__ __forceinline__ DynCoef ens_coef(const EnsMember &e, double dt) {
+    DynCoef c = e.coef;
+    c.dt = dt;
+    c.dtdx = dt * c.rdx;
+    c.dtdy = dt * c.rdy;
+    return c;
+}
+
+// C1 (cgrid) or D1 of every member
+template <int VAR, bool CGRID, class R>
+__global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair_ens(const EnsMember *__restrict__ tab,
+                                                                              EnsIndex ix, int nm,
+                                                                              unsigned long long par, double dt) {
+    int m;
+    unsigned l;
+    if (!ens_find(ix, nm, m, l)) return;
+    const EnsMember &e = tab[m];
+    const int tx = (int)tiles_d(e.g.nx, TX), ty = (int)tiles_d(e.g.ny, TY_PAIR);
+    const int bx = (int)(l % tx), by = (int)((l / tx) % ty), K = (int)(l / tx / ty);
+    const int p = (int)((par >> m) & 1ull);
+    if constexpr (CGRID)
+        pair_body<VAR, true, R>(bx, by, K, e.g, ens_coef(e, dt), e.p[0][p], e.p[1][p], e.p[2][p], e.p[3][p],
+                                nullptr, e.plk, e.net_flux, e.dpc, e.ptc, e.uc, e.vc, e.ua, e.va, nullptr);
+    else
+        pair_body<VAR, false, R>(bx, by, K, e.g, ens_coef(e, dt), e.p[0][p], e.p[1][p], e.uc2, e.vc2, e.heatd,
+                                 nullptr, nullptr, e.p[0][1 - p], e.p[1][1 - p], nullptr, nullptr, nullptr, nullptr,
+                                 e.nfw);
+}
+
+template <class R>
+__global__ __launch_bounds__(NT_WIND, 8) void k_c2_ens(const EnsMember *__restrict__ tab, EnsIndex ix, int nm,
+                                                      unsigned long long par, double dt) {
+    int m;
+    unsigned l;
+    if (!ens_find(ix, nm, m, l)) return;
+    const EnsMember &e = tab[m];
+    const int tx = (int)tiles_d(e.g.nx, TX), ty = (int)tiles_d(e.g.ny, TY_WIND);
+    const int bx = (int)(l % tx), by = (int)((l / tx) % ty), K = (int)(l / tx / ty);
+    const int p = (int)((par >> m) & 1ull);
+    c2_body<R>(bx, by, K, e.g, ens_coef(e, dt), e.uc, e.vc, e.p[2][p], e.p[3][p], e.pkap, e.gz, e.ua, e.va, e.uc2,
+               e.vc2);
+}
+
+template <class R>
+__global__ __launch_bounds__(NT_WIND) void k_d2_ens(const EnsMember *__restrict__ tab, EnsIndex ix, int nm,
+                                                   unsigned long long par, double dt) {
+    int m;
+    unsigned l;
+    if (!ens_find(ix, nm, m, l)) return;
+    const EnsMember &e = tab[m];
+    const int tx = (int)tiles_d(e.g.nx, TX), ty = (int)tiles_d(e.g.ny, TY_WIND);
+    const int bx = (int)(l % tx), by = (int)((l / tx) % ty), K = (int)(l / tx / ty);
+    const int p = (int)((par >> m) & 1ull);
+    d2_body<R>(bx, by, K, e.g, ens_coef(e, dt), e.p[2][p], e.p[3][p], e.uc2, e.vc2, e.pkapb, e.gzb, e.p[2][1 - p],
+               e.p[3][1 - p], e.nfw);
+}
+
+__global__ void __launch_bounds__(NT_WIND) k_bcorners_ens(const EnsMember *__restrict__ tab, EnsIndex ix, int nm) {
+    int m;
+    unsigned l;
+    if (!ens_find(ix, nm, m, l)) return;
+    const EnsMember &e = tab[m];
+    const int tx = (e.g.nx + TX) / TX, ty = (e.g.ny + TY_WIND) / TY_WIND;
+    bcorners_body<ENS_CORNER_LEVELS>((int)(l % tx), (int)((l / tx) % ty), (int)(l / tx / ty), e.g, e.pkap, e.gz,
+                                     e.pkapb, e.gzb);
+}
+
+// the column scans after C1 (pkap, gz, heating rate) or after D1 (plev, pkap,
+// gz and the next C step's playkap), over all columns of every member
+template <bool DSTEP>
+__global__ void __launch_bounds__(64, 4) k_scan_ens(const EnsMember *__restrict__ tab, EnsIndex ix, int nm,
+                                                   unsigned long long par) {
+    int m;
+    unsigned l;
+    if (!ens_find(ix, nm, m, l)) return;
+    const EnsMember &e = tab[m];
+    const int tx = (int)tiles_d(e.g.NI, SC_COLS);
+    const int bx = (int)(l % tx), by = (int)(l / tx);
+    if constexpr (DSTEP) {
+        const int q = 1 - (int)((par >> m) & 1ull);  // the copy D1 wrote
+        scan_cols_body<SC_CMAX>(bx, by, e.g, 0, 0, e.g.NI, e.ak0, e.pktop, e.coef.gravity, e.p[0][q], e.p[1][q],
+                                e.net_flux, e.plev, e.pkap, e.gz, nullptr, e.plk);
+    } else {
+        scan_cols_body<SC_CMAX>(bx, by, e.g, 0, 0, e.g.NI, e.ak0, e.pktop, e.coef.gravity, e.dpc, e.ptc,
+                                e.net_flux, nullptr, e.pkap, e.gz, e.heatd, nullptr);
+    }
+}
+
 #undef D
 
 }  // namespace
+
+static unsigned tiles_h(int n, int t) { return (unsigned)((n + t - 1) / t); }
+unsigned atm_ens_blocks_pair(const Geo &g) { return tiles_h(g.nx, TX) * tiles_h(g.ny, TY_PAIR) * (unsigned)g.L; }
+unsigned atm_ens_blocks_wind(const Geo &g) { return tiles_h(g.nx, TX) * tiles_h(g.ny, TY_WIND) * (unsigned)g.L; }
+unsigned atm_ens_blocks_corner(const Geo &g) {
+    return (unsigned)((g.nx + TX) / TX) * (unsigned)((g.ny + TY_WIND) / TY_WIND) * tiles_h(g.L, ENS_CORNER_LEVELS);
+}
+unsigned atm_ens_blocks_scan(const Geo &g) { return tiles_h(g.NI, SC_COLS) * (unsigned)g.NJ; }
+bool atm_ens_scan_fits(const Geo &g) { return g.L <= SC_PARTS * SC_CMAX; }
+
+void atm_ens_launch_pair(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned long long par,
+                         double dt, bool cgrid, bool mixed, unsigned n) {
+    const dim3 grid(n), blk(NT_PAIR);
+    if (cgrid) {
+        if (mixed)
+            hipLaunchKernelGGL((k_pair_ens<V_PPM_U, true, float>), grid, blk, 0, s, tab, ix, nm, par, dt);
+        else
+            hipLaunchKernelGGL((k_pair_ens<V_PPM_U, true, double>), grid, blk, 0, s, tab, ix, nm, par, dt);
+    } else {
+        if (mixed)
+            hipLaunchKernelGGL((k_pair_ens<V_PPM_NN, false, float>), grid, blk, 0, s, tab, ix, nm, par, dt);
+        else
+            hipLaunchKernelGGL((k_pair_ens<V_PPM_NN, false, double>), grid, blk, 0, s, tab, ix, nm, par, dt);
+    }
+}
+void atm_ens_launch_c2(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned long long par,
+                       double dt, bool mixed, unsigned n) {
+    if (mixed)
+        hipLaunchKernelGGL(k_c2_ens<float>, dim3(n), dim3(NT_WIND), 0, s, tab, ix, nm, par, dt);
+    else
+        hipLaunchKernelGGL(k_c2_ens<double>, dim3(n), dim3(NT_WIND), 0, s, tab, ix, nm, par, dt);
+}
+void atm_ens_launch_d2(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned long long par,
+                       double dt, bool mixed, unsigned n) {
+    if (mixed)
+        hipLaunchKernelGGL(k_d2_ens<float>, dim3(n), dim3(NT_WIND), 0, s, tab, ix, nm, par, dt);
+    else
+        hipLaunchKernelGGL(k_d2_ens<double>, dim3(n), dim3(NT_WIND), 0, s, tab, ix, nm, par, dt);
+}
+void atm_ens_launch_scan(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned long long par,
+                         bool dstep, unsigned n) {
+    if (dstep)
+        hipLaunchKernelGGL(k_scan_ens<true>, dim3(n), dim3(64), 0, s, tab, ix, nm, par);
+    else
+        hipLaunchKernelGGL(k_scan_ens<false>, dim3(n), dim3(64), 0, s, tab, ix, nm, par);
+}
+void atm_ens_launch_corners(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned n) {
+    hipLaunchKernelGGL(k_bcorners_ens, dim3(n), dim3(NT_WIND), 0, s, tab, ix, nm);
+}
 
 // The tuning of the fused path: each candidate of a kernel's launch shape
 // is timed on this context's own grid and stream (one warm-up launch, then
@@ -990,6 +1186,19 @@ int atm_fused_autotune(atm_ctx *c, int reps) {
 // tiles of t cells over n cells (the last may overhang)
 static unsigned tiles(int n, int t) { return (unsigned)((n + t - 1) / t); }
 
+// playkap of the current dp for the C step's heating rate, when dp changed
+// since the last D step's scan left it (a remap, a put)
+int atm_fused_plk(atm_ctx *c) {
+    const Geo &g = c->g;
+    atm_timer_begin(c, ATM_T_SCAN);
+    launch_scans(c, ATM_HALO, ATM_HALO, g.nx, g.ny, c->coef.gravity, c->pbuf[0][c->cur], nullptr, nullptr, nullptr,
+                 nullptr, nullptr, nullptr, c->plk);
+    atm_timer_end(c, ATM_T_SCAN);
+    c->plk_valid = true;
+    ATM_HIP(c, hipGetLastError());
+    return ATM_OK;
+}
+
 int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
     const Geo &g = c->g;
     c->coef.dt = dt;
@@ -997,11 +1206,8 @@ int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
     c->coef.dtdy = dt * c->coef.rdy;
     const DynCoef &k = c->coef;
     if (!c->plk_valid) {
-        atm_timer_begin(c, ATM_T_SCAN);
-        launch_scans(c, ATM_HALO, ATM_HALO, g.nx, g.ny, k.gravity, b.dp, nullptr, nullptr, nullptr, nullptr, nullptr,
-                     nullptr, c->plk);
-        atm_timer_end(c, ATM_T_SCAN);
-        c->plk_valid = true;
+        const int rc = atm_fused_plk(c);
+        if (rc) return rc;
     }
     dim3 grid(tiles(g.nx, TX), tiles(g.ny, TY_PAIR), g.L), blk(NT_PAIR);
     dim3 gridw(tiles(g.nx, TX), tiles(g.ny, TY_WIND), g.L), blkw(NT_WIND);

@@ -18,16 +18,12 @@ struct HaloSet {
     int nk[ATM_HALO_MAXF];
 };
 
-__global__ void k_halo_fill(Geo g, HaloSet set) {
-    const int f = blockIdx.z;
-    double *__restrict__ a = set.a[f];
-    const int nk = set.nk[f];
+// halo cell t of level K of a (nk levels)
+__device__ __forceinline__ void halo_cell(const Geo &g, double *__restrict__ a, int nk, int t, int K) {
     const int H = ATM_HALO;
     const int side = H * g.NJ;       // per side strip
     const int tb = g.nx * H;         // per top/bottom strip
     const int per = 2 * side + 2 * tb;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int K = blockIdx.y;
     if (t >= per || K >= nk) return;
     int I, J;
     if (t < 2 * side) {
@@ -46,6 +42,40 @@ __global__ void k_halo_fill(Geo g, HaloSet set) {
     int Is = I < H ? I + g.nx : (I >= H + g.nx ? I - g.nx : I);
     int Js = J < H ? J + g.ny : (J >= H + g.ny ? J - g.ny : J);
     a[g.ix(I, J, K)] = a[g.ix(Is, Js, K)];
+}
+
+__global__ void k_halo_fill(Geo g, HaloSet set) {
+    const int f = blockIdx.z;
+    halo_cell(g, set.a[f], set.nk[f], blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
+// The halo fills of a batched ensemble stage (atm_fused.hpp): the fields of
+// `kind` for every member, each as the member's own fill launch does them.
+__device__ __forceinline__ unsigned ens_halo_strip_blocks(const Geo &g) {
+    return (unsigned)((2 * ATM_HALO * g.NJ + 2 * g.nx * ATM_HALO + 255) / 256);
+}
+__global__ void __launch_bounds__(256) k_halo_ens(const EnsMember *__restrict__ tab, EnsIndex ix, int nm,
+                                                 unsigned long long par, int kind) {
+    int m;
+    unsigned l;
+    if (!ens_find(ix, nm, m, l)) return;
+    const EnsMember &e = tab[m];
+    const unsigned nb = ens_halo_strip_blocks(e.g);
+    const int bx = (int)(l % nb), K = (int)((l / nb) % (unsigned)e.g.L), f = (int)(l / nb / (unsigned)e.g.L);
+    const int p = (int)((par >> m) & 1ull), L = e.g.L;
+    double *a;
+    int nk = L;
+    if (kind == ENS_HALO_C1) {  // after C1: dpc, ptc, uc, vc and the last level's A winds
+        a = f == 0 ? e.dpc : f == 1 ? e.ptc : f == 2 ? e.uc : f == 3 ? e.vc : f == 4 ? e.ua : e.va;
+        if (f >= 4) nk = 1;
+    } else if (kind == ENS_HALO_CH) {  // cgrid_halo_exchange: the updated C winds
+        a = f == 0 ? e.uc2 : e.vc2;
+    } else if (kind == ENS_HALO_D1) {  // after D1: the copy of dp, pt it wrote
+        a = e.p[f][1 - p];
+    } else {  // dgrid_halo_exchange: the current dp, pt, ud, vd
+        a = e.p[f][p];
+    }
+    halo_cell(e.g, a, nk, bx * 256 + (int)threadIdx.x, K);
 }
 
 // check_stability (mod_fields.f90:69-106): flag bit set when any interior
@@ -116,6 +146,16 @@ int atm_launch_halo_multi(atm_ctx *c, double *const *a, const int *nk, int n) {
     hipLaunchKernelGGL(k_halo_fill, gr, b, 0, c->stream, g, set);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;
+}
+
+unsigned atm_ens_blocks_halo(const Geo &g, int kind) {
+    const int nf = kind == ENS_HALO_C1 ? 6 : kind == ENS_HALO_DH ? 4 : 2;
+    return (unsigned)((2 * ATM_HALO * g.NJ + 2 * g.nx * ATM_HALO + 255) / 256) * (unsigned)g.L * (unsigned)nf;
+}
+
+void atm_ens_launch_halo(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned long long par,
+                         int kind, unsigned n) {
+    hipLaunchKernelGGL(k_halo_ens, dim3(n), dim3(256), 0, s, tab, ix, nm, par, kind);
 }
 
 int atm_launch_halo(atm_ctx *c, double *a, int nk) {

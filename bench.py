@@ -136,7 +136,9 @@ def ensemble_timing(args):
     streams drain every step), then with deferred checks (the value: each
     member's check is queued behind its step and read after the next step is
     issued), then with each member's 10-step period replayed from a HIP graph
-    (radiation eager at the period start, one deferred check per period)."""
+    (radiation eager at the period start, one deferred check per period);
+    then a second ensemble of the same members whose dynamics stages run as
+    one launch per stage for all members (atm_ensemble_*, deferred checks)."""
     from atmos.ensemble import Ensemble
     ens = Ensemble(ENSEMBLE_SPECS, args.dx, args.dt, remap_every=5, radiation_every=10, precision="mixed",
                    nstreams=args.ensemble_streams, autotune=args.autotune == "on", checks="sync")
@@ -182,6 +184,21 @@ def ensemble_timing(args):
            "corner_levels_per_workgroup": {f"{nx}x{ny}x{L}": sorted({mb.model.tuning for mb in ens.members
                                                                      if mb.cells == nx * ny * L})
                                            for _, nx, ny, L in ENSEMBLE_SPECS}}
+    ens.close()
+    ens = Ensemble(ENSEMBLE_SPECS, args.dx, args.dt, remap_every=5, radiation_every=10, precision="mixed",
+                   launch="batched")
+    for _ in range(10):
+        if not ens.step():
+            raise RuntimeError("ensemble member unstable in warm-up")
+    if not ens.finish():
+        raise RuntimeError("ensemble member unstable in warm-up")
+    ens.sync()
+    sec_b = timed(ens.step, n_def)
+    out["batched"] = {"steps": n_def, "value": cells * n_def / sec_b, "ms_per_step": sec_b / n_def * 1e3,
+                      "launch": "one launch per dynamics stage for all 32 members (C1, halo, scans, C2, "
+                                "C halo, D1, halo, scans, corners, D2, D halo) on the first member's "
+                                "stream, joined by events to the members' 4 streams, where remap and "
+                                "radiation run per member; deferred checks"}
     ens.close()
     return out
 

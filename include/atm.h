@@ -264,6 +264,28 @@ int atm_graph_end(atm_ctx *ctx);
 int atm_graph_launch(atm_ctx *ctx, int times);
 int atm_graph_free(atm_ctx *ctx);
 
+/* Batched ensemble steps (no reference counterpart; for ensembles of
+   members with their own grids).  An ensemble binds up to 64 contexts that
+   run the fused path in one precision on one device; each call below does
+   for every member what the single-context call does (atm_cgrid_step,
+   atm_cgrid_halo, atm_dgrid_step, atm_dgrid_halo: same checks, same values
+   bit for bit, same host-side state), with one kernel launch per stage for
+   all members together (a member table indexed by workgroup) on the first
+   member's stream.  Members on other streams are joined by events: each
+   call waits for the work already issued on every member's stream, and
+   every member's later work waits for the call.  Members may differ in nx,
+   ny and nlay (nlay <= 64); they share dt.  Remap, radiation and physics
+   stay per member.  Errors are reported on the member concerned
+   (atm_last_error of that context), else on the first.  The ensemble holds
+   pointers to the members' device arrays: destroy it before any member. */
+typedef struct atm_ensemble atm_ensemble;
+int atm_ensemble_create(atm_ctx *const *members, int n, atm_ensemble **out);
+int atm_ensemble_destroy(atm_ensemble *ens);
+int atm_ensemble_cgrid_step(atm_ensemble *ens, double dt);
+int atm_ensemble_cgrid_halo(atm_ensemble *ens);
+int atm_ensemble_dgrid_step(atm_ensemble *ens, double dt);
+int atm_ensemble_dgrid_halo(atm_ensemble *ens);
+
 /* Tangent-linear and adjoint of one model step (no reference counterpart;
    for gradient computations).  The step is the reference's sequence
    cgrid_dynamics_step(dt/2), cgrid_halo_exchange, dgrid_dynamics_step(dt),

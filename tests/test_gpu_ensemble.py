@@ -42,7 +42,78 @@ def test_members_match_solo_runs(gpu_available, precision):
     ens.close()
 
 
-def test_ensemble_at_the_workload_sizes(gpu_available):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_launches_match_member_launches(gpu_available, precision):
+    """One launch per dynamics stage for all members (atm_ensemble_*: a member
+    table indexed by workgroup) gives every member the bits its own launches
+    give: uneven grids, sizes that are not multiples of the tiles, levels of
+    6 to 12, three remaps and a radiation call in 12 steps."""
+    from atmos.ensemble import Ensemble
+    specs = [(1, 64, 32, 12), (2, 40, 24, 10), (1, 37, 23, 6), (1, 32, 16, 7)]
+    dx, dt, seed, steps = 100e3, 60.0, 91, 12
+    print(f"batched ensemble {precision} {specs} x{steps}", flush=True)
+    a = Ensemble(specs, dx, dt, remap_every=4, radiation_every=10, precision=precision, nstreams=2, seed=seed)
+    b = Ensemble(specs, dx, dt, remap_every=4, radiation_every=10, precision=precision, seed=seed,
+                 launch="batched")
+    for _ in range(steps):
+        assert a.step()
+        assert b.step()
+    assert a.finish() and b.finish()
+    a.sync()
+    b.sync()
+    for i, (ma, mb) in enumerate(zip(a.members, b.members)):
+        assert ma.time == mb.time
+        for name in ("dp", "pt", "ud", "vd", "net_flux", "gz", "pkap"):
+            assert np.array_equal(ma.model.get(name), mb.model.get(name)), (i, name)
+    a.close()
+    b.close()
+
+
+def test_batched_launches_refuse_mixed_members(gpu_available):
+    """The batched steps need every member on the fused path and in one
+    precision; a member that is not is named in the error, at creation or at
+    a later call, and the members' call order is checked as for one model."""
+    from atmos.config import Config
+    from atmos import driver
+    from atmos.ic import make_initial_state
+    from atmos.model import AtmError, BatchedSteps, Model, stream_create, stream_destroy
+    cfg = Config(dt_max=60.0, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12, dt_remap=300.0,
+                 radiation_on=False, physics_on=False)
+    s = stream_create()
+
+    def member(prec="fp64"):
+        m = Model(32, 16, 6, 100e3, precision=prec)
+        m.set_stream(s)
+        driver.init_model(m, make_initial_state(32, 16, 6, 100e3), cfg)
+        return m
+
+    m0, m1 = member(), member("mixed")
+    with pytest.raises(AtmError, match="one precision"):
+        BatchedSteps([m0, m1])
+    m1.close()
+    m = member()
+    m.set_path("reference")
+    with pytest.raises(AtmError, match="fused path"):
+        BatchedSteps([m0, m])
+    m.set_path("fused")
+    bs = BatchedSteps([m0, m])
+    bs.cgrid_dynamics_step(30.0)
+    bs.cgrid_halo_exchange()
+    bs.dgrid_dynamics_step(60.0)
+    with pytest.raises(AtmError, match="must follow"):  # the order of the calls, checked per member
+        bs.cgrid_dynamics_step(30.0)
+    bs.dgrid_halo_exchange()
+    m.set_path("reference")  # a member that leaves the fused path later
+    with pytest.raises(AtmError, match="fused path"):
+        bs.cgrid_dynamics_step(30.0)
+    bs.close()
+    m.close()
+    m0.close()
+    stream_destroy(s)
+
+
+@pytest.mark.parametrize("launch", ["streams", "batched"])
+def test_ensemble_at_the_workload_sizes(gpu_available, launch):
     """The uneven ensemble as the workload states it: 24 members at
     192x192x48, 6 at 256x256x48, 2 at 384x384x64, mixed precision, radiation
     every 10 steps, remap every 5, seeded T noise per member; 10 steps.
@@ -55,9 +126,9 @@ def test_ensemble_at_the_workload_sizes(gpu_available):
     from atmos.model import Model
     specs = [(24, 192, 192, 48), (6, 256, 256, 48), (2, 384, 384, 64)]
     dx, dt, seed, steps = 50e3, 40.0, 500, 10
-    print(f"ensemble at workload sizes {specs} x{steps}", flush=True)
+    print(f"ensemble at workload sizes {specs} x{steps}, {launch}", flush=True)
     ens = Ensemble(specs, dx, dt, remap_every=5, radiation_every=10, precision="mixed", nstreams=4, seed=seed,
-                   t_noise=0.5)
+                   t_noise=0.5, launch=launch)
     assert len(ens.members) == 32
     mass0 = [mb.model.get("dp").sum() for mb in ens.members]
     for _ in range(steps):
