@@ -55,12 +55,18 @@ struct Scr {
 struct WindCol {
     const double *plev, *tplev, *v;
     size_t base, s, plane;
-    __device__ double avg(const double *a, int k) const {
+    struct XR {  // the four values a face coordinate averages
+        double a0, am, ap, am2;
+    };
+    __device__ XR raw(const double *a, int k) const {
         const size_t o = base + (size_t)(k - 1) * plane;
-        return ATM_W7_12 * (a[o] + a[o - s]) - ATM_W1_12 * (a[o + s] + a[o - 2 * s]);
+        return {a[o], a[o - s], a[o + s], a[o - 2 * s]};
     }
-    __device__ double X(int k) const { return avg(plev, k); }
-    __device__ double D(int k) const { return avg(tplev, k); }
+    __device__ double xv(const XR &r) const { return ATM_W7_12 * (r.a0 + r.am) - ATM_W1_12 * (r.ap + r.am2); }
+    __device__ XR xr(int k) const { return raw(plev, k); }
+    __device__ XR dr(int k) const { return raw(tplev, k); }
+    __device__ double X(int k) const { return xv(xr(k)); }
+    __device__ double D(int k) const { return xv(dr(k)); }
     __device__ double V(int k) const { return v[base + (size_t)(k - 1) * plane]; }
 };
 
@@ -69,9 +75,13 @@ struct WindCol {
 struct EnergyCol {
     const double *plev, *tplev, *var;
     size_t base, plane, ncol;
+    using XR = double;
     __device__ double X(int k) const { return plev[base + (size_t)(k - 1) * plane]; }
     __device__ double D(int k) const { return tplev[base + (size_t)(k - 1) * plane]; }
     __device__ double V(int k) const { return var[(size_t)k * ncol]; }
+    __device__ XR xr(int k) const { return X(k); }
+    __device__ XR dr(int k) const { return D(k); }
+    __device__ double xv(XR r) const { return r; }
 };
 
 // Logical block of a 1-D launch, consecutive logical blocks kept on one XCD

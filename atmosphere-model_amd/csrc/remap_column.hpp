@@ -6,7 +6,11 @@
 // A column source P provides X(k) (source face coordinates, k = 1..L+1),
 // D(k) (destination faces) and V(k) (layer values, k = 1..L); lv and gm
 // are strided (ns) scratch arrays of L+2 entries for the eliminated edge
-// values and gam.
+// values and gam.  Face coordinates also come in two halves: xr(k) / dr(k)
+// issue the loads a coordinate is formed from (type P::XR) and xv() forms
+// it, so both passes load the next level's values one level before they
+// use them: the loads are in flight while the current level is computed
+// instead of stalling the column's serial chain every level.
 #pragma once
 
 #include "atm_common.hpp"
@@ -16,6 +20,10 @@ template <class P>
 __host__ __device__ void ppm_forward(const P &p, int L, double *lv, double *gm, size_t ns) {
     double xp = p.X(L + 1), x0 = p.X(L), xm = p.X(L - 1);
     double v0 = p.V(L), vm = p.V(L - 1);
+    // level k - 2 of the next iteration, loaded one iteration ahead
+    int kn = L - 2 >= 1 ? L - 2 : 1;
+    typename P::XR xn = p.xr(kn);
+    double vn = p.V(kn);
     double alpha = (x0 - xm) / (xp - x0);
     double beta = alpha * (.5 + alpha);
     double lvn = (2. * alpha * (1. + alpha) * v0 + vm) / beta;  // level_var(L+1)
@@ -35,9 +43,12 @@ __host__ __device__ void ppm_forward(const P &p, int L, double *lv, double *gm, 
         if (k > 2) {
             xp = x0;
             x0 = xm;
-            xm = p.X(k - 2);
+            xm = p.xv(xn);  // X(k - 2)
             v0 = vm;
-            vm = p.V(k - 2);
+            vm = vn;        // V(k - 2)
+            kn = k - 3 >= 1 ? k - 3 : 1;
+            xn = p.xr(kn);
+            vn = p.V(kn);
         }
     }
     // alpha of k = 2; v0 = V(2), vm = V(1), lvn = level_var(2), g = gam(1)
@@ -59,10 +70,17 @@ __host__ __device__ void ppm_merge(const P &p, int L, const double *lv, const do
     double bsN = 0., lfN = 0.;
     // current layer: btm, top, curvature, faces
     double b = 0., t = 0., cv = 0., sk = p.X(1), sk1 = p.X(2);
+    // what the next advance (to layer k + 1) reads, loaded one layer ahead:
+    // gam(k+3), var(k+2), X(k+2), level_var(k+2); indices past the column
+    // are clamped into it and their values replaced as G and V do
+    auto cl = [&](int i) { return i <= L + 1 ? i : L + 1; };
+    double ng = gm[(size_t)cl(4) * ns], nv = p.V(3 <= L ? 3 : L), nl = lv[(size_t)cl(3) * ns];
+    typename P::XR nx = p.xr(cl(3));
 
-    auto layer = [&]() {
+    // lvj = level_var(k+1) after the elimination sweep
+    auto layer = [&](double lvj) {
         const int j = k + 1;
-        const double bs = lv[(size_t)j * ns] - gC * bsC;  // back substitution (313-315)
+        const double bs = lvj - gC * bsC;  // back substitution (313-315)
         double l = bs;
         if (j == L) {  // 316-317
             l = dmin2(l, dmax2(vD, vC));
@@ -104,21 +122,28 @@ __host__ __device__ void ppm_merge(const P &p, int L, const double *lv, const do
         gB = gC;
         gC = gD;
         gD = gE;
-        gE = G(k + 2);
+        gE = k + 2 <= L ? ng : 0.;         // G(k + 2)
         vC = vD;
-        vD = k + 1 <= L ? p.V(k + 1) : 0.;
+        vD = k + 1 <= L ? nv : 0.;         // V(k + 1)
         bsC = bsN;
         lfC = lfN;
         sk = sk1;
-        sk1 = p.X(k + 1);
-        layer();
+        sk1 = p.xv(nx);                    // X(k + 1)
+        const double lvj = nl;             // level_var(k + 1)
+        ng = gm[(size_t)cl(k + 3) * ns];
+        nv = p.V(k + 2 <= L ? k + 2 : L);
+        nl = lv[(size_t)cl(k + 2) * ns];
+        nx = p.xr(cl(k + 2));
+        layer(lvj);
     };
 
-    layer();
+    layer(lv[(size_t)2 * ns]);
     double dm1 = p.D(1);
+    typename P::XR nd = p.dr(2);  // D(m + 1), loaded one layer ahead
     for (int m = 1; m <= L; m++) {
         const double dm0 = dm1;
-        dm1 = p.D(m + 1);
+        dm1 = p.xv(nd);
+        nd = p.dr(cl(m + 2));
         while (k <= L && sk1 >= dm0) advance();
         double acc = 0.;
         if (k <= L && sk > dm1) {

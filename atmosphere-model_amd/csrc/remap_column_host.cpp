@@ -15,6 +15,10 @@ struct ArrayCol {
     double X(int k) const { return x[k - 1]; }
     double D(int k) const { return d[k - 1]; }
     double V(int k) const { return v[k - 1]; }
+    using XR = double;
+    XR xr(int k) const { return X(k); }
+    XR dr(int k) const { return D(k); }
+    double xv(XR r) const { return r; }
 };
 }  // namespace
 
