@@ -30,7 +30,11 @@ members together (atm_ensemble_*: a member table indexed by workgroup), on
 the first member's stream, joined to the other members' streams by events;
 radiation and the remap stay per member, on the members' own streams, where
 they overlap.  The members share the time step.  Every member's values are
-those of its own launches, bit for bit.
+those of its own launches, bit for bit.  With a graph period as well, the
+batched launches and the members' remaps of a whole period are recorded as
+one graph (atm_ensemble_graph_*) and replayed for every later period, under
+the same rules as the members' own graphs; the recorded time steps and
+remap pattern are checked against each later period's.
 """
 from __future__ import annotations
 
@@ -71,10 +75,9 @@ class Ensemble:
             raise ValueError("checks: 'deferred' or 'sync'")
         if launch not in ("streams", "batched"):
             raise ValueError("launch: 'streams' or 'batched'")
-        if launch == "batched" and graph_period:
-            raise ValueError("batched launches are not recorded into graphs")
         self.launch = launch
         self.batch = None
+        self.batch_pattern = None  # batched graph mode: (dt, remap flags) of the recorded period
         self.remap_every, self.radiation_every = remap_every, radiation_every
         self.checks = checks
         self.graph_period = 0
@@ -169,6 +172,7 @@ class Ensemble:
         schedule: radiation due now or not within the period)."""
         self._validate_period(period)
         self.graph_period = period
+        self.batch_pattern = None  # a batched ensemble records afresh
 
     def period(self) -> bool:
         """Graph mode: graph_period steps of every member (the first call
@@ -176,8 +180,11 @@ class Ensemble:
         after the previous period."""
         if not self.graph_period:
             raise RuntimeError("period() needs graph_period")
-        for mb in self.members:
-            self._member_period(mb)
+        if self.batch is not None:
+            self._batched_period()
+        else:
+            for mb in self.members:
+                self._member_period(mb)
         return self._check()
 
     def finish(self) -> bool:
@@ -240,6 +247,57 @@ class Ensemble:
             raise RuntimeError("the remap pattern of this period differs from the recorded one")
         m.graph_launch(1)
         mb.time = t
+
+    def _batched_period(self):
+        """graph_period steps of every member as one recording of batched
+        and member calls (recorded by the first call, replayed after), with
+        radiation issued eagerly at the period start."""
+        for mb in self.members:
+            m, st, cfg = mb.model, mb.state, mb.config
+            if st.calculate_radiation_this_step:  # due at the period start
+                if cfg.radiation_on:
+                    m.calculate_radiative_heating(mb.time)
+                    m.radiation_halo_exchange()
+                st.calculate_radiation_this_step = False
+        record = self.batch_pattern is None
+        b = self.batch
+        if record:
+            for mb in self.members:  # a recording starts with no check queued
+                if mb.pending:
+                    mb.pending = False
+                    if not mb.model.check_stability_end():
+                        raise FloatingPointError("ensemble member unstable before the graph was recorded")
+            b.graph_begin()
+        times, pattern = [mb.time for mb in self.members], []
+        for k in range(self.graph_period):
+            dts, remaps = [], []
+            for i, mb in enumerate(self.members):
+                dts.append(driver.calculate_dt(times[i], mb.config, mb.state))
+                remaps.append(mb.state.remap_this_step)
+                mb.state.remap_this_step = False
+            dt = dts[0]
+            if any(d != dt for d in dts):
+                raise RuntimeError("batched members must share the time step")
+            if record:
+                b.cgrid_dynamics_step(dt / 2.0)
+                b.cgrid_halo_exchange()
+                b.dgrid_dynamics_step(dt)
+                for mb, remap in zip(self.members, remaps):
+                    if remap:
+                        mb.model.perform_vertical_remapping()
+                b.dgrid_halo_exchange()
+            pattern.append((dt, tuple(remaps)))
+            times = [t + dt for t in times]
+            if k + 1 < self.graph_period and any(mb.state.calculate_radiation_this_step for mb in self.members):
+                raise RuntimeError("radiation fell due inside a graph period")
+        if record:
+            b.graph_end()
+            self.batch_pattern = pattern
+        elif pattern != self.batch_pattern:
+            raise RuntimeError("the time steps or remap pattern of this period differ from the recorded ones")
+        b.graph_launch(1)
+        for mb, t in zip(self.members, times):
+            mb.time = t
 
     def sync(self):
         for mb in self.members:

@@ -36,7 +36,8 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_twostream", "atm_twostream_noalloc", "atm_twostream_last_error", "atm_stream_synchronize", "atm_stream_timer_begin",
            "atm_stream_timer_end", "atm_poison", "atm_check_stability_begin", "atm_check_stability_end",
            "atm_ensemble_create", "atm_ensemble_destroy", "atm_ensemble_cgrid_step", "atm_ensemble_cgrid_halo",
-           "atm_ensemble_dgrid_step", "atm_ensemble_dgrid_halo"]
+           "atm_ensemble_dgrid_step", "atm_ensemble_dgrid_halo", "atm_ensemble_graph_begin",
+           "atm_ensemble_graph_end", "atm_ensemble_graph_launch"]
 PATHS = {"fused": 0, "reference": 1}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 TIMERS = {"c1": 0, "c2": 1, "d1": 2, "d2": 3, "scan": 4, "halo": 5, "remap": 6, "radiation": 7,
@@ -111,7 +112,9 @@ def load_library(path: str = LIB_PATH):
     L.atm_get_tuning.argtypes = [vp, C.POINTER(C.c_int)]
     L.atm_set_tuning.argtypes = [vp, ip]
     L.atm_ensemble_create.argtypes = [C.POINTER(vp), ip, C.POINTER(vp)]
-    for n in ["atm_ensemble_destroy", "atm_ensemble_cgrid_halo", "atm_ensemble_dgrid_halo"]:
+    L.atm_ensemble_graph_launch.argtypes = [vp, ip]
+    for n in ["atm_ensemble_destroy", "atm_ensemble_cgrid_halo", "atm_ensemble_dgrid_halo",
+              "atm_ensemble_graph_begin", "atm_ensemble_graph_end"]:
         getattr(L, n).argtypes = [vp]
     for n in ["atm_ensemble_cgrid_step", "atm_ensemble_dgrid_step"]:
         getattr(L, n).argtypes = [vp, C.c_double]
@@ -171,6 +174,17 @@ class BatchedSteps:
 
     def dgrid_halo_exchange(self):
         self._ck(self._L.atm_ensemble_dgrid_halo(self._h))
+
+    def graph_begin(self):
+        """Record the following batched and member step calls into one HIP
+        graph instead of running them."""
+        self._ck(self._L.atm_ensemble_graph_begin(self._h))
+
+    def graph_end(self):
+        self._ck(self._L.atm_ensemble_graph_end(self._h))
+
+    def graph_launch(self, times=1):
+        self._ck(self._L.atm_ensemble_graph_launch(self._h, int(times)))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -751,18 +751,17 @@ int atm_sync(atm_ctx *c) {
 // back where it started when the recording ends, so that every launch
 // finds the buffers the recording used: whole steps, and on the fused path
 // an even number of buffer switches (each D step and each remap switch).
-int atm_graph_begin(atm_ctx *c) {
-    CHECK_CTX(c);
-    CHECK_NOT_CAPTURING(c);
+// whether a recording may start on this context: at a step boundary, with
+// no deferred check pending
+static int cap_ready(atm_ctx *c) {
     if (!c->sw_init) return atm_fail(c, ATM_ERR_STATE, "atm_init_sw_dyn first");
     if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "a graph starts at a step boundary");
     if (c->chk_pending) return atm_fail(c, ATM_ERR_STATE, "a deferred check is pending: atm_check_stability_end first");
-    ATM_HIP(c, hipStreamSynchronize(c->stream));
-    if (c->gexec) {
-        ATM_HIP(c, hipGraphExecDestroy(c->gexec));
-        c->gexec = nullptr;
-    }
-    ATM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    return ATM_OK;
+}
+
+// the host-side state of a context when its calls start being recorded
+static void cap_enter(atm_ctx *c) {
     c->cap_cur = c->cur;
     c->cap_timing = c->timing;
     c->timing = false;  // timer events are not recorded into the graph
@@ -776,8 +775,67 @@ int atm_graph_begin(atm_ctx *c) {
     c->cap_nfw_tracked = c->nfw_tracked;
     c->nfw_tracked = true;
     c->cap_dts.clear();
-    c->graph_dts.clear();
     c->capturing = true;
+}
+
+// What the recorded calls changed on the host side, for each launch to
+// apply; the context goes back to its state at cap_enter (nothing recorded
+// has run).  Returns whether the recording holds whole steps with an even
+// number of buffer switches.
+static bool cap_leave(atm_ctx *c, GraphEffects &fx) {
+    c->capturing = false;
+    c->timing = c->cap_timing;
+    const bool whole = !c->after_d && c->cur == c->cap_cur;
+    fx.cur = c->cap_cur;
+    fx.plk_end = c->plk_valid;
+    fx.tracks = c->nfw_tracked;
+    fx.dts.clear();
+    fx.dts.swap(c->cap_dts);
+    c->after_d = false;
+    if (c->cur != c->cap_cur) {
+        c->cur = c->cap_cur;
+        atm_point_current(c);
+    }
+    c->plk_valid = c->cap_plk_valid;
+    c->nfw_tracked = c->cap_nfw_tracked;
+    return whole;
+}
+
+// whether a recording made from this context's state may launch now
+static int fx_ready(atm_ctx *c, const GraphEffects &fx) {
+    if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "a graph launches at a step boundary");
+    if (c->cur != fx.cur)
+        return atm_fail(c, ATM_ERR_STATE,
+                        "the current copy of the fields is not the one the graph was recorded on "
+                        "(an odd number of eager D steps + remaps since)");
+    return ATM_OK;
+}
+
+// the host-side effects of one launch of the recorded steps, as if run
+// eagerly: accumulation time in the recorded order, playkap's validity, and
+// the stability tracking (lost when a recorded write does not report)
+static void fx_apply(atm_ctx *c, const GraphEffects &fx) {
+    for (double dt : fx.dts) c->acc_time = c->acc_time + dt;
+    c->plk_valid = fx.plk_end;
+    if (!fx.tracks) atm_untrack(c);
+}
+
+static const char *const ATM_GRAPH_WHOLE =
+    "a graph must hold whole steps with an even number of buffer switches (D steps + remaps)";
+
+int atm_graph_begin(atm_ctx *c) {
+    CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
+    int rc = cap_ready(c);
+    if (rc) return rc;
+    ATM_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->gexec) {
+        ATM_HIP(c, hipGraphExecDestroy(c->gexec));
+        c->gexec = nullptr;
+    }
+    ATM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    cap_enter(c);
+    c->gfx.dts.clear();
     return ATM_OK;
 }
 
@@ -786,28 +844,12 @@ int atm_graph_end(atm_ctx *c) {
     if (!c->capturing) return atm_fail(c, ATM_ERR_STATE, "atm_graph_begin first");
     hipGraph_t gr = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &gr);
-    c->capturing = false;
-    c->timing = c->cap_timing;
-    // Nothing recorded has run: the context goes back to its state at
-    // atm_graph_begin, and what the recorded steps change on the host side
-    // is kept for atm_graph_launch to apply.
-    const bool whole = !c->after_d && c->cur == c->cap_cur;
-    c->graph_plk_end = c->plk_valid;
-    c->graph_tracks = c->nfw_tracked;
-    c->after_d = false;
-    if (c->cur != c->cap_cur) {
-        c->cur = c->cap_cur;
-        atm_point_current(c);
-    }
-    c->plk_valid = c->cap_plk_valid;
-    c->nfw_tracked = c->cap_nfw_tracked;
-    std::vector<double> dts;
-    dts.swap(c->cap_dts);
+    GraphEffects fx;
+    const bool whole = cap_leave(c, fx);
     if (e != hipSuccess || !gr) return atm_fail(c, ATM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     if (!whole) {
         (void)hipGraphDestroy(gr);
-        return atm_fail(c, ATM_ERR_STATE,
-                        "a graph must hold whole steps with an even number of buffer switches (D steps + remaps)");
+        return atm_fail(c, ATM_ERR_STATE, ATM_GRAPH_WHOLE);
     }
     const hipError_t ei = hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0);
     (void)hipGraphDestroy(gr);
@@ -815,7 +857,7 @@ int atm_graph_end(atm_ctx *c) {
         c->gexec = nullptr;
         return atm_fail(c, ATM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
     }
-    c->graph_dts.swap(dts);
+    c->gfx = std::move(fx);
     return ATM_OK;
 }
 
@@ -826,19 +868,11 @@ int atm_graph_launch(atm_ctx *c, int times) {
     if (times < 0) return atm_fail(c, ATM_ERR_ARGUMENT, "times must be >= 0");
     // the recording reads and writes the buffers that were current when it
     // began, at a step boundary
-    if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "a graph launches at a step boundary");
-    if (c->cur != c->cap_cur)
-        return atm_fail(c, ATM_ERR_STATE,
-                        "the current copy of the fields is not the one the graph was recorded on "
-                        "(an odd number of eager D steps + remaps since)");
+    int rc = fx_ready(c, c->gfx);
+    if (rc) return rc;
     for (int i = 0; i < times; i++) {
         ATM_HIP(c, hipGraphLaunch(c->gexec, c->stream));
-        // the host-side effects of the recorded steps, as if run eagerly:
-        // accumulation time in the recorded order, playkap's validity, and
-        // the stability tracking (lost when a recorded write does not report)
-        for (double dt : c->graph_dts) c->acc_time = c->acc_time + dt;
-        c->plk_valid = c->graph_plk_end;
-        if (!c->graph_tracks) atm_untrack(c);
+        fx_apply(c, c->gfx);
     }
     return ATM_OK;
 }
@@ -874,6 +908,11 @@ struct atm_ensemble {
     // and afterwards makes those wait for it (events, one per stream).
     std::vector<hipEvent_t> ev;          // n + 1
     std::vector<hipStream_t> others;     // the other member streams of the current call
+    // a recording of batched and member calls (atm_ensemble_graph_*): one
+    // graph on the launch stream, with every member's host-side effects
+    bool capturing = false;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<GraphEffects> fx;
 };
 
 // the launch stream waits for the work already issued on every other member stream
@@ -904,7 +943,7 @@ static int ens_fork(atm_ensemble *e) {
 static unsigned round8(unsigned long long n) { return (unsigned)((n + 7ull) / 8ull * 8ull); }
 
 // every member on the fused path, in one precision, on one device, with
-// its dynamics initialised, not recording a graph
+// its dynamics initialised, not recording a graph of its own
 static int ens_usable(atm_ensemble *e) {
     atm_ctx *c0 = e->m[0];
     for (atm_ctx *c : e->m) {
@@ -912,9 +951,28 @@ static int ens_usable(atm_ensemble *e) {
         if (!c->fused) return atm_fail(c, ATM_ERR_STATE, "ensemble: every member on the fused path");
         if (c->mixed != c0->mixed) return atm_fail(c, ATM_ERR_STATE, "ensemble: every member in one precision");
         if (c->device != c0->device) return atm_fail(c, ATM_ERR_STATE, "ensemble: every member on one device");
-        if (c->capturing) return atm_fail(c, ATM_ERR_STATE, "ensemble: not while recording a graph");
+        if (c->capturing != e->capturing) return atm_fail(c, ATM_ERR_STATE, "ensemble: not while recording a graph");
     }
     return ATM_OK;
+}
+
+// the distinct member streams other than the launch stream
+static std::vector<hipStream_t> ens_streams(atm_ensemble *e) {
+    std::vector<hipStream_t> v;
+    for (atm_ctx *c : e->m)
+        if (c->stream != e->m[0]->stream && std::find(v.begin(), v.end(), c->stream) == v.end()) v.push_back(c->stream);
+    return v;
+}
+
+// ends a recording that will not be used; the members go back to their
+// state at its start
+static void ens_discard(atm_ensemble *e) {
+    hipGraph_t gr = nullptr;
+    if (hipStreamEndCapture(e->m[0]->stream, &gr) == hipSuccess && gr) (void)hipGraphDestroy(gr);
+    GraphEffects fx;
+    for (atm_ctx *c : e->m)
+        if (c->capturing) cap_leave(c, fx);
+    e->capturing = false;
 }
 
 // bit m: member m's current copy of dp, pt, ud, vd
@@ -999,7 +1057,9 @@ int atm_ensemble_create(atm_ctx *const *members, int n, atm_ensemble **out) {
 
 int atm_ensemble_destroy(atm_ensemble *e) {
     if (!e) return ATM_OK;
+    if (e->capturing) ens_discard(e);
     for (atm_ctx *c : e->m) (void)hipStreamSynchronize(c->stream);
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     for (hipEvent_t ev : e->ev) (void)hipEventDestroy(ev);
     if (e->tab) (void)hipFree(e->tab);
     delete e;
@@ -1111,6 +1171,97 @@ int atm_ensemble_dgrid_halo(atm_ensemble *e) {
     atm_timer_end(c0, ATM_T_HALO);
     ATM_HIP(c0, hipGetLastError());
     for (atm_ctx *c : e->m) c->after_d = false;
+    return ens_fork(e);
+}
+
+// Recording of batched steps: the launch stream starts a capture and the
+// other member streams join it (they wait for an event recorded in it), so
+// the batched launches and each member's own calls (remap, playkap) land in
+// one graph; at the end every member stream is joined back.  Each member's
+// host-side state is handled as atm_graph_begin / end handle one context's.
+int atm_ensemble_graph_begin(atm_ensemble *e) {
+    if (!e) return ATM_ERR_ARGUMENT;
+    if (e->capturing) return atm_fail(e->m[0], ATM_ERR_STATE, "ensemble: already recording");
+    int rc = ens_usable(e);
+    if (rc) return rc;
+    for (atm_ctx *c : e->m) {
+        rc = cap_ready(c);
+        if (rc) return rc;
+    }
+    atm_ctx *c0 = e->m[0];
+    for (atm_ctx *c : e->m) ATM_HIP(c, hipStreamSynchronize(c->stream));
+    if (e->gexec) {
+        ATM_HIP(c0, hipGraphExecDestroy(e->gexec));
+        e->gexec = nullptr;
+    }
+    ATM_HIP(c0, hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
+    const std::vector<hipStream_t> others = ens_streams(e);
+    hipError_t err = others.empty() ? hipSuccess : hipEventRecord(e->ev.back(), c0->stream);
+    for (size_t i = 0; err == hipSuccess && i < others.size(); i++) err = hipStreamWaitEvent(others[i], e->ev.back(), 0);
+    for (atm_ctx *c : e->m) cap_enter(c);
+    e->capturing = true;
+    if (err != hipSuccess) {
+        ens_discard(e);
+        return atm_fail(c0, ATM_ERR_HIP, std::string("ensemble graph: ") + hipGetErrorString(err));
+    }
+    return ATM_OK;
+}
+
+int atm_ensemble_graph_end(atm_ensemble *e) {
+    if (!e) return ATM_ERR_ARGUMENT;
+    atm_ctx *c0 = e->m[0];
+    if (!e->capturing) return atm_fail(c0, ATM_ERR_STATE, "atm_ensemble_graph_begin first");
+    const std::vector<hipStream_t> others = ens_streams(e);
+    hipError_t err = hipSuccess;
+    for (size_t i = 0; err == hipSuccess && i < others.size(); i++) {
+        err = hipEventRecord(e->ev[i], others[i]);
+        if (err == hipSuccess) err = hipStreamWaitEvent(c0->stream, e->ev[i], 0);
+    }
+    hipGraph_t gr = nullptr;
+    const hipError_t ee = hipStreamEndCapture(c0->stream, &gr);
+    e->capturing = false;
+    std::vector<GraphEffects> fx(e->m.size());
+    atm_ctx *broken = nullptr;
+    for (size_t i = 0; i < e->m.size(); i++)
+        if (!cap_leave(e->m[i], fx[i]) && !broken) broken = e->m[i];
+    if (err == hipSuccess) err = ee;
+    if (err != hipSuccess || !gr) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return atm_fail(c0, ATM_ERR_HIP, std::string("ensemble graph: ") + hipGetErrorString(err));
+    }
+    if (broken) {
+        (void)hipGraphDestroy(gr);
+        return atm_fail(broken, ATM_ERR_STATE, ATM_GRAPH_WHOLE);
+    }
+    const hipError_t ei = hipGraphInstantiate(&e->gexec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (ei != hipSuccess) {
+        e->gexec = nullptr;
+        return atm_fail(c0, ATM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    e->fx.swap(fx);
+    return ATM_OK;
+}
+
+// Each launch follows the work already issued on every member stream and
+// precedes what is issued there afterwards.
+int atm_ensemble_graph_launch(atm_ensemble *e, int times) {
+    if (!e) return ATM_ERR_ARGUMENT;
+    atm_ctx *c0 = e->m[0];
+    if (!e->gexec) return atm_fail(c0, ATM_ERR_STATE, "no recorded ensemble graph");
+    if (times < 0) return atm_fail(c0, ATM_ERR_ARGUMENT, "times must be >= 0");
+    int rc = ens_usable(e);
+    if (rc) return rc;
+    for (size_t i = 0; i < e->m.size(); i++) {
+        rc = fx_ready(e->m[i], e->fx[i]);
+        if (rc) return rc;
+    }
+    rc = ens_join(e);
+    if (rc) return rc;
+    for (int t = 0; t < times; t++) {
+        ATM_HIP(c0, hipGraphLaunch(e->gexec, c0->stream));
+        for (size_t i = 0; i < e->m.size(); i++) fx_apply(e->m[i], e->fx[i]);
+    }
     return ens_fork(e);
 }
 

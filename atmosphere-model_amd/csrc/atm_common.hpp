@@ -307,6 +307,17 @@ struct AtmTimer {
     int cap = 0, used = 0;
 };
 
+// What one launch of a recorded graph changes on a context's host side:
+// the buffer parity it was recorded on (and leaves), the accumulation time
+// steps in the recorded order, playkap's validity after the recorded steps
+// and whether every recorded write of dp, pt, ud, vd reports into the
+// stability word.
+struct GraphEffects {
+    int cur = 0;
+    std::vector<double> dts;
+    bool plk_end = false, tracks = false;
+};
+
 struct atm_ctx {
     Geo g;
     double dx, dy;
@@ -377,19 +388,17 @@ struct atm_ctx {
     RadWork *rad = nullptr;
     PhysWork *phys = nullptr;
     LinWork *lin = nullptr;  // tangent-linear / adjoint step (atm_lin_init)
-    // HIP graph of a recorded step sequence (atm_graph_begin / end): while
-    // capturing, the entry points that wait on the device or write memory
-    // from the host refuse; cap_cur is the buffer parity at the start.
-    // Host-side state that recorded calls would advance is kept aside while
-    // recording and applied by each launch instead: the accumulation time
-    // steps (cap_dts -> graph_dts), playkap's validity after the recorded
-    // steps (graph_plk_end) and whether every recorded write of dp, pt, ud,
-    // vd reports into the stability word (graph_tracks).
+    // HIP graph of a recorded step sequence (atm_graph_begin / end, or a
+    // batched ensemble's recording): while capturing, the entry points that
+    // wait on the device or write memory from the host refuse; cap_cur is
+    // the buffer parity at the start.  Host-side state that recorded calls
+    // would advance is kept aside while recording (cap_dts collects the
+    // accumulation time steps) and applied by each launch instead (gfx).
     bool capturing = false;
     int cap_cur = 0;
     bool cap_timing = false, cap_plk_valid = false, cap_nfw_tracked = false;
-    std::vector<double> cap_dts, graph_dts;
-    bool graph_plk_end = false, graph_tracks = false;
+    std::vector<double> cap_dts;
+    GraphEffects gfx;
     hipGraphExec_t gexec = nullptr;
     // time-mean output accumulators (nx, ny, nlay+1, 6) and their time
     double *outf = nullptr;

@@ -138,7 +138,9 @@ def ensemble_timing(args):
     issued), then with each member's 10-step period replayed from a HIP graph
     (radiation eager at the period start, one deferred check per period);
     then a second ensemble of the same members whose dynamics stages run as
-    one launch per stage for all members (atm_ensemble_*, deferred checks)."""
+    one launch per stage for all members (atm_ensemble_*, deferred checks),
+    eagerly and then with each 10-step period replayed from one graph of
+    the batched launches and the members' remaps."""
     from atmos.ensemble import Ensemble
     ens = Ensemble(ENSEMBLE_SPECS, args.dx, args.dt, remap_every=5, radiation_every=10, precision="mixed",
                    nstreams=args.ensemble_streams, autotune=args.autotune == "on", checks="sync")
@@ -199,6 +201,16 @@ def ensemble_timing(args):
                                 "C halo, D1, halo, scans, corners, D2, D halo) on the first member's "
                                 "stream, joined by events to the members' 4 streams, where remap and "
                                 "radiation run per member; deferred checks"}
+    ens.use_graphs(10)
+    if not ens.period() or not ens.finish():  # records the period's graph, runs the first period
+        raise RuntimeError("ensemble member unstable")
+    ens.sync()
+    sec_bg = timed(ens.period, n_per)
+    out["batched"]["graph"] = {"period_steps": 10, "periods": n_per, "value": cells * 10 * n_per / sec_bg,
+                               "ms_per_step": sec_bg / (10 * n_per) * 1e3,
+                               "launch": "the batched launches and the members' remaps of a 10-step period "
+                                         "as one graph on the first member's stream (radiation eager at "
+                                         "the period start, one deferred check per member per period)"}
     ens.close()
     return out
 

@@ -286,6 +286,23 @@ int atm_ensemble_cgrid_halo(atm_ensemble *ens);
 int atm_ensemble_dgrid_step(atm_ensemble *ens, double dt);
 int atm_ensemble_dgrid_halo(atm_ensemble *ens);
 
+/* Graphs of batched steps (no reference counterpart).  Between
+   atm_ensemble_graph_begin and atm_ensemble_graph_end the batched calls
+   above and the members' own step calls (atm_remap, ...) are recorded into
+   one HIP graph instead of run: the first member's stream starts the
+   recording and the other member streams join it.  The rules of
+   atm_graph_begin / end / launch hold for every member (whole steps, an
+   even number of buffer switches, no deferred check pending; each launch
+   applies every member's recorded host-side effects).  Each
+   atm_ensemble_graph_launch follows the work already issued on every
+   member's stream and precedes what is issued there afterwards.  One
+   recording per ensemble; a new atm_ensemble_graph_begin replaces it, and
+   atm_ensemble_destroy frees it.  Members may not record graphs of their
+   own meanwhile. */
+int atm_ensemble_graph_begin(atm_ensemble *ens);
+int atm_ensemble_graph_end(atm_ensemble *ens);
+int atm_ensemble_graph_launch(atm_ensemble *ens, int times);
+
 /* Tangent-linear and adjoint of one model step (no reference counterpart;
    for gradient computations).  The step is the reference's sequence
    cgrid_dynamics_step(dt/2), cgrid_halo_exchange, dgrid_dynamics_step(dt),

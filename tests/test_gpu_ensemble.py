@@ -69,6 +69,39 @@ def test_batched_launches_match_member_launches(gpu_available, precision):
     b.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_graph_periods_match_batched_launches(gpu_available, precision):
+    """Periods of batched steps recorded once as one graph (batched launches,
+    the members' remaps, the stream joins) and replayed give every member
+    the bits of the same steps issued eagerly; a replay after an odd number
+    of eager buffer switches is refused."""
+    from atmos.ensemble import Ensemble
+    from atmos.model import AtmError
+    specs = [(1, 64, 32, 12), (2, 40, 24, 10), (1, 37, 23, 6)]
+    dx, dt, seed, period, periods = 100e3, 60.0, 93, 10, 3
+    print(f"batched graph ensemble {precision} {specs} {periods}x{period}", flush=True)
+    a = Ensemble(specs, dx, dt, remap_every=5, radiation_every=10, precision=precision, nstreams=2, seed=seed,
+                 launch="batched")
+    b = Ensemble(specs, dx, dt, remap_every=5, radiation_every=10, precision=precision, nstreams=2, seed=seed,
+                 launch="batched", graph_period=period)
+    for _ in range(period * periods):
+        assert a.step()
+    for _ in range(periods):
+        assert b.period()
+    assert a.finish() and b.finish()
+    a.sync()
+    b.sync()
+    for i, (ma, mb) in enumerate(zip(a.members, b.members)):
+        assert ma.time == mb.time
+        for name in ("dp", "pt", "ud", "vd", "net_flux", "gz", "pkap"):
+            assert np.array_equal(ma.model.get(name), mb.model.get(name)), (i, name)
+    b._batched_step()  # one D step: the other copy of the fields is current
+    with pytest.raises(AtmError, match="current copy"):
+        b.batch.graph_launch(1)
+    a.close()
+    b.close()
+
+
 def test_batched_launches_refuse_mixed_members(gpu_available):
     """The batched steps need every member on the fused path and in one
     precision; a member that is not is named in the error, at creation or at
