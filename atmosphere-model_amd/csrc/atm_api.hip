@@ -674,12 +674,73 @@ int atm_get_path(atm_ctx *c, int *path) {
     return ATM_OK;
 }
 
+// One fused step (C step, its halo fill, D step) from the current state
+// into the other copy of the fields, with the host-side state left as it
+// was: the next real step reads only the current copy and rewrites every
+// array the trial wrote before reading it (playkap, which the trial's D
+// step leaves for the other copy, is marked stale by the caller).
+static int trial_step(atm_ctx *c, double dt) {
+    FusedBufs cur = bufs_of(c, c->cur), nxt = bufs_of(c, 1 - c->cur);
+    int rc = atm_fused_cgrid(c, cur, .5 * dt);
+    if (!rc) rc = atm_fused_cgrid_halo(c);
+    if (!rc) rc = atm_fused_dgrid(c, cur, nxt, dt);
+    return rc;
+}
+
+// The tuning of the fused path: each candidate launch shape is timed
+// inside whole trial steps on this context's grid, state and stream (one
+// warm-up step, then `reps` steps between two events), so that a shape's
+// effect on its neighbours in the step (the corner planes D2 reads back)
+// counts; the fastest is kept.  Only shapes that give the same values are
+// candidates.  Today: the corner kernel's levels per block (1, 2, 4, 8).
 int atm_autotune(atm_ctx *c, int reps) {
     CHECK_CTX(c);
     CHECK_NOT_CAPTURING(c);
     if (reps < 1) return atm_fail(c, ATM_ERR_ARGUMENT, "reps must be >= 1");
     if (!c->fused) return ATM_OK;
-    return atm_fused_autotune(c, reps);
+    if (!c->sw_init) return atm_fail(c, ATM_ERR_STATE, "atm_init_sw_dyn first");
+    if (c->after_d) return atm_fail(c, ATM_ERR_STATE, "the autotuner runs at a step boundary");
+    // the last step's dt, or a Courant number of about 0.2 at 100 m/s
+    const DynCoef coef = c->coef;
+    const double dt = coef.dt > 0. ? coef.dt : .2 * std::min(c->dx, c->dy) / 100.;
+    const bool timing = c->timing;
+    const int before = c->bc_levels;
+    c->timing = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    int best = c->bc_levels, rc = ATM_OK;
+    float best_ms = 0.f;
+    for (int n : {1, 2, 4, 8}) {
+        if (e != hipSuccess || rc) break;
+        c->bc_levels = n;
+        rc = trial_step(c, dt);
+        if (!rc) e = hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps && !rc && e == hipSuccess; r++) rc = trial_step(c, dt);
+        if (!rc && e == hipSuccess) e = hipEventRecord(e1, c->stream);
+        if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (!rc && e == hipSuccess && (n == 1 || ms < best_ms)) {
+            best = n;
+            best_ms = ms;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    c->coef = coef;
+    c->timing = timing;
+    c->plk_valid = false;
+    if (rc) {
+        c->bc_levels = before;
+        return rc;
+    }
+    if (e != hipSuccess) {
+        c->bc_levels = before;
+        return atm_fail(c, ATM_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+    }
+    c->bc_levels = best;
+    return ATM_OK;
 }
 
 int atm_get_tuning(atm_ctx *c, int *corner_levels) {

@@ -26,7 +26,6 @@ struct FusedBufs {
 int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt);
 int atm_fused_cgrid_halo(atm_ctx *c);
 int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt);
-int atm_fused_autotune(atm_ctx *c, int reps);
 int atm_fused_plk(atm_ctx *c);  // playkap of the current dp (the C step needs it valid)
 
 // ---------------------------------------------------------------------------

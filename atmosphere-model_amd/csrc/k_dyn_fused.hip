@@ -1139,46 +1139,6 @@ void atm_ens_launch_corners(hipStream_t s, const EnsMember *tab, const EnsIndex 
     hipLaunchKernelGGL(k_bcorners_ens, dim3(n), dim3(NT_WIND), 0, s, tab, ix, nm);
 }
 
-// The tuning of the fused path: each candidate of a kernel's launch shape
-// is timed on this context's own grid and stream (one warm-up launch, then
-// `reps` launches between two events), and the fastest is kept.  Only
-// shapes that give the same values are candidates, so tuning never changes
-// results.  Today: the corner kernel's levels per block (1, 2, 4, 8).
-int atm_fused_autotune(atm_ctx *c, int reps) {
-    hipEvent_t e0, e1;
-    ATM_HIP(c, hipEventCreate(&e0));
-    if (hipEventCreate(&e1) != hipSuccess) {
-        (void)hipEventDestroy(e0);
-        return atm_fail(c, ATM_ERR_HIP, "hipEventCreate");
-    }
-    int best = c->bc_levels, rc = ATM_OK;
-    float best_ms = 0.f;
-    for (int n : {1, 2, 4, 8}) {
-        launch_bcorners(c, n);
-        hipError_t e = hipEventRecord(e0, c->stream);
-        for (int r = 0; r < reps && e == hipSuccess; r++) {
-            launch_bcorners(c, n);
-            e = hipGetLastError();
-        }
-        if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
-        float ms = 0.f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-        if (e != hipSuccess) {
-            rc = atm_fail(c, ATM_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
-            break;
-        }
-        if (n == 1 || ms < best_ms) {
-            best = n;
-            best_ms = ms;
-        }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (rc == ATM_OK) c->bc_levels = best;
-    return rc;
-}
-
 // ---------------------------------------------------------------------------
 // Step orchestration of the fused path.  Buffers: prognostic fields have two
 // copies (cur, nxt); the C step writes its own arrays; the D step writes nxt.

@@ -225,16 +225,18 @@ enum { ATM_POISON_PADDING = 1, ATM_POISON_OUTER_HALO = 2 };
 int atm_poison(atm_ctx *ctx, int what, double value);
 
 /* Launch-shape tuning (no counterpart in the reference): atm_autotune times
-   each candidate launch shape of the fused path's tunable kernels on this
-   context's grid and stream (one warm-up launch, then `reps` timed ones)
-   and keeps the fastest; it blocks.  Candidates differ only in how the work
-   is divided, never in the values computed.  Today one knob: the corner
-   kernel's levels per workgroup, 1, 2, 4 or 8 (default 4), which
-   atm_get_tuning reads and atm_set_tuning sets directly.  On the
-   reference-structured path these return ATM_OK and change nothing.
-   The candidates are timed in isolation (the same launch repeated), which
-   can rank them differently from their cost inside a step: at 512x512x64
-   it picks 1 level, while a whole-step comparison favours the default 4. */
+   each candidate launch shape of the fused path's tunable kernels inside
+   whole trial steps (C step, C halo fill, D step) from the context's
+   current state into the other copy of its fields, on its grid and stream
+   (one warm-up step, then `reps` timed ones), and keeps the fastest; it
+   blocks.  The trial steps leave the model where it was: the next real
+   step gives the bits it gives without them.  Candidates differ only in
+   how the work is divided, never in the values computed.  Today one knob:
+   the corner kernel's tile depth, 1, 2, 4 or 8 levels per workgroup
+   (default 4), which atm_get_tuning reads and atm_set_tuning sets
+   directly.  Needs atm_init_sw_dyn and a step boundary (ATM_ERR_STATE
+   otherwise).  On the reference-structured path these return ATM_OK and
+   change nothing. */
 int atm_autotune(atm_ctx *ctx, int reps);
 int atm_get_tuning(atm_ctx *ctx, int *corner_levels);
 int atm_set_tuning(atm_ctx *ctx, int corner_levels);

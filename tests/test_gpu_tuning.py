@@ -1,8 +1,9 @@
 """Launch-shape tuning (atm_autotune / atm_get_tuning / atm_set_tuning): every
 candidate shape of the corner kernel (levels per workgroup 1, 2, 4, 8) gives
 the same bits after steps with remaps, in both precisions and at a level
-count that no group size divides; the autotuner picks one of them; bad
-arguments are refused."""
+count that no group size divides; the autotuner's trial steps leave the run
+unchanged; it picks one of the candidates; bad arguments and calls out of
+order are refused."""
 import numpy as np
 import pytest
 
@@ -11,7 +12,7 @@ pytestmark = pytest.mark.gpu
 DT = 60.0
 
 
-def _run(precision, corner_levels, steps=7):
+def _run(precision, corner_levels, steps=7, tune_after=None):
     from atmos import driver
     from atmos.config import Config
     from atmos.ic import make_initial_state
@@ -25,8 +26,18 @@ def _run(precision, corner_levels, steps=7):
     cfg = Config(dt_max=DT, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12, dt_remap=3 * DT,
                  radiation_on=False, physics_on=False)
     st = driver.init_model(m, ic, cfg)
-    _, n, stable = driver.run_model(m, cfg, st, max_steps=steps)
-    assert stable and n == steps
+    if tune_after is None:
+        _, n, stable = driver.run_model(m, cfg, st, max_steps=steps)
+        assert stable and n == steps
+    else:  # trial steps of the autotuner in the middle of the run
+        t = cfg.t_initial
+        for _ in range(tune_after):
+            t += driver.step(m, cfg, st, t)
+        m.autotune(reps=1)
+        m.set_tuning(corner_levels)
+        for _ in range(steps - tune_after):
+            t += driver.step(m, cfg, st, t)
+        assert m.check_stability()
     return {name: m.get_full(name) for name in ("dp", "pt", "ud", "vd")}
 
 
@@ -40,10 +51,32 @@ def test_corner_levels_do_not_change_results(gpu_available, precision):
             assert np.array_equal(got[name], ref[name]), (n, name)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autotune_trial_steps_leave_the_run_unchanged(gpu_available, precision):
+    """The trial steps write the other copy of the fields and the step's
+    work arrays; a remap right after the next step reads the other copy's
+    wind ring, so the run goes through one (remap every 3 steps, tuned after
+    step 2)."""
+    print(f"autotune mid-run {precision}", flush=True)
+    ref = _run(precision, 4)
+    got = _run(precision, 4, tune_after=2)
+    for name in ref:
+        assert np.array_equal(got[name], ref[name]), name
+
+
 def test_autotune_choice_and_arguments(gpu_available):
+    from atmos import driver
+    from atmos.config import Config
+    from atmos.ic import make_initial_state
     from atmos.model import AtmError, Model
     m = Model(64, 32, 13, 100e3)
     assert m.tuning == 4  # default
+    with pytest.raises(AtmError) as e:  # the trial steps need the dynamics initialised
+        m.autotune(reps=2)
+    assert e.value.code == 5
+    cfg = Config(dt_max=DT, t_final=1e12, dt_output=1e12, dt_radiation=1e12, dt_physics=1e12, dt_remap=3 * DT,
+                 radiation_on=False, physics_on=False)
+    driver.init_model(m, make_initial_state(64, 32, 13, 100e3, seed=5), cfg)
     assert m.autotune(reps=2) in (1, 2, 4, 8)
     for bad in (lambda: m.autotune(reps=0), lambda: m.set_tuning(3)):
         with pytest.raises(AtmError) as e:
