@@ -32,7 +32,7 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_physics_init", "atm_physics_tendencies", "atm_apply_physics", "atm_physics_halo",
            "atm_output_init", "atm_accumulate_output", "atm_output_mean", "atm_stream_create",
            "atm_stream_destroy", "atm_graph_begin", "atm_graph_end", "atm_graph_launch", "atm_graph_free",
-           "atm_autotune", "atm_get_tuning", "atm_set_tuning", "atm_lin_init", "atm_tl_step", "atm_ad_step",
+           "atm_autotune", "atm_get_tuning", "atm_set_tuning", "atm_get_tiles", "atm_set_tiles", "atm_lin_init", "atm_tl_step", "atm_ad_step",
            "atm_twostream", "atm_twostream_noalloc", "atm_twostream_last_error", "atm_stream_synchronize", "atm_stream_timer_begin",
            "atm_stream_timer_end", "atm_poison", "atm_check_stability_begin", "atm_check_stability_end",
            "atm_ensemble_create", "atm_ensemble_destroy", "atm_ensemble_cgrid_step", "atm_ensemble_cgrid_halo",
@@ -111,6 +111,8 @@ def load_library(path: str = LIB_PATH):
     L.atm_autotune.argtypes = [vp, ip]
     L.atm_get_tuning.argtypes = [vp, C.POINTER(C.c_int)]
     L.atm_set_tuning.argtypes = [vp, ip]
+    L.atm_get_tiles.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.atm_set_tiles.argtypes = [vp, ip, ip]
     L.atm_ensemble_create.argtypes = [C.POINTER(vp), ip, C.POINTER(vp)]
     L.atm_ensemble_graph_launch.argtypes = [vp, ip]
     for n in ["atm_ensemble_destroy", "atm_ensemble_cgrid_halo", "atm_ensemble_dgrid_halo",
@@ -357,9 +359,10 @@ class Model:
         self._ck(self._L.atm_set_path(self._h, PATHS[name]))
 
     def autotune(self, reps=5):
-        """Time the fused path's candidate launch shapes on this grid and
-        keep the fastest (results do not depend on the choice); returns the
-        chosen corner-kernel levels per workgroup."""
+        """Time the fused path's candidate launch shapes inside trial steps
+        on this grid and keep the fastest (results do not depend on the
+        choice); returns the chosen corner-kernel levels per workgroup (the
+        chosen tile rows: `tiles`)."""
         self._ck(self._L.atm_autotune(self._h, int(reps)))
         return self.tuning
 
@@ -371,6 +374,16 @@ class Model:
 
     def set_tuning(self, corner_levels):
         self._ck(self._L.atm_set_tuning(self._h, int(corner_levels)))
+
+    @property
+    def tiles(self):
+        """Rows of the dp/pt tiles and of the wind tiles (32 cells wide)."""
+        a, b = C.c_int(0), C.c_int(0)
+        self._ck(self._L.atm_get_tiles(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def set_tiles(self, pair_rows, wind_rows):
+        self._ck(self._L.atm_set_tiles(self._h, int(pair_rows), int(wind_rows)))
 
     def timer_enable(self, on=True):
         self._ck(self._L.atm_timer_enable(self._h, int(bool(on))))

@@ -691,8 +691,10 @@ static int trial_step(atm_ctx *c, double dt) {
 // inside whole trial steps on this context's grid, state and stream (one
 // warm-up step, then `reps` steps between two events), so that a shape's
 // effect on its neighbours in the step (the corner planes D2 reads back)
-// counts; the fastest is kept.  Only shapes that give the same values are
-// candidates.  Today: the corner kernel's levels per block (1, 2, 4, 8).
+// counts; the fastest is kept (the faster of two interleaved timings of
+// each candidate, after a warm-up).  Only shapes that give the same values are
+// candidates: the rows of the dp/pt and of the wind tiles (8, 16) and the
+// corner kernel's levels per block (1, 2, 4, 8).
 int atm_autotune(atm_ctx *c, int reps) {
     CHECK_CTX(c);
     CHECK_NOT_CAPTURING(c);
@@ -704,42 +706,62 @@ int atm_autotune(atm_ctx *c, int reps) {
     const DynCoef coef = c->coef;
     const double dt = coef.dt > 0. ? coef.dt : .2 * std::min(c->dx, c->dy) / 100.;
     const bool timing = c->timing;
-    const int before = c->bc_levels;
+    const int before[3] = {c->tile_pair, c->tile_wind, c->bc_levels};
     c->timing = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
-    int best = c->bc_levels, rc = ATM_OK;
-    float best_ms = 0.f;
-    for (int n : {1, 2, 4, 8}) {
-        if (e != hipSuccess || rc) break;
-        c->bc_levels = n;
+    int rc = ATM_OK;
+    // the trial steps' time with the current choice
+    auto timed = [&](float &ms) {
+        ms = 0.f;
+        if (e != hipSuccess || rc) return false;
         rc = trial_step(c, dt);
         if (!rc) e = hipEventRecord(e0, c->stream);
         for (int r = 0; r < reps && !rc && e == hipSuccess; r++) rc = trial_step(c, dt);
         if (!rc && e == hipSuccess) e = hipEventRecord(e1, c->stream);
         if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
-        float ms = 0.f;
         if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-        if (!rc && e == hipSuccess && (n == 1 || ms < best_ms)) {
-            best = n;
-            best_ms = ms;
-        }
+        return !rc && e == hipSuccess;
+    };
+    // One knob at a time, the others at their current choice: the dp/pt
+    // tile rows, the wind tile rows, then the corner depth.  The GPU first
+    // runs a few trial steps (its clocks ramp up under load), then each
+    // knob's candidates are timed in two interleaved rounds and each keeps
+    // its faster time, so no candidate is timed only before the others.
+    for (int w = 0; w < 3 && !rc; w++) rc = trial_step(c, dt);
+    int *knob[] = {&c->tile_pair, &c->tile_wind, &c->bc_levels};
+    const std::vector<int> cand[] = {{8, 16}, {16, 8}, {1, 2, 4, 8}};
+    for (int q = 0; q < 3; q++) {
+        const int start = *knob[q];
+        std::vector<float> best(cand[q].size(), 0.f);
+        bool ok = true;
+        for (int round = 0; round < 2 && ok; round++)
+            for (size_t i = 0; i < cand[q].size() && ok; i++) {
+                *knob[q] = cand[q][i];
+                float ms;
+                ok = timed(ms);
+                if (ok && (round == 0 || ms < best[i])) best[i] = ms;
+            }
+        *knob[q] = start;
+        if (!ok) break;
+        size_t pick = 0;
+        for (size_t i = 1; i < best.size(); i++)
+            if (best[i] < best[pick]) pick = i;
+        *knob[q] = cand[q][pick];
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     c->coef = coef;
     c->timing = timing;
     c->plk_valid = false;
-    if (rc) {
-        c->bc_levels = before;
-        return rc;
-    }
-    if (e != hipSuccess) {
-        c->bc_levels = before;
+    if (rc || e != hipSuccess) {
+        c->tile_pair = before[0];
+        c->tile_wind = before[1];
+        c->bc_levels = before[2];
+        if (rc) return rc;
         return atm_fail(c, ATM_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
     }
-    c->bc_levels = best;
     return ATM_OK;
 }
 
@@ -756,6 +778,24 @@ int atm_set_tuning(atm_ctx *c, int corner_levels) {
     if (corner_levels != 1 && corner_levels != 2 && corner_levels != 4 && corner_levels != 8)
         return atm_fail(c, ATM_ERR_ARGUMENT, "corner levels per workgroup: 1, 2, 4 or 8");
     c->bc_levels = corner_levels;
+    return ATM_OK;
+}
+
+int atm_get_tiles(atm_ctx *c, int *pair_rows, int *wind_rows) {
+    CHECK_CTX(c);
+    if (!pair_rows || !wind_rows) return atm_fail(c, ATM_ERR_ARGUMENT, "null output");
+    *pair_rows = c->tile_pair;
+    *wind_rows = c->tile_wind;
+    return ATM_OK;
+}
+
+int atm_set_tiles(atm_ctx *c, int pair_rows, int wind_rows) {
+    CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
+    if ((pair_rows != 8 && pair_rows != 16) || (wind_rows != 8 && wind_rows != 16))
+        return atm_fail(c, ATM_ERR_ARGUMENT, "tile rows: 8 or 16");
+    c->tile_pair = pair_rows;
+    c->tile_wind = wind_rows;
     return ATM_OK;
 }
 

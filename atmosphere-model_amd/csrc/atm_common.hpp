@@ -343,6 +343,9 @@ struct atm_ctx {
     bool after_d = false;  // a fused D step ran and its halo fill is pending
     double *uc2 = nullptr, *vc2 = nullptr, *heatd = nullptr;  // fused-path C-step outputs
     int bc_levels = 4;  // fused path: levels per block of the corner kernel (atm_autotune)
+    // fused path: rows of the dp/pt tiles (8 or 16) and of the wind tiles
+    // (16 or 8), each 32 cells wide (atm_set_tiles, atm_autotune)
+    int tile_pair = 8, tile_wind = 16;
     // fused path: playkap of the current dp on the interior (the C step's
     // heating rate needs it); left by the D step's scan, recomputed by the
     // C step when dp changed since (remap, put)

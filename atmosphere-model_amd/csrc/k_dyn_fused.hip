@@ -126,7 +126,7 @@ __device__ __forceinline__ bool own_inside(const Geo &g, int I0, int J0, int tx_
 template <class R, bool CGRID>
 constexpr int pair_waves() { return std::is_same<R, double>::value ? 6 : CGRID ? 7 : 8; }
 
-template <int VAR, bool CGRID, class R>
+template <int VAR, bool CGRID, class R, int TY = TY_PAIR>
 __device__ __forceinline__ void pair_body(int bx, int by, int K, Geo g, DynCoef c, const double *__restrict__ dp,
                                              const double *__restrict__ pt, const double *__restrict__ uw,
                                              const double *__restrict__ vw, const double *__restrict__ heat,
@@ -134,7 +134,7 @@ __device__ __forceinline__ void pair_body(int bx, int by, int K, Geo g, DynCoef 
                                              double *__restrict__ dpo, double *__restrict__ pto,
                                              double *__restrict__ uco, double *__restrict__ vco,
                                              double *__restrict__ ua, double *__restrict__ va, int *nfw) {
-    constexpr int TY = TY_PAIR, NT = NT_PAIR;
+    constexpr int NT = TX * TY;
     constexpr bool FP64 = std::is_same<R, double>::value;
     const CoefR<R> r(c);
     using RUC = Reg<0, TX + 1, -3, TY + 3>;
@@ -301,8 +301,8 @@ __device__ __forceinline__ void pair_body(int bx, int by, int K, Geo g, DynCoef 
 #undef LP
 }
 
-template <int VAR, bool CGRID, class R>
-__global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo g, DynCoef c, const double *__restrict__ dp,
+template <int VAR, bool CGRID, class R, int TY>
+__global__ __launch_bounds__(TX * TY, (pair_waves<R, CGRID>())) void k_pair(Geo g, DynCoef c, const double *__restrict__ dp,
                                              const double *__restrict__ pt, const double *__restrict__ uw,
                                              const double *__restrict__ vw, const double *__restrict__ heat,
                                              const double *__restrict__ plk, const double *__restrict__ nf,
@@ -311,21 +311,21 @@ __global__ __launch_bounds__(NT_PAIR, (pair_waves<R, CGRID>())) void k_pair(Geo 
                                              double *__restrict__ ua, double *__restrict__ va, int *nfw) {
     int bx, by, K;
     xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
-    pair_body<VAR, CGRID, R>(bx, by, K, g, c, dp, pt, uw, vw, heat, plk, nf, dpo, pto, uco, vco, ua, va, nfw);
+    pair_body<VAR, CGRID, R, TY>(bx, by, K, g, c, dp, pt, uw, vw, heat, plk, nf, dpo, pto, uco, vco, ua, va, nfw);
 }
 
 // ---------------------------------------------------------------------------
 // C2: kinetic energy, vorticity, pressure gradient and the uc/vc update of
 // one level (cgrid 294-414).
 // ---------------------------------------------------------------------------
-template <class R>
+template <class R, int TY = TY_WIND>
 __device__ __forceinline__ void c2_body(int bx, int by, int K, Geo g, DynCoef c, const double *__restrict__ uc1,
                                            const double *__restrict__ vc1, const double *__restrict__ ud,
                                            const double *__restrict__ vd, const double *__restrict__ pkap,
                                            const double *__restrict__ gz, const double *__restrict__ ua,
                                            const double *__restrict__ va, double *__restrict__ uco,
                                            double *__restrict__ vco) {
-    constexpr int TY = TY_WIND, NT = NT_WIND;
+    constexpr int NT = TX * TY;
     constexpr bool FP64 = std::is_same<R, double>::value;
     const CoefR<R> r(c);
     using RB = Reg<-3, TX + 3, -3, TY + 3>;
@@ -431,8 +431,10 @@ __device__ __forceinline__ void c2_body(int bx, int by, int K, Geo g, DynCoef c,
 #undef W
 }
 
-template <class R>
-__global__ __launch_bounds__(NT_WIND, 8) void k_c2(Geo g, DynCoef c, const double *__restrict__ uc1,
+// waves per SIMD of the C-step wind kernel: 8 in 32 x 16 tiles; the LDS of
+// 32 x 8 tiles holds 6
+template <class R, int TY>
+__global__ __launch_bounds__(TX * TY, TY == 16 ? 8 : 6) void k_c2(Geo g, DynCoef c, const double *__restrict__ uc1,
                                            const double *__restrict__ vc1, const double *__restrict__ ud,
                                            const double *__restrict__ vd, const double *__restrict__ pkap,
                                            const double *__restrict__ gz, const double *__restrict__ ua,
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(NT_WIND, 8) void k_c2(Geo g, DynCoef c, const doubl
                                            double *__restrict__ vco) {
     int bx, by, K;
     xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
-    c2_body<R>(bx, by, K, g, c, uc1, vc1, ud, vd, pkap, gz, ua, va, uco, vco);
+    c2_body<R, TY>(bx, by, K, g, c, uc1, vc1, ud, vd, pkap, gz, ua, va, uco, vco);
 }
 
 // ---------------------------------------------------------------------------
@@ -452,13 +454,13 @@ __global__ __launch_bounds__(NT_WIND, 8) void k_c2(Geo g, DynCoef c, const doubl
 // formed once per tile, into LDS, before the faces that read it (storage
 // of arrays that are dead or not yet written at that stage).
 // ---------------------------------------------------------------------------
-template <class R>
+template <class R, int TY = TY_WIND>
 __device__ __forceinline__ void d2_body(int bx, int by, int K, Geo g, DynCoef c, const double *__restrict__ ud,
                                            const double *__restrict__ vd, const double *__restrict__ uc,
                                            const double *__restrict__ vc, const double *__restrict__ pkapb,
                                            const double *__restrict__ gzb, double *__restrict__ udo,
                                            double *__restrict__ vdo, int *nfw) {
-    constexpr int TY = TY_WIND, NT = NT_WIND;
+    constexpr int NT = TX * TY;
     const CoefR<R> r(c);
     using RU = Reg<-3, TX + 3, -3, TY + 4>;
     using RV = Reg<-3, TX + 4, -3, TY + 3>;
@@ -620,15 +622,15 @@ __device__ __forceinline__ void d2_body(int bx, int by, int K, Geo g, DynCoef c,
 #undef D2
 }
 
-template <class R>
-__global__ __launch_bounds__(NT_WIND) void k_d2(Geo g, DynCoef c, const double *__restrict__ ud,
+template <class R, int TY>
+__global__ __launch_bounds__(TX * TY) void k_d2(Geo g, DynCoef c, const double *__restrict__ ud,
                                            const double *__restrict__ vd, const double *__restrict__ uc,
                                            const double *__restrict__ vc, const double *__restrict__ pkapb,
                                            const double *__restrict__ gzb, double *__restrict__ udo,
                                            double *__restrict__ vdo, int *nfw) {
     int bx, by, K;
     xcd_tile(gridDim.x, gridDim.y, gridDim.z, bx, by, K);
-    d2_body<R>(bx, by, K, g, c, ud, vd, uc, vc, pkapb, gzb, udo, vdo, nfw);
+    d2_body<R, TY>(bx, by, K, g, c, ud, vd, uc, vc, pkapb, gzb, udo, vdo, nfw);
 }
 
 // B-grid corner values of pkap and gz for the D step (dgrid 606-632): for
@@ -1159,6 +1161,38 @@ int atm_fused_plk(atm_ctx *c) {
     return ATM_OK;
 }
 
+// The stencil kernels at the context's tile heights (atm_set_tiles): dp/pt
+// tiles of 32 x 8 or 32 x 16 cells, wind tiles of 32 x 16 or 32 x 8, one
+// thread per cell.  Every height gives the same values.
+template <int VAR, bool CGRID, class R, class... A>
+static void launch_pair(atm_ctx *c, A... a) {
+    const Geo &g = c->g;
+    if (c->tile_pair == 16)
+        hipLaunchKernelGGL((k_pair<VAR, CGRID, R, 16>), dim3(tiles(g.nx, TX), tiles(g.ny, 16), g.L), dim3(TX * 16), 0,
+                           c->stream, a...);
+    else
+        hipLaunchKernelGGL((k_pair<VAR, CGRID, R, 8>), dim3(tiles(g.nx, TX), tiles(g.ny, 8), g.L), dim3(TX * 8), 0,
+                           c->stream, a...);
+}
+template <class R, class... A>
+static void launch_c2(atm_ctx *c, A... a) {
+    const Geo &g = c->g;
+    if (c->tile_wind == 8)
+        hipLaunchKernelGGL((k_c2<R, 8>), dim3(tiles(g.nx, TX), tiles(g.ny, 8), g.L), dim3(TX * 8), 0, c->stream, a...);
+    else
+        hipLaunchKernelGGL((k_c2<R, 16>), dim3(tiles(g.nx, TX), tiles(g.ny, 16), g.L), dim3(TX * 16), 0, c->stream,
+                           a...);
+}
+template <class R, class... A>
+static void launch_d2(atm_ctx *c, A... a) {
+    const Geo &g = c->g;
+    if (c->tile_wind == 8)
+        hipLaunchKernelGGL((k_d2<R, 8>), dim3(tiles(g.nx, TX), tiles(g.ny, 8), g.L), dim3(TX * 8), 0, c->stream, a...);
+    else
+        hipLaunchKernelGGL((k_d2<R, 16>), dim3(tiles(g.nx, TX), tiles(g.ny, 16), g.L), dim3(TX * 16), 0, c->stream,
+                           a...);
+}
+
 int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
     const Geo &g = c->g;
     c->coef.dt = dt;
@@ -1169,17 +1203,13 @@ int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
         const int rc = atm_fused_plk(c);
         if (rc) return rc;
     }
-    dim3 grid(tiles(g.nx, TX), tiles(g.ny, TY_PAIR), g.L), blk(NT_PAIR);
-    dim3 gridw(tiles(g.nx, TX), tiles(g.ny, TY_WIND), g.L), blkw(NT_WIND);
     atm_timer_begin(c, ATM_T_C1);
     if (c->mixed)
-        hipLaunchKernelGGL((k_pair<V_PPM_U, true, float>), grid, blk, 0, c->stream, g, k, b.dp, b.pt, b.ud, b.vd,
-                           (const double *)nullptr, c->plk, c->net_flux, c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va,
-                           (int *)nullptr);
+        launch_pair<V_PPM_U, true, float>(c, g, k, b.dp, b.pt, b.ud, b.vd, (const double *)nullptr, c->plk,
+                                          c->net_flux, c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va, (int *)nullptr);
     else
-        hipLaunchKernelGGL((k_pair<V_PPM_U, true, double>), grid, blk, 0, c->stream, g, k, b.dp, b.pt, b.ud, b.vd,
-                           (const double *)nullptr, c->plk, c->net_flux, c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va,
-                           (int *)nullptr);
+        launch_pair<V_PPM_U, true, double>(c, g, k, b.dp, b.pt, b.ud, b.vd, (const double *)nullptr, c->plk,
+                                           c->net_flux, c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va, (int *)nullptr);
     atm_timer_end(c, ATM_T_C1);
     {
         double *fill[] = {c->dpc, c->ptc, c->uc, c->vc, c->ua, c->va};
@@ -1195,11 +1225,9 @@ int atm_fused_cgrid(atm_ctx *c, FusedBufs &b, double dt) {
     atm_timer_end(c, ATM_T_SCAN);
     atm_timer_begin(c, ATM_T_C2);
     if (c->mixed)
-        hipLaunchKernelGGL(k_c2<float>, gridw, blkw, 0, c->stream, g, k, c->uc, c->vc, b.ud, b.vd, c->pkap, c->gz, c->ua,
-                           c->va, c->uc2, c->vc2);
+        launch_c2<float>(c, g, k, c->uc, c->vc, b.ud, b.vd, c->pkap, c->gz, c->ua, c->va, c->uc2, c->vc2);
     else
-        hipLaunchKernelGGL(k_c2<double>, gridw, blkw, 0, c->stream, g, k, c->uc, c->vc, b.ud, b.vd, c->pkap, c->gz,
-                           c->ua, c->va, c->uc2, c->vc2);
+        launch_c2<double>(c, g, k, c->uc, c->vc, b.ud, b.vd, c->pkap, c->gz, c->ua, c->va, c->uc2, c->vc2);
     atm_timer_end(c, ATM_T_C2);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;
@@ -1222,17 +1250,16 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
     c->coef.dtdx = dt * c->coef.rdx;
     c->coef.dtdy = dt * c->coef.rdy;
     const DynCoef &k = c->coef;
-    dim3 grid(tiles(g.nx, TX), tiles(g.ny, TY_PAIR), g.L), blk(NT_PAIR);
-    dim3 gridw(tiles(g.nx, TX), tiles(g.ny, TY_WIND), g.L), blkw(NT_WIND);
     atm_timer_begin(c, ATM_T_D1);
     if (c->mixed)
-        hipLaunchKernelGGL((k_pair<V_PPM_NN, false, float>), grid, blk, 0, c->stream, g, k, cur.dp, cur.pt, c->uc2,
-                           c->vc2, c->heatd, (const double *)nullptr, (const double *)nullptr, nxt.dp, nxt.pt,
-                           (double *)nullptr, (double *)nullptr, (double *)nullptr, (double *)nullptr, c->nfw);
+        launch_pair<V_PPM_NN, false, float>(c, g, k, cur.dp, cur.pt, c->uc2, c->vc2, c->heatd, (const double *)nullptr,
+                                            (const double *)nullptr, nxt.dp, nxt.pt, (double *)nullptr,
+                                            (double *)nullptr, (double *)nullptr, (double *)nullptr, c->nfw);
     else
-        hipLaunchKernelGGL((k_pair<V_PPM_NN, false, double>), grid, blk, 0, c->stream, g, k, cur.dp, cur.pt, c->uc2,
-                           c->vc2, c->heatd, (const double *)nullptr, (const double *)nullptr, nxt.dp, nxt.pt,
-                           (double *)nullptr, (double *)nullptr, (double *)nullptr, (double *)nullptr, c->nfw);
+        launch_pair<V_PPM_NN, false, double>(c, g, k, cur.dp, cur.pt, c->uc2, c->vc2, c->heatd,
+                                             (const double *)nullptr, (const double *)nullptr, nxt.dp, nxt.pt,
+                                             (double *)nullptr, (double *)nullptr, (double *)nullptr,
+                                             (double *)nullptr, c->nfw);
     atm_timer_end(c, ATM_T_D1);
     {
         double *fill[] = {nxt.dp, nxt.pt};
@@ -1249,11 +1276,9 @@ int atm_fused_dgrid(atm_ctx *c, FusedBufs &cur, FusedBufs &nxt, double dt) {
     atm_timer_end(c, ATM_T_SCAN);
     atm_timer_begin(c, ATM_T_D2);
     if (c->mixed)
-        hipLaunchKernelGGL(k_d2<float>, gridw, blkw, 0, c->stream, g, k, cur.ud, cur.vd, c->uc2, c->vc2, c->pkapb,
-                           c->gzb, nxt.ud, nxt.vd, c->nfw);
+        launch_d2<float>(c, g, k, cur.ud, cur.vd, c->uc2, c->vc2, c->pkapb, c->gzb, nxt.ud, nxt.vd, c->nfw);
     else
-        hipLaunchKernelGGL(k_d2<double>, gridw, blkw, 0, c->stream, g, k, cur.ud, cur.vd, c->uc2, c->vc2, c->pkapb,
-                           c->gzb, nxt.ud, nxt.vd, c->nfw);
+        launch_d2<double>(c, g, k, cur.ud, cur.vd, c->uc2, c->vc2, c->pkapb, c->gzb, nxt.ud, nxt.vd, c->nfw);
     atm_timer_end(c, ATM_T_D2);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;

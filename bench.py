@@ -92,9 +92,9 @@ def parse():
                     help="general two-stream columns: 262144 x 64, 32 + 32 g-points, FP64 and mixed")
     ap.add_argument("--ts-ncol", type=int, default=262144)
     ap.add_argument("--ts-calls", type=int, default=5)
-    ap.add_argument("--autotune", choices=["on", "off"], default="off",
-                    help="time the fused path's launch-shape candidates on each grid first (results unchanged; "
-                         "off by default: isolated timings have picked shapes slower inside a step)")
+    ap.add_argument("--autotune", choices=["on", "off"], default="on",
+                    help="time the fused path's tile and launch-shape candidates inside trial steps on each "
+                         "grid first and keep the fastest (results unchanged)")
     ap.add_argument("--graph", choices=["on", "off"], default="on",
                     help="also time a recorded 10-step graph against the same steps issued one call at a "
                          "time, at 64x64x20 (the CPU configuration's grid) and at the main grid")
@@ -185,7 +185,10 @@ def ensemble_timing(args):
                      "checks": "one deferred check per member per period"},
            "corner_levels_per_workgroup": {f"{nx}x{ny}x{L}": sorted({mb.model.tuning for mb in ens.members
                                                                      if mb.cells == nx * ny * L})
-                                           for _, nx, ny, L in ENSEMBLE_SPECS}}
+                                           for _, nx, ny, L in ENSEMBLE_SPECS},
+           "tile_rows_dp_pt_wind": {f"{nx}x{ny}x{L}": sorted({mb.model.tiles for mb in ens.members
+                                                              if mb.cells == nx * ny * L})
+                                    for _, nx, ny, L in ENSEMBLE_SPECS}}
     ens.close()
     ens = Ensemble(ENSEMBLE_SPECS, args.dx, args.dt, remap_every=5, radiation_every=10, precision="mixed",
                    launch="batched")
@@ -708,6 +711,7 @@ def main():
         m.set_path(args.path)
     st = driver.init_model(m, ic, cfg)
     tuning = m.autotune() if args.autotune == "on" else m.tuning
+    tile_rows = m.tiles
     time_s = 0.0
     for _ in range(args.warmup):
         time_s += driver.step(m, cfg, st, time_s)
@@ -755,7 +759,8 @@ def main():
         "whole_step_roofline": whole_step_roofline(value),
         "kernels_ms_per_step": {k: v[1] / args.steps for k, v in kernels.items()},
         "adjoint_forward_cost_ratio": None,
-        "tuning": {"corner_levels_per_workgroup": tuning, "autotune": args.autotune}
+        "tuning": {"corner_levels_per_workgroup": tuning, "tile_rows_dp_pt_wind": list(tile_rows),
+                   "autotune": args.autotune}
     }
     if args.radiation_calls > 0:
         out["radiation"] = radiation_timing(m, args, time_s)

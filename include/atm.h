@@ -231,15 +231,20 @@ int atm_poison(atm_ctx *ctx, int what, double value);
    (one warm-up step, then `reps` timed ones), and keeps the fastest; it
    blocks.  The trial steps leave the model where it was: the next real
    step gives the bits it gives without them.  Candidates differ only in
-   how the work is divided, never in the values computed.  Today one knob:
-   the corner kernel's tile depth, 1, 2, 4 or 8 levels per workgroup
-   (default 4), which atm_get_tuning reads and atm_set_tuning sets
-   directly.  Needs atm_init_sw_dyn and a step boundary (ATM_ERR_STATE
-   otherwise).  On the reference-structured path these return ATM_OK and
-   change nothing. */
+   how the work is divided, never in the values computed.  The knobs, tuned
+   one at a time in this order: the rows of the dp/pt tiles (8 or 16,
+   default 8) and of the wind tiles (16 or 8, default 16), each 32 cells
+   wide, which atm_get_tiles reads and atm_set_tiles sets directly; the
+   corner kernel's tile depth, 1, 2, 4 or 8 levels per workgroup (default
+   4), which atm_get_tuning reads and atm_set_tuning sets.  Needs
+   atm_init_sw_dyn and a step boundary (ATM_ERR_STATE otherwise).  On the
+   reference-structured path these return ATM_OK and change nothing.  The
+   batched ensemble calls use the default shapes. */
 int atm_autotune(atm_ctx *ctx, int reps);
 int atm_get_tuning(atm_ctx *ctx, int *corner_levels);
 int atm_set_tuning(atm_ctx *ctx, int corner_levels);
+int atm_get_tiles(atm_ctx *ctx, int *pair_rows, int *wind_rows);
+int atm_set_tiles(atm_ctx *ctx, int pair_rows, int wind_rows);
 
 /* Wait for all work queued on the context. */
 int atm_sync(atm_ctx *ctx);
