@@ -1195,7 +1195,7 @@ int atm_ensemble_cgrid_step(atm_ensemble *e, double dt) {
     atm_ens_launch_halo(s, e->tab, e->ix_halo[ENS_HALO_C1], nm, par, ENS_HALO_C1, e->n_halo[ENS_HALO_C1]);
     atm_timer_end(c0, ATM_T_HALO);
     atm_timer_begin(c0, ATM_T_SCAN);
-    atm_ens_launch_scan(s, e->tab, e->ix_scan, nm, par, false, e->n_scan);
+    atm_ens_launch_scan(s, e->tab, e->ix_scan, nm, par, false, c0->mixed, e->n_scan);
     atm_timer_end(c0, ATM_T_SCAN);
     atm_timer_begin(c0, ATM_T_C2);
     atm_ens_launch_c2(s, e->tab, e->ix_wind, nm, par, dt, c0->mixed, e->n_wind);
@@ -1242,7 +1242,7 @@ int atm_ensemble_dgrid_step(atm_ensemble *e, double dt) {
     atm_ens_launch_halo(s, e->tab, e->ix_halo[ENS_HALO_D1], nm, par, ENS_HALO_D1, e->n_halo[ENS_HALO_D1]);
     atm_timer_end(c0, ATM_T_HALO);
     atm_timer_begin(c0, ATM_T_SCAN);
-    atm_ens_launch_scan(s, e->tab, e->ix_scan, nm, par, true, e->n_scan);
+    atm_ens_launch_scan(s, e->tab, e->ix_scan, nm, par, true, c0->mixed, e->n_scan);
     atm_ens_launch_corners(s, e->tab, e->ix_corner, nm, e->n_corner);
     atm_timer_end(c0, ATM_T_SCAN);
     atm_timer_begin(c0, ATM_T_D2);

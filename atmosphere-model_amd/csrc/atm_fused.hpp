@@ -92,7 +92,7 @@ void atm_ens_launch_c2(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, 
 void atm_ens_launch_d2(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned long long par,
                        double dt, bool mixed, unsigned n);
 void atm_ens_launch_scan(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned long long par,
-                         bool dstep, unsigned n);
+                         bool dstep, bool mixed, unsigned n);
 void atm_ens_launch_corners(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned n);
 void atm_ens_launch_halo(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned long long par,
                          int kind, unsigned n);

@@ -253,7 +253,11 @@ __device__ __forceinline__ void pair_body(int bx, int by, int K, Geo g, DynCoef 
             double hr;
             if (CGRID) {
                 const unsigned o = TOFF(tx, ty);
-                hr = ((nf + TBASE(K + 1))[o] - (nf + kb0)[o]) / (ATM_CP_DRY * dpv / c.gravity) / (plk + kb0)[o];
+                if constexpr (FP64)
+                    hr = ((nf + TBASE(K + 1))[o] - (nf + kb0)[o]) / (ATM_CP_DRY * dpv / c.gravity) / (plk + kb0)[o];
+                else  // the mixed form's heating rate, a tendency, in float
+                    hr = D(R((nf + TBASE(K + 1))[o] - (nf + kb0)[o]) *
+                           rcp(R(ATM_CP_DRY / c.gravity) * R(dpv) * R((plk + kb0)[o])));
             } else {
                 hr = G(heat, tx, ty);
             }
@@ -854,7 +858,11 @@ __global__ void __launch_bounds__(64) k_scan_pg(Geo g, double ptop, double pktop
 // ---------------------------------------------------------------------------
 constexpr int SC_COLS = 16, SC_PARTS = 4;
 
-template <int CMAX>
+// F32: the mixed form, whose heating rate and the playkap it divides by
+// (tendencies of pt) are formed in float: playkap = pk d / log1p(d) with
+// d = (pk1 - pk) / pk in double, so the logarithm of a ratio near 1 keeps
+// float's relative precision; pkap, plev and gz stay double.
+template <int CMAX, bool F32 = false>
 __device__ __forceinline__ void scan_cols_body(int bxs, int bys, Geo g, int i0, int j0, int ni, double ptop, double pktop,
                                                   double gravity, const double *__restrict__ src,
                                                   const double *__restrict__ th, const double *__restrict__ nf,
@@ -922,9 +930,18 @@ __device__ __forceinline__ void scan_cols_body(int bxs, int bys, Geo g, int i0, 
             if (u < n) {
                 const size_t x = xb + (size_t)u * pl;
                 const double pk1 = u + 1 < n ? pk[u + 1] : pk_up;
-                const double plk = (pk1 - pk[u]) / log(pk1 / pk[u]);
-                if (want_heat) heat[x] = (nfv[u + 1] - nfv[u]) / (ATM_CP_DRY * s[u] / gravity) / plk;
-                if (want_plk) plk_out[x] = plk;
+                if constexpr (F32) {
+                    const float d = (float)((pk1 - pk[u]) / pk[u]);
+                    const float plk = (float)pk[u] * d * __builtin_amdgcn_rcpf(log1pf(d));
+                    if (want_heat)
+                        heat[x] = (double)((float)(nfv[u + 1] - nfv[u]) *
+                                           __builtin_amdgcn_rcpf((float)(ATM_CP_DRY / gravity) * (float)s[u] * plk));
+                    if (want_plk) plk_out[x] = (double)plk;
+                } else {
+                    const double plk = (pk1 - pk[u]) / log(pk1 / pk[u]);
+                    if (want_heat) heat[x] = (nfv[u + 1] - nfv[u]) / (ATM_CP_DRY * s[u] / gravity) / plk;
+                    if (want_plk) plk_out[x] = plk;
+                }
             }
     }
     if (!th) return;
@@ -959,14 +976,15 @@ __device__ __forceinline__ void scan_cols_body(int bxs, int bys, Geo g, int i0, 
         if (u < n && col) gz[xb + (size_t)(u + 1) * pl] = tz[u];
 }
 
-template <int CMAX>
+template <int CMAX, bool F32>
 __global__ void __launch_bounds__(64, 4) k_scan_cols(Geo g, int i0, int j0, int ni, double ptop, double pktop,
                                                   double gravity, const double *__restrict__ src,
                                                   const double *__restrict__ th, const double *__restrict__ nf,
                                                   double *__restrict__ plev, double *__restrict__ pkap,
                                                   double *__restrict__ gz, double *__restrict__ heat,
                                                   double *__restrict__ plk_out) {
-    scan_cols_body<CMAX>((int)blockIdx.x, (int)blockIdx.y, g, i0, j0, ni, ptop, pktop, gravity, src, th, nf, plev, pkap, gz, heat, plk_out);
+    scan_cols_body<CMAX, F32>((int)blockIdx.x, (int)blockIdx.y, g, i0, j0, ni, ptop, pktop, gravity, src, th, nf, plev,
+                              pkap, gz, heat, plk_out);
 }
 
 constexpr int SC_CMAX = 16;  // k_scan_cols for nlay <= 64
@@ -977,8 +995,14 @@ void launch_scans(atm_ctx *c, int i0, int j0, int ni, int nj, double gravity, co
                   const double *nf, double *plev, double *pkap, double *gz, double *heat, double *plk_out) {
     const Geo &g = c->g;
     if (g.L <= SC_PARTS * SC_CMAX) {
-        hipLaunchKernelGGL(k_scan_cols<SC_CMAX>, dim3((ni + SC_COLS - 1) / SC_COLS, nj), dim3(64), 0, c->stream, g,
-                           i0, j0, ni, c->ak0, c->pktop, gravity, src, th, nf, plev, pkap, gz, heat, plk_out);
+        if (c->mixed)
+            hipLaunchKernelGGL((k_scan_cols<SC_CMAX, true>), dim3((ni + SC_COLS - 1) / SC_COLS, nj), dim3(64), 0,
+                               c->stream, g, i0, j0, ni, c->ak0, c->pktop, gravity, src, th, nf, plev, pkap, gz, heat,
+                               plk_out);
+        else
+            hipLaunchKernelGGL((k_scan_cols<SC_CMAX, false>), dim3((ni + SC_COLS - 1) / SC_COLS, nj), dim3(64), 0,
+                               c->stream, g, i0, j0, ni, c->ak0, c->pktop, gravity, src, th, nf, plev, pkap, gz, heat,
+                               plk_out);
     } else if (th) {
         hipLaunchKernelGGL(k_scan_pg, dim3((g.NI + 63) / 64, g.NJ), dim3(64), 0, c->stream, g, c->ak0, c->pktop,
                            gravity, src, th, nf, plev, pkap, gz, heat, plk_out);
@@ -1069,7 +1093,7 @@ __global__ void __launch_bounds__(NT_WIND) k_bcorners_ens(const EnsMember *__res
 
 // the column scans after C1 (pkap, gz, heating rate) or after D1 (plev, pkap,
 // gz and the next C step's playkap), over all columns of every member
-template <bool DSTEP>
+template <bool DSTEP, bool F32>
 __global__ void __launch_bounds__(64, 4) k_scan_ens(const EnsMember *__restrict__ tab, EnsIndex ix, int nm,
                                                    unsigned long long par) {
     int m;
@@ -1080,10 +1104,10 @@ __global__ void __launch_bounds__(64, 4) k_scan_ens(const EnsMember *__restrict_
     const int bx = (int)(l % tx), by = (int)(l / tx);
     if constexpr (DSTEP) {
         const int q = 1 - (int)((par >> m) & 1ull);  // the copy D1 wrote
-        scan_cols_body<SC_CMAX>(bx, by, e.g, 0, 0, e.g.NI, e.ak0, e.pktop, e.coef.gravity, e.p[0][q], e.p[1][q],
+        scan_cols_body<SC_CMAX, F32>(bx, by, e.g, 0, 0, e.g.NI, e.ak0, e.pktop, e.coef.gravity, e.p[0][q], e.p[1][q],
                                 e.net_flux, e.plev, e.pkap, e.gz, nullptr, e.plk);
     } else {
-        scan_cols_body<SC_CMAX>(bx, by, e.g, 0, 0, e.g.NI, e.ak0, e.pktop, e.coef.gravity, e.dpc, e.ptc,
+        scan_cols_body<SC_CMAX, F32>(bx, by, e.g, 0, 0, e.g.NI, e.ak0, e.pktop, e.coef.gravity, e.dpc, e.ptc,
                                 e.net_flux, nullptr, e.pkap, e.gz, e.heatd, nullptr);
     }
 }
@@ -1131,11 +1155,15 @@ void atm_ens_launch_d2(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, 
         hipLaunchKernelGGL(k_d2_ens<double>, dim3(n), dim3(NT_WIND), 0, s, tab, ix, nm, par, dt);
 }
 void atm_ens_launch_scan(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned long long par,
-                         bool dstep, unsigned n) {
-    if (dstep)
-        hipLaunchKernelGGL(k_scan_ens<true>, dim3(n), dim3(64), 0, s, tab, ix, nm, par);
+                         bool dstep, bool mixed, unsigned n) {
+    if (dstep && mixed)
+        hipLaunchKernelGGL((k_scan_ens<true, true>), dim3(n), dim3(64), 0, s, tab, ix, nm, par);
+    else if (dstep)
+        hipLaunchKernelGGL((k_scan_ens<true, false>), dim3(n), dim3(64), 0, s, tab, ix, nm, par);
+    else if (mixed)
+        hipLaunchKernelGGL((k_scan_ens<false, true>), dim3(n), dim3(64), 0, s, tab, ix, nm, par);
     else
-        hipLaunchKernelGGL(k_scan_ens<false>, dim3(n), dim3(64), 0, s, tab, ix, nm, par);
+        hipLaunchKernelGGL((k_scan_ens<false, false>), dim3(n), dim3(64), 0, s, tab, ix, nm, par);
 }
 void atm_ens_launch_corners(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned n) {
     hipLaunchKernelGGL(k_bcorners_ens, dim3(n), dim3(NT_WIND), 0, s, tab, ix, nm);

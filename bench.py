@@ -388,7 +388,8 @@ def mixed_timing(args, ic, cfg_kw):
     rad_err = rel(nf["mixed"], nf["fp64"])
     value = nx * ny * L * args.steps / sec
     return {"value": value, "unit": "cell-steps/s", "ms_per_step": sec / args.steps * 1e3,
-            "dtype": "f32 reconstruction and fluxes, f64 updates, scans, pressure gradient, remap",
+            "dtype": "f32 reconstruction, fluxes and heating rate; f64 updates, pressures, pkap, gz, "
+                     "pressure gradient, remap",
             "kernels_ms_per_step": kernels,
             "error_vs_fp64": {"steps": args.mixed_error_steps, "radiation_every": 10, **err},
             "radiation_ms_per_call": rad["ms_per_call"] if rad else None,

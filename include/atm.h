@@ -79,8 +79,10 @@ typedef struct atm_params {
    MIXED: the C- and D-step tile kernels form the PPM reconstructions,
    fluxes, kinetic energy, vorticity and damping terms in float and update
    dp, pt and the winds in double; radiation forms its g-point optics in
-   float and runs the adding method in double; the column scans, pressure
-   gradient, heating rate, remap and physics stay FP64.
+   float and runs the adding method in double; the radiative heating rate
+   (a tendency of pt) and the playkap it divides by are formed in float;
+   the column scans' pressures, pkap and geopotential, the pressure
+   gradient, remap and physics stay FP64.
    Each face flux is one value shared by the two cells it separates, so mass
    is conserved to FP64 rounding.  MIXED runs on the fused path (any grid);
    it has no counterpart in the reference. */
