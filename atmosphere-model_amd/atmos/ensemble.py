@@ -24,15 +24,16 @@ and each remap switches), and radiation may fall due only at period
 starts.  The recorded time steps are those of the first period; the remap
 pattern of every later period is checked against it.
 
-Batched mode (`launch="batched"`): the dynamics stages of a step (C step,
-its halo fill, D step, the D halo fill) run as one launch per stage for all
-members together (atm_ensemble_*: a member table indexed by workgroup), on
-the first member's stream, joined to the other members' streams by events;
-radiation and the remap stay per member, on the members' own streams, where
-they overlap.  The members share the time step.  Every member's values are
+Batched mode (`launch="batched"`): the stages of a step (C step, its halo
+fill, D step, the remap, the D halo fill) run as one launch per stage for
+all members together (atm_ensemble_*: a member table indexed by
+workgroup), on the first member's stream, joined to the other members'
+streams by events; radiation stays per member, on the members' own
+streams, where the members' calls overlap.  The members share the time
+step and the remap schedule.  Every member's values are
 those of its own launches, bit for bit.  With a graph period as well, the
-batched launches and the members' remaps of a whole period are recorded as
-one graph (atm_ensemble_graph_*) and replayed for every later period, under
+batched launches of a whole period are recorded as one graph
+(atm_ensemble_graph_*) and replayed for every later period, under
 the same rules as the members' own graphs; the recorded time steps and
 remap pattern are checked against each later period's.
 """
@@ -62,8 +63,8 @@ class Ensemble:
     autotune: each member times the fused path's launch shapes on its own
     grid at construction (results do not depend on the choice).
     launch: "streams" (each member's own launches, members dealt to nstreams
-    streams) or "batched" (one launch per dynamics stage for all members;
-    remap and radiation on the members' streams)."""
+    streams) or "batched" (one launch per dynamics and remap stage for all
+    members; radiation on the members' streams)."""
 
     def __init__(self, specs, dx: float, dt: float, remap_every: int = 5, radiation_every: int = 10,
                  precision: str = "fp64", nstreams: int = 4, seed: int = 20261205, t_noise: float = 0.5,
@@ -151,14 +152,21 @@ class Ensemble:
         for mb in self.members:
             if mb.config.physics_on:
                 mb.model.apply_physics_tendencies(dt)
-            if mb.state.remap_this_step:
-                mb.model.perform_vertical_remapping()
-                mb.state.remap_this_step = False
+        if self._remap_due([mb.state.remap_this_step for mb in self.members]):
+            b.perform_vertical_remapping()
+        for mb in self.members:
+            mb.state.remap_this_step = False
         b.dgrid_halo_exchange()
         for mb in self.members:
             if mb.config.physics_on:
                 mb.model.physics_halo_exchange()
             mb.time = mb.time + dt
+
+    @staticmethod
+    def _remap_due(flags) -> bool:
+        if any(flags) and not all(flags):
+            raise RuntimeError("batched members must remap together")
+        return all(flags)
 
     def _validate_period(self, P: int):
         if self.remap_every < 1 or P % self.remap_every or (P + P // self.remap_every) % 2:
@@ -278,13 +286,13 @@ class Ensemble:
             dt = dts[0]
             if any(d != dt for d in dts):
                 raise RuntimeError("batched members must share the time step")
+            remap = self._remap_due(remaps)
             if record:
                 b.cgrid_dynamics_step(dt / 2.0)
                 b.cgrid_halo_exchange()
                 b.dgrid_dynamics_step(dt)
-                for mb, remap in zip(self.members, remaps):
-                    if remap:
-                        mb.model.perform_vertical_remapping()
+                if remap:
+                    b.perform_vertical_remapping()
                 b.dgrid_halo_exchange()
             pattern.append((dt, tuple(remaps)))
             times = [t + dt for t in times]

@@ -1001,8 +1001,9 @@ struct atm_ensemble {
     EnsMember *tab = nullptr;  // device
     // per launch kind: the members' workgroup spans and the launch size (a
     // multiple of 8)
-    EnsIndex ix_pair{}, ix_wind{}, ix_corner{}, ix_scan{}, ix_halo[4]{};
-    unsigned n_pair = 0, n_wind = 0, n_corner = 0, n_scan = 0, n_halo[4] = {0, 0, 0, 0};
+    EnsIndex ix_pair{}, ix_wind{}, ix_corner{}, ix_scan{}, ix_halo[4]{}, ix_remap[4]{};
+    unsigned n_pair = 0, n_wind = 0, n_corner = 0, n_scan = 0, n_halo[4] = {0, 0, 0, 0}, n_remap[4] = {0, 0, 0, 0};
+    bool remap_ready = false;  // every member's remap initialised when the table was made
     // The batched launches go to the first member's stream.  Members may run
     // on other streams (their remap and radiation overlap there): a batched
     // call first makes the launch stream wait for every other member stream
@@ -1094,13 +1095,15 @@ int atm_ensemble_create(atm_ctx *const *members, int n, atm_ensemble **out) {
     e->m.assign(members, members + n);
     int rc = ens_usable(e);
     std::vector<EnsMember> h(n);
-    unsigned long long np = 0, nw = 0, nc = 0, ns = 0, nh[4] = {0, 0, 0, 0};
+    unsigned long long np = 0, nw = 0, nc = 0, ns = 0, nh[4] = {0, 0, 0, 0}, nr[4] = {0, 0, 0, 0};
+    e->remap_ready = true;
     for (int i = 0; rc == ATM_OK && i < n; i++) {
         e->ix_pair.first[i] = (unsigned)np;
         e->ix_wind.first[i] = (unsigned)nw;
         e->ix_corner.first[i] = (unsigned)nc;
         e->ix_scan.first[i] = (unsigned)ns;
         for (int k = 0; k < 4; k++) e->ix_halo[k].first[i] = (unsigned)nh[k];
+        for (int k = 0; k < 4; k++) e->ix_remap[k].first[i] = (unsigned)nr[k];
         atm_ctx *c = members[i];
         if (!atm_ens_scan_fits(c->g)) {
             rc = atm_fail(c, ATM_ERR_ARGUMENT, "ensemble: nlay <= 64 for the batched column scans");
@@ -1116,13 +1119,15 @@ int atm_ensemble_create(atm_ctx *const *members, int n, atm_ensemble **out) {
         m.dpc = c->dpc; m.ptc = c->ptc; m.uc = c->uc; m.vc = c->vc; m.ua = c->ua; m.va = c->va;
         m.uc2 = c->uc2; m.vc2 = c->vc2; m.plk = c->plk; m.net_flux = c->net_flux; m.heatd = c->heatd;
         m.plev = c->plev; m.pkap = c->pkap; m.gz = c->gz; m.pkapb = c->pkapb; m.gzb = c->gzb; m.nfw = c->nfw;
+        if (!atm_remap_member(c, m)) e->remap_ready = false;
         np += atm_ens_blocks_pair(c->g);
         nw += atm_ens_blocks_wind(c->g);
         nc += atm_ens_blocks_corner(c->g);
         ns += atm_ens_blocks_scan(c->g);
         for (int k = 0; k < 4; k++) nh[k] += atm_ens_blocks_halo(c->g, k);
+        for (int k = 0; k < 4; k++) nr[k] += atm_ens_blocks_remap(c->g, k);
     }
-    if (rc == ATM_OK && std::max(std::max(np, nw), std::max(nc, ns)) > 0x7fffffffull)
+    if (rc == ATM_OK && std::max(std::max(std::max(np, nw), std::max(nc, ns)), std::max(nr[1], nr[2])) > 0x7fffffffull)
         rc = atm_fail(members[0], ATM_ERR_ARGUMENT, "ensemble: too many workgroups for one launch");
     if (rc == ATM_OK) {
         e->ix_pair.first[n] = (unsigned)np;
@@ -1130,6 +1135,7 @@ int atm_ensemble_create(atm_ctx *const *members, int n, atm_ensemble **out) {
         e->ix_corner.first[n] = (unsigned)nc;
         e->ix_scan.first[n] = (unsigned)ns;
         for (int k = 0; k < 4; k++) e->ix_halo[k].first[n] = (unsigned)nh[k];
+        for (int k = 0; k < 4; k++) e->ix_remap[k].first[n] = (unsigned)nr[k];
     }
     if (rc == ATM_OK) {
         hipError_t err = hipMalloc(&e->tab, sizeof(EnsMember) * n);
@@ -1152,6 +1158,7 @@ int atm_ensemble_create(atm_ctx *const *members, int n, atm_ensemble **out) {
     e->n_corner = round8(nc);
     e->n_scan = round8(ns);
     for (int k = 0; k < 4; k++) e->n_halo[k] = round8(nh[k]);
+    for (int k = 0; k < 4; k++) e->n_remap[k] = round8(nr[k]);
     *out = e;
     return ATM_OK;
 }
@@ -1273,6 +1280,37 @@ int atm_ensemble_dgrid_halo(atm_ensemble *e) {
     ATM_HIP(c0, hipGetLastError());
     for (atm_ctx *c : e->m) c->after_d = false;
     return ens_fork(e);
+}
+
+// perform_vertical_remapping of every member after the batched D step:
+// the four remap stages, each one launch for all members, each member's
+// values those of its own atm_remap.
+int atm_ensemble_remap(atm_ensemble *e) {
+    if (!e) return ATM_ERR_ARGUMENT;
+    int rc = ens_usable(e);
+    if (rc) return rc;
+    atm_ctx *c0 = e->m[0];
+    for (atm_ctx *c : e->m) {
+        if (!c->remap_init) return atm_fail(c, ATM_ERR_STATE, "atm_remap_init first");
+        if (!c->after_d) return atm_fail(c, ATM_ERR_STATE, "ensemble: the remap follows atm_ensemble_dgrid_step");
+    }
+    if (!e->remap_ready)
+        return atm_fail(c0, ATM_ERR_STATE, "ensemble: every member's atm_remap_init before atm_ensemble_create");
+    const unsigned long long par = ens_parity(e);
+    rc = ens_join(e);
+    if (rc) return rc;
+    atm_timer_begin(c0, ATM_T_REMAP);
+    atm_ens_launch_remap(c0->stream, e->tab, e->ix_remap, (int)e->m.size(), par, e->n_remap);
+    atm_timer_end(c0, ATM_T_REMAP);
+    ATM_HIP(c0, hipGetLastError());
+    rc = ens_fork(e);
+    if (rc) return rc;
+    for (atm_ctx *c : e->m) {  // as atm_remap after a fused D step
+        c->plk_valid = false;
+        c->cur = 1 - c->cur;
+        atm_point_current(c);
+    }
+    return ATM_OK;
 }
 
 // Recording of batched steps: the launch stream starts a capture and the

@@ -44,6 +44,9 @@ struct EnsMember {
     double *dpc, *ptc, *uc, *vc, *ua, *va, *uc2, *vc2;
     double *plk, *net_flux, *heatd, *plev, *pkap, *gz, *pkapb, *gzb;
     int *nfw;
+    // the remap's (atm_remap_member; null before atm_remap_init)
+    const double *ak, *bk;
+    double *playkap, *rscr, *tplev, *tpkap, *tplk, *ur, *vr;
 };
 #define ATM_ENS_MAX 64  // members per ensemble (one bit each in a parity mask)
 
@@ -93,6 +96,13 @@ void atm_ens_launch_d2(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, 
                        double dt, bool mixed, unsigned n);
 void atm_ens_launch_scan(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned long long par,
                          bool dstep, bool mixed, unsigned n);
+// The remap of every member (k_remap.hip): the member table's remap
+// fields from a context, the workgroups a member needs in each of the four
+// remap stages (grid, winds, energy inputs, energy), and the four launches.
+bool atm_remap_member(const atm_ctx *c, EnsMember &m);
+unsigned atm_ens_blocks_remap(const Geo &g, int stage);
+void atm_ens_launch_remap(hipStream_t s, const EnsMember *tab, const EnsIndex *ix, int nm, unsigned long long par,
+                          const unsigned *n);
 void atm_ens_launch_corners(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned n);
 void atm_ens_launch_halo(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned long long par,
                          int kind, unsigned n);

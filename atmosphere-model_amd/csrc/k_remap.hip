@@ -23,6 +23,7 @@
 // produced in ascending order only, so the per-layer limiter state is a
 // window of a few registers (gam(k-2..k+2), var(k..k+1), edges(k..k+1)).
 #include "atm_common.hpp"
+#include "atm_fused.hpp"
 #include "remap_column.hpp"
 
 struct RemapWork {
@@ -94,14 +95,16 @@ __device__ __forceinline__ unsigned xcd_block1d() {
 
 // current and target grids and gz over isd-3..ied+3 (42-78).  Levels go in
 // chunks of 4 whose pow / log evaluations are independent.
-__global__ void k_remap_grid(Geo g, double top, const double *__restrict__ ak, const double *__restrict__ bk,
-                             const double *__restrict__ dp, const double *__restrict__ pt,
-                             double *__restrict__ plev, double *__restrict__ pkap, double *__restrict__ playkap,
-                             double *__restrict__ gz, double *__restrict__ tplev, double *__restrict__ tpkap,
-                             double *__restrict__ tplk) {
+// The kernels' bodies take the thread's column (or cell) from the caller:
+// each member of a batched ensemble launch (atm_ens_launch_remap) runs
+// the body its own launch runs.
+__device__ __forceinline__ void remap_grid_body(int I, int J, Geo g, double top, const double *__restrict__ ak,
+                                                const double *__restrict__ bk, const double *__restrict__ dp,
+                                                const double *__restrict__ pt, double *__restrict__ plev,
+                                                double *__restrict__ pkap, double *__restrict__ playkap,
+                                                double *__restrict__ gz, double *__restrict__ tplev,
+                                                double *__restrict__ tpkap, double *__restrict__ tplk) {
     constexpr int CH = 4;
-    const int I = 4 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int J = 4 + blockIdx.y * blockDim.y + threadIdx.y;
     if (I > g.NI - 5 || J > g.NJ - 5) return;
     const int L = g.L;
     const size_t pl = g.plane(), c0 = g.ix2(I, J);
@@ -173,28 +176,33 @@ __global__ void k_remap_grid(Geo g, double top, const double *__restrict__ ak, c
     }
 }
 
+__global__ void k_remap_grid(Geo g, double top, const double *__restrict__ ak, const double *__restrict__ bk,
+                             const double *__restrict__ dp, const double *__restrict__ pt,
+                             double *__restrict__ plev, double *__restrict__ pkap, double *__restrict__ playkap,
+                             double *__restrict__ gz, double *__restrict__ tplev, double *__restrict__ tpkap,
+                             double *__restrict__ tplk) {
+    remap_grid_body(4 + blockIdx.x * blockDim.x + threadIdx.x, 4 + blockIdx.y * blockDim.y + threadIdx.y, g, top, ak,
+                    bk, dp, pt, plev, pkap, playkap, gz, tplev, tpkap, tplk);
+}
+
 // winds on faces over isd-1..ied+1 (80-105): u for blockIdx.y = 0, v for
 // blockIdx.y = 1, so the two columns of a point run on separate threads.
 // The ring of the D-grid winds holds the halo values the previous step's
 // halo exchange left there (the D-grid step updates the interior only): a
 // column outside the interior reads its winds from `ring`, an interior
 // column from `in`.
-__global__ void __launch_bounds__(256) k_remap_winds(Geo g, double *__restrict__ scr, size_t ncol,
-                                                     const double *__restrict__ plev,
-                                                     const double *__restrict__ tplev,
-                                                     const double *__restrict__ ud, const double *__restrict__ vd,
-                                                     const double *__restrict__ udr,
-                                                     const double *__restrict__ vdr, double *__restrict__ ur,
-                                                     double *__restrict__ vr) {
+__device__ __forceinline__ void remap_winds_body(size_t t, bool is_v, Geo g, double *__restrict__ scr, size_t ncol,
+                                                 const double *__restrict__ plev, const double *__restrict__ tplev,
+                                                 const double *__restrict__ ud, const double *__restrict__ vd,
+                                                 const double *__restrict__ udr, const double *__restrict__ vdr,
+                                                 double *__restrict__ ur, double *__restrict__ vr) {
     const int W = g.nx + 2;
-    const size_t t = (size_t)xcd_block1d() * blockDim.x + threadIdx.x;
     if (t >= ncol) return;
     const int I = 6 + (int)(t % W), J = 6 + (int)(t / W);
     const int L = g.L;
     const bool inside = I >= ATM_HALO && I < ATM_HALO + g.nx && J >= ATM_HALO && J < ATM_HALO + g.ny;
     const size_t base = g.ix2(I, J), plane = g.plane();
     const Scr s{scr, ncol, t, L};
-    const bool is_v = blockIdx.y == 1;
     // u: faces averaged along J (stride SI); v: along I (stride 1)
     const WindCol c{plev, tplev, is_v ? (inside ? vd : vdr) : (inside ? ud : udr), base,
                     is_v ? (size_t)1 : (size_t)g.SI, plane};
@@ -205,19 +213,29 @@ __global__ void __launch_bounds__(256) k_remap_winds(Geo g, double *__restrict__
     ppm_merge(c, L, lv, gm, ncol, out);
 }
 
+__global__ void __launch_bounds__(256) k_remap_winds(Geo g, double *__restrict__ scr, size_t ncol,
+                                                     const double *__restrict__ plev,
+                                                     const double *__restrict__ tplev,
+                                                     const double *__restrict__ ud, const double *__restrict__ vd,
+                                                     const double *__restrict__ udr,
+                                                     const double *__restrict__ vdr, double *__restrict__ ur,
+                                                     double *__restrict__ vr) {
+    remap_winds_body((size_t)xcd_block1d() * blockDim.x + threadIdx.x, blockIdx.y == 1, g, scr, ncol, plev, tplev, ud,
+                     vd, udr, vdr, ur, vr);
+}
+
 // The per-level inputs of the energy remap, over the interior columns and
 // all levels at once (one thread per cell; column t = (J-7) nx + (I-7)):
 // the total energy of each layer (111-128) into slot 2 and the kinetic
 // energy of the remapped winds (the term of 137-142) into slot 3, each the
 // reference's expression, so the column pass reads one value per level for
 // each.  The winds of a point outside the interior come from the ring.
-__global__ void k_remap_energy_in(Geo g, double *__restrict__ scr, size_t ncol, const double *__restrict__ playkap,
-                                  const double *__restrict__ gz, const double *__restrict__ pt,
-                                  const double *__restrict__ ud, const double *__restrict__ vd,
-                                  const double *__restrict__ udr, const double *__restrict__ vdr,
-                                  const double *__restrict__ ur, const double *__restrict__ vr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int K = blockIdx.z;
+__device__ __forceinline__ void remap_energy_in_body(int i, int j, int K, Geo g, double *__restrict__ scr,
+                                                    size_t ncol, const double *__restrict__ playkap,
+                                                    const double *__restrict__ gz, const double *__restrict__ pt,
+                                                    const double *__restrict__ ud, const double *__restrict__ vd,
+                                                    const double *__restrict__ udr, const double *__restrict__ vdr,
+                                                    const double *__restrict__ ur, const double *__restrict__ vr) {
     if (i >= g.nx || j >= g.ny) return;
     const int I = ATM_HALO + i, J = ATM_HALO + j, L = g.L;
     const size_t t = (size_t)j * g.nx + i, plane = g.plane(), x = g.ix(I, J, K);
@@ -245,19 +263,27 @@ __global__ void k_remap_energy_in(Geo g, double *__restrict__ scr, size_t ncol, 
     }
 }
 
+__global__ void k_remap_energy_in(Geo g, double *__restrict__ scr, size_t ncol, const double *__restrict__ playkap,
+                                  const double *__restrict__ gz, const double *__restrict__ pt,
+                                  const double *__restrict__ ud, const double *__restrict__ vd,
+                                  const double *__restrict__ udr, const double *__restrict__ vdr,
+                                  const double *__restrict__ ur, const double *__restrict__ vr) {
+    remap_energy_in_body(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y, blockIdx.z, g,
+                         scr, ncol, playkap, gz, pt, ud, vd, udr, vdr, ur, vr);
+}
+
 // Total energy remap, pt reconstruction (111-155) and the write-back of
 // the interior (157-161), from the energies k_remap_energy_in left in the
 // scratch.  With write_winds the remapped winds are copied into ud_out /
 // vd_out here (only when those are not the buffers the energy of
 // neighbouring columns is formed from).
-__global__ void __launch_bounds__(256) k_remap_energy(
-    Geo g, double *__restrict__ scr, size_t ncol, const double *__restrict__ plev,
+__device__ __forceinline__ void remap_energy_body(
+    size_t t, Geo g, double *__restrict__ scr, size_t ncol, const double *__restrict__ plev,
     const double *__restrict__ tplev, const double *__restrict__ tpkap, const double *__restrict__ tplk,
     const double *__restrict__ gz, const double *__restrict__ ur, const double *__restrict__ vr, double *pt_out,
     double *__restrict__ dp_out, double *__restrict__ ud_out, double *__restrict__ vd_out, int write_winds,
     int *nfw) {
     const int W = g.nx;
-    const size_t t = (size_t)xcd_block1d() * blockDim.x + threadIdx.x;
     if (t >= ncol) return;
     const int I = 7 + (int)(t % W), J = 7 + (int)(t / W);
     const int L = g.L;
@@ -287,6 +313,16 @@ __global__ void __launch_bounds__(256) k_remap_energy(
     ppm_merge(c, L, s.slot(0), s.slot(1), ncol, out);
 }
 
+__global__ void __launch_bounds__(256) k_remap_energy(
+    Geo g, double *__restrict__ scr, size_t ncol, const double *__restrict__ plev,
+    const double *__restrict__ tplev, const double *__restrict__ tpkap, const double *__restrict__ tplk,
+    const double *__restrict__ gz, const double *__restrict__ ur, const double *__restrict__ vr, double *pt_out,
+    double *__restrict__ dp_out, double *__restrict__ ud_out, double *__restrict__ vd_out, int write_winds,
+    int *nfw) {
+    remap_energy_body((size_t)xcd_block1d() * blockDim.x + threadIdx.x, g, scr, ncol, plev, tplev, tpkap, tplk, gz, ur,
+                      vr, pt_out, dp_out, ud_out, vd_out, write_winds, nfw);
+}
+
 // interior wind write-back when the output winds are the input buffers
 __global__ void k_remap_write_winds(Geo g, const double *__restrict__ ur, const double *__restrict__ vr,
                                     double *__restrict__ ud, double *__restrict__ vd, int *nfw) {
@@ -302,6 +338,66 @@ __global__ void k_remap_write_winds(Geo g, const double *__restrict__ ur, const 
     atm_report_nonfinite(nfw, v);
 }
 
+// Batched remap of every member of an ensemble (atm_ensemble_remap): a
+// member's span of workgroups runs, stage by stage, the bodies its own
+// launches run (same columns, cells and operations, so the same values),
+// reading the copy of the fields its parity bit names and writing the
+// other, as atm_remap does after a D step.
+constexpr int ENS_RM_BLOCK = 256;
+__device__ __forceinline__ unsigned ens_rm_blocks(size_t n) { return (unsigned)((n + ENS_RM_BLOCK - 1) / ENS_RM_BLOCK); }
+
+__global__ void __launch_bounds__(128) k_remap_grid_ens(const EnsMember *__restrict__ tab, EnsIndex ix, int nm,
+                                                        unsigned long long par) {
+    int m;
+    unsigned l;
+    if (!ens_find(ix, nm, m, l)) return;
+    const EnsMember &e = tab[m];
+    const int q = (int)((par >> m) & 1ull);
+    const unsigned gx = (unsigned)(e.g.nx + 6 + 63) / 64;  // the 64 x 2 blocks of k_remap_grid
+    const int bx = (int)(l % gx), by = (int)(l / gx);
+    remap_grid_body(4 + bx * 64 + (int)threadIdx.x % 64, 4 + by * 2 + (int)threadIdx.x / 64, e.g, e.ak0, e.ak, e.bk,
+                    e.p[0][q], e.p[1][q], e.plev, e.pkap, e.playkap, e.gz, e.tplev, e.tpkap, e.tplk);
+}
+
+__global__ void __launch_bounds__(ENS_RM_BLOCK) k_remap_winds_ens(const EnsMember *__restrict__ tab, EnsIndex ix,
+                                                                  int nm, unsigned long long par) {
+    int m;
+    unsigned l;
+    if (!ens_find(ix, nm, m, l)) return;
+    const EnsMember &e = tab[m];
+    const int q = (int)((par >> m) & 1ull);
+    const size_t nw = (size_t)(e.g.nx + 2) * (e.g.ny + 2);
+    const unsigned nb = ens_rm_blocks(nw);  // u columns, then v columns
+    const bool is_v = l >= nb;
+    remap_winds_body((size_t)(is_v ? l - nb : l) * ENS_RM_BLOCK + threadIdx.x, is_v, e.g, e.rscr, nw, e.plev, e.tplev,
+                     e.p[2][q], e.p[3][q], e.p[2][1 - q], e.p[3][1 - q], e.ur, e.vr);
+}
+
+__global__ void __launch_bounds__(ENS_RM_BLOCK) k_remap_energy_in_ens(const EnsMember *__restrict__ tab, EnsIndex ix,
+                                                                      int nm, unsigned long long par) {
+    int m;
+    unsigned l;
+    if (!ens_find(ix, nm, m, l)) return;
+    const EnsMember &e = tab[m];
+    const int q = (int)((par >> m) & 1ull);
+    const unsigned gx = (unsigned)(e.g.nx + 63) / 64, gy = (unsigned)(e.g.ny + 3) / 4;  // the 64 x 4 blocks
+    const int bx = (int)(l % gx), by = (int)(l / gx % gy), K = (int)(l / gx / gy);
+    remap_energy_in_body(bx * 64 + (int)threadIdx.x % 64, by * 4 + (int)threadIdx.x / 64, K, e.g, e.rscr,
+                         (size_t)e.g.nx * e.g.ny, e.playkap, e.gz, e.p[1][q], e.p[2][q], e.p[3][q], e.p[2][1 - q],
+                         e.p[3][1 - q], e.ur, e.vr);
+}
+
+__global__ void __launch_bounds__(ENS_RM_BLOCK) k_remap_energy_ens(const EnsMember *__restrict__ tab, EnsIndex ix,
+                                                                   int nm, unsigned long long par) {
+    int m;
+    unsigned l;
+    if (!ens_find(ix, nm, m, l)) return;
+    const EnsMember &e = tab[m];
+    const int o = 1 - (int)((par >> m) & 1ull);
+    remap_energy_body((size_t)l * ENS_RM_BLOCK + threadIdx.x, e.g, e.rscr, (size_t)e.g.nx * e.g.ny, e.plev, e.tplev,
+                      e.tpkap, e.tplk, e.gz, e.ur, e.vr, e.p[1][o], e.p[0][o], e.p[2][o], e.p[3][o], 1, e.nfw);
+}
+
 __global__ void k_fill_plane(double *a, size_t n, double v) {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) a[t] = v;
@@ -313,6 +409,40 @@ unsigned blocks8(size_t n, unsigned bs) {
 }
 
 }  // namespace
+
+bool atm_remap_member(const atm_ctx *c, EnsMember &m) {
+    const RemapWork *w = c->remap;
+    if (!w) return false;
+    m.ak = c->ak;
+    m.bk = c->bk;
+    m.playkap = c->playkap;
+    m.rscr = w->scr;
+    m.tplev = w->tplev;
+    m.tpkap = w->tpkap;
+    m.tplk = w->tplk;
+    m.ur = w->uremap;
+    m.vr = w->vremap;
+    return true;
+}
+
+// stages: 0 grid (128-thread blocks), 1 winds, 2 energy inputs, 3 energy
+unsigned atm_ens_blocks_remap(const Geo &g, int stage) {
+    auto blocks = [](size_t n, size_t b) { return (unsigned)((n + b - 1) / b); };
+    switch (stage) {
+        case 0: return blocks(g.nx + 6, 64) * blocks(g.ny + 6, 2);
+        case 1: return 2u * blocks((size_t)(g.nx + 2) * (g.ny + 2), ENS_RM_BLOCK);
+        case 2: return blocks(g.nx, 64) * blocks(g.ny, 4) * (unsigned)g.L;
+        default: return blocks((size_t)g.nx * g.ny, ENS_RM_BLOCK);
+    }
+}
+
+void atm_ens_launch_remap(hipStream_t s, const EnsMember *tab, const EnsIndex *ix, int nm, unsigned long long par,
+                          const unsigned *n) {
+    hipLaunchKernelGGL(k_remap_grid_ens, dim3(n[0]), dim3(128), 0, s, tab, ix[0], nm, par);
+    hipLaunchKernelGGL(k_remap_winds_ens, dim3(n[1]), dim3(ENS_RM_BLOCK), 0, s, tab, ix[1], nm, par);
+    hipLaunchKernelGGL(k_remap_energy_in_ens, dim3(n[2]), dim3(ENS_RM_BLOCK), 0, s, tab, ix[2], nm, par);
+    hipLaunchKernelGGL(k_remap_energy_ens, dim3(n[3]), dim3(ENS_RM_BLOCK), 0, s, tab, ix[3], nm, par);
+}
 
 // Frees what a RemapWork holds; also for one whose allocation stopped part way.
 static void remap_release(RemapWork *w) {

@@ -200,10 +200,10 @@ def ensemble_timing(args):
     ens.sync()
     sec_b = timed(ens.step, n_def)
     out["batched"] = {"steps": n_def, "value": cells * n_def / sec_b, "ms_per_step": sec_b / n_def * 1e3,
-                      "launch": "one launch per dynamics stage for all 32 members (C1, halo, scans, C2, "
-                                "C halo, D1, halo, scans, corners, D2, D halo) on the first member's "
-                                "stream, joined by events to the members' 4 streams, where remap and "
-                                "radiation run per member; deferred checks"}
+                      "launch": "one launch per stage for all 32 members (C1, halo, scans, C2, C halo, "
+                                "D1, halo, scans, corners, D2, remap grid, winds, energy inputs, energy, "
+                                "D halo) on the first member's stream, joined by events to the members' "
+                                "4 streams, where radiation runs per member; deferred checks"}
     ens.use_graphs(10)
     if not ens.period() or not ens.finish():  # records the period's graph, runs the first period
         raise RuntimeError("ensemble member unstable")
@@ -211,9 +211,9 @@ def ensemble_timing(args):
     sec_bg = timed(ens.period, n_per)
     out["batched"]["graph"] = {"period_steps": 10, "periods": n_per, "value": cells * 10 * n_per / sec_bg,
                                "ms_per_step": sec_bg / (10 * n_per) * 1e3,
-                               "launch": "the batched launches and the members' remaps of a 10-step period "
-                                         "as one graph on the first member's stream (radiation eager at "
-                                         "the period start, one deferred check per member per period)"}
+                               "launch": "the batched launches of a 10-step period as one graph on the "
+                                         "first member's stream (radiation eager at the period start, one "
+                                         "deferred check per member per period)"}
     ens.close()
     return out
 

@@ -283,8 +283,8 @@ int atm_graph_free(atm_ctx *ctx);
    member's stream.  Members on other streams are joined by events: each
    call waits for the work already issued on every member's stream, and
    every member's later work waits for the call.  Members may differ in nx,
-   ny and nlay (nlay <= 64); they share dt.  Remap, radiation and physics
-   stay per member.  Errors are reported on the member concerned
+   ny and nlay (nlay <= 64); they share dt.  Radiation and physics stay per
+   member.  Errors are reported on the member concerned
    (atm_last_error of that context), else on the first.  The ensemble holds
    pointers to the members' device arrays: destroy it before any member. */
 typedef struct atm_ensemble atm_ensemble;
@@ -294,6 +294,11 @@ int atm_ensemble_cgrid_step(atm_ensemble *ens, double dt);
 int atm_ensemble_cgrid_halo(atm_ensemble *ens);
 int atm_ensemble_dgrid_step(atm_ensemble *ens, double dt);
 int atm_ensemble_dgrid_halo(atm_ensemble *ens);
+/* perform_vertical_remapping (mod_remap.f90:27-163) of every member after
+   atm_ensemble_dgrid_step, as atm_remap does for one context, with one
+   launch per remap stage for all members; needs every member's
+   atm_remap_init before atm_ensemble_create. */
+int atm_ensemble_remap(atm_ensemble *ens);
 
 /* Graphs of batched steps (no reference counterpart).  Between
    atm_ensemble_graph_begin and atm_ensemble_graph_end the batched calls

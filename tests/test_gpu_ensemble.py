@@ -44,10 +44,11 @@ def test_members_match_solo_runs(gpu_available, precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batched_launches_match_member_launches(gpu_available, precision):
-    """One launch per dynamics stage for all members (atm_ensemble_*: a member
-    table indexed by workgroup) gives every member the bits its own launches
-    give: uneven grids, sizes that are not multiples of the tiles, levels of
-    6 to 12, three remaps and a radiation call in 12 steps."""
+    """One launch per dynamics and remap stage for all members
+    (atm_ensemble_*: a member table indexed by workgroup) gives every member
+    the bits its own launches give: uneven grids, sizes that are not
+    multiples of the tiles, levels of 6 to 12, three remaps and a radiation
+    call in 12 steps."""
     from atmos.ensemble import Ensemble
     specs = [(1, 64, 32, 12), (2, 40, 24, 10), (1, 37, 23, 6), (1, 32, 16, 7)]
     dx, dt, seed, steps = 100e3, 60.0, 91, 12
@@ -105,7 +106,8 @@ def test_batched_graph_periods_match_batched_launches(gpu_available, precision):
 def test_batched_launches_refuse_mixed_members(gpu_available):
     """The batched steps need every member on the fused path and in one
     precision; a member that is not is named in the error, at creation or at
-    a later call, and the members' call order is checked as for one model."""
+    a later call, and the members' call order is checked as for one model
+    (the remap after the D step)."""
     from atmos.config import Config
     from atmos import driver
     from atmos.ic import make_initial_state
@@ -130,11 +132,14 @@ def test_batched_launches_refuse_mixed_members(gpu_available):
         BatchedSteps([m0, m])
     m.set_path("fused")
     bs = BatchedSteps([m0, m])
+    with pytest.raises(AtmError, match="follows"):  # the batched remap comes after the D step
+        bs.perform_vertical_remapping()
     bs.cgrid_dynamics_step(30.0)
     bs.cgrid_halo_exchange()
     bs.dgrid_dynamics_step(60.0)
     with pytest.raises(AtmError, match="must follow"):  # the order of the calls, checked per member
         bs.cgrid_dynamics_step(30.0)
+    bs.perform_vertical_remapping()
     bs.dgrid_halo_exchange()
     m.set_path("reference")  # a member that leaves the fused path later
     with pytest.raises(AtmError, match="fused path"):
