@@ -25,12 +25,11 @@ starts.  The recorded time steps are those of the first period; the remap
 pattern of every later period is checked against it.
 
 Batched mode (`launch="batched"`): the stages of a step (C step, its halo
-fill, D step, the remap, the D halo fill) run as one launch per stage for
-all members together (atm_ensemble_*: a member table indexed by
-workgroup), on the first member's stream, joined to the other members'
-streams by events; radiation stays per member, on the members' own
-streams, where the members' calls overlap.  The members share the time
-step and the remap schedule.  Every member's values are
+fill, D step, the remap, the D halo fill) and radiation run as one launch
+per stage for all members together (atm_ensemble_*: a member table
+indexed by workgroup), on the first member's stream, joined to the other
+members' streams by events.  The members share the time step and the
+remap and radiation schedules.  Every member's values are
 those of its own launches, bit for bit.  With a graph period as well, the
 batched launches of a whole period are recorded as one graph
 (atm_ensemble_graph_*) and replayed for every later period, under
@@ -63,8 +62,8 @@ class Ensemble:
     autotune: each member times the fused path's launch shapes on its own
     grid at construction (results do not depend on the choice).
     launch: "streams" (each member's own launches, members dealt to nstreams
-    streams) or "batched" (one launch per dynamics and remap stage for all
-    members; radiation on the members' streams)."""
+    streams) or "batched" (one launch per dynamics, remap and radiation
+    stage for all members)."""
 
     def __init__(self, specs, dx: float, dt: float, remap_every: int = 5, radiation_every: int = 10,
                  precision: str = "fp64", nstreams: int = 4, seed: int = 20261205, t_noise: float = 0.5,
@@ -127,17 +126,27 @@ class Ensemble:
             mb.time = mb.time + dt
         return self._check()
 
+    def _batched_radiation(self):
+        """calculate_radiative_heating of the members whose radiation is due:
+        one batched call when all of them are (they share the schedule)."""
+        due = [mb.state.calculate_radiation_this_step and mb.config.radiation_on for mb in self.members]
+        if all(due):
+            self.batch.calculate_radiative_heating(self.members[0].time)
+        for mb, d in zip(self.members, due):
+            if d:
+                if not all(due):
+                    mb.model.calculate_radiative_heating(mb.time)
+                mb.model.radiation_halo_exchange()
+            mb.state.calculate_radiation_this_step = False
+
     def _batched_step(self):
-        """driver.step (mod_model.f90:117-183) for every member, its dynamics
-        stages batched over the members."""
+        """driver.step (mod_model.f90:117-183) for every member, its stages
+        batched over the members."""
         dts = []
+        if any(mb.state.calculate_radiation_this_step for mb in self.members):
+            self._batched_radiation()
         for mb in self.members:
             m, cfg, st = mb.model, mb.config, mb.state
-            if st.calculate_radiation_this_step:
-                if cfg.radiation_on:
-                    m.calculate_radiative_heating(mb.time)
-                    m.radiation_halo_exchange()
-                st.calculate_radiation_this_step = False
             dts.append(driver.calculate_dt(mb.time, cfg, st))
             if cfg.physics_on and st.calculate_physics_this_step:
                 m.calculate_physics_tendencies()
@@ -260,13 +269,8 @@ class Ensemble:
         """graph_period steps of every member as one recording of batched
         and member calls (recorded by the first call, replayed after), with
         radiation issued eagerly at the period start."""
-        for mb in self.members:
-            m, st, cfg = mb.model, mb.state, mb.config
-            if st.calculate_radiation_this_step:  # due at the period start
-                if cfg.radiation_on:
-                    m.calculate_radiative_heating(mb.time)
-                    m.radiation_halo_exchange()
-                st.calculate_radiation_this_step = False
+        if any(mb.state.calculate_radiation_this_step for mb in self.members):  # due at the period start
+            self._batched_radiation()
         record = self.batch_pattern is None
         b = self.batch
         if record:

@@ -36,7 +36,7 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_twostream", "atm_twostream_noalloc", "atm_twostream_last_error", "atm_stream_synchronize", "atm_stream_timer_begin",
            "atm_stream_timer_end", "atm_poison", "atm_check_stability_begin", "atm_check_stability_end",
            "atm_ensemble_create", "atm_ensemble_destroy", "atm_ensemble_cgrid_step", "atm_ensemble_cgrid_halo",
-           "atm_ensemble_dgrid_step", "atm_ensemble_dgrid_halo", "atm_ensemble_remap", "atm_ensemble_graph_begin",
+           "atm_ensemble_dgrid_step", "atm_ensemble_dgrid_halo", "atm_ensemble_remap", "atm_ensemble_radiation", "atm_ensemble_graph_begin",
            "atm_ensemble_graph_end", "atm_ensemble_graph_launch"]
 PATHS = {"fused": 0, "reference": 1}
 PRECISIONS = {"fp64": 0, "mixed": 1}
@@ -115,6 +115,7 @@ def load_library(path: str = LIB_PATH):
     L.atm_set_tiles.argtypes = [vp, ip, ip]
     L.atm_ensemble_create.argtypes = [C.POINTER(vp), ip, C.POINTER(vp)]
     L.atm_ensemble_graph_launch.argtypes = [vp, ip]
+    L.atm_ensemble_radiation.argtypes = [vp, C.c_double]
     for n in ["atm_ensemble_destroy", "atm_ensemble_cgrid_halo", "atm_ensemble_dgrid_halo", "atm_ensemble_remap",
               "atm_ensemble_graph_begin", "atm_ensemble_graph_end"]:
         getattr(L, n).argtypes = [vp]
@@ -143,7 +144,8 @@ def stream_destroy(handle: int) -> None:
 class BatchedSteps:
     """The batched ensemble steps (atm_ensemble_*) over Models that run the
     fused path in one precision: each call (the dynamics steps, their halo
-    fills, the remap) does for every member what the Model's own call does,
+    fills, the remap, radiation) does for every member what the Model's own
+    call does,
     with one launch per stage for all of them on the first member's stream
     (joined to the others' streams by events).  Create it after the members'
     initialisation and close it before the members."""
@@ -180,6 +182,9 @@ class BatchedSteps:
 
     def perform_vertical_remapping(self):
         self._ck(self._L.atm_ensemble_remap(self._h))
+
+    def calculate_radiative_heating(self, time):
+        self._ck(self._L.atm_ensemble_radiation(self._h, float(time)))
 
     def graph_begin(self):
         """Record the following batched and member step calls into one HIP

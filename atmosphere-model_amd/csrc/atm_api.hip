@@ -1001,9 +1001,12 @@ struct atm_ensemble {
     EnsMember *tab = nullptr;  // device
     // per launch kind: the members' workgroup spans and the launch size (a
     // multiple of 8)
-    EnsIndex ix_pair{}, ix_wind{}, ix_corner{}, ix_scan{}, ix_halo[4]{}, ix_remap[4]{};
-    unsigned n_pair = 0, n_wind = 0, n_corner = 0, n_scan = 0, n_halo[4] = {0, 0, 0, 0}, n_remap[4] = {0, 0, 0, 0};
-    bool remap_ready = false;  // every member's remap initialised when the table was made
+    EnsIndex ix_pair{}, ix_wind{}, ix_corner{}, ix_scan{}, ix_halo[4]{}, ix_remap[4]{}, ix_rad[3]{};
+    unsigned n_pair = 0, n_wind = 0, n_corner = 0, n_scan = 0, n_halo[4] = {0, 0, 0, 0}, n_remap[4] = {0, 0, 0, 0},
+             n_rad[3] = {0, 0, 0};
+    // every member's remap / radiation initialised when the table was made
+    bool remap_ready = false, rad_ready = false;
+    int lmax = 0;  // the members' largest nlay
     // The batched launches go to the first member's stream.  Members may run
     // on other streams (their remap and radiation overlap there): a batched
     // call first makes the launch stream wait for every other member stream
@@ -1095,8 +1098,9 @@ int atm_ensemble_create(atm_ctx *const *members, int n, atm_ensemble **out) {
     e->m.assign(members, members + n);
     int rc = ens_usable(e);
     std::vector<EnsMember> h(n);
-    unsigned long long np = 0, nw = 0, nc = 0, ns = 0, nh[4] = {0, 0, 0, 0}, nr[4] = {0, 0, 0, 0};
+    unsigned long long np = 0, nw = 0, nc = 0, ns = 0, nh[4] = {0, 0, 0, 0}, nr[4] = {0, 0, 0, 0}, nd[3] = {0, 0, 0};
     e->remap_ready = true;
+    e->rad_ready = true;
     for (int i = 0; rc == ATM_OK && i < n; i++) {
         e->ix_pair.first[i] = (unsigned)np;
         e->ix_wind.first[i] = (unsigned)nw;
@@ -1104,6 +1108,7 @@ int atm_ensemble_create(atm_ctx *const *members, int n, atm_ensemble **out) {
         e->ix_scan.first[i] = (unsigned)ns;
         for (int k = 0; k < 4; k++) e->ix_halo[k].first[i] = (unsigned)nh[k];
         for (int k = 0; k < 4; k++) e->ix_remap[k].first[i] = (unsigned)nr[k];
+        for (int k = 0; k < 3; k++) e->ix_rad[k].first[i] = (unsigned)nd[k];
         atm_ctx *c = members[i];
         if (!atm_ens_scan_fits(c->g)) {
             rc = atm_fail(c, ATM_ERR_ARGUMENT, "ensemble: nlay <= 64 for the batched column scans");
@@ -1120,14 +1125,18 @@ int atm_ensemble_create(atm_ctx *const *members, int n, atm_ensemble **out) {
         m.uc2 = c->uc2; m.vc2 = c->vc2; m.plk = c->plk; m.net_flux = c->net_flux; m.heatd = c->heatd;
         m.plev = c->plev; m.pkap = c->pkap; m.gz = c->gz; m.pkapb = c->pkapb; m.gzb = c->gzb; m.nfw = c->nfw;
         if (!atm_remap_member(c, m)) e->remap_ready = false;
+        if (!atm_rad_member(c, m)) e->rad_ready = false;
+        e->lmax = std::max(e->lmax, c->g.L);
         np += atm_ens_blocks_pair(c->g);
         nw += atm_ens_blocks_wind(c->g);
         nc += atm_ens_blocks_corner(c->g);
         ns += atm_ens_blocks_scan(c->g);
         for (int k = 0; k < 4; k++) nh[k] += atm_ens_blocks_halo(c->g, k);
         for (int k = 0; k < 4; k++) nr[k] += atm_ens_blocks_remap(c->g, k);
+        for (int k = 0; k < 3; k++) nd[k] += atm_ens_blocks_rad(c->g, k);
     }
-    if (rc == ATM_OK && std::max(std::max(std::max(np, nw), std::max(nc, ns)), std::max(nr[1], nr[2])) > 0x7fffffffull)
+    if (rc == ATM_OK && std::max(std::max(std::max(np, nw), std::max(nc, ns)),
+                                 std::max(std::max(nr[1], nr[2]), std::max(nd[1], nd[2]))) > 0x7fffffffull)
         rc = atm_fail(members[0], ATM_ERR_ARGUMENT, "ensemble: too many workgroups for one launch");
     if (rc == ATM_OK) {
         e->ix_pair.first[n] = (unsigned)np;
@@ -1136,6 +1145,7 @@ int atm_ensemble_create(atm_ctx *const *members, int n, atm_ensemble **out) {
         e->ix_scan.first[n] = (unsigned)ns;
         for (int k = 0; k < 4; k++) e->ix_halo[k].first[n] = (unsigned)nh[k];
         for (int k = 0; k < 4; k++) e->ix_remap[k].first[n] = (unsigned)nr[k];
+        for (int k = 0; k < 3; k++) e->ix_rad[k].first[n] = (unsigned)nd[k];
     }
     if (rc == ATM_OK) {
         hipError_t err = hipMalloc(&e->tab, sizeof(EnsMember) * n);
@@ -1159,6 +1169,7 @@ int atm_ensemble_create(atm_ctx *const *members, int n, atm_ensemble **out) {
     e->n_scan = round8(ns);
     for (int k = 0; k < 4; k++) e->n_halo[k] = round8(nh[k]);
     for (int k = 0; k < 4; k++) e->n_remap[k] = round8(nr[k]);
+    for (int k = 0; k < 3; k++) e->n_rad[k] = round8(nd[k]);
     *out = e;
     return ATM_OK;
 }
@@ -1311,6 +1322,40 @@ int atm_ensemble_remap(atm_ensemble *e) {
         atm_point_current(c);
     }
     return ATM_OK;
+}
+
+// calculate_radiative_heating of every member (atm_radiation for each):
+// the profiles, the net flux's zero fill and the column solves, each one
+// launch for all members; one wait for every member's range flag, an error
+// reported on the first member whose lookup left the tables.
+int atm_ensemble_radiation(atm_ensemble *e, double time) {
+    (void)time;  // as atm_radiation: unused by solar_irradiance (mod_solar.f90:26-35)
+    if (!e) return ATM_ERR_ARGUMENT;
+    int rc = ens_usable(e);
+    if (rc) return rc;
+    atm_ctx *c0 = e->m[0];
+    for (atm_ctx *c : e->m) {
+        CHECK_NOT_CAPTURING(c);
+        if (!c->rad_init) return atm_fail(c, ATM_ERR_STATE, "atm_radiation_init first");
+    }
+    if (!e->rad_ready)
+        return atm_fail(c0, ATM_ERR_STATE, "ensemble: every member's atm_radiation_init before atm_ensemble_create");
+    const unsigned long long par = ens_parity(e);
+    rc = ens_join(e);
+    if (rc) return rc;
+    const hipStream_t s = c0->stream;
+    for (atm_ctx *c : e->m) ATM_HIP(c0, hipMemsetAsync(c->flag, 0, sizeof(int), s));
+    atm_timer_begin(c0, ATM_T_RAD);
+    atm_ens_launch_rad(s, e->tab, e->ix_rad, (int)e->m.size(), par, e->n_rad, e->lmax, c0->mixed);
+    atm_timer_end(c0, ATM_T_RAD);
+    ATM_HIP(c0, hipGetLastError());
+    for (atm_ctx *c : e->m)
+        ATM_HIP(c0, hipMemcpyAsync(c->flag_host, c->flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    ATM_HIP(c0, hipStreamSynchronize(s));
+    for (atm_ctx *c : e->m)
+        if (*c->flag_host & 2)
+            return atm_fail(c, ATM_ERR_TABLE_RANGE, "gas-optics lookup outside the tabulated pressure/temperature range");
+    return ens_fork(e);
 }
 
 // Recording of batched steps: the launch stream starts a capture and the

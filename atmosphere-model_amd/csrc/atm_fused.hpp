@@ -47,6 +47,12 @@ struct EnsMember {
     // the remap's (atm_remap_member; null before atm_remap_init)
     const double *ak, *bk;
     double *playkap, *rscr, *tplev, *tpkap, *tplk, *ur, *vr;
+    // the radiation's (atm_rad_member; null before atm_radiation_init)
+    const float *rtab[8];
+    double rgrav;
+    double *play, *lT, *vT;
+    const double *ts, *radius;
+    int *rflag;
 };
 #define ATM_ENS_MAX 64  // members per ensemble (one bit each in a parity mask)
 
@@ -103,6 +109,13 @@ bool atm_remap_member(const atm_ctx *c, EnsMember &m);
 unsigned atm_ens_blocks_remap(const Geo &g, int stage);
 void atm_ens_launch_remap(hipStream_t s, const EnsMember *tab, const EnsIndex *ix, int nm, unsigned long long par,
                           const unsigned *n);
+// The radiation of every member (k_rad.hip): the member table's radiation
+// fields, the workgroups of the three stages (profiles, the net flux's zero
+// fill, columns) and the launches (lmax: the members' largest nlay).
+bool atm_rad_member(const atm_ctx *c, EnsMember &m);
+unsigned atm_ens_blocks_rad(const Geo &g, int stage);
+void atm_ens_launch_rad(hipStream_t s, const EnsMember *tab, const EnsIndex *ix, int nm, unsigned long long par,
+                        const unsigned *n, int lmax, bool mixed);
 void atm_ens_launch_corners(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned n);
 void atm_ens_launch_halo(hipStream_t s, const EnsMember *tab, const EnsIndex &ix, int nm, unsigned long long par,
                          int kind, unsigned n);

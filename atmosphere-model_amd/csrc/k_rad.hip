@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "atm_common.hpp"
+#include "atm_fused.hpp"
 #include "gas_optics_tables.inc"
 #include "rad_chain.hpp"
 
@@ -68,11 +69,13 @@ __device__ __forceinline__ bool in_range(double x, const float *xs, int n) {
 }
 
 // layer and level profiles over is+4..ie-4 (53-101)
-__global__ void k_rad_profiles(Geo g, const double *__restrict__ dp, const double *__restrict__ pt,
-                               double *__restrict__ plev, double *__restrict__ pkap, double *__restrict__ play,
-                               double *__restrict__ playkap, double *__restrict__ lT, double *__restrict__ vT) {
-    const int I = 4 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int J = 4 + blockIdx.y * blockDim.y + threadIdx.y;
+// (the kernels' bodies take the thread's column from the caller: each
+// member of a batched ensemble launch runs the body its own launch runs)
+__device__ __forceinline__ void rad_profiles_body(int I, int J, Geo g, const double *__restrict__ dp,
+                                                  const double *__restrict__ pt, double *__restrict__ plev,
+                                                  double *__restrict__ pkap, double *__restrict__ play,
+                                                  double *__restrict__ playkap, double *__restrict__ lT,
+                                                  double *__restrict__ vT) {
     if (I > g.NI - 5 || J > g.NJ - 5) return;
     const int L = g.L;
     double p1v = plev[g.ix(I, J, L)];
@@ -109,6 +112,13 @@ __global__ void k_rad_profiles(Geo g, const double *__restrict__ dp, const doubl
         pg = log(plev[g.ix(I, J, 0)]);
         vT[g.ix(I, J, 0)] = (lT[g.ix(I, J, 0)] * (a2 - pg) + lT[g.ix(I, J, 1)] * (pg - a1)) / (a2 - a1);
     }
+}
+
+__global__ void k_rad_profiles(Geo g, const double *__restrict__ dp, const double *__restrict__ pt,
+                               double *__restrict__ plev, double *__restrict__ pkap, double *__restrict__ play,
+                               double *__restrict__ playkap, double *__restrict__ lT, double *__restrict__ vT) {
+    rad_profiles_body(4 + blockIdx.x * blockDim.x + threadIdx.x, 4 + blockIdx.y * blockDim.y + threadIdx.y, g, dp, pt,
+                      plev, pkap, play, playkap, lT, vT);
 }
 
 // ---------------------------------------------------------------------------
@@ -206,15 +216,12 @@ __device__ __forceinline__ LinPT lin_pT(const Tables &t, double p, double T, int
 // kernel fits 3 waves per SIMD (LDS 12.4 KB and <= 168 registers at
 // nlay = 64), which the launch bound asks for.
 template <int NL, class R>
-__global__ void __launch_bounds__(64, NL == 1 ? 3 : 1) k_rad_columns(Geo g, Tables t, double gravity, const double *__restrict__ dp,
-                                                    const double *__restrict__ play, const double *__restrict__ lT,
-                                                    const double *__restrict__ vT, const double *__restrict__ ts,
-                                                    const double *__restrict__ radius, double *__restrict__ net_flux,
-                                                    int *flag) {
+__device__ __forceinline__ void rad_columns_body(int col, Geo g, const Tables &t, double gravity,
+                                                 const double *__restrict__ dp, const double *__restrict__ play,
+                                                 const double *__restrict__ lT, const double *__restrict__ vT,
+                                                 const double *__restrict__ ts, const double *__restrict__ radius,
+                                                 double *__restrict__ net_flux, int *flag) {
     extern __shared__ double lds[];
-    // consecutive columns (in i) on one XCD: they share the 128-byte lines
-    // of every level they read
-    const int col = (int)xcd_block(blockIdx.x, gridDim.x);
     const int I = ATM_HALO + col % g.nx, J = ATM_HALO + col / g.nx;
     const int lane = threadIdx.x;
     const int L = g.L, N = L + 1;
@@ -378,6 +385,59 @@ __global__ void __launch_bounds__(64, NL == 1 ? 3 : 1) k_rad_columns(Geo g, Tabl
     if (err) atomicOr(flag, 2);
 }
 
+template <int NL, class R>
+__global__ void __launch_bounds__(64, NL == 1 ? 3 : 1) k_rad_columns(Geo g, Tables t, double gravity, const double *__restrict__ dp,
+                                                    const double *__restrict__ play, const double *__restrict__ lT,
+                                                    const double *__restrict__ vT, const double *__restrict__ ts,
+                                                    const double *__restrict__ radius, double *__restrict__ net_flux,
+                                                    int *flag) {
+    // consecutive columns (in i) on one XCD: they share the 128-byte lines
+    // of every level they read
+    rad_columns_body<NL, R>((int)xcd_block(blockIdx.x, gridDim.x), g, t, gravity, dp, play, lT, vT, ts, radius,
+                            net_flux, flag);
+}
+
+// Batched radiation of every member of an ensemble (atm_ensemble_radiation):
+// a member's span of workgroups runs the bodies its own launches run, on
+// the copy of the fields its parity bit names.
+__device__ __forceinline__ Tables ens_tables(const EnsMember &e) {
+    return Tables{e.rtab[0], e.rtab[1], e.rtab[2], e.rtab[3], e.rtab[4], e.rtab[5], e.rtab[6], e.rtab[7]};
+}
+
+__global__ void __launch_bounds__(128) k_rad_profiles_ens(const EnsMember *__restrict__ tab, EnsIndex ix, int nm,
+                                                          unsigned long long par) {
+    int m;
+    unsigned l;
+    if (!ens_find(ix, nm, m, l)) return;
+    const EnsMember &e = tab[m];
+    const int q = (int)((par >> m) & 1ull);
+    const unsigned gx = (unsigned)(e.g.nx + 6 + 63) / 64;  // the 64 x 2 blocks of k_rad_profiles
+    rad_profiles_body(4 + (int)(l % gx) * 64 + (int)threadIdx.x % 64, 4 + (int)(l / gx) * 2 + (int)threadIdx.x / 64,
+                      e.g, e.p[0][q], e.p[1][q], e.plev, e.pkap, e.play, e.playkap, e.lT, e.vT);
+}
+
+// the net flux's zero fill (the memset before a member's own column launch)
+__global__ void __launch_bounds__(256) k_rad_zero_ens(const EnsMember *__restrict__ tab, EnsIndex ix, int nm) {
+    int m;
+    unsigned l;
+    if (!ens_find(ix, nm, m, l)) return;
+    const EnsMember &e = tab[m];
+    const size_t i = (size_t)l * 256 + threadIdx.x;
+    if (i < e.g.plane() * (size_t)(e.g.L + 1)) e.net_flux[i] = 0.;
+}
+
+template <class R>
+__global__ void __launch_bounds__(64, 3) k_rad_columns_ens(const EnsMember *__restrict__ tab, EnsIndex ix, int nm,
+                                                           unsigned long long par) {
+    int m;
+    unsigned l;
+    if (!ens_find(ix, nm, m, l)) return;
+    const EnsMember &e = tab[m];
+    const int q = (int)((par >> m) & 1ull);
+    rad_columns_body<1, R>((int)l, e.g, ens_tables(e), e.rgrav, e.p[0][q], e.play, e.lT, e.vT, e.ts, e.radius,
+                           e.net_flux, e.rflag);
+}
+
 }  // namespace
 
 // Frees what a RadWork holds; also for one whose allocation stopped part way.
@@ -460,4 +520,41 @@ int atm_rad_run(atm_ctx *c, double time) {
     atm_timer_end(c, ATM_T_RAD);
     ATM_HIP(c, hipGetLastError());
     return ATM_OK;
+}
+
+bool atm_rad_member(const atm_ctx *c, EnsMember &m) {
+    const RadWork *w = c->rad;
+    if (!w) return false;
+    const Tables &t = w->t;
+    const float *tt[8] = {t.P, t.T, t.sw, t.ray, t.lw, t.planckT, t.planck, t.frac};
+    for (int i = 0; i < 8; i++) m.rtab[i] = tt[i];
+    m.rgrav = c->prm.gravity;
+    m.play = c->play;
+    m.lT = w->layer_T;
+    m.vT = w->level_T;
+    m.ts = c->ts;
+    m.radius = c->radius;
+    m.rflag = c->flag;
+    return true;
+}
+
+// stages: 0 profiles (128-thread blocks), 1 net-flux zero fill, 2 columns
+unsigned atm_ens_blocks_rad(const Geo &g, int stage) {
+    auto blocks = [](size_t n, size_t b) { return (unsigned)((n + b - 1) / b); };
+    switch (stage) {
+        case 0: return blocks(g.nx + 6, 64) * blocks(g.ny + 6, 2);
+        case 1: return blocks(g.plane() * (size_t)(g.L + 1), 256);
+        default: return (unsigned)(g.nx * g.ny);
+    }
+}
+
+void atm_ens_launch_rad(hipStream_t s, const EnsMember *tab, const EnsIndex *ix, int nm, unsigned long long par,
+                        const unsigned *n, int lmax, bool mixed) {
+    hipLaunchKernelGGL(k_rad_profiles_ens, dim3(n[0]), dim3(128), 0, s, tab, ix[0], nm, par);
+    hipLaunchKernelGGL(k_rad_zero_ens, dim3(n[1]), dim3(256), 0, s, tab, ix[1], nm);
+    const size_t lds = sizeof(double) * (RARR * RG + 4) * (lmax + 1);
+    if (mixed)
+        hipLaunchKernelGGL(k_rad_columns_ens<float>, dim3(n[2]), dim3(64), lds, s, tab, ix[2], nm, par);
+    else
+        hipLaunchKernelGGL(k_rad_columns_ens<double>, dim3(n[2]), dim3(64), lds, s, tab, ix[2], nm, par);
 }

@@ -200,10 +200,10 @@ def ensemble_timing(args):
     ens.sync()
     sec_b = timed(ens.step, n_def)
     out["batched"] = {"steps": n_def, "value": cells * n_def / sec_b, "ms_per_step": sec_b / n_def * 1e3,
-                      "launch": "one launch per stage for all 32 members (C1, halo, scans, C2, C halo, "
-                                "D1, halo, scans, corners, D2, remap grid, winds, energy inputs, energy, "
-                                "D halo) on the first member's stream, joined by events to the members' "
-                                "4 streams, where radiation runs per member; deferred checks"}
+                      "launch": "one launch per stage for all 32 members (radiation profiles, zero fill, "
+                                "columns; C1, halo, scans, C2, C halo, D1, halo, scans, corners, D2, remap "
+                                "grid, winds, energy inputs, energy, D halo) on the first member's stream, "
+                                "joined by events to the members' 4 streams; deferred checks"}
     ens.use_graphs(10)
     if not ens.period() or not ens.finish():  # records the period's graph, runs the first period
         raise RuntimeError("ensemble member unstable")

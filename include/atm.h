@@ -283,8 +283,7 @@ int atm_graph_free(atm_ctx *ctx);
    member's stream.  Members on other streams are joined by events: each
    call waits for the work already issued on every member's stream, and
    every member's later work waits for the call.  Members may differ in nx,
-   ny and nlay (nlay <= 64); they share dt.  Radiation and physics stay per
-   member.  Errors are reported on the member concerned
+   ny and nlay (nlay <= 64); they share dt.  Physics stays per member.  Errors are reported on the member concerned
    (atm_last_error of that context), else on the first.  The ensemble holds
    pointers to the members' device arrays: destroy it before any member. */
 typedef struct atm_ensemble atm_ensemble;
@@ -299,6 +298,13 @@ int atm_ensemble_dgrid_halo(atm_ensemble *ens);
    launch per remap stage for all members; needs every member's
    atm_remap_init before atm_ensemble_create. */
 int atm_ensemble_remap(atm_ensemble *ens);
+/* calculate_radiative_heating (mod_radiation.f90:41-156) of every member,
+   as atm_radiation does for one context, with one launch per stage for all
+   members and one wait for their lookups' range flags (ATM_ERR_TABLE_RANGE
+   on the first member concerned); needs every member's atm_radiation_init
+   before atm_ensemble_create.  The halo fill of the net flux stays per
+   member (atm_radiation_halo). */
+int atm_ensemble_radiation(atm_ensemble *ens, double time);
 
 /* Graphs of batched steps (no reference counterpart).  Between
    atm_ensemble_graph_begin and atm_ensemble_graph_end the batched calls
