@@ -1360,8 +1360,9 @@ int atm_ensemble_radiation(atm_ensemble *e, double time) {
 
 // Recording of batched steps: the launch stream starts a capture and the
 // other member streams join it (they wait for an event recorded in it), so
-// the batched launches and each member's own calls (remap, playkap) land in
-// one graph; at the end every member stream is joined back.  Each member's
+// the batched launches and any member's own calls (its playkap scan, an
+// atm_remap of its own) land in one graph; at the end every member stream
+// is joined back.  Each member's
 // host-side state is handled as atm_graph_begin / end handle one context's.
 int atm_ensemble_graph_begin(atm_ensemble *e) {
     if (!e) return ATM_ERR_ARGUMENT;

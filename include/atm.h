@@ -307,9 +307,10 @@ int atm_ensemble_remap(atm_ensemble *ens);
 int atm_ensemble_radiation(atm_ensemble *ens, double time);
 
 /* Graphs of batched steps (no reference counterpart).  Between
-   atm_ensemble_graph_begin and atm_ensemble_graph_end the batched calls
-   above and the members' own step calls (atm_remap, ...) are recorded into
-   one HIP graph instead of run: the first member's stream starts the
+   atm_ensemble_graph_begin and atm_ensemble_graph_end the batched step
+   calls above (atm_ensemble_radiation excepted: it waits on the device)
+   and the members' own step calls are recorded into one HIP graph instead
+   of run: the first member's stream starts the
    recording and the other member streams join it.  The rules of
    atm_graph_begin / end / launch hold for every member (whole steps, an
    even number of buffer switches, no deferred check pending; each launch
