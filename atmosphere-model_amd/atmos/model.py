@@ -28,7 +28,7 @@ EXPORTS = ["atm_create", "atm_destroy", "atm_last_error", "atm_set_stream", "atm
            "atm_cgrid_step", "atm_cgrid_halo", "atm_dgrid_step", "atm_dgrid_halo",
            "atm_remap_init", "atm_remap", "atm_radiation_init", "atm_radiation",
            "atm_radiation_halo", "atm_halo_exchange", "atm_check_stability", "atm_sync",
-           "atm_build_info", "atm_set_path", "atm_get_path", "atm_timer_enable", "atm_timer_read",
+           "atm_build_info", "atm_set_path", "atm_get_path", "atm_timer_enable", "atm_timer_select", "atm_timer_read",
            "atm_physics_init", "atm_physics_tendencies", "atm_apply_physics", "atm_physics_halo",
            "atm_output_init", "atm_accumulate_output", "atm_output_mean", "atm_stream_create",
            "atm_stream_destroy", "atm_graph_begin", "atm_graph_end", "atm_graph_launch", "atm_graph_free",
@@ -103,6 +103,7 @@ def load_library(path: str = LIB_PATH):
     L.atm_set_path.argtypes = [vp, ip]
     L.atm_get_path.argtypes = [vp, C.POINTER(C.c_int)]
     L.atm_timer_enable.argtypes = [vp, ip]
+    L.atm_timer_select.argtypes = [vp, C.c_uint]
     L.atm_poison.argtypes = [vp, ip, C.c_double]
     L.atm_timer_read.argtypes = [vp, ip, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     L.atm_lin_init.argtypes = [vp]
@@ -396,6 +397,11 @@ class Model:
 
     def timer_enable(self, on=True):
         self._ck(self._L.atm_timer_enable(self._h, int(bool(on))))
+
+    def timer_select(self, names=None):
+        """Time only these kernel groups (keys of TIMERS; None: all)."""
+        mask = (1 << len(TIMERS)) - 1 if names is None else sum(1 << TIMERS[n] for n in set(names))
+        self._ck(self._L.atm_timer_select(self._h, C.c_uint(mask)))
 
     def poison(self, padding=True, outer_halo=True, value=float("nan")):
         """Write `value` into memory no kernel may read for a result inside

@@ -113,7 +113,7 @@ static FusedBufs bufs_of(atm_ctx *c, int which) {
 }
 
 void atm_timer_begin(atm_ctx *c, int id) {
-    if (!c->timing) return;
+    if (!c->timing || !((c->timer_mask >> id) & 1u)) return;
     AtmTimer &t = c->timers[id];
     if (t.used + 2 > t.cap) {
         int ncap = t.cap ? 2 * t.cap : 256;
@@ -128,7 +128,7 @@ void atm_timer_begin(atm_ctx *c, int id) {
 }
 
 void atm_timer_end(atm_ctx *c, int id) {
-    if (!c->timing) return;
+    if (!c->timing || !((c->timer_mask >> id) & 1u)) return;
     AtmTimer &t = c->timers[id];
     (void)hipEventRecord(t.ev[t.used + 1], c->stream);
     t.used += 2;
@@ -816,6 +816,16 @@ int atm_timer_enable(atm_ctx *c, int on) {
     ATM_HIP(c, hipStreamSynchronize(c->stream));
     for (auto &t : c->timers) t.used = 0;
     c->timing = on != 0;
+    return ATM_OK;
+}
+
+int atm_timer_select(atm_ctx *c, unsigned groups) {
+    CHECK_CTX(c);
+    CHECK_NOT_CAPTURING(c);
+    if (groups >> ATM_T_COUNT) return atm_fail(c, ATM_ERR_ARGUMENT, "timer groups: bits 0-8");
+    ATM_HIP(c, hipStreamSynchronize(c->stream));
+    for (auto &t : c->timers) t.used = 0;
+    c->timer_mask = groups;
     return ATM_OK;
 }
 

@@ -352,6 +352,7 @@ struct atm_ctx {
     double *plk = nullptr;
     bool plk_valid = false;
     bool timing = false;
+    unsigned timer_mask = ~0u;  // the kernel groups timed while timing (bit per ATM_T_*)
     AtmTimer timers[ATM_T_COUNT];
     double *ak = nullptr, *bk = nullptr;  // device, nlay+1
     // mod_fields

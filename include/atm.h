@@ -210,6 +210,10 @@ int atm_get_path(atm_ctx *ctx, int *path);
    8 physics. */
 int atm_timer_enable(atm_ctx *ctx, int on);
 int atm_timer_read(atm_ctx *ctx, int kernel, int64_t *launches, double *total_ms);
+/* The kernel groups timed while timing is enabled: bit g for group g
+   above (default all); fewer groups record fewer events in the stream.
+   Resets the counters. */
+int atm_timer_select(atm_ctx *ctx, unsigned groups);
 
 /* Read-extent diagnostics (no counterpart in the reference; for tests).
    atm_poison writes `value` (a NaN, say) into device memory that no kernel
