@@ -342,19 +342,35 @@ def mixed_timing(args, ic, cfg_kw):
         t += driver.step(m, cfg, st, t)
         assert m.check_stability()
     m.sync()
+    # timed as the FP64 steps are: events around the roofline's candidates
+    # only, each step's check read once the next step is issued
+    m.timer_select(ROOFLINE_GROUPS)
     m.timer_enable(True)
     t0 = time.perf_counter()
+    pending = False
     for _ in range(args.steps):
         t += driver.step(m, cfg, st, t)
-        if not m.check_stability():
+        stable = m.check_stability_end() if pending else True
+        m.check_stability_begin()
+        pending = True
+        if not stable:
             raise SystemExit("mixed: unstable")
+    if pending and not m.check_stability_end():
+        raise SystemExit("mixed: unstable")
     m.sync()
     sec = time.perf_counter() - t0
+    # every group over a short pass after the timed steps
+    n_groups = max(10, args.steps // 5 // args.remap_every * args.remap_every)
+    m.timer_select(None)
+    m.timer_enable(True)
+    for _ in range(n_groups):
+        t += driver.step(m, cfg, st, t)
+        assert m.check_stability()
     kernels = {}
     for name in ("c1", "c2", "d1", "d2", "scan", "halo", "remap"):
         n, ms = m.timer_read(name)
         if n:
-            kernels[name] = ms / args.steps
+            kernels[name] = ms / n_groups
     m.timer_enable(False)
     rad = radiation_timing(m, args, t) if args.radiation_calls > 0 else None
     del m
